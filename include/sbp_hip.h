@@ -1,0 +1,163 @@
+/*
+ * sbp_hip.h -- C-ABI of the MI355X (gfx950) SBP-SAT hot path.
+ *
+ * The reference drives its matrix-free right-hand side through PETSc's TS:
+ *   PetscErrorCode rhs(TS ts, PetscReal t, Vec v_src, Vec v_dst, void *ctx)
+ * registered with TSSetRHSFunction (code/src/time_stepping/ts_rk.cpp:21) and
+ * advanced by ts_rk4(da, t_end, dt, v, rhs, ctx) (code/include/time_stepping/ts_rk.h:26,
+ * code/src/time_stepping/ts_rk.cpp:54-65).  Every entry point below replaces one
+ * of those pieces; the reference interface each one stands in for is cited at
+ * the declaration.  Plain pointers and sizes only: no HIP, PETSc or C++ types.
+ *
+ * Arrays crossing the boundary use the reference's own single-rank layout,
+ * AoS v[j][i][c] at flat index C*(i + nx*j) + c (code/include/grids/layout.h:118-127,
+ * code/src/grids/create_layout.cpp:39-41), in double precision.
+ *
+ * Return codes mirror PetscErrorCode: 0 on success, a negative SBP_E* code on
+ * failure; sbp_last_error() then describes the failure (thread-local string).
+ */
+#ifndef SBP_HIP_H
+#define SBP_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- problem kinds (one per reference demo family) -------------------- */
+typedef enum {
+  SBP_ADVECTION_1D  = 1, /* u_t + a u_x = 0          demo/advection/advection_1D_sim.cpp  */
+  SBP_ADVECTION_2D  = 2, /* u_t + a u_x + b u_y = 0  demo/advection/advection_2D_sim.cpp  */
+  SBP_ACOUSTIC_2D   = 3, /* [u,v,p] first-order acoustics  demo/wave, demo/wave_hom       */
+  SBP_REFLECTION_1D = 4  /* [u;v]_t = [v_x;u_x]      demo/reflection/reflection_sim.cpp   */
+} sbp_physics;
+
+/* Material model of SBP_ACOUSTIC_2D. */
+typedef enum {
+  SBP_MAT_UNIT    = 0, /* rho = K = 1, no forcing: demo/wave_hom/wave_eq_hom_rhs.h          */
+  SBP_MAT_REF_RHO = 1, /* rho = 2 + x*y, K = 1, manufactured forcing: demo/wave/wave_eq_rhs.h:42-64 */
+  SBP_MAT_ARRAYS  = 2  /* caller-supplied rho^-1(x,y) and K(x,y) per grid point              */
+} sbp_material;
+
+/* Boundary treatment. SBP_BC_DEFAULT selects the reference's own choice:
+ * upwind inflow SAT for advection (advection_rhs.h:94-119,460-528), projection
+ * u=0 for reflection (reflection_rhs.h:108-120), free-surface SAT without rho^-1
+ * for acoustics (wave_eq_rhs.h:417-468). */
+typedef enum {
+  SBP_BC_DEFAULT           = 0,
+  SBP_BC_FREE_SURFACE_REF  = 1, /* p = 0 SAT exactly as wave_eq_rhs.h:425,438,452,466 (no rho^-1) */
+  SBP_BC_FREE_SURFACE      = 2, /* p = 0 SAT carrying rho^-1: energy-conserving for any rho      */
+  SBP_BC_WALL              = 3  /* v.n = 0 SAT on the p equation carrying K: energy-conserving    */
+} sbp_bc;
+
+/* Arithmetic form of the device state. */
+typedef enum {
+  SBP_FP64       = 0, /* everything in FP64 (the reference's PetscScalar)                     */
+  SBP_FP32       = 1, /* stencils, RK4 stages and state in FP32                               */
+  SBP_FP32_KAHAN = 2  /* FP32 bulk; u^{n+1} = u^n + du accumulated with Kahan compensation  */
+} sbp_precision;
+
+typedef struct sbp_problem {
+  int physics;          /* sbp_physics                                                   */
+  int order;            /* SBP interior order 2, 4 or 6 (code/include/sbpops/op_defs.h:18-30) */
+  int nx, ny;           /* grid points per direction (ny = 1 for the 1D problems)        */
+  double xl, xr;        /* domain [xl,xr] x [yl,yr]; hi = (N-1)/(xr-xl) as in the drivers */
+  double yl, yr;
+  double ax, ay;        /* advection velocities a, b (advection_2D_sim.cpp:74-75)        */
+  int material;         /* sbp_material (acoustics only)                                 */
+  int bc;               /* sbp_bc                                                         */
+  int precision;        /* sbp_precision                                                  */
+  const double *rho_inv;/* nx*ny values [j][i], SBP_MAT_ARRAYS only; copied at create    */
+  const double *bulk;   /* nx*ny values [j][i] of K, SBP_MAT_ARRAYS only; copied         */
+} sbp_problem;
+
+/* Error codes. */
+enum {
+  SBP_OK          = 0,
+  SBP_EINVAL      = -1, /* bad argument / unsupported combination      */
+  SBP_EHIP        = -2, /* HIP runtime error                           */
+  SBP_ENOMEM      = -3, /* device or host allocation failed            */
+  SBP_ENODEV      = -4, /* no usable gfx950 device                     */
+  SBP_ESTATE      = -5  /* call out of sequence (e.g. no state loaded) */
+};
+
+typedef struct sbp_ctx sbp_ctx; /* opaque: device buffers, stream, graphs */
+
+/* ---- context ------------------------------------------------------------
+ * Replaces the per-demo AppCtx (e.g. demo/wave/wave_eq_sim.cpp:36-45,119-127)
+ * together with the DMDA that sizes its vectors (wave_eq_sim.cpp:99-106). */
+int  sbp_create(const sbp_problem *desc, sbp_ctx **out);
+int  sbp_destroy(sbp_ctx *ctx);
+/* Number of components C and total values nx*ny*C of one AoS vector. */
+int  sbp_components(const sbp_ctx *ctx);
+long sbp_vector_length(const sbp_ctx *ctx);
+
+/* ---- right-hand side ----------------------------------------------------
+ * dst = F(t, src) on host AoS arrays.  Replaces the rhs_serial callbacks
+ * (advection_1D_sim.cpp:212-228, advection_2D_sim.cpp:232-249,
+ *  wave_eq_sim.cpp:291-308, wave_eq_hom_sim.cpp:291-308, reflection_sim.cpp:249-265),
+ * i.e. the region kernels of partitioned_rhs/rhs.h:363-398 plus the SAT / BC
+ * functors of partitioned_rhs/boundary_conditions.h:69-88. */
+int  sbp_rhs(sbp_ctx *ctx, double t, const double *src, double *dst);
+
+/* ---- time stepping ------------------------------------------------------
+ * Classical RK4 with fixed dt for nsteps steps, v (host AoS) in/out.
+ * Replaces ts_rk4 with an integer step count (ts_rk.cpp:54-65: TSRK4,
+ * TSADAPTNONE); the stage times t+c_s*dt, c = (0,1/2,1/2,1), and the update
+ * v += dt/6 (k1 + 2(k2+k3) + k4) follow RK4_custom
+ * (code/include/time_stepping/time_stepping.h:127-142,267-269). */
+int  sbp_rk4_advance(sbp_ctx *ctx, double *v, double t0, double dt, long nsteps);
+
+/* ---- device-resident state (the time loop stays on the GPU) -------------- */
+int  sbp_set_state(sbp_ctx *ctx, const double *v);  /* host AoS -> device state */
+int  sbp_get_state(sbp_ctx *ctx, double *v);        /* device state -> host AoS */
+/* Advance the resident state nsteps RK4 steps from time t0 (asynchronous). */
+int  sbp_step(sbp_ctx *ctx, double t0, double dt, long nsteps);
+int  sbp_synchronize(sbp_ctx *ctx);
+/* Run nsteps resident steps bracketed by HIP events on the context's stream;
+ * *ms receives the elapsed event time.  Used by the benchmark. */
+int  sbp_time_steps(sbp_ctx *ctx, double t0, double dt, long nsteps, double *ms);
+/* Time each of the four stage kernels separately (HIP events around every
+ * launch, nsteps steps); ms_stage[4] receives the mean duration per launch. */
+int  sbp_time_stages(sbp_ctx *ctx, double dt, long nsteps, double ms_stage[4]);
+
+/* ---- norms ----------------------------------------------------------------
+ * Discrete energy in the SBP norm H = Hx (x) Hy (code/include/sbpops/H_central.h:39-103
+ * gives the 1D weights; the reference's 2D norm is a stub, H_central.h:177-183):
+ *   E = sum_{j,i} Hx_i Hy_j sum_c w_c(i,j) v_c(i,j)^2
+ * with w = (rho, rho, 1/K) for acoustics and w = 1 otherwise.  FP64. */
+int  sbp_energy(sbp_ctx *ctx, const double *v, double *E);      /* host AoS vector  */
+int  sbp_state_energy(sbp_ctx *ctx, double *E);                  /* resident state   */
+/* sqrt(sum_c v_c^T H v_c): get_norm_1D (H_central.h:70-103) and its 2D analogue. */
+int  sbp_hnorm(sbp_ctx *ctx, const double *v, double *norm);
+
+/* Error norms of vec_util.h:47-61: l2 = sqrt(hx*hy)*||v-w||_2, max = ||v-w||_inf. */
+int  sbp_error_norms(const sbp_ctx *ctx, const double *v, const double *w,
+                     double *l2, double *maxerr);
+
+/* ---- batched ensemble ---------------------------------------------------
+ * Many independent 2D acoustic problems of uneven size advanced together, one
+ * persistent kernel per RK4 stage, the whole step captured in a hipGraph.
+ * No counterpart in the reference (one problem per MPI job there). */
+typedef struct sbp_batch sbp_batch;
+int  sbp_batch_create(const sbp_problem *descs, int count, sbp_batch **out);
+int  sbp_batch_destroy(sbp_batch *b);
+int  sbp_batch_set_state(sbp_batch *b, int k, const double *v);
+int  sbp_batch_get_state(sbp_batch *b, int k, double *v);
+/* dt[k] per instance; nsteps RK4 steps for every instance. */
+int  sbp_batch_step(sbp_batch *b, const double *dt, long nsteps);
+int  sbp_batch_time_steps(sbp_batch *b, const double *dt, long nsteps, double *ms);
+/* Pick the tile shape by timing candidates on this ensemble; writes the
+ * chosen configuration index to *choice. */
+int  sbp_batch_autotune(sbp_batch *b, const double *dt, int *choice);
+int  sbp_batch_synchronize(sbp_batch *b);
+
+/* ---- diagnostics ---------------------------------------------------------- */
+const char *sbp_last_error(void);
+/* Name of the device the library runs on (e.g. "gfx950"), or "" without one. */
+const char *sbp_device_arch(void);
+int  sbp_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SBP_HIP_H */

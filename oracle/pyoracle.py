@@ -1,0 +1,148 @@
+"""ctypes binding of oracle/liborc.so and oracle/libcpubase.so (test infrastructure).
+
+Problem descriptions are duck-typed: any object with the fields of
+``sbp_problem`` (physics, order, nx, ny, xl, xr, yl, yr, ax, ay, material, bc,
+precision, rho_inv, bulk) is accepted.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+class OrcProblem(C.Structure):
+    _fields_ = [("physics", C.c_int), ("order", C.c_int), ("nx", C.c_int), ("ny", C.c_int),
+                ("xl", C.c_double), ("xr", C.c_double), ("yl", C.c_double), ("yr", C.c_double),
+                ("ax", C.c_double), ("ay", C.c_double),
+                ("material", C.c_int), ("bc", C.c_int), ("precision", C.c_int),
+                ("rho_inv", C.POINTER(C.c_double)), ("bulk", C.POINTER(C.c_double))]
+
+
+class OrcOps(C.Structure):
+    _fields_ = [("order", C.c_int), ("r", C.c_int), ("cls", C.c_int), ("w", C.c_int),
+                ("inner", C.c_double * 7), ("Q", (C.c_double * 9) * 6),
+                ("hq", C.c_double * 6), ("hqinv", C.c_double * 6)]
+
+
+_libs = {}
+_DP = C.POINTER(C.c_double)
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def _load(name):
+    if name not in _libs:
+        path = os.path.join(HERE, name)
+        if not os.path.exists(path):
+            build()
+        lib = C.CDLL(path)
+        P = C.POINTER(OrcProblem)
+        lib.orc_get_ops.argtypes = [C.c_int, C.POINTER(OrcOps)]
+        lib.orc_validate.argtypes = [P]
+        lib.orc_d1.argtypes = [C.POINTER(OrcOps), _DP, C.c_long, C.c_int, C.c_int, C.c_double]
+        lib.orc_d1.restype = C.c_double
+        lib.orc_rhs.argtypes = [P, C.c_double, _DP, _DP]
+        lib.orc_rk4.argtypes = [P, _DP, C.c_double, C.c_double, C.c_long]
+        lib.orc_rk4_par.argtypes = [P, _DP, C.c_double, C.c_double, C.c_long, C.c_int]
+        lib.orc_energy.argtypes = [P, _DP]
+        lib.orc_energy.restype = C.c_double
+        lib.orc_hnorm.argtypes = [P, _DP]
+        lib.orc_hnorm.restype = C.c_double
+        lib.orc_error_norms.argtypes = [P, _DP, _DP, _DP, _DP]
+        _libs[name] = lib
+    return _libs[name]
+
+
+def lib(fast: bool = False):
+    return _load("libcpubase.so" if fast else "liborc.so")
+
+
+class _Desc:
+    """Keeps the material arrays alive while the struct points at them."""
+
+    def __init__(self, spec):
+        self.keep = []
+        s = OrcProblem()
+        for f, _ in OrcProblem._fields_:
+            if f in ("rho_inv", "bulk"):
+                arr = getattr(spec, f, None)
+                if arr is None:
+                    setattr(s, f, None)
+                else:
+                    a = np.ascontiguousarray(arr, dtype=np.float64)
+                    self.keep.append(a)
+                    setattr(s, f, a.ctypes.data_as(_DP))
+            else:
+                setattr(s, f, getattr(spec, f))
+        self.s = s
+
+    @property
+    def ref(self):
+        return C.byref(self.s)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_DP)
+
+
+def ops(order: int) -> OrcOps:
+    o = OrcOps()
+    if lib().orc_get_ops(order, C.byref(o)):
+        raise ValueError(order)
+    return o
+
+
+def validate(spec) -> bool:
+    return lib().orc_validate(_Desc(spec).ref) == 0
+
+
+def d1(order: int, v: np.ndarray, i: int, hi: float = 1.0) -> float:
+    o = ops(order)
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    return lib().orc_d1(C.byref(o), _ptr(v), 1, len(v), i, hi)
+
+
+def rhs(spec, t: float, v: np.ndarray) -> np.ndarray:
+    d = _Desc(spec)
+    src = np.ascontiguousarray(v, dtype=np.float64)
+    dst = np.zeros_like(src)
+    if lib().orc_rhs(d.ref, t, _ptr(src), _ptr(dst)):
+        raise ValueError("orc_rhs rejected the problem")
+    return dst
+
+
+def rk4(spec, v: np.ndarray, t0: float, dt: float, nsteps: int, threads: int = 1,
+        fast: bool = False) -> np.ndarray:
+    d = _Desc(spec)
+    out = np.array(v, dtype=np.float64, copy=True)
+    rc = lib(fast).orc_rk4_par(d.ref, _ptr(out), t0, dt, nsteps, threads)
+    if rc:
+        raise ValueError("orc_rk4 rejected the problem")
+    return out
+
+
+def energy(spec, v: np.ndarray) -> float:
+    d = _Desc(spec)
+    return lib().orc_energy(d.ref, _ptr(np.ascontiguousarray(v, dtype=np.float64)))
+
+
+def hnorm(spec, v: np.ndarray) -> float:
+    d = _Desc(spec)
+    return lib().orc_hnorm(d.ref, _ptr(np.ascontiguousarray(v, dtype=np.float64)))
+
+
+def error_norms(spec, v: np.ndarray, w: np.ndarray):
+    d = _Desc(spec)
+    l2 = C.c_double()
+    mx = C.c_double()
+    lib().orc_error_norms(d.ref, _ptr(np.ascontiguousarray(v, dtype=np.float64)),
+                          _ptr(np.ascontiguousarray(w, dtype=np.float64)),
+                          C.byref(l2), C.byref(mx))
+    return l2.value, mx.value

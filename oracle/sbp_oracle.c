@@ -1,0 +1,625 @@
+/*
+ * sbp_oracle.c -- CPU restatement of the reference SBP-SAT hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY (see sbp_oracle.h).  Built twice from this one
+ * source by oracle/Makefile:
+ *   liborc.so      strict IEEE (-O2 -ffp-contract=off, no fast-math): the
+ *                  parity oracle for the HIP path;
+ *   libcpubase.so  the reference's opt flags (-O3 -ffast-math, code/Makefile:17)
+ *                  with pthreads: the timed CPU baseline of the benchmark.
+ *
+ * Every function cites the reference lines it restates.  The region split of
+ * partitioned_rhs/rhs.h:363-398 (ll, li, lr, il, ii, ir, rl, ri, rr) is kept
+ * as a per-direction choice left / interior / right of the closure, which
+ * selects exactly the same stencil for every grid point.
+ */
+#define _POSIX_C_SOURCE 200809L
+#include "sbp_oracle.h"
+
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* PETSC_PI as used by wave_eq_rhs.h:54,63 and wave_eq_sim.cpp:226-228. */
+#define ORC_PI 3.14159265358979323846264338327950288419716939937510582097494459
+
+/* ------------------------------------------------------------------------ */
+/* Operator tables                                                            */
+/* ------------------------------------------------------------------------ */
+/* Coefficients of the central SBP first-derivative operators, their norms
+ * and inverse norms.  The rational numbers are the operators' definitions and
+ * must be identical to code/include/sbpops/D1_central.h:243-273 (stencils),
+ * H_central.h:193-212 (closure_quad) and HI_central.h:128-147
+ * (closure_invquad).  They are written as the same decimal quotients so the
+ * compiled doubles are bit-identical. */
+int orc_get_ops(int order, orc_ops *op)
+{
+  memset(op, 0, sizeof(*op));
+  op->order = order;
+  if (order == 2) {
+    op->r = 1; op->cls = 1; op->w = 2;
+    op->inner[0] = -1./2; op->inner[1] = 0; op->inner[2] = 1./2;
+    op->Q[0][0] = -1.; op->Q[0][1] = 1;
+    op->hq[0] = 1./2;
+    op->hqinv[0] = 2.;
+    return 0;
+  }
+  if (order == 4) {
+    static const double q4[4][6] = {
+      {-24./17, 59./34, -4./17, -3./34, 0, 0},
+      {-1./2, 0, 1./2, 0, 0, 0},
+      {4./43, -59./86, 0, 59./86, -4./43, 0},
+      {3./98, 0, -59./98, 0, 32./49, -4./49}};
+    static const double in4[5] = {1./12, -2./3, 0., 2./3, -1./12};
+    static const double h4[4] = {17./48, 59./48, 43./48, 49./48};
+    static const double hi4[4] = {48./17, 48./59, 48./43, 48./49};
+    op->r = 2; op->cls = 4; op->w = 6;
+    memcpy(op->inner, in4, sizeof(in4));
+    for (int a = 0; a < 4; a++)
+      for (int b = 0; b < 6; b++) op->Q[a][b] = q4[a][b];
+    memcpy(op->hq, h4, sizeof(h4));
+    memcpy(op->hqinv, hi4, sizeof(hi4));
+    return 0;
+  }
+  if (order == 6) {
+    static const double q6[6][9] = {
+      {-21600./13649, 104009./54596, 30443./81894, -33311./27298, 16863./27298, -15025./163788, 0, 0, 0},
+      {-104009./240260, 0, -311./72078, 20229./24026, -24337./48052, 36661./360390, 0, 0, 0},
+      {-30443./162660, 311./32532, 0, -11155./16266, 41287./32532, -21999./54220, 0, 0, 0},
+      {33311./107180, -20229./21436, 485./1398, 0, 4147./21436, 25427./321540, 72./5359, 0, 0},
+      {-16863./78770, 24337./31508, -41287./47262, -4147./15754, 0, 342523./472620, -1296./7877, 144./7877, 0},
+      {15025./525612, -36661./262806, 21999./87602, -25427./262806, -342523./525612, 0, 32400./43801, -6480./43801, 720./43801}};
+    static const double in6[7] = {-1./60, 3./20, -3./4, 0, 3./4, -3./20, 1./60};
+    static const double h6[6] = {13649./43200, 12013./8640, 2711./4320, 5359./4320, 7877./8640, 43801./43200};
+    static const double hi6[6] = {43200./13649, 8640./12013, 4320./2711, 4320./5359, 8640./7877, 43200./43801};
+    op->r = 3; op->cls = 6; op->w = 9;
+    memcpy(op->inner, in6, sizeof(in6));
+    for (int a = 0; a < 6; a++)
+      for (int b = 0; b < 9; b++) op->Q[a][b] = q6[a][b];
+    memcpy(op->hq, h6, sizeof(h6));
+    memcpy(op->hqinv, hi6, sizeof(hi6));
+    return 0;
+  }
+  return -1;
+}
+
+int orc_components(const sbp_problem *p)
+{
+  switch (p->physics) {
+    case SBP_ADVECTION_1D: case SBP_ADVECTION_2D: return 1;
+    case SBP_ACOUSTIC_2D: return 3;   /* [u, v, p]: wave_eq_rhs.h:12-16 */
+    case SBP_REFLECTION_1D: return 2; /* [u, v]: reflection_rhs.h:8   */
+  }
+  return 0;
+}
+
+static int is_1d(const sbp_problem *p)
+{
+  return p->physics == SBP_ADVECTION_1D || p->physics == SBP_REFLECTION_1D;
+}
+
+int orc_validate(const sbp_problem *p)
+{
+  orc_ops op;
+  if (!p || orc_get_ops(p->order, &op)) return -1;
+  if (orc_components(p) == 0) return -1;
+  int taps = op.w > 2 * op.r + 1 ? op.w : 2 * op.r + 1;
+  int nmin = 2 * op.cls > taps ? 2 * op.cls : taps;
+  if (p->nx < nmin) return -1;
+  if (is_1d(p)) { if (p->ny != 1) return -1; }
+  else if (p->ny < nmin) return -1;
+  if (p->physics == SBP_ACOUSTIC_2D && p->material == SBP_MAT_ARRAYS &&
+      (!p->rho_inv || !p->bulk)) return -1;
+  return 0;
+}
+
+double orc_hi(int n, double lo, double hi_end) { return (n - 1) / (hi_end - lo); }
+
+/* ------------------------------------------------------------------------ */
+/* D1 application: D1_central.h:57-104 (1D) and :119-231 (2D).               */
+/* kind 0 = left closure, 1 = interior, 2 = right closure.                   */
+/* ------------------------------------------------------------------------ */
+static inline double d1_kind(const orc_ops *op, const double *v, long s, int n, int i,
+                             double hi, int kind)
+{
+  double u = 0;
+  if (kind == 0) {               /* apply_left: sum_is Q[i][is] v[is]            */
+    for (int k = 0; k < op->w; k++) u += op->Q[i][k] * v[k * s];
+  } else if (kind == 1) {        /* apply_interior: sum stencil[is] v[i-r+is]   */
+    for (int k = 0; k < 2 * op->r + 1; k++) u += op->inner[k] * v[(long)(i - op->r + k) * s];
+  } else {                       /* apply_right: u -= Q[N-i-1][w-is-1] v[N-w+is] */
+    for (int k = 0; k < op->w; k++)
+      u -= op->Q[n - i - 1][op->w - k - 1] * v[(long)(n - op->w + k) * s];
+  }
+  return hi * u;
+}
+
+static inline int region(const orc_ops *op, int n, int i)
+{
+  return i < op->cls ? 0 : (i >= n - op->cls ? 2 : 1);
+}
+
+double orc_d1(const orc_ops *op, const double *v, long stride, int n, int i, double hi)
+{
+  return d1_kind(op, v, stride, n, i, hi, region(op, n, i));
+}
+
+/* ------------------------------------------------------------------------ */
+/* Problem helpers                                                            */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+  const sbp_problem *p;
+  orc_ops op;
+  int C, nx, ny;
+  double hix, hiy;
+} orc_setup;
+
+static void setup(const sbp_problem *p, orc_setup *s)
+{
+  s->p = p;
+  orc_get_ops(p->order, &s->op);
+  s->C = orc_components(p);
+  s->nx = p->nx;
+  s->ny = p->ny;
+  s->hix = orc_hi(p->nx, p->xl, p->xr);
+  s->hiy = is_1d(p) ? 0.0 : orc_hi(p->ny, p->yl, p->yr);
+}
+
+/* rho_inv of wave_eq_rhs.h:42-46 (x = xl + i/hi, not i*h). */
+static inline double ref_rho_inv(const orc_setup *s, int i, int j)
+{
+  double x = s->p->xl + i / s->hix;
+  double y = s->p->yl + j / s->hiy;
+  return 1. / (2 + x * y);
+}
+
+/* forcing_u / forcing_v of wave_eq_rhs.h:51-64. */
+static inline double ref_forcing_u(const orc_setup *s, int i, int j, double t)
+{
+  double x = s->p->xl + i / s->hix;
+  double y = s->p->yl + j / s->hiy;
+  return -(3 * ORC_PI * cos(5 * ORC_PI * t) * cos(3 * ORC_PI * x) * sin(4 * ORC_PI * y) * (x * y + 1)) / (x * y + 2);
+}
+
+static inline double ref_forcing_v(const orc_setup *s, int i, int j, double t)
+{
+  double x = s->p->xl + i / s->hix;
+  double y = s->p->yl + j / s->hiy;
+  return -(4 * ORC_PI * cos(5 * ORC_PI * t) * cos(4 * ORC_PI * y) * sin(3 * ORC_PI * x) * (x * y + 1)) / (x * y + 2);
+}
+
+static inline double mat_rinv(const orc_setup *s, int i, int j)
+{
+  if (s->p->material == SBP_MAT_REF_RHO) return ref_rho_inv(s, i, j);
+  if (s->p->material == SBP_MAT_ARRAYS) return s->p->rho_inv[(long)j * s->nx + i];
+  return 1.0;
+}
+
+static inline double mat_bulk(const orc_setup *s, int i, int j)
+{
+  if (s->p->material == SBP_MAT_ARRAYS) return s->p->bulk[(long)j * s->nx + i];
+  return 1.0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Region kernels over a block [i0,i1) x [j0,j1)                             */
+/* ------------------------------------------------------------------------ */
+
+/* 1D advection: advection_{l,i,r} (advection_rhs.h:13-51): dst = -a D1 u. */
+static void adv1d_block(const orc_setup *s, const double *src, double *dst, int i0, int i1)
+{
+  const double a = s->p->ax;
+  for (int i = i0; i < i1; i++)
+    dst[i] = -a * orc_d1(&s->op, src, 1, s->nx, i, s->hix);
+}
+
+/* 1D reflection: reflection_{l,i,r} (reflection_rhs.h:10-47):
+ * dst(i,1) = D1 src(:,0), dst(i,0) = D1 src(:,1). */
+static void refl_block(const orc_setup *s, const double *src, double *dst, int i0, int i1)
+{
+  for (int i = i0; i < i1; i++) {
+    dst[2 * i + 1] = orc_d1(&s->op, src + 0, 2, s->nx, i, s->hix);
+    dst[2 * i + 0] = orc_d1(&s->op, src + 1, 2, s->nx, i, s->hix);
+  }
+}
+
+/* 2D advection: advection_{ll..rr} (advection_rhs.h:156-393):
+ * dst = -(a_x D_x u + a_y D_y u). */
+static void adv2d_block(const orc_setup *s, const double *src, double *dst,
+                        int i0, int i1, int j0, int j1)
+{
+  const int nx = s->nx, ny = s->ny;
+  const double ax = s->p->ax, ay = s->p->ay;
+  for (int j = j0; j < j1; j++) {
+    const int ky = region(&s->op, ny, j);
+    const double *row = src + (long)nx * j;
+    const double *col0 = src;
+    for (int i = i0; i < i1; i++) {
+      const int kx = region(&s->op, nx, i);
+      double dx = d1_kind(&s->op, row, 1, nx, i, s->hix, kx);
+      double dy = d1_kind(&s->op, col0 + i, nx, ny, j, s->hiy, ky);
+      dst[(long)nx * j + i] = -(ax * dx + ay * dy);
+    }
+  }
+}
+
+/* 2D acoustics: wave_eq_{ll..rr} (wave_eq_rhs.h:76-322) and
+ * wave_eq_hom_{ll..rr} (wave_eq_hom_rhs.h:49-296):
+ *   F_u = -rho^-1 D_x p (+ f_u),  F_v = -rho^-1 D_y p (+ f_v),  F_p = -(D_x u) - (D_y v).
+ * With caller material arrays the pressure row carries K: F_p = -K (D_x u + D_y v). */
+static void acoustic_block(const orc_setup *s, double t, const double *src, double *dst,
+                           int i0, int i1, int j0, int j1)
+{
+  const int nx = s->nx, ny = s->ny;
+  const long rs = 3L * nx; /* row stride */
+  const int mat = s->p->material;
+  for (int j = j0; j < j1; j++) {
+    const int ky = region(&s->op, ny, j);
+    const double *row = src + rs * j;
+    for (int i = i0; i < i1; i++) {
+      const int kx = region(&s->op, nx, i);
+      const double *col = src + 3L * i;
+      double dxp = d1_kind(&s->op, row + 2, 3, nx, i, s->hix, kx);
+      double dyp = d1_kind(&s->op, col + 2, rs, ny, j, s->hiy, ky);
+      double dxu = d1_kind(&s->op, row + 0, 3, nx, i, s->hix, kx);
+      double dyv = d1_kind(&s->op, col + 1, rs, ny, j, s->hiy, ky);
+      double *F = dst + rs * j + 3L * i;
+      if (mat == SBP_MAT_UNIT) {
+        F[0] = -dxp;
+        F[1] = -dyp;
+        F[2] = -dxu - dyv;
+      } else if (mat == SBP_MAT_REF_RHO) {
+        double ri = ref_rho_inv(s, i, j);
+        F[0] = -ri * dxp + ref_forcing_u(s, i, j, t);
+        F[1] = -ri * dyp + ref_forcing_v(s, i, j, t);
+        F[2] = -dxu - dyv;
+      } else {
+        double ri = mat_rinv(s, i, j), K = mat_bulk(s, i, j);
+        F[0] = -ri * dxp;
+        F[1] = -ri * dyp;
+        F[2] = -K * (dxu + dyv);
+      }
+    }
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Boundary conditions on the part of the boundary inside a block            */
+/* ------------------------------------------------------------------------ */
+
+/* SAT_bc_l / SAT_bc_r (advection_rhs.h:95-119).  The right SAT is written
+ * with the correct inverse weight closure_invquad[0]; the reference reads
+ * closure_invquad[nx-1] there but multiplies it by tau = 0 for a >= 0. */
+static void adv1d_bc(const orc_setup *s, const double *src, double *dst, int i0, int i1)
+{
+  const int nx = s->nx;
+  const double hq = s->hix * s->op.hqinv[0];
+  if (i0 == 0) {
+    double a_l = s->p->ax;
+    double tau = -0.5 * (a_l + fabs(a_l));
+    dst[0] += 0.5 * tau * (hq * src[0]);
+  }
+  if (i1 == nx) {
+    double a_r = s->p->ax;
+    double tau = 0.5 * (a_r - fabs(a_r));
+    dst[nx - 1] += 0.5 * tau * (hq * src[nx - 1]);
+  }
+}
+
+/* proj_dirichlet_bc_{l,r} (reflection_rhs.h:108-120): u_t = 0 at both ends. */
+static void refl_bc(const orc_setup *s, double *dst, int i0, int i1)
+{
+  if (i0 == 0) dst[0] = 0.0;
+  if (i1 == s->nx) dst[2 * (s->nx - 1)] = 0.0;
+}
+
+/* SAT_bc_{west,south,east,north} (advection_rhs.h:461-528), applied in the
+ * bc_serial order W, S, E, N (boundary_conditions.h:84-87).  The south SAT
+ * uses the y-spacing and closure_invquad[0]; the reference uses
+ * hi[0]*closure_invquad[i] there, multiplied by tau_s = 0 for b <= 0. */
+static void adv2d_bc(const orc_setup *s, const double *src, double *dst,
+                     int i0, int i1, int j0, int j1)
+{
+  const int nx = s->nx, ny = s->ny;
+  const double hqx = s->hix * s->op.hqinv[0], hqy = s->hiy * s->op.hqinv[0];
+  const double ax = s->p->ax, ay = s->p->ay;
+  if (i0 == 0)
+    for (int j = j0; j < j1; j++) {
+      double tau = -0.5 * (ax + fabs(ax));
+      dst[(long)nx * j] += 0.5 * tau * (hqx * src[(long)nx * j]);
+    }
+  if (j0 == 0)
+    for (int i = i0; i < i1; i++) {
+      double tau = -0.5 * (ay + fabs(ay));
+      dst[i] += 0.5 * tau * (hqy * src[i]);
+    }
+  if (i1 == nx)
+    for (int j = j0; j < j1; j++) {
+      double tau = 0.5 * (ax - fabs(ax));
+      long k = (long)nx * j + nx - 1;
+      dst[k] += 0.5 * tau * (hqx * src[k]);
+    }
+  if (j1 == ny)
+    for (int i = i0; i < i1; i++) {
+      double tau = 0.5 * (ay - fabs(ay));
+      long k = (long)nx * (ny - 1) + i;
+      dst[k] += 0.5 * tau * (hqy * src[k]);
+    }
+}
+
+/* Acoustic boundary terms, W, S, E, N order as bc_serial.
+ *  FREE_SURFACE_REF: free_surface_bc_* of wave_eq_rhs.h:417-468 -- F_u -= hix qinv0 p (W),
+ *                    F_v -= hiy qinv0 p (S), F_u += ... (E), F_v += ... (N).
+ *  FREE_SURFACE:     the same terms multiplied by rho^-1 at the boundary point.
+ *  WALL:             normal velocity zero, imposed on the pressure equation:
+ *                    F_p -= K hix qinv0 u (W), F_p -= K hiy qinv0 v (S),
+ *                    F_p += K hix qinv0 u (E), F_p += K hiy qinv0 v (N). */
+static void acoustic_bc(const orc_setup *s, const double *src, double *dst,
+                        int i0, int i1, int j0, int j1)
+{
+  const int nx = s->nx, ny = s->ny;
+  const long rs = 3L * nx;
+  const double hqx = s->hix * s->op.hqinv[0], hqy = s->hiy * s->op.hqinv[0];
+  int bc = s->p->bc == SBP_BC_DEFAULT ? SBP_BC_FREE_SURFACE_REF : s->p->bc;
+#define Q(j, i, c) src[rs * (j) + 3L * (i) + (c)]
+#define F(j, i, c) dst[rs * (j) + 3L * (i) + (c)]
+  if (bc == SBP_BC_FREE_SURFACE_REF) {
+    if (i0 == 0)      for (int j = j0; j < j1; j++) F(j, 0, 0) -= hqx * Q(j, 0, 2);
+    if (j0 == 0)      for (int i = i0; i < i1; i++) F(0, i, 1) -= hqy * Q(0, i, 2);
+    if (i1 == nx)     for (int j = j0; j < j1; j++) F(j, nx - 1, 0) += hqx * Q(j, nx - 1, 2);
+    if (j1 == ny)     for (int i = i0; i < i1; i++) F(ny - 1, i, 1) += hqy * Q(ny - 1, i, 2);
+  } else if (bc == SBP_BC_FREE_SURFACE) {
+    if (i0 == 0)  for (int j = j0; j < j1; j++) F(j, 0, 0) -= mat_rinv(s, 0, j) * (hqx * Q(j, 0, 2));
+    if (j0 == 0)  for (int i = i0; i < i1; i++) F(0, i, 1) -= mat_rinv(s, i, 0) * (hqy * Q(0, i, 2));
+    if (i1 == nx) for (int j = j0; j < j1; j++) F(j, nx - 1, 0) += mat_rinv(s, nx - 1, j) * (hqx * Q(j, nx - 1, 2));
+    if (j1 == ny) for (int i = i0; i < i1; i++) F(ny - 1, i, 1) += mat_rinv(s, i, ny - 1) * (hqy * Q(ny - 1, i, 2));
+  } else { /* WALL */
+    if (i0 == 0)  for (int j = j0; j < j1; j++) F(j, 0, 2) -= mat_bulk(s, 0, j) * (hqx * Q(j, 0, 0));
+    if (j0 == 0)  for (int i = i0; i < i1; i++) F(0, i, 2) -= mat_bulk(s, i, 0) * (hqy * Q(0, i, 1));
+    if (i1 == nx) for (int j = j0; j < j1; j++) F(j, nx - 1, 2) += mat_bulk(s, nx - 1, j) * (hqx * Q(j, nx - 1, 0));
+    if (j1 == ny) for (int i = i0; i < i1; i++) F(ny - 1, i, 2) += mat_bulk(s, i, ny - 1) * (hqy * Q(ny - 1, i, 1));
+  }
+#undef Q
+#undef F
+}
+
+/* Right-hand side on one block: region kernels then boundary functors
+ * (the *_serial + *_bc_serial pair of every rhs_serial callback). */
+static void rhs_block(const orc_setup *s, double t, const double *src, double *dst,
+                      int i0, int i1, int j0, int j1)
+{
+  switch (s->p->physics) {
+    case SBP_ADVECTION_1D:
+      adv1d_block(s, src, dst, i0, i1);
+      adv1d_bc(s, src, dst, i0, i1);
+      break;
+    case SBP_REFLECTION_1D:
+      refl_block(s, src, dst, i0, i1);
+      refl_bc(s, dst, i0, i1);
+      break;
+    case SBP_ADVECTION_2D:
+      adv2d_block(s, src, dst, i0, i1, j0, j1);
+      adv2d_bc(s, src, dst, i0, i1, j0, j1);
+      break;
+    case SBP_ACOUSTIC_2D:
+      acoustic_block(s, t, src, dst, i0, i1, j0, j1);
+      acoustic_bc(s, src, dst, i0, i1, j0, j1);
+      break;
+  }
+}
+
+int orc_rhs(const sbp_problem *p, double t, const double *src, double *dst)
+{
+  if (orc_validate(p)) return -1;
+  orc_setup s;
+  setup(p, &s);
+  rhs_block(&s, t, src, dst, 0, s.nx, 0, s.ny);
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* RK4: RK4_custom (time_stepping.h:98-152), apply_WAXPY (:232-252),        */
+/* rk4_util (:254-278).                                                       */
+/* ------------------------------------------------------------------------ */
+int orc_rk4(const sbp_problem *p, double *v, double t0, double dt, long nsteps)
+{
+  return orc_rk4_par(p, v, t0, dt, nsteps, 1);
+}
+
+typedef struct {
+  const orc_setup *s;
+  double *v, *k1, *k2, *k3, *k4, *tmp;
+  double t0, dt;
+  long nsteps, len;
+  int i0, i1, j0, j1;
+  pthread_barrier_t *bar;
+  int nthreads;
+} orc_worker;
+
+static inline void waxpy_block(const orc_setup *s, double *w, double alpha, const double *x,
+                               const double *y, int i0, int i1, int j0, int j1)
+{
+  const int C = s->C;
+  for (int j = j0; j < j1; j++) {
+    long b = (long)C * ((long)s->nx * j + i0), e = (long)C * ((long)s->nx * j + i1);
+    for (long k = b; k < e; k++) w[k] = y[k] + alpha * x[k];
+  }
+}
+
+static inline void rk4_util_block(const orc_setup *s, double *v, const double *k1,
+                                  const double *k2, const double *k3, const double *k4,
+                                  double alpha, int i0, int i1, int j0, int j1)
+{
+  const int C = s->C;
+  for (int j = j0; j < j1; j++) {
+    long b = (long)C * ((long)s->nx * j + i0), e = (long)C * ((long)s->nx * j + i1);
+    for (long k = b; k < e; k++) v[k] += alpha * (k1[k] + 2 * (k2[k] + k3[k]) + k4[k]);
+  }
+}
+
+static inline void sync_workers(orc_worker *w)
+{
+  if (w->nthreads > 1) pthread_barrier_wait(w->bar);
+}
+
+static void *rk4_worker(void *arg)
+{
+  orc_worker *w = (orc_worker *)arg;
+  const orc_setup *s = w->s;
+  const double dt = w->dt, dt2 = 0.5 * dt, dt6 = dt / 6;
+  double t = w->t0;
+  for (long n = 0; n < w->nsteps; n++) {
+    rhs_block(s, t, w->v, w->k1, w->i0, w->i1, w->j0, w->j1);
+    sync_workers(w); /* k1 complete before its use is irrelevant (pointwise), but v
+                        must stay untouched until every block has read its halo */
+    waxpy_block(s, w->tmp, dt2, w->k1, w->v, w->i0, w->i1, w->j0, w->j1);
+    sync_workers(w);
+    rhs_block(s, t + dt2, w->tmp, w->k2, w->i0, w->i1, w->j0, w->j1);
+    sync_workers(w);
+    waxpy_block(s, w->tmp, dt2, w->k2, w->v, w->i0, w->i1, w->j0, w->j1);
+    sync_workers(w);
+    rhs_block(s, t + dt2, w->tmp, w->k3, w->i0, w->i1, w->j0, w->j1);
+    sync_workers(w);
+    waxpy_block(s, w->tmp, dt, w->k3, w->v, w->i0, w->i1, w->j0, w->j1);
+    sync_workers(w);
+    rhs_block(s, t + dt, w->tmp, w->k4, w->i0, w->i1, w->j0, w->j1);
+    sync_workers(w);
+    rk4_util_block(s, w->v, w->k1, w->k2, w->k3, w->k4, dt6, w->i0, w->i1, w->j0, w->j1);
+    sync_workers(w);
+    t = t + dt;
+  }
+  return NULL;
+}
+
+/* Near-square px x py factorisation of n ranks, as PETSC_DECIDE does for
+ * DMDACreate2d (wave_eq_sim.cpp:100-101): px/py close to nx/ny. */
+static void decompose(int n, int nx, int ny, int is1d, int *px, int *py)
+{
+  if (is1d) { *px = n; *py = 1; return; }
+  int best = 1; double bestscore = 1e300;
+  for (int a = 1; a <= n; a++) {
+    if (n % a) continue;
+    int b = n / a;
+    double sx = (double)nx / a, sy = (double)ny / b;
+    double score = sx > sy ? sx / sy : sy / sx;
+    if (score < bestscore) { bestscore = score; best = a; }
+  }
+  *px = best; *py = n / best;
+}
+
+int orc_rk4_par(const sbp_problem *p, double *v, double t0, double dt, long nsteps, int nthreads)
+{
+  if (orc_validate(p)) return -1;
+  orc_setup s;
+  setup(p, &s);
+  long len = (long)s.C * s.nx * s.ny;
+  double *buf = (double *)malloc(sizeof(double) * 5 * len);
+  if (!buf) return -1;
+  if (nthreads < 1) nthreads = 1;
+  int px, py;
+  decompose(nthreads, s.nx, s.ny, is_1d(p), &px, &py);
+  int nt = px * py;
+  orc_worker *ws = (orc_worker *)calloc(nt, sizeof(orc_worker));
+  pthread_t *th = (pthread_t *)calloc(nt, sizeof(pthread_t));
+  pthread_barrier_t bar;
+  if (nt > 1) pthread_barrier_init(&bar, NULL, nt);
+  for (int b = 0; b < py; b++)
+    for (int a = 0; a < px; a++) {
+      orc_worker *w = &ws[b * px + a];
+      w->s = &s; w->v = v;
+      w->k1 = buf; w->k2 = buf + len; w->k3 = buf + 2 * len; w->k4 = buf + 3 * len;
+      w->tmp = buf + 4 * len;
+      w->t0 = t0; w->dt = dt; w->nsteps = nsteps; w->len = len;
+      w->i0 = (int)((long)s.nx * a / px); w->i1 = (int)((long)s.nx * (a + 1) / px);
+      w->j0 = (int)((long)s.ny * b / py); w->j1 = (int)((long)s.ny * (b + 1) / py);
+      w->bar = &bar; w->nthreads = nt;
+    }
+  if (nt == 1) {
+    rk4_worker(&ws[0]);
+  } else {
+    for (int k = 0; k < nt; k++) pthread_create(&th[k], NULL, rk4_worker, &ws[k]);
+    for (int k = 0; k < nt; k++) pthread_join(th[k], NULL);
+    pthread_barrier_destroy(&bar);
+  }
+  free(th);
+  free(ws);
+  free(buf);
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Norms                                                                      */
+/* ------------------------------------------------------------------------ */
+/* Diagonal SBP norm weight of index i on n points (H_central.h:39-58: h*q_i
+ * in the closures, h in the interior, mirrored on the right). */
+static inline double hweight(const orc_ops *op, int n, int i, double h)
+{
+  if (i < op->cls) return h * op->hq[i];
+  if (i >= n - op->cls) return h * op->hq[n - 1 - i];
+  return h;
+}
+
+double orc_energy(const sbp_problem *p, const double *v)
+{
+  if (orc_validate(p)) return -1;
+  orc_setup s;
+  setup(p, &s);
+  const double hx = 1.0 / s.hix, hy = is_1d(p) ? 1.0 : 1.0 / s.hiy;
+  double E = 0;
+  for (int j = 0; j < s.ny; j++) {
+    double wy = is_1d(p) ? 1.0 : hweight(&s.op, s.ny, j, hy);
+    for (int i = 0; i < s.nx; i++) {
+      double wx = hweight(&s.op, s.nx, i, hx);
+      const double *q = v + (long)s.C * ((long)s.nx * j + i);
+      double e = 0;
+      if (p->physics == SBP_ACOUSTIC_2D) {
+        double rho = 1.0 / mat_rinv(&s, i, j);
+        double kinv = 1.0 / mat_bulk(&s, i, j);
+        e = rho * (q[0] * q[0] + q[1] * q[1]) + kinv * q[2] * q[2];
+      } else {
+        for (int c = 0; c < s.C; c++) e += q[c] * q[c];
+      }
+      E += wx * wy * e;
+    }
+  }
+  return E;
+}
+
+/* get_norm_1D (H_central.h:70-103) and its 2D tensor-product analogue. */
+double orc_hnorm(const sbp_problem *p, const double *v)
+{
+  if (orc_validate(p)) return -1;
+  orc_setup s;
+  setup(p, &s);
+  const double hx = 1.0 / s.hix, hy = is_1d(p) ? 1.0 : 1.0 / s.hiy;
+  double u = 0;
+  for (int c = 0; c < s.C; c++)
+    for (int j = 0; j < s.ny; j++) {
+      double wy = is_1d(p) ? 1.0 : hweight(&s.op, s.ny, j, hy);
+      for (int i = 0; i < s.nx; i++) {
+        double x = v[(long)s.C * ((long)s.nx * j + i) + c];
+        u += wy * hweight(&s.op, s.nx, i, hx) * x * x;
+      }
+    }
+  return sqrt(u);
+}
+
+/* error_l2 / error_max (vec_util.h:47-61). */
+void orc_error_norms(const sbp_problem *p, const double *v, const double *w,
+                     double *l2, double *maxerr)
+{
+  orc_setup s;
+  setup(p, &s);
+  long len = (long)s.C * s.nx * s.ny;
+  double ss = 0, mx = 0;
+  for (long k = 0; k < len; k++) {
+    double d = fabs(v[k] - w[k]);
+    ss += d * d;
+    if (d > mx) mx = d;
+  }
+  double hprod = 1.0 / s.hix;
+  if (!is_1d(p)) hprod *= 1.0 / s.hiy;
+  if (l2) *l2 = sqrt(hprod) * sqrt(ss);
+  if (maxerr) *maxerr = mx;
+}
