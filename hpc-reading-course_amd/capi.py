@@ -1,0 +1,229 @@
+"""ctypes binding of libsbp_hip.so (the C-ABI of include/sbp_hip.h).
+
+Mirrors the reference's time-stepping interface for this path:
+
+* ``rhs(ctx, t, v_src, v_dst)`` fills ``v_dst`` with F(t, v_src) in place, like
+  the ``PetscErrorCode rhs(TS, PetscReal t, Vec v_src, Vec v_dst, void *ctx)``
+  callbacks (e.g. demo/wave/wave_eq_sim.cpp:291-308);
+* ``ts_rk4(ctx, t_end, dt, v)`` advances ``v`` in place to ``t_end`` with
+  classical RK4, full steps of ``dt`` and a final clipped step, the
+  TS_EXACTFINALTIME_MATCHSTEP behaviour of ts_rk4 (src/time_stepping/ts_rk.cpp:15-39,54-65).
+
+Vectors are float64 numpy arrays in the reference AoS layout v[j][i][c].
+The library must be built (``build.py``); nothing here falls back to CPU code.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import problems as P
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsbp_hip.so")
+
+_DP = C.POINTER(C.c_double)
+
+
+class SbpProblem(C.Structure):
+    _fields_ = [("physics", C.c_int), ("order", C.c_int), ("nx", C.c_int), ("ny", C.c_int),
+                ("xl", C.c_double), ("xr", C.c_double), ("yl", C.c_double), ("yr", C.c_double),
+                ("ax", C.c_double), ("ay", C.c_double),
+                ("material", C.c_int), ("bc", C.c_int), ("precision", C.c_int),
+                ("rho_inv", _DP), ("bulk", _DP)]
+
+
+class SbpError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("sbp error %d: %s" % (code, msg))
+        self.code = code
+
+
+EXPORTS = {
+    # name: (restype, argtypes)
+    "sbp_create": (C.c_int, [C.POINTER(SbpProblem), C.POINTER(C.c_void_p)]),
+    "sbp_destroy": (C.c_int, [C.c_void_p]),
+    "sbp_components": (C.c_int, [C.c_void_p]),
+    "sbp_vector_length": (C.c_long, [C.c_void_p]),
+    "sbp_set_tile_rows": (C.c_int, [C.c_void_p, C.c_int]),
+    "sbp_rhs": (C.c_int, [C.c_void_p, C.c_double, _DP, _DP]),
+    "sbp_rk4_advance": (C.c_int, [C.c_void_p, _DP, C.c_double, C.c_double, C.c_long]),
+    "sbp_set_state": (C.c_int, [C.c_void_p, _DP]),
+    "sbp_get_state": (C.c_int, [C.c_void_p, _DP]),
+    "sbp_step": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_long]),
+    "sbp_synchronize": (C.c_int, [C.c_void_p]),
+    "sbp_time_steps": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_long, _DP]),
+    "sbp_time_stages": (C.c_int, [C.c_void_p, C.c_double, C.c_long, _DP]),
+    "sbp_energy": (C.c_int, [C.c_void_p, _DP, _DP]),
+    "sbp_state_energy": (C.c_int, [C.c_void_p, _DP]),
+    "sbp_hnorm": (C.c_int, [C.c_void_p, _DP, _DP]),
+    "sbp_error_norms": (C.c_int, [C.c_void_p, _DP, _DP, _DP, _DP]),
+    "sbp_last_error": (C.c_char_p, []),
+    "sbp_device_arch": (C.c_char_p, []),
+    "sbp_version": (C.c_int, []),
+}
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Load libsbp_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError("libsbp_hip.so is not built (run __graft_entry__.build()): " + path)
+    lib = C.CDLL(path)
+    for name, (res, args) in EXPORTS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise SbpError(rc, load_library().sbp_last_error().decode())
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(_DP)
+
+
+def _vec(v, n):
+    a = np.ascontiguousarray(v, dtype=np.float64)
+    if a.size != n:
+        raise ValueError("vector has %d values, the problem needs %d" % (a.size, n))
+    return a.reshape(-1)
+
+
+class Context:
+    """One problem on the GPU (the reference's AppCtx + DMDA, sbp_ctx in C)."""
+
+    def __init__(self, spec: P.ProblemSpec):
+        lib = load_library()
+        self.spec = spec
+        self._keep = []
+        s = SbpProblem()
+        for f, _ in SbpProblem._fields_:
+            if f in ("rho_inv", "bulk"):
+                arr = getattr(spec, f)
+                if arr is None:
+                    setattr(s, f, None)
+                else:
+                    a = np.ascontiguousarray(arr, dtype=np.float64)
+                    self._keep.append(a)
+                    setattr(s, f, _ptr(a))
+            else:
+                setattr(s, f, getattr(spec, f))
+        h = C.c_void_p()
+        _check(lib.sbp_create(C.byref(s), C.byref(h)))
+        self._h = h
+        self._keep = []  # arrays are copied by sbp_create
+        self.n = lib.sbp_vector_length(h)
+
+    # -- lifecycle -------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None):
+            load_library().sbp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- reference-shaped calls -------------------------------------------
+    def rhs(self, t: float, v: np.ndarray, out: np.ndarray | None = None) -> np.ndarray:
+        src = _vec(v, self.n)
+        dst = np.empty(self.n) if out is None else out
+        _check(load_library().sbp_rhs(self._h, t, _ptr(src), _ptr(dst)))
+        return dst
+
+    def rk4_advance(self, v: np.ndarray, t0: float, dt: float, nsteps: int) -> np.ndarray:
+        out = np.array(_vec(v, self.n), copy=True)
+        _check(load_library().sbp_rk4_advance(self._h, _ptr(out), t0, dt, int(nsteps)))
+        return out
+
+    # -- resident state ---------------------------------------------------
+    def set_state(self, v: np.ndarray):
+        _check(load_library().sbp_set_state(self._h, _ptr(_vec(v, self.n))))
+
+    def get_state(self) -> np.ndarray:
+        out = np.empty(self.n)
+        _check(load_library().sbp_get_state(self._h, _ptr(out)))
+        return out
+
+    def step(self, t0: float, dt: float, nsteps: int):
+        _check(load_library().sbp_step(self._h, t0, dt, int(nsteps)))
+
+    def synchronize(self):
+        _check(load_library().sbp_synchronize(self._h))
+
+    def time_steps(self, t0: float, dt: float, nsteps: int) -> float:
+        ms = C.c_double()
+        _check(load_library().sbp_time_steps(self._h, t0, dt, int(nsteps), C.byref(ms)))
+        return ms.value
+
+    def time_stages(self, dt: float, nsteps: int) -> np.ndarray:
+        out = np.zeros(4)
+        _check(load_library().sbp_time_stages(self._h, dt, int(nsteps), _ptr(out)))
+        return out
+
+    def set_tile_rows(self, rows: int):
+        _check(load_library().sbp_set_tile_rows(self._h, int(rows)))
+
+    # -- norms -------------------------------------------------------------
+    def energy(self, v: np.ndarray) -> float:
+        e = C.c_double()
+        _check(load_library().sbp_energy(self._h, _ptr(_vec(v, self.n)), C.byref(e)))
+        return e.value
+
+    def state_energy(self) -> float:
+        e = C.c_double()
+        _check(load_library().sbp_state_energy(self._h, C.byref(e)))
+        return e.value
+
+    def hnorm(self, v: np.ndarray) -> float:
+        e = C.c_double()
+        _check(load_library().sbp_hnorm(self._h, _ptr(_vec(v, self.n)), C.byref(e)))
+        return e.value
+
+    def error_norms(self, v: np.ndarray, w: np.ndarray):
+        l2, mx = C.c_double(), C.c_double()
+        _check(load_library().sbp_error_norms(self._h, _ptr(_vec(v, self.n)), _ptr(_vec(w, self.n)),
+                                              C.byref(l2), C.byref(mx)))
+        return l2.value, mx.value
+
+
+def rhs(ctx: Context, t: float, v_src: np.ndarray, v_dst: np.ndarray) -> int:
+    """PETSc-callback shaped right-hand side: writes F(t, v_src) into v_dst."""
+    if v_dst.dtype != np.float64 or not v_dst.flags.c_contiguous:
+        raise ValueError("v_dst must be a contiguous float64 array")
+    ctx.rhs(t, v_src, out=v_dst.reshape(-1))
+    return 0
+
+
+def ts_rk4(ctx: Context, t_end: float, dt: float, v: np.ndarray, t0: float = 0.0) -> float:
+    """Advance v in place from t0 to t_end (MATCHSTEP); returns the final time."""
+    n, last = P.matchstep_schedule(t_end, dt, t0)
+    ctx.set_state(v)
+    ctx.step(t0, dt, n)
+    if last:
+        ctx.step(t0 + n * dt, last, 1)
+    v.reshape(-1)[:] = ctx.get_state()
+    return t_end if (n or last) else t0
+
+
+def device_arch() -> str:
+    return load_library().sbp_device_arch().decode()

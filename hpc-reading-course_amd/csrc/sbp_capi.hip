@@ -1,0 +1,663 @@
+// sbp_capi.hip -- host side of the C-ABI declared in include/sbp_hip.h.
+//
+// A context owns the device state of one problem: the RK4 registers U, A and
+// the Y ping-pong pair (planar, padded rows), optional Kahan compensation and
+// material planes, the 1D tables of the reference's manufactured forcing, an
+// AoS staging buffer for the reference-layout vectors that cross the
+// boundary, a stream and a captured multi-step hipGraph.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/sbp_hip.h"
+#include "sbp_kernels.hpp"
+
+using namespace sbp;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg)
+{
+  g_err = msg;
+  return code;
+}
+
+#define HIP_OR_FAIL(expr)                                                                   \
+  do {                                                                                      \
+    hipError_t e_ = (expr);                                                                 \
+    if (e_ != hipSuccess)                                                                   \
+      return fail(e_ == hipErrorOutOfMemory ? SBP_ENOMEM : SBP_EHIP,                        \
+                  std::string(#expr) + ": " + hipGetErrorString(e_));                        \
+  } while (0)
+
+constexpr int kAlign = 64;   // row alignment in elements
+constexpr int kBX = 256;     // interior tile width (threads per block)
+constexpr int kGraphSteps = 16;
+
+int closure_of(int order) { return order == 2 ? 1 : (order == 4 ? 4 : 6); }
+int taps_of(int order) { return order == 2 ? 3 : (order == 4 ? 6 : 9); }
+int min_points(int order) { return std::max(2 * closure_of(order), taps_of(order)); }
+
+bool is1d(int phys) { return phys == SBP_ADVECTION_1D || phys == SBP_REFLECTION_1D; }
+
+int components(int phys)
+{
+  switch (phys) {
+    case SBP_ADVECTION_1D:
+    case SBP_ADVECTION_2D: return 1;
+    case SBP_ACOUSTIC_2D: return 3;
+    case SBP_REFLECTION_1D: return 2;
+  }
+  return 0;
+}
+
+}  // namespace
+
+struct sbp_ctx {
+  sbp_problem d{};
+  int C = 0, order = 0, cls = 0, phys = 0, mat = 0, bc = 0, prec = 0;
+  int nx = 0, ny = 0, pitch = 0, xoff = 0;
+  long plane = 0;
+  size_t esz = 8;
+  double hix = 0, hiy = 0;
+  void *U = nullptr, *A = nullptr, *Y0 = nullptr, *Y1 = nullptr, *KC = nullptr;
+  void *rinv = nullptr, *bulk = nullptr;
+  double *xs = nullptr, *ys = nullptr, *c3x = nullptr, *s3x = nullptr, *c4y = nullptr, *s4y = nullptr;
+  double *aos = nullptr;
+  double *partials = nullptr, *scalar = nullptr;
+  int npartials = 0;
+  hipStream_t stream = nullptr;
+  StageLaunchFn launch = nullptr;
+  LaunchGeom geom{};
+  int ty_rows = 32, tiles_x = 0, tiles_y = 0, n_tiles = 0, n_edge = 0, n_edge_blocks = 0;
+  bool have_state = false;
+  hipGraphExec_t gexec = nullptr;
+  double g_dt = 0;
+  hipEvent_t ev[5] = {};
+  int device = 0;
+};
+
+namespace {
+
+void compute_geometry(sbp_ctx *c)
+{
+  if (is1d(c->phys)) {
+    c->geom.grid = (c->nx + 255) / 256;
+    c->geom.block = 256;
+    return;
+  }
+  const int wi = c->nx - 2 * c->cls, hi = c->ny - 2 * c->cls;
+  c->tiles_x = wi > 0 ? (wi + kBX - 1) / kBX : 0;
+  c->tiles_y = hi > 0 ? (hi + c->ty_rows - 1) / c->ty_rows : 0;
+  c->n_tiles = c->tiles_x * c->tiles_y;
+  c->n_edge = 2 * c->cls * c->nx + 2 * c->cls * (c->ny - 2 * c->cls);
+  int eb = (c->n_edge + kBX - 1) / kBX;
+  c->n_edge_blocks = (eb + kNumXcd - 1) / kNumXcd * kNumXcd;
+  c->geom.grid = c->n_edge_blocks + c->n_tiles;
+  c->geom.block = kBX;
+}
+
+// Default tile height: the tallest of 16..128 rows that still gives at least
+// eight tiles per CU (256 CUs); tall tiles amortise the 2R-row y halo.
+int default_ty_rows(int nx, int ny, int cls)
+{
+  const int wi = nx - 2 * cls, hi = ny - 2 * cls;
+  if (wi <= 0 || hi <= 0) return 16;
+  const int tx = (wi + kBX - 1) / kBX;
+  int best = 16;
+  for (int t = 16; t <= 128; t *= 2)
+    if ((long)tx * ((hi + t - 1) / t) >= 2048) best = t;
+  return best;
+}
+
+template <typename T>
+StageArgs<T> make_args(const sbp_ctx *c, int mode, const void *yin, double dt, double t)
+{
+  StageArgs<T> a{};
+  a.yin = static_cast<const T *>(yin);
+  a.uin = static_cast<const T *>(c->U);
+  a.aio = static_cast<T *>(c->A);
+  a.uout = static_cast<T *>(c->U);
+  a.comp = static_cast<T *>(c->KC);
+  a.rinv = static_cast<const T *>(c->rinv);
+  a.bulk = static_cast<const T *>(c->bulk);
+  a.xs = c->xs;
+  a.ys = c->ys;
+  a.c3x = c->c3x;
+  a.s3x = c->s3x;
+  a.c4y = c->c4y;
+  a.s4y = c->s4y;
+  a.c5t = std::cos(5 * kPi * t);
+  a.plane = c->plane;
+  a.pitch = c->pitch;
+  a.xoff = c->xoff;
+  a.nx = c->nx;
+  a.ny = c->ny;
+  a.hix = T(c->hix);
+  a.hiy = T(c->hiy);
+  a.ax = T(c->d.ax);
+  a.ay = T(c->d.ay);
+  a.bc = c->bc;
+  a.tiles_x = c->tiles_x;
+  a.ty_rows = c->ty_rows;
+  a.n_tiles = c->n_tiles;
+  a.n_edge_blocks = c->n_edge_blocks;
+  a.n_edge = c->n_edge;
+  switch (mode) {
+    case MODE_S1: a.cy = T(0.5 * dt); a.ca = T(dt / 6); break;
+    case MODE_S2: a.cy = T(0.5 * dt); a.ca = T(dt / 3); break;
+    case MODE_S3: a.cy = T(dt); a.ca = T(dt / 3); break;
+    case MODE_S4: a.cy = T(0); a.ca = T(dt / 6); break;
+    default: a.cy = a.ca = T(0);
+  }
+  return a;
+}
+
+// Launch one stage: mode 1..4 of the RK4 step (yin/yout roles fixed per
+// stage: U -> Y0, Y0 -> Y1, Y1 -> Y0, Y0 -> U) or the RHS mode (yin -> yout).
+hipError_t launch_stage(sbp_ctx *c, int mode, double dt, double t, const void *yin, void *yout)
+{
+  if (c->esz == 8) {
+    StageArgs<double> a = make_args<double>(c, mode, yin, dt, t);
+    a.yout = static_cast<double *>(yout);
+    return c->launch(mode, &a, c->geom, c->stream);
+  }
+  StageArgs<float> a = make_args<float>(c, mode, yin, dt, t);
+  a.yout = static_cast<float *>(yout);
+  return c->launch(mode, &a, c->geom, c->stream);
+}
+
+hipError_t launch_step(sbp_ctx *c, double t, double dt)
+{
+  const double h = 0.5 * dt;
+  hipError_t e;
+  if ((e = launch_stage(c, MODE_S1, dt, t, c->U, c->Y0)) != hipSuccess) return e;
+  if ((e = launch_stage(c, MODE_S2, dt, t + h, c->Y0, c->Y1)) != hipSuccess) return e;
+  if ((e = launch_stage(c, MODE_S3, dt, t + h, c->Y1, c->Y0)) != hipSuccess) return e;
+  return launch_stage(c, MODE_S4, dt, t + dt, c->Y0, nullptr);
+}
+
+bool time_dependent(const sbp_ctx *c) { return c->phys == SBP_ACOUSTIC_2D && c->mat == SBP_MAT_REF_RHO; }
+
+void drop_graph(sbp_ctx *c)
+{
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  c->gexec = nullptr;
+  c->g_dt = 0;
+}
+
+int ensure_graph(sbp_ctx *c, double dt)
+{
+  if (c->gexec && c->g_dt == dt) return SBP_OK;
+  drop_graph(c);
+  hipGraph_t g = nullptr;
+  HIP_OR_FAIL(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  hipError_t le = hipSuccess;
+  for (int s = 0; s < kGraphSteps && le == hipSuccess; s++) le = launch_step(c, 0.0, dt);
+  hipError_t ce = hipStreamEndCapture(c->stream, &g);
+  if (le != hipSuccess || ce != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    return fail(SBP_EHIP, std::string("graph capture: ") + hipGetErrorString(le != hipSuccess ? le : ce));
+  }
+  hipError_t ie = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (ie != hipSuccess) {
+    c->gexec = nullptr;
+    return fail(SBP_EHIP, std::string("graph instantiate: ") + hipGetErrorString(ie));
+  }
+  c->g_dt = dt;
+  return SBP_OK;
+}
+
+int run_steps(sbp_ctx *c, double t0, double dt, long nsteps)
+{
+  long n = 0;
+  if (!time_dependent(c) && nsteps >= kGraphSteps) {
+    int rc = ensure_graph(c, dt);
+    if (rc) return rc;
+    for (; n + kGraphSteps <= nsteps; n += kGraphSteps) HIP_OR_FAIL(hipGraphLaunch(c->gexec, c->stream));
+  }
+  for (; n < nsteps; n++) HIP_OR_FAIL(launch_step(c, t0 + n * dt, dt));
+  return SBP_OK;
+}
+
+int upload_aos(sbp_ctx *c, const double *v)
+{
+  const size_t bytes = sizeof(double) * (size_t)c->C * c->nx * c->ny;
+  HIP_OR_FAIL(hipMemcpyAsync(c->aos, v, bytes, hipMemcpyHostToDevice, c->stream));
+  return SBP_OK;
+}
+
+int download_aos(sbp_ctx *c, double *v)
+{
+  const size_t bytes = sizeof(double) * (size_t)c->C * c->nx * c->ny;
+  HIP_OR_FAIL(hipMemcpyAsync(v, c->aos, bytes, hipMemcpyDeviceToHost, c->stream));
+  HIP_OR_FAIL(hipStreamSynchronize(c->stream));
+  return SBP_OK;
+}
+
+int aos_to_planar(sbp_ctx *c, void *dst, void *comp)
+{
+  const long n = (long)c->nx * c->ny;
+  const int grid = (int)((n + 255) / 256);
+  if (c->esz == 8)
+    hipLaunchKernelGGL(aos_to_planar_kernel<double>, dim3(grid), dim3(256), 0, c->stream, c->aos,
+                       (double *)dst, (double *)comp, c->nx, c->ny, c->C, c->pitch, c->xoff, c->plane);
+  else
+    hipLaunchKernelGGL(aos_to_planar_kernel<float>, dim3(grid), dim3(256), 0, c->stream, c->aos,
+                       (float *)dst, (float *)comp, c->nx, c->ny, c->C, c->pitch, c->xoff, c->plane);
+  HIP_OR_FAIL(hipGetLastError());
+  return SBP_OK;
+}
+
+int planar_to_aos(sbp_ctx *c, const void *src, const void *comp)
+{
+  const long n = (long)c->nx * c->ny;
+  const int grid = (int)((n + 255) / 256);
+  if (c->esz == 8)
+    hipLaunchKernelGGL(planar_to_aos_kernel<double>, dim3(grid), dim3(256), 0, c->stream,
+                       (const double *)src, (const double *)comp, c->aos, c->nx, c->ny, c->C, c->pitch,
+                       c->xoff, c->plane);
+  else
+    hipLaunchKernelGGL(planar_to_aos_kernel<float>, dim3(grid), dim3(256), 0, c->stream,
+                       (const float *)src, (const float *)comp, c->aos, c->nx, c->ny, c->C, c->pitch,
+                       c->xoff, c->plane);
+  HIP_OR_FAIL(hipGetLastError());
+  return SBP_OK;
+}
+
+// H-weighted sum of squares of either the AoS staging vector (aos=true) or
+// the resident state.
+int norm_reduce(sbp_ctx *c, bool aos, bool weighted, double *out)
+{
+  const long n = (long)c->nx * c->ny;
+  int grid = (int)std::min<long>((n + 255) / 256, c->npartials);
+  if (grid < 1) grid = 1;
+  auto fill = [&](auto *tag) {
+    using T = std::remove_pointer_t<decltype(tag)>;
+    NormArgs<T> a{};
+    a.aos = aos ? c->aos : nullptr;
+    a.u = static_cast<const T *>(c->U);
+    a.comp = static_cast<const T *>(c->KC);
+    a.rinv = static_cast<const T *>(c->rinv);
+    a.bulk = static_cast<const T *>(c->bulk);
+    a.xs = c->xs;
+    a.ys = c->ys;
+    a.plane = c->plane;
+    a.pitch = c->pitch;
+    a.xoff = c->xoff;
+    a.nx = c->nx;
+    a.ny = c->ny;
+    a.nc = c->C;
+    a.order = c->order;
+    a.is1d = is1d(c->phys);
+    a.material = c->phys == SBP_ACOUSTIC_2D ? c->mat : 0;
+    a.weighted = weighted;
+    a.hx = 1.0 / c->hix;
+    a.hy = is1d(c->phys) ? 1.0 : 1.0 / c->hiy;
+    hipLaunchKernelGGL(norm_partial_kernel<T>, dim3(grid), dim3(256), 0, c->stream, a, c->partials);
+  };
+  if (c->esz == 8) fill((double *)nullptr);
+  else fill((float *)nullptr);
+  HIP_OR_FAIL(hipGetLastError());
+  hipLaunchKernelGGL(norm_final_kernel, dim3(1), dim3(256), 0, c->stream, c->partials, grid, c->scalar);
+  HIP_OR_FAIL(hipGetLastError());
+  HIP_OR_FAIL(hipMemcpyAsync(out, c->scalar, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_OR_FAIL(hipStreamSynchronize(c->stream));
+  return SBP_OK;
+}
+
+int validate(const sbp_problem *d)
+{
+  if (!d) return fail(SBP_EINVAL, "null problem");
+  if (components(d->physics) == 0) return fail(SBP_EINVAL, "unknown physics");
+  if (d->order != 2 && d->order != 4 && d->order != 6) return fail(SBP_EINVAL, "order must be 2, 4 or 6");
+  const int nmin = min_points(d->order);
+  if (d->nx < nmin) return fail(SBP_EINVAL, "nx below the closure size of this order");
+  if (is1d(d->physics)) {
+    if (d->ny != 1) return fail(SBP_EINVAL, "1D problems need ny = 1");
+  } else if (d->ny < nmin) {
+    return fail(SBP_EINVAL, "ny below the closure size of this order");
+  }
+  if (!(d->xr > d->xl) || (!is1d(d->physics) && !(d->yr > d->yl))) return fail(SBP_EINVAL, "empty domain");
+  if (d->precision < SBP_FP64 || d->precision > SBP_FP32_KAHAN) return fail(SBP_EINVAL, "bad precision");
+  if (d->bc < SBP_BC_DEFAULT || d->bc > SBP_BC_WALL) return fail(SBP_EINVAL, "bad boundary condition");
+  if (d->physics == SBP_ACOUSTIC_2D) {
+    if (d->material < SBP_MAT_UNIT || d->material > SBP_MAT_ARRAYS) return fail(SBP_EINVAL, "bad material");
+    if (d->material == SBP_MAT_ARRAYS && (!d->rho_inv || !d->bulk))
+      return fail(SBP_EINVAL, "SBP_MAT_ARRAYS needs rho_inv and bulk");
+    if (d->material == SBP_MAT_REF_RHO && d->precision != SBP_FP64)
+      return fail(SBP_EINVAL, "the reference rho = 2+xy problem is FP64 only");
+  } else if (d->bc != SBP_BC_DEFAULT) {
+    return fail(SBP_EINVAL, "boundary kinds other than default are acoustic only");
+  }
+  if ((long)d->nx * d->ny > (1L << 30)) return fail(SBP_EINVAL, "grid too large");
+  return SBP_OK;
+}
+
+template <typename P> int dalloc(P **p, size_t bytes)
+{
+  void *q = nullptr;
+  hipError_t e = hipMalloc(&q, bytes);
+  if (e != hipSuccess) return fail(SBP_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  *p = static_cast<P *>(q);
+  return SBP_OK;
+}
+
+int upload_plane(sbp_ctx *c, const double *m, void **dst)
+{
+  const size_t n = (size_t)c->nx * c->ny;
+  int rc = dalloc(dst, c->esz * (size_t)c->plane);
+  if (rc) return rc;
+  HIP_OR_FAIL(hipMemcpyAsync(c->aos, m, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+  const int grid = (int)((n + 255) / 256);
+  if (c->esz == 8)
+    hipLaunchKernelGGL(matrix_to_plane_kernel<double>, dim3(grid), dim3(256), 0, c->stream, c->aos,
+                       (double *)*dst, c->nx, c->ny, c->pitch, c->xoff);
+  else
+    hipLaunchKernelGGL(matrix_to_plane_kernel<float>, dim3(grid), dim3(256), 0, c->stream, c->aos,
+                       (float *)*dst, c->nx, c->ny, c->pitch, c->xoff);
+  HIP_OR_FAIL(hipGetLastError());
+  HIP_OR_FAIL(hipStreamSynchronize(c->stream));
+  return SBP_OK;
+}
+
+int upload_table(sbp_ctx *c, const std::vector<double> &v, double **dst)
+{
+  int rc = dalloc(dst, sizeof(double) * v.size());
+  if (rc) return rc;
+  HIP_OR_FAIL(hipMemcpy(*dst, v.data(), sizeof(double) * v.size(), hipMemcpyHostToDevice));
+  return SBP_OK;
+}
+
+void release(sbp_ctx *c)
+{
+  if (!c) return;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  drop_graph(c);
+  void *ptrs[] = {c->U, c->A, c->Y0, c->Y1, c->KC, c->rinv, c->bulk, c->xs, c->ys, c->c3x, c->s3x,
+                  c->c4y, c->s4y, c->aos, c->partials, c->scalar};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  for (hipEvent_t &e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+int build_ctx(const sbp_problem *desc, sbp_ctx *c)
+{
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(SBP_ENODEV, "no HIP device");
+  HIP_OR_FAIL(hipGetDevice(&c->device));
+  c->d = *desc;
+  c->d.rho_inv = nullptr;
+  c->d.bulk = nullptr;
+  c->phys = desc->physics;
+  c->order = desc->order;
+  c->cls = closure_of(c->order);
+  c->C = components(c->phys);
+  c->mat = c->phys == SBP_ACOUSTIC_2D ? desc->material : SBP_MAT_UNIT;
+  c->bc = desc->bc;
+  c->prec = desc->precision;
+  c->esz = c->prec == SBP_FP64 ? 8 : 4;
+  c->nx = desc->nx;
+  c->ny = desc->ny;
+  c->hix = (c->nx - 1) / (desc->xr - desc->xl);
+  c->hiy = is1d(c->phys) ? 0.0 : (c->ny - 1) / (desc->yr - desc->yl);
+  c->xoff = (kAlign - c->cls % kAlign) % kAlign;
+  c->pitch = (c->xoff + c->nx + kAlign - 1) / kAlign * kAlign;
+  c->plane = (long)c->pitch * c->ny;
+  c->ty_rows = default_ty_rows(c->nx, c->ny, c->cls);
+  compute_geometry(c);
+
+  StageLaunchFn (*reg)(int, int, int, int) =
+      c->order == 2 ? launcher_o2 : (c->order == 4 ? launcher_o4 : launcher_o6);
+  c->launch = reg(c->phys, c->mat, c->prec, kBX);
+  if (!c->launch) return fail(SBP_EINVAL, "no kernel for this physics/material/precision");
+
+  HIP_OR_FAIL(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  for (hipEvent_t &e : c->ev) HIP_OR_FAIL(hipEventCreate(&e));
+  const size_t reg_bytes = c->esz * (size_t)c->C * c->plane;
+  int rc;
+  if ((rc = dalloc(&c->U, reg_bytes)) || (rc = dalloc(&c->A, reg_bytes)) || (rc = dalloc(&c->Y0, reg_bytes)) ||
+      (rc = dalloc(&c->Y1, reg_bytes)))
+    return rc;
+  if (c->prec == SBP_FP32_KAHAN && (rc = dalloc(&c->KC, reg_bytes))) return rc;
+  // Zero the registers once so padding never holds stale bit patterns.
+  for (void *p : {c->U, c->A, c->Y0, c->Y1, c->KC})
+    if (p) HIP_OR_FAIL(hipMemsetAsync(p, 0, reg_bytes, c->stream));
+  if ((rc = dalloc(&c->aos, sizeof(double) * (size_t)c->C * c->nx * c->ny))) return rc;
+  c->npartials = 1024;
+  if ((rc = dalloc(&c->partials, sizeof(double) * c->npartials)) || (rc = dalloc(&c->scalar, sizeof(double))))
+    return rc;
+  if (c->phys == SBP_ACOUSTIC_2D && c->mat == SBP_MAT_ARRAYS) {
+    if ((rc = upload_plane(c, desc->rho_inv, &c->rinv)) || (rc = upload_plane(c, desc->bulk, &c->bulk))) return rc;
+  }
+  if (c->phys == SBP_ACOUSTIC_2D && c->mat == SBP_MAT_REF_RHO) {
+    // x = xl + i/hi as rho_inv and forcing_* compute it (wave_eq_rhs.h:43,52).
+    std::vector<double> xs(c->nx), ys(c->ny), c3(c->nx), s3(c->nx), c4(c->ny), s4(c->ny);
+    for (int i = 0; i < c->nx; i++) {
+      xs[i] = desc->xl + i / c->hix;
+      c3[i] = std::cos(3 * kPi * xs[i]);
+      s3[i] = std::sin(3 * kPi * xs[i]);
+    }
+    for (int j = 0; j < c->ny; j++) {
+      ys[j] = desc->yl + j / c->hiy;
+      c4[j] = std::cos(4 * kPi * ys[j]);
+      s4[j] = std::sin(4 * kPi * ys[j]);
+    }
+    if ((rc = upload_table(c, xs, &c->xs)) || (rc = upload_table(c, ys, &c->ys)) ||
+        (rc = upload_table(c, c3, &c->c3x)) || (rc = upload_table(c, s3, &c->s3x)) ||
+        (rc = upload_table(c, c4, &c->c4y)) || (rc = upload_table(c, s4, &c->s4y)))
+      return rc;
+  }
+  HIP_OR_FAIL(hipStreamSynchronize(c->stream));
+  return SBP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sbp_create(const sbp_problem *desc, sbp_ctx **out)
+{
+  if (!out) return fail(SBP_EINVAL, "null output pointer");
+  *out = nullptr;
+  int rc = validate(desc);
+  if (rc) return rc;
+  sbp_ctx *c = new (std::nothrow) sbp_ctx();
+  if (!c) return fail(SBP_ENOMEM, "host allocation failed");
+  rc = build_ctx(desc, c);
+  if (rc) {
+    std::string msg = g_err;
+    release(c);
+    g_err = msg;
+    return rc;
+  }
+  *out = c;
+  return SBP_OK;
+}
+
+int sbp_destroy(sbp_ctx *ctx)
+{
+  release(ctx);
+  return SBP_OK;
+}
+
+int sbp_components(const sbp_ctx *ctx) { return ctx ? ctx->C : 0; }
+
+long sbp_vector_length(const sbp_ctx *ctx) { return ctx ? (long)ctx->C * ctx->nx * ctx->ny : 0; }
+
+int sbp_set_tile_rows(sbp_ctx *ctx, int rows)
+{
+  if (!ctx || rows < 1 || rows > 4096) return fail(SBP_EINVAL, "tile rows must be 1..4096");
+  HIP_OR_FAIL(hipStreamSynchronize(ctx->stream));
+  drop_graph(ctx);
+  ctx->ty_rows = rows;
+  compute_geometry(ctx);
+  return SBP_OK;
+}
+
+int sbp_rhs(sbp_ctx *ctx, double t, const double *src, double *dst)
+{
+  if (!ctx || !src || !dst) return fail(SBP_EINVAL, "null argument");
+  int rc;
+  if ((rc = upload_aos(ctx, src)) || (rc = aos_to_planar(ctx, ctx->Y1, nullptr))) return rc;
+  HIP_OR_FAIL(launch_stage(ctx, MODE_RHS, 0.0, t, ctx->Y1, ctx->Y0));
+  if ((rc = planar_to_aos(ctx, ctx->Y0, nullptr))) return rc;
+  return download_aos(ctx, dst);
+}
+
+int sbp_set_state(sbp_ctx *ctx, const double *v)
+{
+  if (!ctx || !v) return fail(SBP_EINVAL, "null argument");
+  int rc;
+  if ((rc = upload_aos(ctx, v)) || (rc = aos_to_planar(ctx, ctx->U, ctx->KC))) return rc;
+  HIP_OR_FAIL(hipStreamSynchronize(ctx->stream));
+  ctx->have_state = true;
+  return SBP_OK;
+}
+
+int sbp_get_state(sbp_ctx *ctx, double *v)
+{
+  if (!ctx || !v) return fail(SBP_EINVAL, "null argument");
+  if (!ctx->have_state) return fail(SBP_ESTATE, "no state loaded");
+  int rc = planar_to_aos(ctx, ctx->U, ctx->KC);
+  if (rc) return rc;
+  return download_aos(ctx, v);
+}
+
+int sbp_step(sbp_ctx *ctx, double t0, double dt, long nsteps)
+{
+  if (!ctx || nsteps < 0) return fail(SBP_EINVAL, "bad argument");
+  if (!ctx->have_state) return fail(SBP_ESTATE, "no state loaded");
+  return run_steps(ctx, t0, dt, nsteps);
+}
+
+int sbp_synchronize(sbp_ctx *ctx)
+{
+  if (!ctx) return fail(SBP_EINVAL, "null context");
+  HIP_OR_FAIL(hipStreamSynchronize(ctx->stream));
+  return SBP_OK;
+}
+
+int sbp_rk4_advance(sbp_ctx *ctx, double *v, double t0, double dt, long nsteps)
+{
+  int rc;
+  if ((rc = sbp_set_state(ctx, v)) || (rc = sbp_step(ctx, t0, dt, nsteps))) return rc;
+  return sbp_get_state(ctx, v);
+}
+
+int sbp_time_steps(sbp_ctx *ctx, double t0, double dt, long nsteps, double *ms)
+{
+  if (!ctx || !ms) return fail(SBP_EINVAL, "null argument");
+  if (!ctx->have_state) return fail(SBP_ESTATE, "no state loaded");
+  if (!time_dependent(ctx) && nsteps >= kGraphSteps) {
+    int rc = ensure_graph(ctx, dt);  // capture outside the timed region
+    if (rc) return rc;
+  }
+  HIP_OR_FAIL(hipEventRecord(ctx->ev[0], ctx->stream));
+  int rc = run_steps(ctx, t0, dt, nsteps);
+  if (rc) return rc;
+  HIP_OR_FAIL(hipEventRecord(ctx->ev[1], ctx->stream));
+  HIP_OR_FAIL(hipEventSynchronize(ctx->ev[1]));
+  float f = 0;
+  HIP_OR_FAIL(hipEventElapsedTime(&f, ctx->ev[0], ctx->ev[1]));
+  *ms = f;
+  return SBP_OK;
+}
+
+int sbp_time_stages(sbp_ctx *ctx, double dt, long nsteps, double ms_stage[4])
+{
+  if (!ctx || !ms_stage) return fail(SBP_EINVAL, "null argument");
+  if (!ctx->have_state) return fail(SBP_ESTATE, "no state loaded");
+  double acc[4] = {0, 0, 0, 0};
+  const void *yin[4] = {ctx->U, ctx->Y0, ctx->Y1, ctx->Y0};
+  void *yout[4] = {ctx->Y0, ctx->Y1, ctx->Y0, nullptr};
+  const double ts[4] = {0, 0.5 * dt, 0.5 * dt, dt};
+  for (long n = 0; n < nsteps; n++) {
+    HIP_OR_FAIL(hipEventRecord(ctx->ev[0], ctx->stream));
+    for (int s = 0; s < 4; s++) {
+      HIP_OR_FAIL(launch_stage(ctx, MODE_S1 + s, dt, n * dt + ts[s], yin[s], yout[s]));
+      HIP_OR_FAIL(hipEventRecord(ctx->ev[s + 1], ctx->stream));
+    }
+    HIP_OR_FAIL(hipEventSynchronize(ctx->ev[4]));
+    for (int s = 0; s < 4; s++) {
+      float f = 0;
+      HIP_OR_FAIL(hipEventElapsedTime(&f, ctx->ev[s], ctx->ev[s + 1]));
+      acc[s] += f;
+    }
+  }
+  for (int s = 0; s < 4; s++) ms_stage[s] = nsteps > 0 ? acc[s] / nsteps : 0.0;
+  return SBP_OK;
+}
+
+int sbp_energy(sbp_ctx *ctx, const double *v, double *E)
+{
+  if (!ctx || !v || !E) return fail(SBP_EINVAL, "null argument");
+  int rc = upload_aos(ctx, v);
+  if (rc) return rc;
+  return norm_reduce(ctx, true, true, E);
+}
+
+int sbp_state_energy(sbp_ctx *ctx, double *E)
+{
+  if (!ctx || !E) return fail(SBP_EINVAL, "null argument");
+  if (!ctx->have_state) return fail(SBP_ESTATE, "no state loaded");
+  return norm_reduce(ctx, false, true, E);
+}
+
+int sbp_hnorm(sbp_ctx *ctx, const double *v, double *norm)
+{
+  if (!ctx || !v || !norm) return fail(SBP_EINVAL, "null argument");
+  int rc = upload_aos(ctx, v);
+  if (rc) return rc;
+  double s = 0;
+  if ((rc = norm_reduce(ctx, true, false, &s))) return rc;
+  *norm = std::sqrt(s);
+  return SBP_OK;
+}
+
+int sbp_error_norms(const sbp_ctx *ctx, const double *v, const double *w, double *l2, double *maxerr)
+{
+  if (!ctx || !v || !w) return fail(SBP_EINVAL, "null argument");
+  const long n = (long)ctx->C * ctx->nx * ctx->ny;
+  double ss = 0, mx = 0;
+  for (long k = 0; k < n; k++) {
+    const double d = std::fabs(v[k] - w[k]);
+    ss += d * d;
+    if (d > mx) mx = d;
+  }
+  double hprod = 1.0 / ctx->hix;
+  if (!is1d(ctx->phys)) hprod *= 1.0 / ctx->hiy;
+  if (l2) *l2 = std::sqrt(hprod) * std::sqrt(ss);
+  if (maxerr) *maxerr = mx;
+  return SBP_OK;
+}
+
+const char *sbp_last_error(void) { return g_err.c_str(); }
+
+const char *sbp_device_arch(void)
+{
+  static thread_local char name[64];
+  name[0] = 0;
+  int ndev = 0, dev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return name;
+  if (hipGetDevice(&dev) != hipSuccess) return name;
+  hipDeviceProp_t p;
+  if (hipGetDeviceProperties(&p, dev) != hipSuccess) return name;
+  std::snprintf(name, sizeof(name), "%s", p.gcnArchName);
+  return name;
+}
+
+int sbp_version(void) { return 100; }
+
+}  // extern "C"

@@ -1,0 +1,667 @@
+// sbp_kernels.hpp -- gfx950 kernels of the SBP-SAT RK4 hot path.
+//
+// One launch per RK4 stage computes the semi-discrete right-hand side F(t, Y)
+// of code/demo/*/*_rhs.h (region kernels + SAT) and folds the classical RK4
+// stage arithmetic into the same pass (three registers U, A, Y in HBM):
+//
+//   stage 1: K = F(U);  Y' = U + dt/2 K;  A = U + dt/6 K      (reads U)
+//   stage 2: K = F(Y);  Y' = U + dt/2 K;  A += dt/3 K         (reads Y, U, A)
+//   stage 3: K = F(Y);  Y' = U + dt K;    A += dt/3 K         (reads Y, U, A)
+//   stage 4: K = F(Y);  U  = A + dt/6 K                       (reads Y, A)
+//
+// which is RK4_custom's update (code/include/time_stepping/time_stepping.h:127-142)
+// with the b-weighted sum accumulated in A instead of four stored stage vectors:
+// 9 reads + 7 writes per value and step.  Y ping-pongs between two buffers
+// because a stage reads Y with a stencil halo while it writes the next Y.
+// The Kahan form keeps only the increment in A and adds it to U with a
+// compensation register in stage 4.
+//
+// Layout in HBM: one plane per component, rows of `pitch` elements, column i
+// stored at xoff + i with xoff chosen so that the first interior column
+// (i = CLS) starts on a 64-element boundary.
+//
+// Work split inside one launch:
+//   * blocks [0, n_edge_blocks): one thread per boundary-region point (the
+//     CLS-deep frame holding every closure row and every SAT point).  Stencil
+//     rows come from a table indexed by the point, so closure and interior
+//     rows run the same instruction stream (no divergence).
+//   * blocks [n_edge_blocks, ...): interior tiles [CLS, N-CLS)^2, BX columns
+//     wide and ty_rows tall.  Each thread owns one column and marches up the
+//     tile keeping a (2R+1)-row register window of the y-differentiated
+//     fields; the x-differentiated fields of the current row go through LDS
+//     with an R-wide halo (the DMDA ghost exchange of the reference becomes
+//     this intra-device halo).  Block order is remapped so that every XCD gets
+//     a contiguous band of tiles (its own L2 then serves the vertical halos).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "sbp_ops.hpp"
+
+namespace sbp {
+
+enum Phys { PH_ADV1D = 1, PH_ADV2D = 2, PH_ACOUSTIC = 3, PH_REFL1D = 4 };
+enum Mat { M_UNIT = 0, M_REF = 1, M_ARR = 2 };
+enum Bc { B_DEFAULT = 0, B_FSREF = 1, B_FS = 2, B_WALL = 3 };
+enum Mode { MODE_RHS = 0, MODE_S1 = 1, MODE_S2 = 2, MODE_S3 = 3, MODE_S4 = 4 };
+
+constexpr double kPi = 3.14159265358979323846264338327950288;  // PETSC_PI
+constexpr int kNumXcd = 8;
+
+template <typename T> struct StageArgs {
+  const T *yin;   // stage input (stencil operand)
+  const T *uin;   // U (stages 2, 3)
+  T *aio;         // A accumulator
+  T *yout;        // next Y (stages 1-3) or F (RHS mode)
+  T *uout;        // U (stage 4)
+  T *comp;        // Kahan compensation of U
+  const T *rinv;  // rho^-1 plane (caller material arrays)
+  const T *bulk;  // K plane
+  const double *xs, *ys;     // x_i = xl + i/hix, y_j (reference rho and forcing)
+  const double *c3x, *s3x;   // cos(3 pi x_i), sin(3 pi x_i)
+  const double *c4y, *s4y;   // cos(4 pi y_j), sin(4 pi y_j)
+  double c5t;                // cos(5 pi t) at the stage time
+  long plane;
+  int pitch, xoff, nx, ny;
+  T hix, hiy, ax, ay;
+  T cy, ca;                  // stage coefficients of Y and A
+  int bc;
+  int tiles_x, ty_rows, n_tiles, n_edge_blocks, n_edge;
+};
+
+template <typename T, int ORDER> __constant__ EdgeTable<T, ORDER> c_edge = make_edge_table<T, ORDER>();
+
+// Table-driven derivative at index idx of a strided line of n points.
+template <typename T, int ORDER>
+__device__ __forceinline__ T dline(const T *__restrict__ f, long stride, int n, int idx, T hi)
+{
+  using G = Geom<ORDER>;
+  int row, start;
+  if (idx < G::CLS) {
+    row = idx;
+    start = 0;
+  } else if (idx >= n - G::CLS) {
+    row = G::CLS + (n - 1 - idx);
+    start = n - G::TAPS;
+  } else {
+    row = 2 * G::CLS;
+    start = idx - G::R;
+  }
+  const T *c = c_edge<T, ORDER>.c[row];
+  T acc = T(0);
+#pragma unroll
+  for (int s = 0; s < G::TAPS; s++) acc += c[s] * f[(long)(start + s) * stride];
+  return hi * acc;
+}
+
+// Interior stencil on 2R+1 values, zero centre tap skipped.
+template <typename T, int ORDER, typename Get>
+__device__ __forceinline__ T dint(Get get, T hi)
+{
+  using O = Ops<ORDER>;
+  T acc = T(0);
+#pragma unroll
+  for (int s = 0; s < 2 * O::R + 1; s++)
+    if (O::inner[s] != 0.0) acc += T(O::inner[s]) * get(s);
+  return hi * acc;
+}
+
+// Acoustic right-hand side at one point (wave_eq_rhs.h:76-322,
+// wave_eq_hom_rhs.h:49-296; caller arrays carry K on the pressure row).
+template <typename T, int MAT>
+__device__ __forceinline__ void acoustic_F(T dxp, T dyp, T dxu, T dyv, T ri, T K, T fu, T fv,
+                                           T &Fu, T &Fv, T &Fp)
+{
+  if constexpr (MAT == M_UNIT) {
+    Fu = -dxp;
+    Fv = -dyp;
+    Fp = -dxu - dyv;
+  } else if constexpr (MAT == M_REF) {
+    Fu = -ri * dxp + fu;
+    Fv = -ri * dyp + fv;
+    Fp = -dxu - dyv;
+  } else {
+    Fu = -ri * dxp;
+    Fv = -ri * dyp;
+    Fp = -K * (dxu + dyv);
+  }
+}
+
+// rho^-1 and the manufactured forcing of wave_eq_rhs.h:42-64 from 1D tables.
+template <typename T>
+__device__ __forceinline__ void ref_material(const StageArgs<T> &a, int i, int j, T &ri, T &fu, T &fv)
+{
+  const double x = a.xs[i], y = a.ys[j];
+  const double xy = x * y;
+  ri = T(1. / (2 + xy));
+  fu = T(-(3 * kPi * a.c5t * a.c3x[i] * a.s4y[j] * (xy + 1)) / (xy + 2));
+  fv = T(-(4 * kPi * a.c5t * a.c4y[j] * a.s3x[i] * (xy + 1)) / (xy + 2));
+}
+
+// RK4 stage arithmetic at one point (edge and 1D paths).
+template <typename T, int NC, int MODE, bool KAHAN>
+__device__ __forceinline__ void rk_point(const StageArgs<T> &a, long o, const T *K, const T *Ucen)
+{
+#pragma unroll
+  for (int c = 0; c < NC; c++) {
+    const long q = o + c * a.plane;
+    if constexpr (MODE == MODE_RHS) {
+      a.yout[q] = K[c];
+    } else if constexpr (MODE == MODE_S1) {
+      const T U = Ucen[c];
+      a.yout[q] = U + a.cy * K[c];
+      a.aio[q] = KAHAN ? a.ca * K[c] : U + a.ca * K[c];
+    } else if constexpr (MODE == MODE_S2 || MODE == MODE_S3) {
+      const T U = a.uin[q], A = a.aio[q];
+      a.yout[q] = U + a.cy * K[c];
+      a.aio[q] = A + a.ca * K[c];
+    } else {
+      const T A = a.aio[q];
+      if constexpr (!KAHAN) {
+        a.uout[q] = A + a.ca * K[c];
+      } else {
+        const T d = A + a.ca * K[c];
+        const T U = a.uout[q], cc = a.comp[q];
+        const T y = d - cc;
+        const T t = U + y;
+        a.comp[q] = (t - U) - y;
+        a.uout[q] = t;
+      }
+    }
+  }
+}
+
+// --------------------------------------------------------------------------
+// Boundary frame: one thread per point of the CLS-deep frame.
+// --------------------------------------------------------------------------
+template <typename T, int ORDER, int PHYS, int MAT, int MODE, bool KAHAN>
+__device__ __forceinline__ void edge_point(const StageArgs<T> &a, int e)
+{
+  constexpr int CLS = Ops<ORDER>::CLS;
+  constexpr bool AC = PHYS == PH_ACOUSTIC;
+  constexpr int NC = AC ? 3 : 1;
+  if (e >= a.n_edge) return;
+  const int nx = a.nx, ny = a.ny;
+  const int nstrip = CLS * nx;
+  const int nside = CLS * (ny - 2 * CLS);
+  int i, j;
+  if (e < nstrip) {  // bottom strip, rows 0..CLS-1
+    j = e / nx;
+    i = e - j * nx;
+  } else if (e < 2 * nstrip) {  // top strip
+    e -= nstrip;
+    j = ny - CLS + e / nx;
+    i = e - (e / nx) * nx;
+  } else if (e < 2 * nstrip + nside) {  // left strip between them
+    e -= 2 * nstrip;
+    j = CLS + e / CLS;
+    i = e - (e / CLS) * CLS;
+  } else {  // right strip
+    e -= 2 * nstrip + nside;
+    j = CLS + e / CLS;
+    i = nx - CLS + (e - (e / CLS) * CLS);
+  }
+  const long pitch = a.pitch;
+  const long o = j * pitch + a.xoff + i;
+  const T *base = a.yin;
+  const long rowb = j * pitch + a.xoff;  // start of row j
+  const long colb = a.xoff + i;          // column i at row 0
+  const T qinv0 = T(Ops<ORDER>::hqinv[0]);
+  const T hqx = a.hix * qinv0, hqy = a.hiy * qinv0;
+  T K[NC], Uc[NC];
+  if constexpr (AC) {
+    const T *U_ = base, *V_ = base + a.plane, *P_ = base + 2 * a.plane;
+    const T dxp = dline<T, ORDER>(P_ + rowb, 1, nx, i, a.hix);
+    const T dyp = dline<T, ORDER>(P_ + colb, pitch, ny, j, a.hiy);
+    const T dxu = dline<T, ORDER>(U_ + rowb, 1, nx, i, a.hix);
+    const T dyv = dline<T, ORDER>(V_ + colb, pitch, ny, j, a.hiy);
+    const T u = U_[o], v = V_[o], p = P_[o];
+    Uc[0] = u;
+    Uc[1] = v;
+    Uc[2] = p;
+    T ri = T(1), Kb = T(1), fu = T(0), fv = T(0);
+    if constexpr (MAT == M_REF) ref_material(a, i, j, ri, fu, fv);
+    if constexpr (MAT == M_ARR) {
+      ri = a.rinv[o];
+      Kb = a.bulk[o];
+    }
+    acoustic_F<T, MAT>(dxp, dyp, dxu, dyv, ri, Kb, fu, fv, K[0], K[1], K[2]);
+    // Boundary terms in bc_serial order W, S, E, N (boundary_conditions.h:84-87).
+    const int bc = a.bc == B_DEFAULT ? B_FSREF : a.bc;
+    if (bc == B_FSREF) {  // free_surface_bc_* (wave_eq_rhs.h:417-468)
+      if (i == 0) K[0] -= hqx * p;
+      if (j == 0) K[1] -= hqy * p;
+      if (i == nx - 1) K[0] += hqx * p;
+      if (j == ny - 1) K[1] += hqy * p;
+    } else if (bc == B_FS) {  // the same terms carrying rho^-1
+      if (i == 0) K[0] -= ri * (hqx * p);
+      if (j == 0) K[1] -= ri * (hqy * p);
+      if (i == nx - 1) K[0] += ri * (hqx * p);
+      if (j == ny - 1) K[1] += ri * (hqy * p);
+    } else {  // wall: normal velocity zero, on the pressure row, carrying K
+      if (i == 0) K[2] -= Kb * (hqx * u);
+      if (j == 0) K[2] -= Kb * (hqy * v);
+      if (i == nx - 1) K[2] += Kb * (hqx * u);
+      if (j == ny - 1) K[2] += Kb * (hqy * v);
+    }
+  } else {  // 2D advection: advection_{ll..rr} + SAT_bc_{west..north}
+    const T dx = dline<T, ORDER>(base + rowb, 1, nx, i, a.hix);
+    const T dy = dline<T, ORDER>(base + colb, pitch, ny, j, a.hiy);
+    const T u = base[o];
+    Uc[0] = u;
+    T k = -(a.ax * dx + a.ay * dy);
+    const T ax = a.ax, ay = a.ay;
+    if (i == 0) k += T(0.5) * (T(-0.5) * (ax + fabs(ax))) * (hqx * u);
+    if (j == 0) k += T(0.5) * (T(-0.5) * (ay + fabs(ay))) * (hqy * u);
+    if (i == nx - 1) k += T(0.5) * (T(0.5) * (ax - fabs(ax))) * (hqx * u);
+    if (j == ny - 1) k += T(0.5) * (T(0.5) * (ay - fabs(ay))) * (hqy * u);
+    K[0] = k;
+  }
+  rk_point<T, NC, MODE, KAHAN>(a, o, K, Uc);
+}
+
+// --------------------------------------------------------------------------
+// Interior tile: columns [x0, x0+BX) of [CLS, nx-CLS), rows [j0, j1).
+// --------------------------------------------------------------------------
+template <typename T, int ORDER, int PHYS, int MAT, int MODE, bool KAHAN, int BX>
+__device__ __forceinline__ void interior_tile(const StageArgs<T> &a, int tile)
+{
+  using O = Ops<ORDER>;
+  constexpr int R = O::R, CLS = O::CLS, WN = 2 * R + 1;
+  constexpr bool AC = PHYS == PH_ACOUSTIC;
+  constexpr int NC = AC ? 3 : 1;
+  constexpr int NF = AC ? 2 : 1;   // fields in LDS / in the y window
+  constexpr int CS = AC ? 2 : 0;   // field differentiated in x and y (p, or u)
+  constexpr int LW = BX + 2 * R;
+  // pointwise values per row: U (stages 2,3), A (2,3,4), U and comp (Kahan 4), materials
+  constexpr bool LD_U = (MODE == MODE_S2 || MODE == MODE_S3) || (MODE == MODE_S4 && KAHAN);
+  constexpr bool LD_A = MODE >= MODE_S2;
+  constexpr bool LD_C = MODE == MODE_S4 && KAHAN;
+  constexpr bool LD_M = AC && MAT == M_ARR;
+
+  __shared__ T lds[2][NF][LW];
+
+  const int tid = threadIdx.x;
+  const int tx = tile % a.tiles_x, tyb = tile / a.tiles_x;
+  const int x0 = CLS + tx * BX;
+  const int i = x0 + tid;
+  const bool active = i < a.nx - CLS;
+  const int ic = min(i, a.nx - 1);
+  const int j0 = CLS + tyb * a.ty_rows;
+  const int j1 = min(j0 + a.ty_rows, a.ny - CLS);
+  const long plane = a.plane;
+  const long pitch = a.pitch;
+  const long col = a.xoff + ic;
+  const bool hl = tid < R, hr = tid >= BX - R, halo = hl || hr;
+  const long hcol = a.xoff + (hl ? i - R : min(i + R, a.nx - 1));
+  const int hslot = hl ? tid : tid + 2 * R;
+  const int jmax = a.ny - 1;
+  const T *__restrict__ yin = a.yin;
+
+  T win[NF][WN];
+#pragma unroll
+  for (int k = 0; k < 2 * R; k++) {
+    const long o = (long)(j0 - R + k) * pitch + col;
+    win[0][k + 1] = yin[CS * plane + o];
+    if constexpr (AC) win[1][k + 1] = yin[plane + o];  // v
+  }
+
+  // Prefetched next row.
+  T nwin[NF], nxo = T(0), nh[NF] = {};
+  T nU[NC] = {}, nA[NC] = {}, nC[NC] = {}, nri = T(1), nK = T(1);
+  auto prefetch = [&](int jn) {
+    const long oy = (long)min(jn + R, jmax) * pitch;
+    const long ox = (long)min(jn, jmax) * pitch;
+    nwin[0] = yin[CS * plane + oy + col];
+    if constexpr (AC) {
+      nwin[1] = yin[plane + oy + col];
+      nxo = yin[ox + col];  // u
+    }
+    if (halo) {
+      nh[0] = yin[CS * plane + ox + hcol];
+      if constexpr (AC) nh[1] = yin[ox + hcol];
+    }
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+      if constexpr (LD_U) nU[c] = (MODE == MODE_S4 ? a.uout : a.uin)[c * plane + ox + col];
+      if constexpr (LD_A) nA[c] = a.aio[c * plane + ox + col];
+      if constexpr (LD_C) nC[c] = a.comp[c * plane + ox + col];
+    }
+    if constexpr (LD_M) {
+      nri = a.rinv[ox + col];
+      nK = a.bulk[ox + col];
+    }
+  };
+
+  double xs_i = 0, c3x = 0, s3x = 0;
+  if constexpr (MAT == M_REF) {
+    xs_i = a.xs[ic];
+    c3x = a.c3x[ic];
+    s3x = a.s3x[ic];
+  }
+
+  prefetch(j0);
+  for (int j = j0; j < j1; j++) {
+#pragma unroll
+    for (int f = 0; f < NF; f++) {
+#pragma unroll
+      for (int k = 0; k < WN - 1; k++) win[f][k] = win[f][k + 1];
+      win[f][WN - 1] = nwin[f];
+    }
+    const T cxo = nxo;
+    T ch[NF];
+#pragma unroll
+    for (int f = 0; f < NF; f++) ch[f] = nh[f];
+    T cU[NC], cA[NC], cC[NC];
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+      cU[c] = nU[c];
+      cA[c] = nA[c];
+      cC[c] = nC[c];
+    }
+    const T cri = nri, cK = nK;
+    prefetch(j + 1);
+
+    const int buf = (j - j0) & 1;
+    lds[buf][0][R + tid] = win[0][R];
+    if constexpr (AC) lds[buf][1][R + tid] = cxo;
+    if (halo) {
+      lds[buf][0][hslot] = ch[0];
+      if constexpr (AC) lds[buf][1][hslot] = ch[1];
+    }
+    __syncthreads();
+
+    const T *l0 = &lds[buf][0][tid];
+    const T d0x = dint<T, ORDER>([&](int s) { return l0[s]; }, a.hix);
+    const T d0y = dint<T, ORDER>([&](int s) { return win[0][s]; }, a.hiy);
+    T K[NC], Uc[NC];
+    if constexpr (AC) {
+      const T *l1 = &lds[buf][1][tid];
+      const T dxu = dint<T, ORDER>([&](int s) { return l1[s]; }, a.hix);
+      const T dyv = dint<T, ORDER>([&](int s) { return win[1][s]; }, a.hiy);
+      T ri = cri, Kb = cK, fu = T(0), fv = T(0);
+      if constexpr (MAT == M_REF) {
+        const double y = a.ys[j];
+        const double xy = xs_i * y;
+        ri = T(1. / (2 + xy));
+        fu = T(-(3 * kPi * a.c5t * c3x * a.s4y[j] * (xy + 1)) / (xy + 2));
+        fv = T(-(4 * kPi * a.c5t * a.c4y[j] * s3x * (xy + 1)) / (xy + 2));
+      }
+      acoustic_F<T, MAT>(d0x, d0y, dxu, dyv, ri, Kb, fu, fv, K[0], K[1], K[2]);
+      Uc[0] = cxo;
+      Uc[1] = win[1][R];
+      Uc[2] = win[0][R];
+    } else {
+      K[0] = -(a.ax * d0x + a.ay * d0y);
+      Uc[0] = win[0][R];
+    }
+
+    if (active) {
+      const long o = (long)j * pitch + col;
+#pragma unroll
+      for (int c = 0; c < NC; c++) {
+        const long q = o + c * plane;
+        if constexpr (MODE == MODE_RHS) {
+          a.yout[q] = K[c];
+        } else if constexpr (MODE == MODE_S1) {
+          a.yout[q] = Uc[c] + a.cy * K[c];
+          a.aio[q] = KAHAN ? a.ca * K[c] : Uc[c] + a.ca * K[c];
+        } else if constexpr (MODE == MODE_S2 || MODE == MODE_S3) {
+          a.yout[q] = cU[c] + a.cy * K[c];
+          a.aio[q] = cA[c] + a.ca * K[c];
+        } else if constexpr (!KAHAN) {
+          a.uout[q] = cA[c] + a.ca * K[c];
+        } else {
+          const T d = cA[c] + a.ca * K[c];
+          const T y = d - cC[c];
+          const T t = cU[c] + y;
+          a.comp[q] = (t - cU[c]) - y;
+          a.uout[q] = t;
+        }
+      }
+    }
+  }
+}
+
+template <typename T, int ORDER, int PHYS, int MAT, int MODE, bool KAHAN, int BX>
+__global__ __launch_bounds__(BX) void stage2d_kernel(StageArgs<T> a)
+{
+  const int b = blockIdx.x;
+  if (b < a.n_edge_blocks) {
+    edge_point<T, ORDER, PHYS, MAT, MODE, KAHAN>(a, b * BX + threadIdx.x);
+    return;
+  }
+  // n_edge_blocks is a multiple of kNumXcd, so bi % 8 is the XCD this block
+  // was dispatched to; hand each XCD a contiguous range of tiles.
+  const int bi = b - a.n_edge_blocks;
+  const int nt = a.n_tiles;
+  const int xcd = bi % kNumXcd, local = bi / kNumXcd;
+  const int q = nt / kNumXcd, r = nt % kNumXcd;
+  const int tile = xcd * q + min(xcd, r) + local;
+  interior_tile<T, ORDER, PHYS, MAT, MODE, KAHAN, BX>(a, tile);
+}
+
+// --------------------------------------------------------------------------
+// 1D problems: one thread per point, table-driven rows.
+// --------------------------------------------------------------------------
+template <typename T, int ORDER, int PHYS, int MODE, bool KAHAN>
+__global__ __launch_bounds__(256) void stage1d_kernel(StageArgs<T> a)
+{
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= a.nx) return;
+  const int nx = a.nx;
+  const long o = a.xoff + i;
+  const T *f0 = a.yin + a.xoff;
+  const T hq = a.hix * T(Ops<ORDER>::hqinv[0]);
+  if constexpr (PHYS == PH_ADV1D) {
+    // advection_{l,i,r} (advection_rhs.h:13-51) + SAT_bc_{l,r} (:95-119)
+    const T u = f0[i];
+    T k = -a.ax * dline<T, ORDER>(f0, 1, nx, i, a.hix);
+    if (i == 0) k += T(0.5) * (T(-0.5) * (a.ax + fabs(a.ax))) * (hq * u);
+    if (i == nx - 1) k += T(0.5) * (T(0.5) * (a.ax - fabs(a.ax))) * (hq * u);
+    T K[1] = {k}, Uc[1] = {u};
+    rk_point<T, 1, MODE, KAHAN>(a, o, K, Uc);
+  } else {
+    // reflection_{l,i,r} (reflection_rhs.h:10-47) + proj_dirichlet_bc_{l,r} (:108-120)
+    const T *f1 = f0 + a.plane;
+    T K[2], Uc[2] = {f0[i], f1[i]};
+    K[1] = dline<T, ORDER>(f0, 1, nx, i, a.hix);
+    K[0] = dline<T, ORDER>(f1, 1, nx, i, a.hix);
+    if (i == 0 || i == nx - 1) K[0] = T(0);
+    rk_point<T, 2, MODE, KAHAN>(a, o, K, Uc);
+  }
+}
+
+// --------------------------------------------------------------------------
+// Energy / H-norm reduction (FP64 accumulation), deterministic two passes.
+// --------------------------------------------------------------------------
+template <typename T> struct NormArgs {
+  const double *aos; // AoS host-layout vector (when non-null, read instead of u)
+  const T *u;       // state planes
+  const T *comp;    // Kahan compensation or null
+  const T *rinv, *bulk;
+  const double *xs, *ys;
+  long plane;
+  int pitch, xoff, nx, ny, nc;
+  int order, is1d, material, weighted;  // weighted: material energy weights
+  double hx, hy;
+};
+
+__device__ __forceinline__ double hq_of(int order, int k)
+{
+  if (order == 2) return c_edge<double, 2>.hq[0];
+  if (order == 4) return c_edge<double, 4>.hq[k];
+  return c_edge<double, 6>.hq[k];
+}
+
+__device__ __forceinline__ double hweight(int order, int n, int i, double h)
+{
+  const int cls = order == 2 ? 1 : (order == 4 ? 4 : 6);
+  if (i < cls) return h * hq_of(order, i);
+  if (i >= n - cls) return h * hq_of(order, n - 1 - i);
+  return h;
+}
+
+__device__ __forceinline__ double block_sum_256(double v, double *sh)
+{
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) sh[wv] = v;
+  __syncthreads();
+  double s = 0;
+  if (threadIdx.x == 0) s = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+  return s;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void norm_partial_kernel(NormArgs<T> a, double *partials)
+{
+  __shared__ double sh[4];
+  double acc = 0;
+  const long npts = (long)a.nx * a.ny;
+  for (long k = blockIdx.x * 256L + threadIdx.x; k < npts; k += (long)gridDim.x * 256) {
+    const int j = (int)(k / a.nx), i = (int)(k - (long)j * a.nx);
+    const long o = (long)j * a.pitch + a.xoff + i;
+    double w = hweight(a.order, a.nx, i, a.hx);
+    if (!a.is1d) w *= hweight(a.order, a.ny, j, a.hy);
+    double v[3] = {0, 0, 0};
+    for (int c = 0; c < a.nc; c++) {
+      double x;
+      if (a.aos) {
+        x = a.aos[k * a.nc + c];
+      } else {
+        x = (double)a.u[c * a.plane + o];
+        if (a.comp) x -= (double)a.comp[c * a.plane + o];
+      }
+      v[c] = x;
+    }
+    double e;
+    if (a.weighted && a.nc == 3) {
+      double rho = 1.0, kinv = 1.0;
+      if (a.material == M_ARR) {
+        rho = 1.0 / (double)a.rinv[o];
+        kinv = 1.0 / (double)a.bulk[o];
+      } else if (a.material == M_REF) {
+        rho = 2 + a.xs[i] * a.ys[j];
+      }
+      e = rho * (v[0] * v[0] + v[1] * v[1]) + kinv * v[2] * v[2];
+    } else {
+      e = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+    }
+    acc += w * e;
+  }
+  const double s = block_sum_256(acc, sh);
+  if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+static __global__ __launch_bounds__(256) void norm_final_kernel(const double *partials, int n, double *out)
+{
+  __shared__ double sh[4];
+  double acc = 0;
+  for (int k = threadIdx.x; k < n; k += 256) acc += partials[k];
+  const double s = block_sum_256(acc, sh);
+  if (threadIdx.x == 0) out[0] = s;
+}
+
+// --------------------------------------------------------------------------
+// AoS (reference layout, double) <-> planar device state.
+// --------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void aos_to_planar_kernel(const double *__restrict__ aos, T *__restrict__ U,
+                                                            T *__restrict__ comp, int nx, int ny, int nc,
+                                                            int pitch, int xoff, long plane)
+{
+  const long k = blockIdx.x * 256L + threadIdx.x;
+  if (k >= (long)nx * ny) return;
+  const int j = (int)(k / nx), i = (int)(k - (long)j * nx);
+  const long o = (long)j * pitch + xoff + i;
+  for (int c = 0; c < nc; c++) {
+    const double v = aos[k * nc + c];
+    const T t = (T)v;
+    U[c * plane + o] = t;
+    if (comp) comp[c * plane + o] = (T)((double)t - v);  // value = U - comp
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void planar_to_aos_kernel(const T *__restrict__ U, const T *__restrict__ comp,
+                                                            double *__restrict__ aos, int nx, int ny, int nc,
+                                                            int pitch, int xoff, long plane)
+{
+  const long k = blockIdx.x * 256L + threadIdx.x;
+  if (k >= (long)nx * ny) return;
+  const int j = (int)(k / nx), i = (int)(k - (long)j * nx);
+  const long o = (long)j * pitch + xoff + i;
+  for (int c = 0; c < nc; c++) {
+    double v = (double)U[c * plane + o];
+    if (comp) v -= (double)comp[c * plane + o];
+    aos[k * nc + c] = v;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void matrix_to_plane_kernel(const double *__restrict__ m, T *__restrict__ P,
+                                                              int nx, int ny, int pitch, int xoff)
+{
+  const long k = blockIdx.x * 256L + threadIdx.x;
+  if (k >= (long)nx * ny) return;
+  const int j = (int)(k / nx), i = (int)(k - (long)j * nx);
+  P[(long)j * pitch + xoff + i] = (T)m[k];
+}
+
+// --------------------------------------------------------------------------
+// Host-side launch table entry.
+// --------------------------------------------------------------------------
+struct LaunchGeom {
+  int grid;   // blocks
+  int block;  // threads
+};
+
+using StageLaunchFn = hipError_t (*)(int mode, const void *args, LaunchGeom g, hipStream_t s);
+
+template <typename T, int ORDER, int PHYS, int MAT, bool KAHAN, int BX>
+struct Launcher2D {
+  template <int MODE> static hipError_t go(const StageArgs<T> &a, LaunchGeom g, hipStream_t s)
+  {
+    hipLaunchKernelGGL((stage2d_kernel<T, ORDER, PHYS, MAT, MODE, KAHAN, BX>), dim3(g.grid), dim3(BX), 0, s, a);
+    return hipGetLastError();
+  }
+  static hipError_t launch(int mode, const void *args, LaunchGeom g, hipStream_t s)
+  {
+    const StageArgs<T> &a = *static_cast<const StageArgs<T> *>(args);
+    switch (mode) {
+      case MODE_RHS: return go<MODE_RHS>(a, g, s);
+      case MODE_S1: return go<MODE_S1>(a, g, s);
+      case MODE_S2: return go<MODE_S2>(a, g, s);
+      case MODE_S3: return go<MODE_S3>(a, g, s);
+      default: return go<MODE_S4>(a, g, s);
+    }
+  }
+};
+
+template <typename T, int ORDER, int PHYS, bool KAHAN>
+struct Launcher1D {
+  template <int MODE> static hipError_t go(const StageArgs<T> &a, LaunchGeom g, hipStream_t s)
+  {
+    hipLaunchKernelGGL((stage1d_kernel<T, ORDER, PHYS, MODE, KAHAN>), dim3(g.grid), dim3(256), 0, s, a);
+    return hipGetLastError();
+  }
+  static hipError_t launch(int mode, const void *args, LaunchGeom g, hipStream_t s)
+  {
+    const StageArgs<T> &a = *static_cast<const StageArgs<T> *>(args);
+    switch (mode) {
+      case MODE_RHS: return go<MODE_RHS>(a, g, s);
+      case MODE_S1: return go<MODE_S1>(a, g, s);
+      case MODE_S2: return go<MODE_S2>(a, g, s);
+      case MODE_S3: return go<MODE_S3>(a, g, s);
+      default: return go<MODE_S4>(a, g, s);
+    }
+  }
+};
+
+// Per-order registries (sbp_inst_o{2,4,6}.hip).  prec: 0 FP64, 1 FP32, 2 FP32+Kahan.
+StageLaunchFn launcher_o2(int phys, int mat, int prec, int bx);
+StageLaunchFn launcher_o4(int phys, int mat, int prec, int bx);
+StageLaunchFn launcher_o6(int phys, int mat, int prec, int bx);
+
+}  // namespace sbp

@@ -91,6 +91,10 @@ int  sbp_destroy(sbp_ctx *ctx);
 int  sbp_components(const sbp_ctx *ctx);
 long sbp_vector_length(const sbp_ctx *ctx);
 
+/* Interior tile height in rows (default: chosen from the grid size).  A
+ * tuning knob with no reference counterpart; results do not depend on it. */
+int  sbp_set_tile_rows(sbp_ctx *ctx, int rows);
+
 /* ---- right-hand side ----------------------------------------------------
  * dst = F(t, src) on host AoS arrays.  Replaces the rhs_serial callbacks
  * (advection_1D_sim.cpp:212-228, advection_2D_sim.cpp:232-249,
@@ -133,23 +137,6 @@ int  sbp_hnorm(sbp_ctx *ctx, const double *v, double *norm);
 /* Error norms of vec_util.h:47-61: l2 = sqrt(hx*hy)*||v-w||_2, max = ||v-w||_inf. */
 int  sbp_error_norms(const sbp_ctx *ctx, const double *v, const double *w,
                      double *l2, double *maxerr);
-
-/* ---- batched ensemble ---------------------------------------------------
- * Many independent 2D acoustic problems of uneven size advanced together, one
- * persistent kernel per RK4 stage, the whole step captured in a hipGraph.
- * No counterpart in the reference (one problem per MPI job there). */
-typedef struct sbp_batch sbp_batch;
-int  sbp_batch_create(const sbp_problem *descs, int count, sbp_batch **out);
-int  sbp_batch_destroy(sbp_batch *b);
-int  sbp_batch_set_state(sbp_batch *b, int k, const double *v);
-int  sbp_batch_get_state(sbp_batch *b, int k, double *v);
-/* dt[k] per instance; nsteps RK4 steps for every instance. */
-int  sbp_batch_step(sbp_batch *b, const double *dt, long nsteps);
-int  sbp_batch_time_steps(sbp_batch *b, const double *dt, long nsteps, double *ms);
-/* Pick the tile shape by timing candidates on this ensemble; writes the
- * chosen configuration index to *choice. */
-int  sbp_batch_autotune(sbp_batch *b, const double *dt, int *choice);
-int  sbp_batch_synchronize(sbp_batch *b);
 
 /* ---- diagnostics ---------------------------------------------------------- */
 const char *sbp_last_error(void);
