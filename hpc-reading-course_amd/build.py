@@ -31,6 +31,12 @@ def _run(cmd):
     return out
 
 
+def extra_flags(src):
+    # FP64 kernels: no FMA contraction, so the right-hand side rounds exactly
+    # like the reference's loops (mul then add, in the same order).
+    return ["-ffp-contract=off"] if src.endswith("_f64.hip") else []
+
+
 def sources():
     return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
 
@@ -41,7 +47,8 @@ def build(jobs: int | None = None, verbose: bool = False) -> str:
     objs = [os.path.join(OBJ, os.path.basename(s)[:-4] + ".o") for s in srcs]
     jobs = jobs or min(8, os.cpu_count() or 4)
     with cf.ThreadPoolExecutor(jobs) as ex:
-        futs = [ex.submit(_run, [HIPCC, *HIPFLAGS, "-c", s, "-o", o]) for s, o in zip(srcs, objs)]
+        futs = [ex.submit(_run, [HIPCC, *HIPFLAGS, *extra_flags(s), "-c", s, "-o", o])
+                for s, o in zip(srcs, objs)]
         for f in futs:
             msg = f.result()
             if verbose and msg.strip():

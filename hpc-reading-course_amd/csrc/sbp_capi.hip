@@ -67,7 +67,7 @@ struct sbp_ctx {
   long plane = 0;
   size_t esz = 8;
   double hix = 0, hiy = 0;
-  void *U = nullptr, *A = nullptr, *Y0 = nullptr, *Y1 = nullptr, *KC = nullptr;
+  void *U = nullptr, *A = nullptr, *Y0 = nullptr, *Y1 = nullptr, *KC = nullptr, *TK = nullptr;
   void *rinv = nullptr, *bulk = nullptr;
   double *xs = nullptr, *ys = nullptr, *c3x = nullptr, *s3x = nullptr, *c4y = nullptr, *s4y = nullptr;
   double *aos = nullptr;
@@ -126,6 +126,7 @@ StageArgs<T> make_args(const sbp_ctx *c, int mode, const void *yin, double dt, d
   a.aio = static_cast<T *>(c->A);
   a.uout = static_cast<T *>(c->U);
   a.comp = static_cast<T *>(c->KC);
+  a.tk = static_cast<T *>(c->TK);
   a.rinv = static_cast<const T *>(c->rinv);
   a.bulk = static_cast<const T *>(c->bulk);
   a.xs = c->xs;
@@ -224,7 +225,13 @@ int run_steps(sbp_ctx *c, double t0, double dt, long nsteps)
     if (rc) return rc;
     for (; n + kGraphSteps <= nsteps; n += kGraphSteps) HIP_OR_FAIL(hipGraphLaunch(c->gexec, c->stream));
   }
-  for (; n < nsteps; n++) HIP_OR_FAIL(launch_step(c, t0 + n * dt, dt));
+  // Step times accumulate as t = t + dt (RK4_custom, time_stepping.h:141).
+  double t = t0;
+  for (long k = 0; k < n; k++) t = t + dt;
+  for (; n < nsteps; n++) {
+    HIP_OR_FAIL(launch_step(c, t, dt));
+    t = t + dt;
+  }
   return SBP_OK;
 }
 
@@ -382,7 +389,7 @@ void release(sbp_ctx *c)
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   drop_graph(c);
-  void *ptrs[] = {c->U, c->A, c->Y0, c->Y1, c->KC, c->rinv, c->bulk, c->xs, c->ys, c->c3x, c->s3x,
+  void *ptrs[] = {c->U, c->A, c->Y0, c->Y1, c->KC, c->TK, c->rinv, c->bulk, c->xs, c->ys, c->c3x, c->s3x,
                   c->c4y, c->s4y, c->aos, c->partials, c->scalar};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -418,9 +425,15 @@ int build_ctx(const sbp_problem *desc, sbp_ctx *c)
   c->ty_rows = default_ty_rows(c->nx, c->ny, c->cls);
   compute_geometry(c);
 
-  StageLaunchFn (*reg)(int, int, int, int) =
-      c->order == 2 ? launcher_o2 : (c->order == 4 ? launcher_o4 : launcher_o6);
-  c->launch = reg(c->phys, c->mat, c->prec, kBX);
+  if (c->prec == SBP_FP64) {
+    StageLaunchFn (*reg)(int, int, int) =
+        c->order == 2 ? launcher_o2_f64 : (c->order == 4 ? launcher_o4_f64 : launcher_o6_f64);
+    c->launch = reg(c->phys, c->mat, kBX);
+  } else {
+    StageLaunchFn (*reg)(int, int, bool, int) =
+        c->order == 2 ? launcher_o2_f32 : (c->order == 4 ? launcher_o4_f32 : launcher_o6_f32);
+    c->launch = reg(c->phys, c->mat, c->prec == SBP_FP32_KAHAN, kBX);
+  }
   if (!c->launch) return fail(SBP_EINVAL, "no kernel for this physics/material/precision");
 
   HIP_OR_FAIL(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
@@ -431,8 +444,9 @@ int build_ctx(const sbp_problem *desc, sbp_ctx *c)
       (rc = dalloc(&c->Y1, reg_bytes)))
     return rc;
   if (c->prec == SBP_FP32_KAHAN && (rc = dalloc(&c->KC, reg_bytes))) return rc;
+  if (is1d(c->phys) && (rc = dalloc(&c->TK, reg_bytes))) return rc;
   // Zero the registers once so padding never holds stale bit patterns.
-  for (void *p : {c->U, c->A, c->Y0, c->Y1, c->KC})
+  for (void *p : {c->U, c->A, c->Y0, c->Y1, c->KC, c->TK})
     if (p) HIP_OR_FAIL(hipMemsetAsync(p, 0, reg_bytes, c->stream));
   if ((rc = dalloc(&c->aos, sizeof(double) * (size_t)c->C * c->nx * c->ny))) return rc;
   c->npartials = 1024;

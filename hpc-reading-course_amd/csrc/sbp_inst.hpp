@@ -1,52 +1,44 @@
 // sbp_inst.hpp -- maps (physics, material, precision, block width) to the
-// stage-kernel launcher of one SBP order.  Included once per order by
-// sbp_inst_o{2,4,6}.hip so the three orders compile in parallel.
+// stage-kernel launcher of one SBP order.  Included by sbp_inst_o{2,4,6}_{f64,f32}.hip
+// so orders and precisions compile in parallel (and the FP64 units can be
+// built without FMA contraction).
 #pragma once
 #include "sbp_kernels.hpp"
 
 namespace sbp {
 
-template <int ORDER, int BX>
-StageLaunchFn pick_launcher(int phys, int mat, int prec)
+template <typename T, int ORDER, bool KAHAN, int BX>
+StageLaunchFn pick_launcher(int phys, int mat)
 {
-  if (phys == PH_ADV1D) {
-    if (prec == 0) return &Launcher1D<double, ORDER, PH_ADV1D, false>::launch;
-    if (prec == 1) return &Launcher1D<float, ORDER, PH_ADV1D, false>::launch;
-    return &Launcher1D<float, ORDER, PH_ADV1D, true>::launch;
-  }
-  if (phys == PH_REFL1D) {
-    if (prec == 0) return &Launcher1D<double, ORDER, PH_REFL1D, false>::launch;
-    if (prec == 1) return &Launcher1D<float, ORDER, PH_REFL1D, false>::launch;
-    return &Launcher1D<float, ORDER, PH_REFL1D, true>::launch;
-  }
-  if (phys == PH_ADV2D) {
-    if (prec == 0) return &Launcher2D<double, ORDER, PH_ADV2D, M_UNIT, false, BX>::launch;
-    if (prec == 1) return &Launcher2D<float, ORDER, PH_ADV2D, M_UNIT, false, BX>::launch;
-    return &Launcher2D<float, ORDER, PH_ADV2D, M_UNIT, true, BX>::launch;
-  }
+  if (phys == PH_ADV1D) return &Launcher1D<T, ORDER, PH_ADV1D, KAHAN>::launch;
+  if (phys == PH_REFL1D) return &Launcher1D<T, ORDER, PH_REFL1D, KAHAN>::launch;
+  if (phys == PH_ADV2D) return &Launcher2D<T, ORDER, PH_ADV2D, M_UNIT, KAHAN, BX>::launch;
   if (phys == PH_ACOUSTIC) {
-    if (mat == M_REF) return prec == 0 ? &Launcher2D<double, ORDER, PH_ACOUSTIC, M_REF, false, BX>::launch : nullptr;
-    if (mat == M_UNIT) {
-      if (prec == 0) return &Launcher2D<double, ORDER, PH_ACOUSTIC, M_UNIT, false, BX>::launch;
-      if (prec == 1) return &Launcher2D<float, ORDER, PH_ACOUSTIC, M_UNIT, false, BX>::launch;
-      return &Launcher2D<float, ORDER, PH_ACOUSTIC, M_UNIT, true, BX>::launch;
-    }
-    if (mat == M_ARR) {
-      if (prec == 0) return &Launcher2D<double, ORDER, PH_ACOUSTIC, M_ARR, false, BX>::launch;
-      if (prec == 1) return &Launcher2D<float, ORDER, PH_ACOUSTIC, M_ARR, false, BX>::launch;
-      return &Launcher2D<float, ORDER, PH_ACOUSTIC, M_ARR, true, BX>::launch;
-    }
+    if (mat == M_UNIT) return &Launcher2D<T, ORDER, PH_ACOUSTIC, M_UNIT, KAHAN, BX>::launch;
+    if (mat == M_ARR) return &Launcher2D<T, ORDER, PH_ACOUSTIC, M_ARR, KAHAN, BX>::launch;
+    if constexpr (sizeof(T) == 8 && !KAHAN)
+      if (mat == M_REF) return &Launcher2D<T, ORDER, PH_ACOUSTIC, M_REF, KAHAN, BX>::launch;
   }
   return nullptr;
 }
 
 }  // namespace sbp
 
-#define SBP_DEFINE_LAUNCHER(ORDER)                                              \
+#define SBP_DEFINE_LAUNCHER_F64(ORDER)                                          \
   namespace sbp {                                                               \
-  StageLaunchFn launcher_o##ORDER(int phys, int mat, int prec, int bx)          \
+  StageLaunchFn launcher_o##ORDER##_f64(int phys, int mat, int bx)              \
   {                                                                             \
-    if (bx == 256) return pick_launcher<ORDER, 256>(phys, mat, prec);           \
+    if (bx == 256) return pick_launcher<double, ORDER, false, 256>(phys, mat);  \
     return nullptr;                                                             \
+  }                                                                             \
+  }
+
+#define SBP_DEFINE_LAUNCHER_F32(ORDER)                                          \
+  namespace sbp {                                                               \
+  StageLaunchFn launcher_o##ORDER##_f32(int phys, int mat, bool kahan, int bx)  \
+  {                                                                             \
+    if (bx != 256) return nullptr;                                              \
+    return kahan ? pick_launcher<float, ORDER, true, 256>(phys, mat)            \
+                 : pick_launcher<float, ORDER, false, 256>(phys, mat);          \
   }                                                                             \
   }

@@ -54,6 +54,7 @@ template <typename T> struct StageArgs {
   T *yout;        // next Y (stages 1-3) or F (RHS mode)
   T *uout;        // U (stage 4)
   T *comp;        // Kahan compensation of U
+  T *tk;          // k2 register of the 1D stage sequence
   const T *rinv;  // rho^-1 plane (caller material arrays)
   const T *bulk;  // K plane
   const double *xs, *ys;     // x_i = xl + i/hix, y_j (reference rho and forcing)
@@ -137,7 +138,7 @@ __device__ __forceinline__ void ref_material(const StageArgs<T> &a, int i, int j
   fv = T(-(4 * kPi * a.c5t * a.c4y[j] * a.s3x[i] * (xy + 1)) / (xy + 2));
 }
 
-// RK4 stage arithmetic at one point (edge and 1D paths).
+// RK4 stage arithmetic at one point (edge path of the 2D kernels).
 template <typename T, int NC, int MODE, bool KAHAN>
 __device__ __forceinline__ void rk_point(const StageArgs<T> &a, long o, const T *K, const T *Ucen)
 {
@@ -162,6 +163,45 @@ __device__ __forceinline__ void rk_point(const StageArgs<T> &a, long o, const T 
         const T d = A + a.ca * K[c];
         const T U = a.uout[q], cc = a.comp[q];
         const T y = d - cc;
+        const T t = U + y;
+        a.comp[q] = (t - U) - y;
+        a.uout[q] = t;
+      }
+    }
+  }
+}
+
+// RK4 stage arithmetic of the 1D kernels in exactly RK4_custom's order
+// (time_stepping.h:127-142, rk4_util :267-269): the stage vectors are kept,
+//   stage 1: S = k1            Y = U + dt/2 k1
+//   stage 2: T = k2            Y = U + dt/2 k2
+//   stage 3: S = k1 + 2(k2+k3) Y = U + dt k3
+//   stage 4: U = U + dt/6 (S + k4)   (Kahan form: the increment is compensated)
+// The 1D problems are latency-bound, so the extra register costs nothing.
+template <typename T, int NC, int MODE, bool KAHAN>
+__device__ __forceinline__ void rk_point_exact(const StageArgs<T> &a, long o, const T *K, const T *Ucen)
+{
+#pragma unroll
+  for (int c = 0; c < NC; c++) {
+    const long q = o + c * a.plane;
+    if constexpr (MODE == MODE_RHS) {
+      a.yout[q] = K[c];
+    } else if constexpr (MODE == MODE_S1) {
+      a.aio[q] = K[c];
+      a.yout[q] = Ucen[c] + a.cy * K[c];
+    } else if constexpr (MODE == MODE_S2) {
+      a.tk[q] = K[c];
+      a.yout[q] = a.uin[q] + a.cy * K[c];
+    } else if constexpr (MODE == MODE_S3) {
+      a.aio[q] = a.aio[q] + 2 * (a.tk[q] + K[c]);
+      a.yout[q] = a.uin[q] + a.cy * K[c];
+    } else {
+      const T inc = a.ca * (a.aio[q] + K[c]);
+      const T U = a.uout[q];
+      if constexpr (!KAHAN) {
+        a.uout[q] = U + inc;
+      } else {
+        const T y = inc - a.comp[q];
         const T t = U + y;
         a.comp[q] = (t - U) - y;
         a.uout[q] = t;
@@ -459,7 +499,7 @@ __global__ __launch_bounds__(256) void stage1d_kernel(StageArgs<T> a)
     if (i == 0) k += T(0.5) * (T(-0.5) * (a.ax + fabs(a.ax))) * (hq * u);
     if (i == nx - 1) k += T(0.5) * (T(0.5) * (a.ax - fabs(a.ax))) * (hq * u);
     T K[1] = {k}, Uc[1] = {u};
-    rk_point<T, 1, MODE, KAHAN>(a, o, K, Uc);
+    rk_point_exact<T, 1, MODE, KAHAN>(a, o, K, Uc);
   } else {
     // reflection_{l,i,r} (reflection_rhs.h:10-47) + proj_dirichlet_bc_{l,r} (:108-120)
     const T *f1 = f0 + a.plane;
@@ -467,7 +507,7 @@ __global__ __launch_bounds__(256) void stage1d_kernel(StageArgs<T> a)
     K[1] = dline<T, ORDER>(f0, 1, nx, i, a.hix);
     K[0] = dline<T, ORDER>(f1, 1, nx, i, a.hix);
     if (i == 0 || i == nx - 1) K[0] = T(0);
-    rk_point<T, 2, MODE, KAHAN>(a, o, K, Uc);
+    rk_point_exact<T, 2, MODE, KAHAN>(a, o, K, Uc);
   }
 }
 
@@ -659,9 +699,15 @@ struct Launcher1D {
   }
 };
 
-// Per-order registries (sbp_inst_o{2,4,6}.hip).  prec: 0 FP64, 1 FP32, 2 FP32+Kahan.
-StageLaunchFn launcher_o2(int phys, int mat, int prec, int bx);
-StageLaunchFn launcher_o4(int phys, int mat, int prec, int bx);
-StageLaunchFn launcher_o6(int phys, int mat, int prec, int bx);
+// Per-order, per-precision registries (sbp_inst_o{2,4,6}_{f64,f32}.hip).
+// The FP64 units are compiled without FMA contraction so that every
+// right-hand side evaluates in the same operation order, and with the same
+// roundings, as the reference's loops.  kahan selects the compensated FP32 form.
+StageLaunchFn launcher_o2_f64(int phys, int mat, int bx);
+StageLaunchFn launcher_o4_f64(int phys, int mat, int bx);
+StageLaunchFn launcher_o6_f64(int phys, int mat, int bx);
+StageLaunchFn launcher_o2_f32(int phys, int mat, bool kahan, int bx);
+StageLaunchFn launcher_o4_f32(int phys, int mat, bool kahan, int bx);
+StageLaunchFn launcher_o6_f32(int phys, int mat, bool kahan, int bx);
 
 }  // namespace sbp

@@ -1,7 +1,14 @@
 """HIP path vs the CPU oracle (FP64), through the C-ABI.
 
-Bar: max-norm difference relative to the max norm of the oracle result
-  <= 1e-12 for every FP64 right-hand side and RK4 run (tolerance stated per test).
+Bars (stated per test):
+* FP64 right-hand sides (every physics, order, material, boundary kind and
+  ragged grid) are BIT-IDENTICAL to the oracle: the FP64 kernels are built
+  without FMA contraction and evaluate every stencil, material and SAT term in
+  the oracle's (the reference's) operation order.
+* 1D RK4 runs are bit-identical too (RK4_custom's combination order).
+* 2D RK4 runs accumulate the RK4 weights in a fused register (A = U + dt/6 k1
+  + dt/3 k2 + ...) instead of RK4_custom's k1 + 2(k2+k3) + k4: max-norm
+  difference relative to the oracle's max norm <= 1e-12.
 """
 import os
 import sys
@@ -80,7 +87,7 @@ def test_rhs_2d(capi, orc, case):
     ref = orc.rhs(spec, t, v)
     with capi.Context(spec) as ctx:
         got = ctx.rhs(t, v)
-    assert relerr(got, ref) <= TOL
+    assert np.array_equal(got, ref), relerr(got, ref)
 
 
 @pytest.mark.parametrize("order", [2, 4, 6])
@@ -94,7 +101,7 @@ def test_rhs_1d(capi, orc, kind, order, n):
     ref = orc.rhs(spec, 0.0, v)
     with capi.Context(spec) as ctx:
         got = ctx.rhs(0.0, v)
-    assert relerr(got, ref) <= TOL
+    assert np.array_equal(got, ref), relerr(got, ref)
 
 
 RK_CASES = [
@@ -132,7 +139,7 @@ def test_rk4_1d(capi, orc, kind, order):
     ref = orc.rk4(spec, v0, 0.0, dt, 50)
     with capi.Context(spec) as ctx:
         got = ctx.rk4_advance(v0, 0.0, dt, 50)
-    assert relerr(got, ref) <= TOL
+    assert np.array_equal(got, ref), relerr(got, ref)
 
 
 def test_config_1d_advection_full(capi, orc):
@@ -145,7 +152,7 @@ def test_config_1d_advection_full(capi, orc):
     with capi.Context(spec) as ctx:
         got = ctx.rk4_advance(v0, 0.0, dt, n)
         e_gpu = ctx.energy(got)
-    assert relerr(got, ref) <= TOL
+    assert np.array_equal(got, ref), relerr(got, ref)
     assert e_gpu == pytest.approx(orc.energy(spec, ref), rel=1e-13)
 
 
