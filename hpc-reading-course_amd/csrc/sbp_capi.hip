@@ -104,16 +104,17 @@ void compute_geometry(sbp_ctx *c)
   c->geom.block = kBX;
 }
 
-// Default tile height: the tallest of 16..128 rows that still gives at least
-// eight tiles per CU (256 CUs); tall tiles amortise the 2R-row y halo.
+// Default tile height: the tallest of 16..64 rows that still gives at least
+// four tiles per CU (256 CUs); tall tiles amortise the 2R-row y halo
+// (measured on 4097^2: 64 rows beat 16/32/128).
 int default_ty_rows(int nx, int ny, int cls)
 {
   const int wi = nx - 2 * cls, hi = ny - 2 * cls;
   if (wi <= 0 || hi <= 0) return 16;
   const int tx = (wi + kBX - 1) / kBX;
   int best = 16;
-  for (int t = 16; t <= 128; t *= 2)
-    if ((long)tx * ((hi + t - 1) / t) >= 2048) best = t;
+  for (int t = 16; t <= 64; t *= 2)
+    if ((long)tx * ((hi + t - 1) / t) >= 1024) best = t;
   return best;
 }
 

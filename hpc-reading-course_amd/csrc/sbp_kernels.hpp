@@ -302,7 +302,40 @@ __device__ __forceinline__ void edge_point(const StageArgs<T> &a, int e)
 // --------------------------------------------------------------------------
 // Interior tile: columns [x0, x0+BX) of [CLS, nx-CLS), rows [j0, j1).
 // --------------------------------------------------------------------------
-template <typename T, int ORDER, int PHYS, int MAT, int MODE, bool KAHAN, int BX>
+
+// Memory-access policy of the interior tiles (tuned on gfx950):
+//   PF   rows of loads kept in flight ahead of the row being computed;
+//   NTP  nontemporal loads of the pointwise operands (U, A, compensation,
+//        materials), read exactly once;
+//   NTS  nontemporal stores of every output;
+//   NTY  nontemporal loads of the stencil operand Y.
+template <int PF_, bool NTP_, bool NTS_, bool NTY_> struct TilePolicy {
+  static constexpr int PF = PF_;
+  static constexpr bool NTP = NTP_, NTS = NTS_, NTY = NTY_;
+};
+using DefaultPolicy = TilePolicy<1, true, true, false>;
+
+template <bool NT, typename T> __device__ __forceinline__ T ldg(const T *p)
+{
+  if constexpr (NT) return __builtin_nontemporal_load(p);
+  else return *p;
+}
+
+template <bool NT, typename T> __device__ __forceinline__ void stg(T *p, T v)
+{
+  if constexpr (NT) __builtin_nontemporal_store(v, p);
+  else *p = v;
+}
+
+template <typename T, int NF, int NC> struct RowRegs {
+  T win[NF];  // y-differentiated fields at row j+R
+  T xo;       // x-only field (u) at row j
+  T h[NF];    // halo column of the x-differentiated fields at row j
+  T U[NC], A[NC], C[NC];
+  T ri, K;
+};
+
+template <typename T, int ORDER, int PHYS, int MAT, int MODE, bool KAHAN, int BX, class POL>
 __device__ __forceinline__ void interior_tile(const StageArgs<T> &a, int tile)
 {
   using O = Ops<ORDER>;
@@ -312,6 +345,7 @@ __device__ __forceinline__ void interior_tile(const StageArgs<T> &a, int tile)
   constexpr int NF = AC ? 2 : 1;   // fields in LDS / in the y window
   constexpr int CS = AC ? 2 : 0;   // field differentiated in x and y (p, or u)
   constexpr int LW = BX + 2 * R;
+  constexpr int PF = POL::PF;
   // pointwise values per row: U (stages 2,3), A (2,3,4), U and comp (Kahan 4), materials
   constexpr bool LD_U = (MODE == MODE_S2 || MODE == MODE_S3) || (MODE == MODE_S4 && KAHAN);
   constexpr bool LD_A = MODE >= MODE_S2;
@@ -336,41 +370,45 @@ __device__ __forceinline__ void interior_tile(const StageArgs<T> &a, int tile)
   const int hslot = hl ? tid : tid + 2 * R;
   const int jmax = a.ny - 1;
   const T *__restrict__ yin = a.yin;
+  const T *__restrict__ usrc = MODE == MODE_S4 ? a.uout : a.uin;
 
   T win[NF][WN];
 #pragma unroll
   for (int k = 0; k < 2 * R; k++) {
     const long o = (long)(j0 - R + k) * pitch + col;
-    win[0][k + 1] = yin[CS * plane + o];
-    if constexpr (AC) win[1][k + 1] = yin[plane + o];  // v
+    win[0][k + 1] = ldg<POL::NTY>(yin + CS * plane + o);
+    if constexpr (AC) win[1][k + 1] = ldg<POL::NTY>(yin + plane + o);  // v
   }
 
-  // Prefetched next row.
-  T nwin[NF], nxo = T(0), nh[NF] = {};
-  T nU[NC] = {}, nA[NC] = {}, nC[NC] = {}, nri = T(1), nK = T(1);
-  auto prefetch = [&](int jn) {
+  RowRegs<T, NF, NC> q[PF];
+  auto fetch = [&](RowRegs<T, NF, NC> &r, int jn) {
     const long oy = (long)min(jn + R, jmax) * pitch;
     const long ox = (long)min(jn, jmax) * pitch;
-    nwin[0] = yin[CS * plane + oy + col];
+    r.win[0] = ldg<POL::NTY>(yin + CS * plane + oy + col);
     if constexpr (AC) {
-      nwin[1] = yin[plane + oy + col];
-      nxo = yin[ox + col];  // u
+      r.win[1] = ldg<POL::NTY>(yin + plane + oy + col);
+      r.xo = ldg<POL::NTY>(yin + ox + col);  // u
     }
     if (halo) {
-      nh[0] = yin[CS * plane + ox + hcol];
-      if constexpr (AC) nh[1] = yin[ox + hcol];
+      r.h[0] = yin[CS * plane + ox + hcol];
+      if constexpr (AC) r.h[1] = yin[ox + hcol];
     }
 #pragma unroll
     for (int c = 0; c < NC; c++) {
-      if constexpr (LD_U) nU[c] = (MODE == MODE_S4 ? a.uout : a.uin)[c * plane + ox + col];
-      if constexpr (LD_A) nA[c] = a.aio[c * plane + ox + col];
-      if constexpr (LD_C) nC[c] = a.comp[c * plane + ox + col];
+      if constexpr (LD_U) r.U[c] = ldg<POL::NTP>(usrc + c * plane + ox + col);
+      if constexpr (LD_A) r.A[c] = ldg<POL::NTP>(a.aio + c * plane + ox + col);
+      if constexpr (LD_C) r.C[c] = ldg<POL::NTP>(a.comp + c * plane + ox + col);
     }
     if constexpr (LD_M) {
-      nri = a.rinv[ox + col];
-      nK = a.bulk[ox + col];
+      r.ri = ldg<POL::NTP>(a.rinv + ox + col);
+      r.K = ldg<POL::NTP>(a.bulk + ox + col);
     }
   };
+#pragma unroll
+  for (int k = 0; k < PF; k++) {
+    q[k] = RowRegs<T, NF, NC>{};
+    fetch(q[k], j0 + k);
+  }
 
   double xs_i = 0, c3x = 0, s3x = 0;
   if constexpr (MAT == M_REF) {
@@ -379,34 +417,24 @@ __device__ __forceinline__ void interior_tile(const StageArgs<T> &a, int tile)
     s3x = a.s3x[ic];
   }
 
-  prefetch(j0);
   for (int j = j0; j < j1; j++) {
+    const RowRegs<T, NF, NC> cur = q[0];
+#pragma unroll
+    for (int k = 0; k < PF - 1; k++) q[k] = q[k + 1];
+    fetch(q[PF - 1], j + PF);
 #pragma unroll
     for (int f = 0; f < NF; f++) {
 #pragma unroll
       for (int k = 0; k < WN - 1; k++) win[f][k] = win[f][k + 1];
-      win[f][WN - 1] = nwin[f];
+      win[f][WN - 1] = cur.win[f];
     }
-    const T cxo = nxo;
-    T ch[NF];
-#pragma unroll
-    for (int f = 0; f < NF; f++) ch[f] = nh[f];
-    T cU[NC], cA[NC], cC[NC];
-#pragma unroll
-    for (int c = 0; c < NC; c++) {
-      cU[c] = nU[c];
-      cA[c] = nA[c];
-      cC[c] = nC[c];
-    }
-    const T cri = nri, cK = nK;
-    prefetch(j + 1);
 
     const int buf = (j - j0) & 1;
     lds[buf][0][R + tid] = win[0][R];
-    if constexpr (AC) lds[buf][1][R + tid] = cxo;
+    if constexpr (AC) lds[buf][1][R + tid] = cur.xo;
     if (halo) {
-      lds[buf][0][hslot] = ch[0];
-      if constexpr (AC) lds[buf][1][hslot] = ch[1];
+      lds[buf][0][hslot] = cur.h[0];
+      if constexpr (AC) lds[buf][1][hslot] = cur.h[1];
     }
     __syncthreads();
 
@@ -418,7 +446,7 @@ __device__ __forceinline__ void interior_tile(const StageArgs<T> &a, int tile)
       const T *l1 = &lds[buf][1][tid];
       const T dxu = dint<T, ORDER>([&](int s) { return l1[s]; }, a.hix);
       const T dyv = dint<T, ORDER>([&](int s) { return win[1][s]; }, a.hiy);
-      T ri = cri, Kb = cK, fu = T(0), fv = T(0);
+      T ri = cur.ri, Kb = cur.K, fu = T(0), fv = T(0);
       if constexpr (MAT == M_REF) {
         const double y = a.ys[j];
         const double xy = xs_i * y;
@@ -427,7 +455,7 @@ __device__ __forceinline__ void interior_tile(const StageArgs<T> &a, int tile)
         fv = T(-(4 * kPi * a.c5t * a.c4y[j] * s3x * (xy + 1)) / (xy + 2));
       }
       acoustic_F<T, MAT>(d0x, d0y, dxu, dyv, ri, Kb, fu, fv, K[0], K[1], K[2]);
-      Uc[0] = cxo;
+      Uc[0] = cur.xo;
       Uc[1] = win[1][R];
       Uc[2] = win[0][R];
     } else {
@@ -437,32 +465,33 @@ __device__ __forceinline__ void interior_tile(const StageArgs<T> &a, int tile)
 
     if (active) {
       const long o = (long)j * pitch + col;
+      constexpr bool NS = POL::NTS;
 #pragma unroll
       for (int c = 0; c < NC; c++) {
-        const long q = o + c * plane;
+        const long qo = o + c * plane;
         if constexpr (MODE == MODE_RHS) {
-          a.yout[q] = K[c];
+          stg<NS>(a.yout + qo, K[c]);
         } else if constexpr (MODE == MODE_S1) {
-          a.yout[q] = Uc[c] + a.cy * K[c];
-          a.aio[q] = KAHAN ? a.ca * K[c] : Uc[c] + a.ca * K[c];
+          stg<NS>(a.yout + qo, Uc[c] + a.cy * K[c]);
+          stg<NS>(a.aio + qo, KAHAN ? a.ca * K[c] : Uc[c] + a.ca * K[c]);
         } else if constexpr (MODE == MODE_S2 || MODE == MODE_S3) {
-          a.yout[q] = cU[c] + a.cy * K[c];
-          a.aio[q] = cA[c] + a.ca * K[c];
+          stg<NS>(a.yout + qo, cur.U[c] + a.cy * K[c]);
+          stg<NS>(a.aio + qo, cur.A[c] + a.ca * K[c]);
         } else if constexpr (!KAHAN) {
-          a.uout[q] = cA[c] + a.ca * K[c];
+          stg<NS>(a.uout + qo, cur.A[c] + a.ca * K[c]);
         } else {
-          const T d = cA[c] + a.ca * K[c];
-          const T y = d - cC[c];
-          const T t = cU[c] + y;
-          a.comp[q] = (t - cU[c]) - y;
-          a.uout[q] = t;
+          const T d = cur.A[c] + a.ca * K[c];
+          const T y = d - cur.C[c];
+          const T t = cur.U[c] + y;
+          stg<NS>(a.comp + qo, (t - cur.U[c]) - y);
+          stg<NS>(a.uout + qo, t);
         }
       }
     }
   }
 }
 
-template <typename T, int ORDER, int PHYS, int MAT, int MODE, bool KAHAN, int BX>
+template <typename T, int ORDER, int PHYS, int MAT, int MODE, bool KAHAN, int BX, class POL = DefaultPolicy>
 __global__ __launch_bounds__(BX) void stage2d_kernel(StageArgs<T> a)
 {
   const int b = blockIdx.x;
@@ -477,7 +506,7 @@ __global__ __launch_bounds__(BX) void stage2d_kernel(StageArgs<T> a)
   const int xcd = bi % kNumXcd, local = bi / kNumXcd;
   const int q = nt / kNumXcd, r = nt % kNumXcd;
   const int tile = xcd * q + min(xcd, r) + local;
-  interior_tile<T, ORDER, PHYS, MAT, MODE, KAHAN, BX>(a, tile);
+  interior_tile<T, ORDER, PHYS, MAT, MODE, KAHAN, BX, POL>(a, tile);
 }
 
 // --------------------------------------------------------------------------

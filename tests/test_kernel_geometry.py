@@ -27,8 +27,8 @@ def default_ty(nx, ny, cls):
     tx = (wi + BX - 1) // BX
     best = 16
     t = 16
-    while t <= 128:
-        if tx * ((hi + t - 1) // t) >= 2048:
+    while t <= 64:
+        if tx * ((hi + t - 1) // t) >= 1024:
             best = t
         t *= 2
     return best
@@ -112,8 +112,10 @@ def check(nx, ny, order, ty=None):
         hc = np.where(hl, i - r, np.minimum(i + r, nx - 1))
         assert ic.min() >= 0 and ic.max() < nx
         assert hc[hl | hr].min() >= 0 and hc[hl | hr].max() < nx
-        # rows touched: prologue j0-R..j0+R-1, prefetch min(jn+R, ny-1) and min(jn, ny-1), jn <= j1
-        rows = list(range(j0 - r, j0 + r)) + [min(jn + r, ny - 1) for jn in range(j0, j1 + 1)]
+        # rows touched: prologue j0-R..j0+R-1, prefetch min(jn+R, ny-1) and min(jn, ny-1)
+        # for jn up to j1 + PF - 1 (PF <= 3 rows in flight)
+        rows = list(range(j0 - r, j0 + r)) + [min(jn + r, ny - 1) for jn in range(j0, j1 + 3)]
+        rows += [min(jn, ny - 1) for jn in range(j0, j1 + 3)]
         assert min(rows) >= 0 and max(rows) < ny
         # LDS neighbours read by active threads map to real columns
         for t in tid[active]:
@@ -149,7 +151,7 @@ def test_geometry_tile_heights(ty):
 def test_geometry_large_grid_parameters():
     g = geometry(4097, 4097, 4)
     assert g["tiles_x"] == 16 and g["n_edge_blocks"] % NXCD == 0
-    assert g["n_tiles"] >= 2048
+    assert g["n_tiles"] >= 1024 and g["ty"] == 64
     # remap is a bijection on the big grid too
     nt = g["n_tiles"]
     assert sorted(xcd_tile(b, nt) for b in range(nt)) == list(range(nt))
