@@ -78,6 +78,37 @@ def measure(spec, v0, dt, steps, warmup):
         bytes_per_dof=bpd, achieved_gbs=achieved, frac=achieved / PEAK_HBM_GBS, energy=energy)
 
 
+def batched(count=256, steps=500, warmup=10, prec=1):
+    """Ensemble of `count` acoustic instances, N ~ U{129..1025}, order 4, FP32,
+    seeded materials and sources; autotuned launch shape; graph-replayed steps."""
+    P = pkg("problems")
+    capi = pkg("capi")
+    t = time.perf_counter()
+    ens = P.batched_ensemble(count, precision=prec)
+    setup_s = time.perf_counter() - t
+    specs = [e[0] for e in ens]
+    dts = [e[2] for e in ens]
+    with capi.Batch(specs) as b:
+        for k, (_, v0, _) in enumerate(ens):
+            b.set_state(k, v0)
+        choice = b.autotune(dts)
+        cfg = b.get_config()
+        b.step(dts, warmup)
+        b.synchronize()
+        t0 = time.perf_counter()
+        ev_ms = b.time_steps(dts, steps)
+        wall = time.perf_counter() - t0
+        e_end = [b.energy(k) for k in range(min(count, 4))]
+    dofs = sum(sp.nx * sp.ny * 3 for sp in specs)
+    bpd = bytes_per_dof(prec, materials=True)
+    achieved = dofs * bpd * steps / (ev_ms * 1e-3) / 1e9
+    return {"instances": count, "dofs": dofs, "steps": steps, "precision": ["fp64", "fp32", "fp32_kahan"][prec],
+            "dof_updates_per_s": dofs * steps / wall, "ms_per_step": wall / steps * 1e3,
+            "event_ms_per_step": ev_ms / steps, "bytes_per_dof": bpd, "achieved_gbs": achieved,
+            "frac": achieved / PEAK_HBM_GBS, "autotune_choice": choice, "config": cfg,
+            "setup_s": setup_s, "energy_head": e_end}
+
+
 def cpu_baseline(spec, v0, dt, budget_s=12.0):
     from oracle import pyoracle
     threads = min(16, os.cpu_count() or 1)
@@ -138,6 +169,9 @@ def main():
                 "bytes_per_dof": m3["bytes_per_dof"], "achieved_gbs": m3["achieved_gbs"],
                 "frac": m3["frac"], "stage_ms": m3["stage_ms"]}
             log(name, secondary["acoustic2d_wall_%dsq_order%d_%s" % (args.n, args.order, name)])
+        bm = batched()
+        secondary["batched_256x_N129-1025_order4_fp32"] = bm
+        log("batched:", bm)
 
     cpu = None
     if not args.no_cpu:

@@ -60,6 +60,20 @@ EXPORTS = {
     "sbp_state_energy": (C.c_int, [C.c_void_p, _DP]),
     "sbp_hnorm": (C.c_int, [C.c_void_p, _DP, _DP]),
     "sbp_error_norms": (C.c_int, [C.c_void_p, _DP, _DP, _DP, _DP]),
+    "sbp_batch_create": (C.c_int, [C.POINTER(SbpProblem), C.c_int, C.POINTER(C.c_void_p)]),
+    "sbp_batch_destroy": (C.c_int, [C.c_void_p]),
+    "sbp_batch_count": (C.c_int, [C.c_void_p]),
+    "sbp_batch_set_state": (C.c_int, [C.c_void_p, C.c_int, _DP]),
+    "sbp_batch_get_state": (C.c_int, [C.c_void_p, C.c_int, _DP]),
+    "sbp_batch_step": (C.c_int, [C.c_void_p, _DP, C.c_long]),
+    "sbp_batch_synchronize": (C.c_int, [C.c_void_p]),
+    "sbp_batch_time_steps": (C.c_int, [C.c_void_p, _DP, C.c_long, _DP]),
+    "sbp_batch_energy": (C.c_int, [C.c_void_p, C.c_int, _DP]),
+    "sbp_batch_num_configs": (C.c_int, []),
+    "sbp_batch_set_config": (C.c_int, [C.c_void_p, C.c_int]),
+    "sbp_batch_get_config": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                       C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "sbp_batch_autotune": (C.c_int, [C.c_void_p, _DP, C.POINTER(C.c_int)]),
     "sbp_last_error": (C.c_char_p, []),
     "sbp_device_arch": (C.c_char_p, []),
     "sbp_version": (C.c_int, []),
@@ -204,6 +218,107 @@ class Context:
         _check(load_library().sbp_error_norms(self._h, _ptr(_vec(v, self.n)), _ptr(_vec(w, self.n)),
                                               C.byref(l2), C.byref(mx)))
         return l2.value, mx.value
+
+
+def _problem_struct(spec, keep):
+    s = SbpProblem()
+    for f, _ in SbpProblem._fields_:
+        if f in ("rho_inv", "bulk"):
+            arr = getattr(spec, f)
+            if arr is None:
+                setattr(s, f, None)
+            else:
+                a = np.ascontiguousarray(arr, dtype=np.float64)
+                keep.append(a)
+                setattr(s, f, _ptr(a))
+        else:
+            setattr(s, f, getattr(spec, f))
+    return s
+
+
+class Batch:
+    """Ensemble of independent 2D acoustic problems advanced together
+    (sbp_batch in C): one persistent kernel per RK4 stage, graph-replayed steps."""
+
+    def __init__(self, specs):
+        lib = load_library()
+        self.specs = list(specs)
+        keep = []
+        arr = (SbpProblem * len(self.specs))()
+        for k, sp in enumerate(self.specs):
+            arr[k] = _problem_struct(sp, keep)
+        h = C.c_void_p()
+        _check(lib.sbp_batch_create(arr, len(self.specs), C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load_library().sbp_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __len__(self):
+        return len(self.specs)
+
+    def set_state(self, k: int, v: np.ndarray):
+        _check(load_library().sbp_batch_set_state(self._h, k, _ptr(_vec(v, self.specs[k].length))))
+
+    def get_state(self, k: int) -> np.ndarray:
+        out = np.empty(self.specs[k].length)
+        _check(load_library().sbp_batch_get_state(self._h, k, _ptr(out)))
+        return out
+
+    def _dt(self, dt):
+        d = np.ascontiguousarray(np.broadcast_to(np.asarray(dt, dtype=np.float64), (len(self.specs),)))
+        return d
+
+    def step(self, dt, nsteps: int):
+        d = self._dt(dt)
+        _check(load_library().sbp_batch_step(self._h, _ptr(d), int(nsteps)))
+
+    def synchronize(self):
+        _check(load_library().sbp_batch_synchronize(self._h))
+
+    def time_steps(self, dt, nsteps: int) -> float:
+        d = self._dt(dt)
+        ms = C.c_double()
+        _check(load_library().sbp_batch_time_steps(self._h, _ptr(d), int(nsteps), C.byref(ms)))
+        return ms.value
+
+    def energy(self, k: int) -> float:
+        e = C.c_double()
+        _check(load_library().sbp_batch_energy(self._h, k, C.byref(e)))
+        return e.value
+
+    @staticmethod
+    def num_configs() -> int:
+        return load_library().sbp_batch_num_configs()
+
+    def set_config(self, cfg: int):
+        _check(load_library().sbp_batch_set_config(self._h, int(cfg)))
+
+    def get_config(self):
+        v = [C.c_int() for _ in range(4)]
+        _check(load_library().sbp_batch_get_config(self._h, *[C.byref(x) for x in v]))
+        return dict(config=v[0].value, tile_width=v[1].value, tile_rows=v[2].value,
+                    blocks_per_cu=v[3].value)
+
+    def autotune(self, dt) -> int:
+        d = self._dt(dt)
+        c = C.c_int()
+        _check(load_library().sbp_batch_autotune(self._h, _ptr(d), C.byref(c)))
+        return c.value
 
 
 def rhs(ctx: Context, t: float, v_src: np.ndarray, v_dst: np.ndarray) -> int:

@@ -1,0 +1,125 @@
+// sbp_batch.hpp -- persistent stage kernel of the batched acoustic ensemble.
+//
+// Many independent 2D acoustic problems of different sizes share one launch
+// per RK4 stage.  The host flattens the work of every instance into one list
+// of (instance, item) pairs -- the boundary-frame blocks and the interior
+// tiles of sbp_kernels.hpp -- and a fixed grid of persistent blocks walks
+// that list.  Each XCD takes a contiguous range of the list (items of one
+// instance stay together in one L2).  The per-instance descriptors live in
+// device memory, so a captured graph stays valid when the time steps change.
+#pragma once
+#include "sbp_kernels.hpp"
+
+namespace sbp {
+
+template <typename T> struct BatchInst {
+  T *U, *A, *Y0, *Y1, *comp;
+  const T *rinv, *bulk;
+  long plane;
+  int pitch, xoff, nx, ny;
+  T hix, hiy;
+  double dt;
+  int bc, tiles_x, ty_rows, n_tiles, n_edge_blocks, n_edge;
+};
+
+template <typename T, int MODE>
+__device__ __forceinline__ StageArgs<T> batch_args(const BatchInst<T> &d)
+{
+  StageArgs<T> a{};
+  a.uin = d.U;
+  a.aio = d.A;
+  a.uout = d.U;
+  a.comp = d.comp;
+  a.rinv = d.rinv;
+  a.bulk = d.bulk;
+  a.plane = d.plane;
+  a.pitch = d.pitch;
+  a.xoff = d.xoff;
+  a.nx = d.nx;
+  a.ny = d.ny;
+  a.hix = d.hix;
+  a.hiy = d.hiy;
+  a.bc = d.bc;
+  a.tiles_x = d.tiles_x;
+  a.ty_rows = d.ty_rows;
+  a.n_tiles = d.n_tiles;
+  a.n_edge_blocks = d.n_edge_blocks;
+  a.n_edge = d.n_edge;
+  const double dt = d.dt;
+  if constexpr (MODE == MODE_S1) {
+    a.yin = d.U;
+    a.yout = d.Y0;
+    a.cy = T(0.5 * dt);
+    a.ca = T(dt / 6);
+  } else if constexpr (MODE == MODE_S2) {
+    a.yin = d.Y0;
+    a.yout = d.Y1;
+    a.cy = T(0.5 * dt);
+    a.ca = T(dt / 3);
+  } else if constexpr (MODE == MODE_S3) {
+    a.yin = d.Y1;
+    a.yout = d.Y0;
+    a.cy = T(dt);
+    a.ca = T(dt / 3);
+  } else {
+    a.yin = d.Y0;
+    a.yout = nullptr;
+    a.ca = T(dt / 6);
+  }
+  return a;
+}
+
+template <typename T, int ORDER, int MAT, int MODE, bool KAHAN, int BX>
+__global__ __launch_bounds__(BX) void batch_stage_kernel(const BatchInst<T> *__restrict__ inst,
+                                                         const int2 *__restrict__ work, int nwork)
+{
+  // gridDim.x is a multiple of kNumXcd: block b (on XCD b % 8) visits list
+  // positions w = b, b + G, ...; position w maps into XCD (w % 8)'s chunk.
+  const int q = nwork / kNumXcd, r = nwork % kNumXcd;
+  for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+    const int xcd = w % kNumXcd, local = w / kNumXcd;
+    const int2 it = work[xcd * q + min(xcd, r) + local];
+    const StageArgs<T> a = batch_args<T, MODE>(inst[it.x]);
+    if (it.y < a.n_edge_blocks)
+      edge_point<T, ORDER, PH_ACOUSTIC, MAT, MODE, KAHAN>(a, it.y * BX + threadIdx.x);
+    else
+      interior_tile<T, ORDER, PH_ACOUSTIC, MAT, MODE, KAHAN, BX, DefaultPolicy>(a, it.y - a.n_edge_blocks);
+    __syncthreads();  // the next item reuses the LDS row buffers
+  }
+}
+
+using BatchLaunchFn = hipError_t (*)(int mode, const void *inst, const int2 *work, int nwork, int grid,
+                                     hipStream_t s);
+
+template <typename T, int ORDER, int MAT, bool KAHAN, int BX> struct BatchLauncher {
+  template <int MODE>
+  static hipError_t go(const void *inst, const int2 *work, int nwork, int grid, hipStream_t s)
+  {
+    hipLaunchKernelGGL((batch_stage_kernel<T, ORDER, MAT, MODE, KAHAN, BX>), dim3(grid), dim3(BX), 0, s,
+                       static_cast<const BatchInst<T> *>(inst), work, nwork);
+    return hipGetLastError();
+  }
+  static hipError_t launch(int mode, const void *inst, const int2 *work, int nwork, int grid, hipStream_t s)
+  {
+    switch (mode) {
+      case MODE_S1: return go<MODE_S1>(inst, work, nwork, grid, s);
+      case MODE_S2: return go<MODE_S2>(inst, work, nwork, grid, s);
+      case MODE_S3: return go<MODE_S3>(inst, work, nwork, grid, s);
+      default: return go<MODE_S4>(inst, work, nwork, grid, s);
+    }
+  }
+};
+
+template <typename T, bool KAHAN, int BX> BatchLaunchFn pick_batch(int order, int mat)
+{
+  const bool arr = mat == M_ARR;
+  if (order == 2) return arr ? &BatchLauncher<T, 2, M_ARR, KAHAN, BX>::launch : &BatchLauncher<T, 2, M_UNIT, KAHAN, BX>::launch;
+  if (order == 4) return arr ? &BatchLauncher<T, 4, M_ARR, KAHAN, BX>::launch : &BatchLauncher<T, 4, M_UNIT, KAHAN, BX>::launch;
+  return arr ? &BatchLauncher<T, 6, M_ARR, KAHAN, BX>::launch : &BatchLauncher<T, 6, M_UNIT, KAHAN, BX>::launch;
+}
+
+// sbp_batch_f64.hip / sbp_batch_f32.hip
+BatchLaunchFn batch_launcher_f64(int order, int mat, int bx);
+BatchLaunchFn batch_launcher_f32(int order, int mat, bool kahan, int bx);
+
+}  // namespace sbp

@@ -15,27 +15,25 @@
 #include <vector>
 
 #include "../../include/sbp_hip.h"
+#include "sbp_host.hpp"
 #include "sbp_kernels.hpp"
 
 using namespace sbp;
 
 namespace {
-
 thread_local std::string g_err;
+}
 
-int fail(int code, const std::string &msg)
+int sbp::host::fail(int code, const std::string &msg)
 {
   g_err = msg;
   return code;
 }
 
-#define HIP_OR_FAIL(expr)                                                                   \
-  do {                                                                                      \
-    hipError_t e_ = (expr);                                                                 \
-    if (e_ != hipSuccess)                                                                   \
-      return fail(e_ == hipErrorOutOfMemory ? SBP_ENOMEM : SBP_EHIP,                        \
-                  std::string(#expr) + ": " + hipGetErrorString(e_));                        \
-  } while (0)
+namespace {
+
+using sbp::host::fail;
+#define HIP_OR_FAIL(expr) SBP_HIP_OR_FAIL(expr)
 
 constexpr int kAlign = 64;   // row alignment in elements
 constexpr int kBX = 256;     // interior tile width (threads per block)

@@ -138,6 +138,34 @@ int  sbp_hnorm(sbp_ctx *ctx, const double *v, double *norm);
 int  sbp_error_norms(const sbp_ctx *ctx, const double *v, const double *w,
                      double *l2, double *maxerr);
 
+/* ---- batched ensemble ---------------------------------------------------
+ * Many independent 2D acoustic problems (uneven sizes, own materials, own dt)
+ * advanced together: one persistent kernel per RK4 stage walks a flattened
+ * list of every instance's tiles, and nsteps are replayed from a captured
+ * hipGraph.  No counterpart in the reference, which runs one problem per MPI
+ * job (code/batch.sh:1-7); each instance's arithmetic is the single-problem
+ * path's (sbp_rk4_advance).  All instances share order, precision and
+ * material model (SBP_MAT_UNIT or SBP_MAT_ARRAYS). */
+typedef struct sbp_batch sbp_batch;
+int  sbp_batch_create(const sbp_problem *descs, int count, sbp_batch **out);
+int  sbp_batch_destroy(sbp_batch *b);
+int  sbp_batch_count(const sbp_batch *b);
+int  sbp_batch_set_state(sbp_batch *b, int k, const double *v);   /* host AoS of instance k */
+int  sbp_batch_get_state(sbp_batch *b, int k, double *v);
+/* nsteps RK4 steps of every instance, dt[k] per instance (asynchronous). */
+int  sbp_batch_step(sbp_batch *b, const double *dt, long nsteps);
+int  sbp_batch_synchronize(sbp_batch *b);
+int  sbp_batch_time_steps(sbp_batch *b, const double *dt, long nsteps, double *ms);
+int  sbp_batch_energy(sbp_batch *b, int k, double *E);
+/* Launch-shape candidates (tile width, tile rows, persistent blocks per CU). */
+int  sbp_batch_num_configs(void);
+int  sbp_batch_set_config(sbp_batch *b, int config);
+int  sbp_batch_get_config(const sbp_batch *b, int *config, int *tile_width, int *tile_rows,
+                          int *blocks_per_cu);
+/* Time every candidate on this ensemble (resident states are saved and
+ * restored) and keep the fastest; *choice receives its index. */
+int  sbp_batch_autotune(sbp_batch *b, const double *dt, int *choice);
+
 /* ---- diagnostics ---------------------------------------------------------- */
 const char *sbp_last_error(void);
 /* Name of the device the library runs on (e.g. "gfx950"), or "" without one. */
