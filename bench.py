@@ -41,10 +41,21 @@ def log(*a):
 
 
 def bytes_per_dof(precision, materials=False, C=3):
-    """Algorithmic HBM bytes per DOF-update of the fused 3-register RK4."""
+    """Algorithmic HBM bytes per DOF-update of the stage kernels as built:
+    9 reads + 4 writes per value and step (U, Y2, Y3, Y4 registers); the Kahan
+    form adds the compensation read and write (15 words)."""
     w = 8 if precision == 0 else 4
-    words = 16 if precision != 2 else 19  # Kahan: 11 reads + 8 writes
+    words = 13 if precision != 2 else 15
     extra = (4 * 2 / C) if materials else 0.0  # rho^-1 and K read in each of 4 stages
+    return (words + extra) * w
+
+
+def bytes_per_dof_3reg(precision, materials=False, C=3):
+    """The 16-word accounting of a fused 3-register RK4 (U, Y, accumulator):
+    9 reads + 7 writes; 19 words with Kahan (11 reads + 8 writes)."""
+    w = 8 if precision == 0 else 4
+    words = 16 if precision != 2 else 19
+    extra = (4 * 2 / C) if materials else 0.0
     return (words + extra) * w
 
 
@@ -72,10 +83,13 @@ def measure(spec, v0, dt, steps, warmup):
     bpd = bytes_per_dof(spec.precision, spec.material == 2, spec.components)
     step_kernel_s = float(np.sum(stage_ms)) * 1e-3
     achieved = dofs * bpd / step_kernel_s / 1e9
+    bpd16 = bytes_per_dof_3reg(spec.precision, spec.material == 2, spec.components)
+    eq16 = dofs * bpd16 / step_kernel_s / 1e9
     return dict(
         wall_s=wall, event_ms=ev_ms, ms_per_step=wall / steps * 1e3,
         dof_updates_per_s=dofs * steps / wall, stage_ms=[float(x) for x in stage_ms],
-        bytes_per_dof=bpd, achieved_gbs=achieved, frac=achieved / PEAK_HBM_GBS, energy=energy)
+        bytes_per_dof=bpd, achieved_gbs=achieved, frac=achieved / PEAK_HBM_GBS, energy=energy,
+        bytes_per_dof_3reg=bpd16, achieved_3reg_equiv_gbs=eq16, frac_3reg_equiv=eq16 / PEAK_HBM_GBS)
 
 
 def batched(count=256, steps=500, warmup=10, prec=1):
@@ -135,6 +149,8 @@ def main():
     ap.add_argument("--order", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-secondary", action="store_true")
+    ap.add_argument("--copy-only", action="store_true",
+                    help="only run the 1 GiB plain copy kernel (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the SBP-SAT path here is built and timed on a single "
@@ -146,6 +162,11 @@ def main():
     capi = pkg("capi")
     capi.load_library()
     log("device:", capi.device_arch())
+    copy_gbs = capi.copy_bandwidth(1 << 30, 5)
+    log("plain copy of 1 GiB: %.0f GB/s" % copy_gbs)
+    if args.copy_only:
+        print(json.dumps({"copy_1GiB_gbs": copy_gbs}))
+        return
 
     spec, v0, dt = P.baseline_acoustic(args.n, args.order)
     log("workload %dx%d order %d FP64: %d warmup + %d timed steps" % (spec.nx, spec.ny, spec.order,
@@ -200,7 +221,10 @@ def main():
                      "unit": "GB/s", "frac": main_m["frac"], "traffic": None,
                      "kernel": "stage2d_kernel x4 (one fused RK4 step)",
                      "bytes_per_dof": main_m["bytes_per_dof"],
-                     "stage_ms": main_m["stage_ms"]},
+                     "bytes_per_dof_3reg_scheme": main_m["bytes_per_dof_3reg"],
+                     "frac_3reg_equiv": main_m["frac_3reg_equiv"],
+                     "stage_ms": main_m["stage_ms"],
+                     "copy_ceiling_gbs": copy_gbs},
         "cpu_baseline": cpu,
         "secondary": secondary,
     }

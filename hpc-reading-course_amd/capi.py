@@ -77,6 +77,7 @@ EXPORTS = {
     "sbp_last_error": (C.c_char_p, []),
     "sbp_device_arch": (C.c_char_p, []),
     "sbp_version": (C.c_int, []),
+    "sbp_copy_bandwidth": (C.c_int, [C.c_long, C.c_int, _DP]),
 }
 
 _lib = None
@@ -271,11 +272,17 @@ class Batch:
     def __len__(self):
         return len(self.specs)
 
+    def _index(self, k):
+        if not 0 <= k < len(self.specs):
+            raise SbpError(-1, "instance index out of range")
+        return k
+
     def set_state(self, k: int, v: np.ndarray):
-        _check(load_library().sbp_batch_set_state(self._h, k, _ptr(_vec(v, self.specs[k].length))))
+        n = self.specs[self._index(k)].length
+        _check(load_library().sbp_batch_set_state(self._h, k, _ptr(_vec(v, n))))
 
     def get_state(self, k: int) -> np.ndarray:
-        out = np.empty(self.specs[k].length)
+        out = np.empty(self.specs[self._index(k)].length)
         _check(load_library().sbp_batch_get_state(self._h, k, _ptr(out)))
         return out
 
@@ -338,6 +345,13 @@ def ts_rk4(ctx: Context, t_end: float, dt: float, v: np.ndarray, t0: float = 0.0
         ctx.step(t0 + n * dt, last, 1)
     v.reshape(-1)[:] = ctx.get_state()
     return t_end if (n or last) else t0
+
+
+def copy_bandwidth(nbytes: int = 1 << 30, reps: int = 5) -> float:
+    """GB/s of a plain device copy of nbytes (read + write counted)."""
+    g = C.c_double()
+    _check(load_library().sbp_copy_bandwidth(int(nbytes), int(reps), C.byref(g)))
+    return g.value
 
 
 def device_arch() -> str:

@@ -27,6 +27,9 @@ __device__ __forceinline__ StageArgs<T> batch_args(const BatchInst<T> &d)
 {
   StageArgs<T> a{};
   a.uin = d.U;
+  a.y2 = d.Y0;
+  a.y3 = d.Y1;
+  a.third = T(1.0 / 3.0);
   a.aio = d.A;
   a.uout = d.U;
   a.comp = d.comp;
@@ -58,11 +61,11 @@ __device__ __forceinline__ StageArgs<T> batch_args(const BatchInst<T> &d)
     a.ca = T(dt / 3);
   } else if constexpr (MODE == MODE_S3) {
     a.yin = d.Y1;
-    a.yout = d.Y0;
+    a.yout = d.A;  // Y4
     a.cy = T(dt);
     a.ca = T(dt / 3);
   } else {
-    a.yin = d.Y0;
+    a.yin = d.A;
     a.yout = nullptr;
     a.ca = T(dt / 6);
   }

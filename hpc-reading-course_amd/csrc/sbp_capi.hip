@@ -122,6 +122,9 @@ StageArgs<T> make_args(const sbp_ctx *c, int mode, const void *yin, double dt, d
   StageArgs<T> a{};
   a.yin = static_cast<const T *>(yin);
   a.uin = static_cast<const T *>(c->U);
+  a.y2 = static_cast<const T *>(c->Y0);
+  a.y3 = static_cast<const T *>(c->Y1);
+  a.third = T(1.0 / 3.0);
   a.aio = static_cast<T *>(c->A);
   a.uout = static_cast<T *>(c->U);
   a.comp = static_cast<T *>(c->KC);
@@ -160,8 +163,26 @@ StageArgs<T> make_args(const sbp_ctx *c, int mode, const void *yin, double dt, d
   return a;
 }
 
-// Launch one stage: mode 1..4 of the RK4 step (yin/yout roles fixed per
-// stage: U -> Y0, Y0 -> Y1, Y1 -> Y0, Y0 -> U) or the RHS mode (yin -> yout).
+// Stage operand and output of stage s (1..4).
+//   2D: U -> Y0 (=Y2), Y0 -> Y1 (=Y3), Y1 -> A (=Y4), A -> U (reads Y0, Y1)
+//   1D: U -> Y0, Y0 -> Y1, Y1 -> Y0, Y0 -> U (stage vectors in A and TK)
+void stage_io(const sbp_ctx *c, int s, const void **yin, void **yout)
+{
+  if (is1d(c->phys)) {
+    const void *in[4] = {c->U, c->Y0, c->Y1, c->Y0};
+    void *out[4] = {c->Y0, c->Y1, c->Y0, nullptr};
+    *yin = in[s - 1];
+    *yout = out[s - 1];
+  } else {
+    const void *in[4] = {c->U, c->Y0, c->Y1, c->A};
+    void *out[4] = {c->Y0, c->Y1, c->A, nullptr};
+    *yin = in[s - 1];
+    *yout = out[s - 1];
+  }
+}
+
+// Launch one stage (mode 1..4 of the RK4 step, roles from stage_io) or the
+// RHS mode (yin -> yout).
 hipError_t launch_stage(sbp_ctx *c, int mode, double dt, double t, const void *yin, void *yout)
 {
   if (c->esz == 8) {
@@ -177,11 +198,15 @@ hipError_t launch_stage(sbp_ctx *c, int mode, double dt, double t, const void *y
 hipError_t launch_step(sbp_ctx *c, double t, double dt)
 {
   const double h = 0.5 * dt;
-  hipError_t e;
-  if ((e = launch_stage(c, MODE_S1, dt, t, c->U, c->Y0)) != hipSuccess) return e;
-  if ((e = launch_stage(c, MODE_S2, dt, t + h, c->Y0, c->Y1)) != hipSuccess) return e;
-  if ((e = launch_stage(c, MODE_S3, dt, t + h, c->Y1, c->Y0)) != hipSuccess) return e;
-  return launch_stage(c, MODE_S4, dt, t + dt, c->Y0, nullptr);
+  const double ts[4] = {t, t + h, t + h, t + dt};  // stage times (RK4_custom)
+  hipError_t e = hipSuccess;
+  for (int s = 1; s <= 4 && e == hipSuccess; s++) {
+    const void *yin;
+    void *yout;
+    stage_io(c, s, &yin, &yout);
+    e = launch_stage(c, s, dt, ts[s - 1], yin, yout);
+  }
+  return e;
 }
 
 bool time_dependent(const sbp_ctx *c) { return c->phys == SBP_ACOUSTIC_2D && c->mat == SBP_MAT_REF_RHO; }
@@ -593,13 +618,14 @@ int sbp_time_stages(sbp_ctx *ctx, double dt, long nsteps, double ms_stage[4])
   if (!ctx || !ms_stage) return fail(SBP_EINVAL, "null argument");
   if (!ctx->have_state) return fail(SBP_ESTATE, "no state loaded");
   double acc[4] = {0, 0, 0, 0};
-  const void *yin[4] = {ctx->U, ctx->Y0, ctx->Y1, ctx->Y0};
-  void *yout[4] = {ctx->Y0, ctx->Y1, ctx->Y0, nullptr};
   const double ts[4] = {0, 0.5 * dt, 0.5 * dt, dt};
   for (long n = 0; n < nsteps; n++) {
     HIP_OR_FAIL(hipEventRecord(ctx->ev[0], ctx->stream));
     for (int s = 0; s < 4; s++) {
-      HIP_OR_FAIL(launch_stage(ctx, MODE_S1 + s, dt, n * dt + ts[s], yin[s], yout[s]));
+      const void *yin;
+      void *yout;
+      stage_io(ctx, s + 1, &yin, &yout);
+      HIP_OR_FAIL(launch_stage(ctx, MODE_S1 + s, dt, n * dt + ts[s], yin, yout));
       HIP_OR_FAIL(hipEventRecord(ctx->ev[s + 1], ctx->stream));
     }
     HIP_OR_FAIL(hipEventSynchronize(ctx->ev[4]));
@@ -669,6 +695,51 @@ const char *sbp_device_arch(void)
   if (hipGetDeviceProperties(&p, dev) != hipSuccess) return name;
   std::snprintf(name, sizeof(name), "%s", p.gcnArchName);
   return name;
+}
+
+// Plain copy kernel (vectorised, one 16-byte element per thread) used to
+// calibrate HBM counters and to report the achievable streaming bandwidth.
+static __global__ __launch_bounds__(256) void copy_probe_kernel(const double2 *__restrict__ src,
+                                                                double2 *__restrict__ dst, long n)
+{
+  const long i = blockIdx.x * 256L + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+
+int sbp_copy_bandwidth(long bytes, int reps, double *gbs)
+{
+  if (bytes < 1024 || reps < 1 || !gbs) return fail(SBP_EINVAL, "bad argument");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(SBP_ENODEV, "no HIP device");
+  const long n = bytes / (long)sizeof(double2);
+  double2 *a = nullptr, *b = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = SBP_OK;
+  float best = 1e30f;
+  if (hipMalloc(&a, n * sizeof(double2)) != hipSuccess || hipMalloc(&b, n * sizeof(double2)) != hipSuccess) {
+    rc = fail(SBP_ENOMEM, "copy probe allocation failed");
+  } else if (hipMemset(a, 0, n * sizeof(double2)) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+             hipEventCreate(&e1) != hipSuccess) {
+    rc = fail(SBP_EHIP, "copy probe setup failed");
+  } else {
+    const int grid = (int)((n + 255) / 256);
+    for (int r = 0; r <= reps && rc == SBP_OK; r++) {
+      (void)hipEventRecord(e0, nullptr);
+      hipLaunchKernelGGL(copy_probe_kernel, dim3(grid), dim3(256), 0, nullptr, a, b, n);
+      (void)hipEventRecord(e1, nullptr);
+      if (hipEventSynchronize(e1) != hipSuccess) rc = fail(SBP_EHIP, "copy probe failed");
+      float ms = 0;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      if (r > 0 && ms < best) best = ms;  // first launch is a warm-up
+    }
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (a) (void)hipFree(a);
+  if (b) (void)hipFree(b);
+  if (rc) return rc;
+  *gbs = 2.0 * n * sizeof(double2) / (best * 1e-3) / 1e9;
+  return SBP_OK;
 }
 
 int sbp_version(void) { return 100; }

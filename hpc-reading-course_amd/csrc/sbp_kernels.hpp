@@ -2,19 +2,23 @@
 //
 // One launch per RK4 stage computes the semi-discrete right-hand side F(t, Y)
 // of code/demo/*/*_rhs.h (region kernels + SAT) and folds the classical RK4
-// stage arithmetic into the same pass (three registers U, A, Y in HBM):
+// stage arithmetic into the same pass.  The 2D kernels keep four registers in
+// HBM (U, Y2, Y3, Y4) and rebuild the RK4 combination in the last stage:
 //
-//   stage 1: K = F(U);  Y' = U + dt/2 K;  A = U + dt/6 K      (reads U)
-//   stage 2: K = F(Y);  Y' = U + dt/2 K;  A += dt/3 K         (reads Y, U, A)
-//   stage 3: K = F(Y);  Y' = U + dt K;    A += dt/3 K         (reads Y, U, A)
-//   stage 4: K = F(Y);  U  = A + dt/6 K                       (reads Y, A)
+//   stage 1: K1 = F(U);   Y2 = U + dt/2 K1                         (reads U)
+//   stage 2: K2 = F(Y2);  Y3 = U + dt/2 K2                         (reads Y2, U)
+//   stage 3: K3 = F(Y3);  Y4 = U + dt K3                           (reads Y3, U)
+//   stage 4: K4 = F(Y4);  U += ((Y2-U) + 2(Y3-U) + (Y4-U))/3 + dt/6 K4
+//                                                                  (reads Y4, U, Y2, Y3)
 //
-// which is RK4_custom's update (code/include/time_stepping/time_stepping.h:127-142)
-// with the b-weighted sum accumulated in A instead of four stored stage vectors:
-// 9 reads + 7 writes per value and step.  Y ping-pongs between two buffers
-// because a stage reads Y with a stencil halo while it writes the next Y.
-// The Kahan form keeps only the increment in A and adds it to U with a
-// compensation register in stage 4.
+// i.e. RK4_custom's update U + dt/6 (K1 + 2(K2+K3) + K4)
+// (code/include/time_stepping/time_stepping.h:127-142) with dt/2 K1 = Y2-U,
+// dt/2 K2 = Y3-U, dt K3 = Y4-U recovered exactly (each difference of two
+// nearby floating-point numbers is exact).  9 reads + 4 writes per value and
+// step instead of the 16 of a 3-register (U, Y, accumulator) scheme.  The
+// Kahan form compensates the final U += increment with one more register.
+// The 1D kernels keep the stage vectors and follow RK4_custom's operation
+// order exactly.
 //
 // Layout in HBM: one plane per component, rows of `pitch` elements, column i
 // stored at xoff + i with xoff chosen so that the first interior column
@@ -50,9 +54,11 @@ constexpr int kNumXcd = 8;
 template <typename T> struct StageArgs {
   const T *yin;   // stage input (stencil operand)
   const T *uin;   // U (stages 2, 3)
-  T *aio;         // A accumulator
+  const T *y2;    // Y2 (stage 4 of the 2D kernels)
+  const T *y3;    // Y3 (stage 4 of the 2D kernels)
+  T *aio;         // S register of the 1D stage sequence
   T *yout;        // next Y (stages 1-3) or F (RHS mode)
-  T *uout;        // U (stage 4)
+  T *uout;        // U (stage 4, read and written)
   T *comp;        // Kahan compensation of U
   T *tk;          // k2 register of the 1D stage sequence
   const T *rinv;  // rho^-1 plane (caller material arrays)
@@ -64,7 +70,8 @@ template <typename T> struct StageArgs {
   long plane;
   int pitch, xoff, nx, ny;
   T hix, hiy, ax, ay;
-  T cy, ca;                  // stage coefficients of Y and A
+  T cy, ca;                  // stage coefficients: Y = U + cy K; dt/6 (stage 4)
+  T third;                   // 1/3 (stage 4)
   int bc;
   int tiles_x, ty_rows, n_tiles, n_edge_blocks, n_edge;
 };
@@ -138,9 +145,16 @@ __device__ __forceinline__ void ref_material(const StageArgs<T> &a, int i, int j
   fv = T(-(4 * kPi * a.c5t * a.c4y[j] * a.s3x[i] * (xy + 1)) / (xy + 2));
 }
 
-// RK4 stage arithmetic at one point (edge path of the 2D kernels).
+// RK4 stage arithmetic at one point (edge path of the 2D kernels).  Ycen are
+// the centre values of the stage operand: U in stage 1, Y4 in stage 4.
+template <typename T, bool KAHAN>
+__device__ __forceinline__ T stage4_increment(const StageArgs<T> &a, T U, T Y2, T Y3, T Y4, T K)
+{
+  return ((Y2 - U) + 2 * (Y3 - U) + (Y4 - U)) * a.third + a.ca * K;
+}
+
 template <typename T, int NC, int MODE, bool KAHAN>
-__device__ __forceinline__ void rk_point(const StageArgs<T> &a, long o, const T *K, const T *Ucen)
+__device__ __forceinline__ void rk_point(const StageArgs<T> &a, long o, const T *K, const T *Ycen)
 {
 #pragma unroll
   for (int c = 0; c < NC; c++) {
@@ -148,21 +162,16 @@ __device__ __forceinline__ void rk_point(const StageArgs<T> &a, long o, const T 
     if constexpr (MODE == MODE_RHS) {
       a.yout[q] = K[c];
     } else if constexpr (MODE == MODE_S1) {
-      const T U = Ucen[c];
-      a.yout[q] = U + a.cy * K[c];
-      a.aio[q] = KAHAN ? a.ca * K[c] : U + a.ca * K[c];
+      a.yout[q] = Ycen[c] + a.cy * K[c];
     } else if constexpr (MODE == MODE_S2 || MODE == MODE_S3) {
-      const T U = a.uin[q], A = a.aio[q];
-      a.yout[q] = U + a.cy * K[c];
-      a.aio[q] = A + a.ca * K[c];
+      a.yout[q] = a.uin[q] + a.cy * K[c];
     } else {
-      const T A = a.aio[q];
+      const T U = a.uout[q];
+      const T inc = stage4_increment<T, KAHAN>(a, U, a.y2[q], a.y3[q], Ycen[c], K[c]);
       if constexpr (!KAHAN) {
-        a.uout[q] = A + a.ca * K[c];
+        a.uout[q] = U + inc;
       } else {
-        const T d = A + a.ca * K[c];
-        const T U = a.uout[q], cc = a.comp[q];
-        const T y = d - cc;
+        const T y = inc - a.comp[q];
         const T t = U + y;
         a.comp[q] = (t - U) - y;
         a.uout[q] = t;
@@ -331,7 +340,7 @@ template <typename T, int NF, int NC> struct RowRegs {
   T win[NF];  // y-differentiated fields at row j+R
   T xo;       // x-only field (u) at row j
   T h[NF];    // halo column of the x-differentiated fields at row j
-  T U[NC], A[NC], C[NC];
+  T U[NC], Y2[NC], Y3[NC], C[NC];
   T ri, K;
 };
 
@@ -346,9 +355,10 @@ __device__ __forceinline__ void interior_tile(const StageArgs<T> &a, int tile)
   constexpr int CS = AC ? 2 : 0;   // field differentiated in x and y (p, or u)
   constexpr int LW = BX + 2 * R;
   constexpr int PF = POL::PF;
-  // pointwise values per row: U (stages 2,3), A (2,3,4), U and comp (Kahan 4), materials
-  constexpr bool LD_U = (MODE == MODE_S2 || MODE == MODE_S3) || (MODE == MODE_S4 && KAHAN);
-  constexpr bool LD_A = MODE >= MODE_S2;
+  // pointwise values per row: U (stages 2-4), Y2 and Y3 (stage 4), the
+  // compensation (Kahan stage 4), materials
+  constexpr bool LD_U = MODE >= MODE_S2;
+  constexpr bool LD_Y = MODE == MODE_S4;
   constexpr bool LD_C = MODE == MODE_S4 && KAHAN;
   constexpr bool LD_M = AC && MAT == M_ARR;
 
@@ -396,7 +406,10 @@ __device__ __forceinline__ void interior_tile(const StageArgs<T> &a, int tile)
 #pragma unroll
     for (int c = 0; c < NC; c++) {
       if constexpr (LD_U) r.U[c] = ldg<POL::NTP>(usrc + c * plane + ox + col);
-      if constexpr (LD_A) r.A[c] = ldg<POL::NTP>(a.aio + c * plane + ox + col);
+      if constexpr (LD_Y) {
+        r.Y2[c] = ldg<POL::NTP>(a.y2 + c * plane + ox + col);
+        r.Y3[c] = ldg<POL::NTP>(a.y3 + c * plane + ox + col);
+      }
       if constexpr (LD_C) r.C[c] = ldg<POL::NTP>(a.comp + c * plane + ox + col);
     }
     if constexpr (LD_M) {
@@ -473,18 +486,18 @@ __device__ __forceinline__ void interior_tile(const StageArgs<T> &a, int tile)
           stg<NS>(a.yout + qo, K[c]);
         } else if constexpr (MODE == MODE_S1) {
           stg<NS>(a.yout + qo, Uc[c] + a.cy * K[c]);
-          stg<NS>(a.aio + qo, KAHAN ? a.ca * K[c] : Uc[c] + a.ca * K[c]);
         } else if constexpr (MODE == MODE_S2 || MODE == MODE_S3) {
           stg<NS>(a.yout + qo, cur.U[c] + a.cy * K[c]);
-          stg<NS>(a.aio + qo, cur.A[c] + a.ca * K[c]);
-        } else if constexpr (!KAHAN) {
-          stg<NS>(a.uout + qo, cur.A[c] + a.ca * K[c]);
         } else {
-          const T d = cur.A[c] + a.ca * K[c];
-          const T y = d - cur.C[c];
-          const T t = cur.U[c] + y;
-          stg<NS>(a.comp + qo, (t - cur.U[c]) - y);
-          stg<NS>(a.uout + qo, t);
+          const T inc = stage4_increment<T, KAHAN>(a, cur.U[c], cur.Y2[c], cur.Y3[c], Uc[c], K[c]);
+          if constexpr (!KAHAN) {
+            stg<NS>(a.uout + qo, cur.U[c] + inc);
+          } else {
+            const T y = inc - cur.C[c];
+            const T t = cur.U[c] + y;
+            stg<NS>(a.comp + qo, (t - cur.U[c]) - y);
+            stg<NS>(a.uout + qo, t);
+          }
         }
       }
     }

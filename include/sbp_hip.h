@@ -171,6 +171,10 @@ const char *sbp_last_error(void);
 /* Name of the device the library runs on (e.g. "gfx950"), or "" without one. */
 const char *sbp_device_arch(void);
 int  sbp_version(void);
+/* Best-of-reps bandwidth (read + write bytes / time) of a plain device copy
+ * of `bytes` bytes: the achievable streaming ceiling, and the known-size
+ * kernel that calibrates HBM traffic counters.  No reference counterpart. */
+int  sbp_copy_bandwidth(long bytes, int reps, double *gbs);
 
 #ifdef __cplusplus
 }
