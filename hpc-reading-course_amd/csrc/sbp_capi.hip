@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "../../include/sbp_hip.h"
+#include "sbp_ctx.hpp"
 #include "sbp_host.hpp"
 #include "sbp_kernels.hpp"
 
@@ -58,29 +59,6 @@ int components(int phys)
 
 }  // namespace
 
-struct sbp_ctx {
-  sbp_problem d{};
-  int C = 0, order = 0, cls = 0, phys = 0, mat = 0, bc = 0, prec = 0;
-  int nx = 0, ny = 0, pitch = 0, xoff = 0;
-  long plane = 0;
-  size_t esz = 8;
-  double hix = 0, hiy = 0;
-  void *U = nullptr, *A = nullptr, *Y0 = nullptr, *Y1 = nullptr, *KC = nullptr, *TK = nullptr;
-  void *rinv = nullptr, *bulk = nullptr;
-  double *xs = nullptr, *ys = nullptr, *c3x = nullptr, *s3x = nullptr, *c4y = nullptr, *s4y = nullptr;
-  double *aos = nullptr;
-  double *partials = nullptr, *scalar = nullptr;
-  int npartials = 0;
-  hipStream_t stream = nullptr;
-  StageLaunchFn launch = nullptr;
-  LaunchGeom geom{};
-  int ty_rows = 32, tiles_x = 0, tiles_y = 0, n_tiles = 0, n_edge = 0, n_edge_blocks = 0;
-  bool have_state = false;
-  hipGraphExec_t gexec = nullptr;
-  double g_dt = 0;
-  hipEvent_t ev[5] = {};
-  int device = 0;
-};
 
 namespace {
 
@@ -304,9 +282,10 @@ int planar_to_aos(sbp_ctx *c, const void *src, const void *comp)
   return SBP_OK;
 }
 
-// H-weighted sum of squares of either the AoS staging vector (aos=true) or
-// the resident state.
-int norm_reduce(sbp_ctx *c, bool aos, bool weighted, double *out)
+// H-weighted sum of squares of either the AoS staging vector (aos=true) or a
+// planar state (u, comp) laid out like the registers.  The result lands in
+// c->scalar on the device; with out != nullptr it is also copied back.
+int norm_reduce_planar(sbp_ctx *c, bool aos, bool weighted, const void *u, const void *comp, double *out)
 {
   const long n = (long)c->nx * c->ny;
   int grid = (int)std::min<long>((n + 255) / 256, c->npartials);
@@ -315,8 +294,8 @@ int norm_reduce(sbp_ctx *c, bool aos, bool weighted, double *out)
     using T = std::remove_pointer_t<decltype(tag)>;
     NormArgs<T> a{};
     a.aos = aos ? c->aos : nullptr;
-    a.u = static_cast<const T *>(c->U);
-    a.comp = static_cast<const T *>(c->KC);
+    a.u = static_cast<const T *>(u);
+    a.comp = static_cast<const T *>(comp);
     a.rinv = static_cast<const T *>(c->rinv);
     a.bulk = static_cast<const T *>(c->bulk);
     a.xs = c->xs;
@@ -340,9 +319,16 @@ int norm_reduce(sbp_ctx *c, bool aos, bool weighted, double *out)
   HIP_OR_FAIL(hipGetLastError());
   hipLaunchKernelGGL(norm_final_kernel, dim3(1), dim3(256), 0, c->stream, c->partials, grid, c->scalar);
   HIP_OR_FAIL(hipGetLastError());
-  HIP_OR_FAIL(hipMemcpyAsync(out, c->scalar, sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  HIP_OR_FAIL(hipStreamSynchronize(c->stream));
+  if (out) {
+    HIP_OR_FAIL(hipMemcpyAsync(out, c->scalar, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_OR_FAIL(hipStreamSynchronize(c->stream));
+  }
   return SBP_OK;
+}
+
+int norm_reduce(sbp_ctx *c, bool aos, bool weighted, double *out)
+{
+  return norm_reduce_planar(c, aos, weighted, c->U, c->KC, out);
 }
 
 int validate(const sbp_problem *d)
@@ -502,6 +488,13 @@ int build_ctx(const sbp_problem *desc, sbp_ctx *c)
 }
 
 }  // namespace
+
+hipError_t sbp::host::launch_step(sbp_ctx *c, double t, double dt) { return ::launch_step(c, t, dt); }
+
+int sbp::host::energy_of(sbp_ctx *c, const void *u, const void *comp, double *out)
+{
+  return norm_reduce_planar(c, false, true, u, comp, out);
+}
 
 extern "C" {
 

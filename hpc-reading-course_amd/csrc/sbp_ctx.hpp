@@ -1,0 +1,43 @@
+// sbp_ctx.hpp -- the single-problem context shared by the host translation
+// units (C-ABI, learned corrector).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "../../include/sbp_hip.h"
+#include "sbp_kernels.hpp"
+
+struct sbp_ctx {
+  sbp_problem d{};
+  int C = 0, order = 0, cls = 0, phys = 0, mat = 0, bc = 0, prec = 0;
+  int nx = 0, ny = 0, pitch = 0, xoff = 0;
+  long plane = 0;
+  size_t esz = 8;
+  double hix = 0, hiy = 0;
+  void *U = nullptr, *A = nullptr, *Y0 = nullptr, *Y1 = nullptr, *KC = nullptr, *TK = nullptr;
+  void *rinv = nullptr, *bulk = nullptr;
+  double *xs = nullptr, *ys = nullptr, *c3x = nullptr, *s3x = nullptr, *c4y = nullptr, *s4y = nullptr;
+  double *aos = nullptr;
+  double *partials = nullptr, *scalar = nullptr;
+  int npartials = 0;
+  hipStream_t stream = nullptr;
+  sbp::StageLaunchFn launch = nullptr;
+  sbp::LaunchGeom geom{};
+  int ty_rows = 32, tiles_x = 0, tiles_y = 0, n_tiles = 0, n_edge = 0, n_edge_blocks = 0;
+  bool have_state = false;
+  hipGraphExec_t gexec = nullptr;
+  double g_dt = 0;
+  hipEvent_t ev[5] = {};
+  int device = 0;
+};
+
+
+namespace sbp {
+namespace host {
+// Launch one RK4 stage / one whole step of a context (sbp_capi.hip).
+hipError_t launch_step(sbp_ctx *c, double t, double dt);
+// H-weighted energy of a planar state laid out like the context's registers
+// (U and optional compensation); result left in c->scalar (device) and, when
+// out != nullptr, copied to the host.
+int energy_of(sbp_ctx *c, const void *u, const void *comp, double *out);
+}  // namespace host
+}  // namespace sbp
