@@ -25,6 +25,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libsbp_hip.so")
 
 _DP = C.POINTER(C.c_double)
+_FP = C.POINTER(C.c_float)
 
 
 class SbpProblem(C.Structure):
@@ -74,6 +75,21 @@ EXPORTS = {
     "sbp_batch_get_config": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                        C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "sbp_batch_autotune": (C.c_int, [C.c_void_p, _DP, C.POINTER(C.c_int)]),
+    "sbp_get_state_strided": (C.c_int, [C.c_void_p, C.c_int, _DP]),
+    "sbp_batch_get_state_strided": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _DP]),
+    "sbp_corr_param_count": (C.c_int, []),
+    "sbp_corr_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
+    "sbp_corr_destroy": (C.c_int, [C.c_void_p]),
+    "sbp_corr_set_params": (C.c_int, [C.c_void_p, _FP, C.c_int]),
+    "sbp_corr_get_params": (C.c_int, [C.c_void_p, _FP, C.c_int]),
+    "sbp_corr_set_normalization": (C.c_int, [C.c_void_p, _FP, _FP, _FP]),
+    "sbp_corr_forward": (C.c_int, [C.c_void_p, _FP, C.c_int, _FP]),
+    "sbp_corr_gradients": (C.c_int, [C.c_void_p, _FP, _FP, C.c_int, _FP, _DP]),
+    "sbp_corr_load_dataset": (C.c_int, [C.c_void_p, _FP, _FP, C.c_int]),
+    "sbp_corr_train_step": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.c_int, C.c_float, _DP]),
+    "sbp_corr_synchronize": (C.c_int, [C.c_void_p]),
+    "sbp_corr_advance": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_long, C.c_double,
+                                   C.POINTER(C.c_int)]),
     "sbp_last_error": (C.c_char_p, []),
     "sbp_device_arch": (C.c_char_p, []),
     "sbp_version": (C.c_int, []),
@@ -195,6 +211,14 @@ class Context:
         _check(load_library().sbp_time_stages(self._h, dt, int(nsteps), _ptr(out)))
         return out
 
+    def get_state_strided(self, stride: int) -> np.ndarray:
+        """State at every stride-th point (injection onto the nested grid)."""
+        cx = (self.spec.nx - 1) // stride + 1
+        cy = (self.spec.ny - 1) // stride + 1
+        out = np.empty(cx * cy * self.spec.components)
+        _check(load_library().sbp_get_state_strided(self._h, int(stride), _ptr(out)))
+        return out
+
     def set_tile_rows(self, rows: int):
         _check(load_library().sbp_set_tile_rows(self._h, int(rows)))
 
@@ -286,6 +310,14 @@ class Batch:
         _check(load_library().sbp_batch_get_state(self._h, k, _ptr(out)))
         return out
 
+    def get_state_strided(self, k: int, stride: int) -> np.ndarray:
+        sp = self.specs[self._index(k)]
+        cx = (sp.nx - 1) // stride + 1
+        cy = (sp.ny - 1) // stride + 1
+        out = np.empty(cx * cy * 3)
+        _check(load_library().sbp_batch_get_state_strided(self._h, k, int(stride), _ptr(out)))
+        return out
+
     def _dt(self, dt):
         d = np.ascontiguousarray(np.broadcast_to(np.asarray(dt, dtype=np.float64), (len(self.specs),)))
         return d
@@ -326,6 +358,100 @@ class Batch:
         c = C.c_int()
         _check(load_library().sbp_batch_autotune(self._h, _ptr(d), C.byref(c)))
         return c.value
+
+
+def _fptr(a: np.ndarray):
+    return a.ctypes.data_as(_FP)
+
+
+def _f32(a, shape=None):
+    x = np.ascontiguousarray(a, dtype=np.float32)
+    if shape is not None and x.shape != tuple(shape):
+        raise ValueError("array has shape %s, expected %s" % (x.shape, tuple(shape)))
+    return x
+
+
+class Corrector:
+    """Learned coarse-grid corrector (sbp_corrector in C): a 3-layer 3x3
+    conv net on bf16 MFMA; raw inputs (u, v, p, rho, K), output the
+    correction of (u, v, p) in units of out_scale."""
+
+    def __init__(self, nx: int, ny: int, batch: int = 1):
+        lib = load_library()
+        self.nx, self.ny, self.batch = nx, ny, batch
+        h = C.c_void_p()
+        _check(lib.sbp_corr_create(nx, ny, batch, C.byref(h)))
+        self._h = h
+        self.n_params = lib.sbp_corr_param_count()
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load_library().sbp_corr_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def params(self) -> np.ndarray:
+        out = np.empty(self.n_params, dtype=np.float32)
+        _check(load_library().sbp_corr_get_params(self._h, _fptr(out), self.n_params))
+        return out
+
+    @params.setter
+    def params(self, p):
+        x = _f32(p, (self.n_params,))
+        _check(load_library().sbp_corr_set_params(self._h, _fptr(x), self.n_params))
+
+    def set_normalization(self, in_shift, in_scale, out_scale):
+        a, b, c = _f32(in_shift, (5,)), _f32(in_scale, (5,)), _f32(out_scale, (3,))
+        _check(load_library().sbp_corr_set_normalization(self._h, _fptr(a), _fptr(b), _fptr(c)))
+
+    def forward(self, raw) -> np.ndarray:
+        raw = _f32(raw)
+        nb = raw.shape[0]
+        _f32(raw, (nb, self.ny, self.nx, 5))
+        out = np.empty((nb, self.ny, self.nx, 3), dtype=np.float32)
+        _check(load_library().sbp_corr_forward(self._h, _fptr(raw), nb, _fptr(out)))
+        return out
+
+    def gradients(self, raw, target):
+        raw = _f32(raw)
+        nb = raw.shape[0]
+        _f32(raw, (nb, self.ny, self.nx, 5))
+        tgt = _f32(target, (nb, self.ny, self.nx, 3))
+        g = np.empty(self.n_params, dtype=np.float32)
+        loss = C.c_double()
+        _check(load_library().sbp_corr_gradients(self._h, _fptr(raw), _fptr(tgt), nb, _fptr(g), C.byref(loss)))
+        return loss.value, g
+
+    def load_dataset(self, raw, target):
+        raw = _f32(raw)
+        n = raw.shape[0]
+        _f32(raw, (n, self.ny, self.nx, 5))
+        tgt = _f32(target, (n, self.ny, self.nx, 3))
+        _check(load_library().sbp_corr_load_dataset(self._h, _fptr(raw), _fptr(tgt), n))
+
+    def train_step(self, idx, lr: float) -> float:
+        ix = np.ascontiguousarray(idx, dtype=np.int32)
+        loss = C.c_double()
+        _check(load_library().sbp_corr_train_step(self._h, ix.ctypes.data_as(C.POINTER(C.c_int)), len(ix),
+                                                   float(lr), C.byref(loss)))
+        return loss.value
+
+    def advance(self, ctx: "Context", t0: float, dt: float, nsteps: int, bound: float) -> int:
+        rej = C.c_int()
+        _check(load_library().sbp_corr_advance(self._h, ctx._h, t0, dt, int(nsteps), float(bound), C.byref(rej)))
+        return rej.value
 
 
 def rhs(ctx: Context, t: float, v_src: np.ndarray, v_dst: np.ndarray) -> int:

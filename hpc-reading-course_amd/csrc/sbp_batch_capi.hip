@@ -461,6 +461,31 @@ int sbp_batch_get_state(sbp_batch *b, int k, double *v)
   return SBP_OK;
 }
 
+int sbp_batch_get_state_strided(sbp_batch *b, int k, int stride, double *v)
+{
+  int rc = check_index(b, k);
+  if (rc) return rc;
+  if (!v || stride < 1) return fail(SBP_EINVAL, "bad argument");
+  if (!b->have_state[k]) return fail(SBP_ESTATE, "no state loaded for this instance");
+  const InstHost &h = b->inst[k];
+  const int cx = (h.nx - 1) / stride + 1, cy = (h.ny - 1) / stride + 1;
+  const long n = (long)cx * cy;
+  const int grid = (int)((n + 255) / 256);
+  if (b->esz == 8)
+    hipLaunchKernelGGL(planar_to_aos_strided_kernel<double>, dim3(grid), dim3(256), 0, b->stream,
+                       static_cast<const double *>(b->U) + 3 * h.off, (const double *)nullptr, b->aos, h.nx, h.ny, 3,
+                       h.pitch, h.xoff, h.plane, stride);
+  else
+    hipLaunchKernelGGL(planar_to_aos_strided_kernel<float>, dim3(grid), dim3(256), 0, b->stream,
+                       static_cast<const float *>(b->U) + 3 * h.off,
+                       b->KC ? static_cast<const float *>(b->KC) + 3 * h.off : nullptr, b->aos, h.nx, h.ny, 3, h.pitch,
+                       h.xoff, h.plane, stride);
+  HIP_OR_FAIL(hipGetLastError());
+  HIP_OR_FAIL(hipMemcpyAsync(v, b->aos, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, b->stream));
+  HIP_OR_FAIL(hipStreamSynchronize(b->stream));
+  return SBP_OK;
+}
+
 int sbp_batch_step(sbp_batch *b, const double *dt, long nsteps)
 {
   if (!b || !dt || nsteps < 0) return fail(SBP_EINVAL, "bad argument");

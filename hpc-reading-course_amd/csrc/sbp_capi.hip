@@ -566,6 +566,27 @@ int sbp_get_state(sbp_ctx *ctx, double *v)
   return download_aos(ctx, v);
 }
 
+int sbp_get_state_strided(sbp_ctx *ctx, int stride, double *v)
+{
+  if (!ctx || !v || stride < 1) return fail(SBP_EINVAL, "bad argument");
+  if (!ctx->have_state) return fail(SBP_ESTATE, "no state loaded");
+  const int cx = (ctx->nx - 1) / stride + 1, cy = (ctx->ny - 1) / stride + 1;
+  const long n = (long)cx * cy;
+  const int grid = (int)((n + 255) / 256);
+  if (ctx->esz == 8)
+    hipLaunchKernelGGL(planar_to_aos_strided_kernel<double>, dim3(grid), dim3(256), 0, ctx->stream,
+                       (const double *)ctx->U, (const double *)ctx->KC, ctx->aos, ctx->nx, ctx->ny, ctx->C, ctx->pitch,
+                       ctx->xoff, ctx->plane, stride);
+  else
+    hipLaunchKernelGGL(planar_to_aos_strided_kernel<float>, dim3(grid), dim3(256), 0, ctx->stream,
+                       (const float *)ctx->U, (const float *)ctx->KC, ctx->aos, ctx->nx, ctx->ny, ctx->C, ctx->pitch,
+                       ctx->xoff, ctx->plane, stride);
+  HIP_OR_FAIL(hipGetLastError());
+  HIP_OR_FAIL(hipMemcpyAsync(v, ctx->aos, sizeof(double) * ctx->C * n, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_OR_FAIL(hipStreamSynchronize(ctx->stream));
+  return SBP_OK;
+}
+
 int sbp_step(sbp_ctx *ctx, double t0, double dt, long nsteps)
 {
   if (!ctx || nsteps < 0) return fail(SBP_EINVAL, "bad argument");

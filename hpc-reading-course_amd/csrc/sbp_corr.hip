@@ -1,0 +1,808 @@
+// sbp_corr.hip -- learned coarse-grid corrector: a 3-layer 3x3 convolutional
+// network (5 -> 32 -> 32 -> 3 channels, ReLU) applied to the state after each
+// coarse RK4 step.  Forward, backward and inference run on bf16 MFMA
+// (v_mfma_f32_16x16x16_bf16) with FP32 accumulation; master weights and Adam
+// moments are FP32.  No counterpart in the reference: the plain fine-grid
+// solve of the same path is its fallback and judge, and a correction that
+// increases the discrete energy by more than a set bound is rejected.
+//
+// Activations are channels-last [image][y][x][channel] in bf16, channels
+// padded to a multiple of 4 (inputs: 5 -> 8).  A convolution is a GEMM with
+// M = 16 pixels of one image row, N = output channels, K = 9 taps x input
+// channels (padded to a multiple of 16); one wavefront owns a 16-pixel
+// segment.  Backward-data is the same kernel with the weights transposed and
+// the taps mirrored; weight gradients are a GEMM reduced over pixels, written
+// as per-wavefront partials and summed in a fixed order (deterministic).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/sbp_hip.h"
+#include "sbp_ctx.hpp"
+#include "sbp_host.hpp"
+
+using namespace sbp;
+using sbp::host::fail;
+#define HIP_OR_FAIL(expr) SBP_HIP_OR_FAIL(expr)
+
+namespace {
+
+using bf16x4 = __attribute__((ext_vector_type(4))) short;
+using f32x4 = __attribute__((ext_vector_type(4))) float;
+typedef unsigned short bf16;
+
+// Network shape.
+constexpr int C_IN = 5, C_INP = 8, C_H = 32, C_OUT = 3, C_OUTP = 16;
+constexpr int KP1 = 80;   // 9*8 = 72 padded to 80
+constexpr int KP2 = 288;  // 9*32
+constexpr int KP3 = 288;
+constexpr int KP3T = 144;  // backward-data of layer 3: 9*16
+constexpr int N_W1 = C_H * C_IN * 9, N_B1 = C_H;
+constexpr int N_W2 = C_H * C_H * 9, N_B2 = C_H;
+constexpr int N_W3 = C_OUT * C_H * 9, N_B3 = C_OUT;
+constexpr int OFF_W1 = 0, OFF_B1 = OFF_W1 + N_W1, OFF_W2 = OFF_B1 + N_B1, OFF_B2 = OFF_W2 + N_W2;
+constexpr int OFF_W3 = OFF_B2 + N_B2, OFF_B3 = OFF_W3 + N_W3, N_PARAMS = OFF_B3 + N_B3;
+constexpr int WG_WAVES = 1024;  // wavefronts of the weight-gradient reduction
+
+__device__ __forceinline__ bf16 f2bf(float f)
+{
+  unsigned u = __float_as_uint(f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (bf16)((u >> 16) | 0x40u);  // quiet NaN
+  u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
+  return (bf16)(u >> 16);
+}
+
+__device__ __forceinline__ float bf2f(bf16 h) { return __uint_as_float(((unsigned)h) << 16); }
+
+// --------------------------------------------------------------------------
+// conv3x3 as MFMA GEMM.  MODE 0: + bias, ReLU, bf16 out (COUT_P channels);
+// MODE 1: + bias, fp32 out (first out_ch channels); MODE 2: fp32 out, no bias.
+// wt is [COUT_P][KP] bf16 with k = tap * CIN_P + ci, tap = 3*(dy+1) + (dx+1).
+// --------------------------------------------------------------------------
+template <int CIN_P, int COUT_P, int KP, int MODE>
+__global__ __launch_bounds__(256) void conv3x3_kernel(const bf16 *__restrict__ in, int nimg, int H, int W,
+                                                      const bf16 *__restrict__ wt, const float *__restrict__ bias,
+                                                      bf16 *__restrict__ out_bf, float *__restrict__ out_f, int out_ch)
+{
+  static_assert(CIN_P % 4 == 0 && COUT_P % 16 == 0 && KP % 16 == 0, "MFMA tiling");
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int segs = (W + 15) / 16;
+  const long seg = (long)blockIdx.x * 4 + wv;
+  if (seg >= (long)nimg * H * segs) return;  // whole wavefront; no barriers below
+  const int sx = (int)(seg % segs);
+  const long row = seg / segs;
+  const int y = (int)(row % H), img = (int)(row / H);
+  const int x0 = sx * 16;
+  const int m = lane & 15, kg = lane >> 4;
+  constexpr int NT = COUT_P / 16;
+  f32x4 acc[NT];
+#pragma unroll
+  for (int t = 0; t < NT; t++) acc[t] = f32x4{0, 0, 0, 0};
+  const long img_base = (long)img * H * W;
+#pragma unroll 2
+  for (int ks = 0; ks < KP / 16; ks++) {
+    const int k0 = ks * 16 + kg * 4;
+    const int tap = k0 / CIN_P, ci = k0 - tap * CIN_P;
+    bf16x4 a = {0, 0, 0, 0};
+    if (tap < 9) {
+      const int yy = y + tap / 3 - 1, xx = x0 + m + tap % 3 - 1;
+      if (yy >= 0 && yy < H && xx >= 0 && xx < W)
+        a = *reinterpret_cast<const bf16x4 *>(in + (img_base + (long)yy * W + xx) * CIN_P + ci);
+    }
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+      const bf16x4 b = *reinterpret_cast<const bf16x4 *>(wt + (long)(t * 16 + m) * KP + k0);
+      acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
+    }
+  }
+  // acc[t][i] = C[row 4*kg+i][col m]: pixel x0 + 4*kg + i, output channel 16*t + m
+#pragma unroll
+  for (int t = 0; t < NT; t++) {
+    const int co = t * 16 + m;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int px = x0 + 4 * kg + i;
+      if (px >= W) continue;
+      float v = acc[t][i];
+      if constexpr (MODE != 2) v += bias[co];
+      const long p = img_base + (long)y * W + px;
+      if constexpr (MODE == 0) {
+        out_bf[p * COUT_P + co] = f2bf(v > 0.f ? v : 0.f);
+      } else {
+        if (co < out_ch) out_f[p * out_ch + co] = v;
+      }
+    }
+  }
+}
+
+// Weight gradient dW[co][k] = sum_p dY[p][co] * patch(X)[p][k]: GEMM with
+// M = output channels, N = K-index, reduced over pixels (16 per MFMA step).
+template <int CIN_P, int COUT_P, int KP>
+__global__ __launch_bounds__(256) void wgrad_kernel(const bf16 *__restrict__ X, const bf16 *__restrict__ dY, int nimg,
+                                                    int H, int W, float *__restrict__ partial)
+{
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const long wid = (long)blockIdx.x * 4 + wv, nw = (long)gridDim.x * 4;
+  constexpr int MT = COUT_P / 16, NT = KP / 16;
+  const int r = lane & 15, kg = lane >> 4;
+  f32x4 acc[MT][NT];
+#pragma unroll
+  for (int a = 0; a < MT; a++)
+#pragma unroll
+    for (int b = 0; b < NT; b++) acc[a][b] = f32x4{0, 0, 0, 0};
+  const int segs = (W + 15) / 16;
+  const long nseg = (long)nimg * H * segs;
+  for (long seg = wid; seg < nseg; seg += nw) {
+    const int sx = (int)(seg % segs);
+    const long row = seg / segs;
+    const int y = (int)(row % H), img = (int)(row / H);
+    const int x0 = sx * 16;
+    const long img_base = (long)img * H * W;
+    bf16x4 a[MT];
+#pragma unroll
+    for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int px = x0 + 4 * kg + i;
+        a[mt][i] = px < W ? (short)dY[(img_base + (long)y * W + px) * COUT_P + mt * 16 + r] : (short)0;
+      }
+#pragma unroll
+    for (int nt = 0; nt < NT; nt++) {
+      const int kidx = nt * 16 + r;
+      const int tap = kidx / CIN_P, ci = kidx - tap * CIN_P;
+      bf16x4 b = {0, 0, 0, 0};
+      if (tap < 9) {
+        const int yy = y + tap / 3 - 1;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          const int px = x0 + 4 * kg + i, xx = px + tap % 3 - 1;
+          if (px < W && yy >= 0 && yy < H && xx >= 0 && xx < W)
+            b[i] = (short)X[(img_base + (long)yy * W + xx) * CIN_P + ci];
+        }
+      }
+#pragma unroll
+      for (int mt = 0; mt < MT; mt++) acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[mt], b, acc[mt][nt], 0, 0, 0);
+    }
+  }
+  float *P = partial + wid * COUT_P * KP;
+#pragma unroll
+  for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+    for (int nt = 0; nt < NT; nt++)
+#pragma unroll
+      for (int i = 0; i < 4; i++) P[(mt * 16 + 4 * kg + i) * KP + nt * 16 + r] = acc[mt][nt][i];
+}
+
+// Sum the per-wavefront partials in a fixed order and scatter the real
+// entries into the flat gradient [co][ci][ky][kx].
+__global__ __launch_bounds__(256) void wgrad_reduce_kernel(const float *__restrict__ partial, int nw, int cout,
+                                                           int cin, int cin_p, int cout_p, int kp,
+                                                           float *__restrict__ grad)
+{
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  if (e >= cout * cin * 9) return;
+  const int co = e / (cin * 9), rem = e % (cin * 9), ci = rem / 9, tap = rem % 9;
+  const long idx = (long)co * kp + tap * cin_p + ci;
+  float s = 0.f;
+  for (int w = 0; w < nw; w++) s += partial[(long)w * cout_p * kp + idx];
+  grad[e] = s;
+}
+
+// Bias gradient db[co] = sum_p dY[p][co] (first cout of cout_p channels).
+__global__ __launch_bounds__(256) void bgrad_kernel(const bf16 *__restrict__ dY, long npix, int cout, int cout_p,
+                                                    float *__restrict__ grad)
+{
+  __shared__ float sh[256];
+  const int co = blockIdx.x;
+  float s = 0.f;
+  for (long p = threadIdx.x; p < npix; p += 256) s += bf2f(dY[p * cout_p + co]);
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) grad[co] = sh[0];
+  (void)cout;
+}
+
+// dPre = dA * (A > 0) in bf16 (A holds post-ReLU activations).
+__global__ __launch_bounds__(256) void relu_back_kernel(const float *__restrict__ dA, const bf16 *__restrict__ A,
+                                                        long n, bf16 *__restrict__ dpre)
+{
+  const long i = blockIdx.x * 256L + threadIdx.x;
+  if (i < n) dpre[i] = f2bf(bf2f(A[i]) > 0.f ? dA[i] : 0.f);
+}
+
+// MSE loss over the C_OUT real channels and its gradient 2(y - t)/N in bf16
+// (padded channels zero).  Per-block partial sums of squares in FP64.
+__global__ __launch_bounds__(256) void loss_kernel(const float *__restrict__ y, const float *__restrict__ t, long npix,
+                                                   double inv_n, bf16 *__restrict__ dY, double *__restrict__ partial)
+{
+  __shared__ double sh[256];
+  double s = 0;
+  for (long p = blockIdx.x * 256L + threadIdx.x; p < npix; p += (long)gridDim.x * 256) {
+#pragma unroll
+    for (int c = 0; c < C_OUTP; c++) {
+      float g = 0.f;
+      if (c < C_OUT) {
+        const float d = y[p * C_OUT + c] - t[p * C_OUT + c];
+        s += (double)d * d;
+        g = (float)(2.0 * inv_n) * d;
+      }
+      dY[p * C_OUTP + c] = f2bf(g);
+    }
+  }
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+
+__global__ void sum_kernel(const double *__restrict__ partial, int n, double scale, double *__restrict__ out)
+{
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    double s = 0;
+    for (int k = 0; k < n; k++) s += partial[k];
+    out[0] = s * scale;
+  }
+}
+
+// Adam on the FP32 master parameters.
+__global__ __launch_bounds__(256) void adam_kernel(float *__restrict__ w, float *__restrict__ m, float *__restrict__ v,
+                                                   const float *__restrict__ g, int n, float lr, float b1, float b2,
+                                                   float eps, float c1, float c2)
+{
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.f - b1) * gi;
+  const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// Master parameters -> padded bf16 weights (forward and mirrored-transposed
+// backward-data copies) and FP32 padded biases.
+__global__ __launch_bounds__(256) void prepare_weights_kernel(const float *__restrict__ p, bf16 *__restrict__ w1,
+                                                              bf16 *__restrict__ w2, bf16 *__restrict__ w3,
+                                                              bf16 *__restrict__ w2t, bf16 *__restrict__ w3t,
+                                                              float *__restrict__ b1, float *__restrict__ b2,
+                                                              float *__restrict__ b3)
+{
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  // forward layer 1: [32][80]
+  if (i < C_H * KP1) {
+    const int co = i / KP1, k = i % KP1, tap = k / C_INP, ci = k % C_INP;
+    w1[i] = (tap < 9 && ci < C_IN) ? f2bf(p[OFF_W1 + (co * C_IN + ci) * 9 + tap]) : (bf16)0;
+  }
+  if (i < C_H * KP2) {
+    const int co = i / KP2, k = i % KP2, tap = k / C_H, ci = k % C_H;
+    w2[i] = f2bf(p[OFF_W2 + (co * C_H + ci) * 9 + tap]);
+    // backward-data of layer 2: W2T[ci][tap'*32 + co] = W2[co][8-tap'][ci]
+    const int ci2 = i / KP2, k2 = i % KP2, tp = k2 / C_H, co2 = k2 % C_H;
+    w2t[i] = f2bf(p[OFF_W2 + (co2 * C_H + ci2) * 9 + (8 - tp)]);
+  }
+  if (i < C_OUTP * KP3) {
+    const int co = i / KP3, k = i % KP3, tap = k / C_H, ci = k % C_H;
+    w3[i] = co < C_OUT ? f2bf(p[OFF_W3 + (co * C_H + ci) * 9 + tap]) : (bf16)0;
+  }
+  if (i < C_H * KP3T) {
+    // backward-data of layer 3: W3T[ci][tap'*16 + co] = W3[co][8-tap'][ci]
+    const int ci = i / KP3T, k = i % KP3T, tp = k / C_OUTP, co = k % C_OUTP;
+    w3t[i] = co < C_OUT ? f2bf(p[OFF_W3 + (co * C_H + ci) * 9 + (8 - tp)]) : (bf16)0;
+  }
+  if (i < C_H) {
+    b1[i] = p[OFF_B1 + i];
+    b2[i] = p[OFF_B2 + i];
+  }
+  if (i < C_OUTP) b3[i] = i < C_OUT ? p[OFF_B3 + i] : 0.f;
+}
+
+// Raw per-point quantities (u, v, p, rho, K) -> normalised bf16 features.
+__global__ __launch_bounds__(256) void pack_raw_kernel(const float *__restrict__ raw, long npix,
+                                                       const float *__restrict__ shift_scale, bf16 *__restrict__ X)
+{
+  const long p = blockIdx.x * 256L + threadIdx.x;
+  if (p >= npix) return;
+#pragma unroll
+  for (int c = 0; c < C_INP; c++)
+    X[p * C_INP + c] = c < C_IN ? f2bf((raw[p * C_IN + c] - shift_scale[c]) * shift_scale[C_IN + c]) : (bf16)0;
+}
+
+// Gather dataset samples (raw inputs and targets) into a contiguous batch.
+__global__ __launch_bounds__(256) void gather_kernel(const float *__restrict__ raw_all, const float *__restrict__ tgt_all,
+                                                     const int *__restrict__ idx, int nb, long npix_img,
+                                                     const float *__restrict__ shift_scale, const float *__restrict__ out_scale,
+                                                     bf16 *__restrict__ X, float *__restrict__ T)
+{
+  const long p = blockIdx.x * 256L + threadIdx.x;
+  if (p >= (long)nb * npix_img) return;
+  const int b = (int)(p / npix_img);
+  const long q = (long)idx[b] * npix_img + (p - (long)b * npix_img);
+#pragma unroll
+  for (int c = 0; c < C_INP; c++)
+    X[p * C_INP + c] = c < C_IN ? f2bf((raw_all[q * C_IN + c] - shift_scale[c]) * shift_scale[C_IN + c]) : (bf16)0;
+#pragma unroll
+  for (int c = 0; c < C_OUT; c++) T[p * C_OUT + c] = tgt_all[q * C_OUT + c] / out_scale[c];
+}
+
+// Features of a solver context's resident state.
+template <typename T>
+__global__ __launch_bounds__(256) void pack_state_kernel(const T *__restrict__ U, const T *__restrict__ rinv,
+                                                         const T *__restrict__ bulk, int nx, int ny, int pitch, int xoff,
+                                                         long plane, const float *__restrict__ shift_scale,
+                                                         bf16 *__restrict__ X)
+{
+  const long p = blockIdx.x * 256L + threadIdx.x;
+  if (p >= (long)nx * ny) return;
+  const int j = (int)(p / nx), i = (int)(p - (long)j * nx);
+  const long o = (long)j * pitch + xoff + i;
+  float raw[C_IN];
+  raw[0] = (float)U[o];
+  raw[1] = (float)U[plane + o];
+  raw[2] = (float)U[2 * plane + o];
+  raw[3] = rinv ? (float)(1.0 / (double)rinv[o]) : 1.f;
+  raw[4] = bulk ? (float)bulk[o] : 1.f;
+#pragma unroll
+  for (int c = 0; c < C_INP; c++)
+    X[p * C_INP + c] = c < C_IN ? f2bf((raw[c] - shift_scale[c]) * shift_scale[C_IN + c]) : (bf16)0;
+}
+
+// Corrected candidate: Y = U + out_scale * net (into the scratch register).
+template <typename T>
+__global__ __launch_bounds__(256) void candidate_kernel(const T *__restrict__ U, const float *__restrict__ net, int nx,
+                                                        int ny, int pitch, int xoff, long plane,
+                                                        const float *__restrict__ out_scale, T *__restrict__ Y)
+{
+  const long p = blockIdx.x * 256L + threadIdx.x;
+  if (p >= (long)nx * ny) return;
+  const int j = (int)(p / nx), i = (int)(p - (long)j * nx);
+  const long o = (long)j * pitch + xoff + i;
+#pragma unroll
+  for (int c = 0; c < 3; c++) Y[c * plane + o] = U[c * plane + o] + (T)(net[p * C_OUT + c] * out_scale[c]);
+}
+
+// Accept the candidate when E_new <= E_old * (1 + bound); count rejections.
+template <typename T>
+__global__ __launch_bounds__(256) void accept_kernel(const T *__restrict__ Y, T *__restrict__ U, long n,
+                                                     const double *__restrict__ e_old, const double *__restrict__ e_new,
+                                                     double bound, int *__restrict__ rejected)
+{
+  const bool ok = e_new[0] <= e_old[0] * (1.0 + bound) && e_new[0] == e_new[0];
+  const long i = blockIdx.x * 256L + threadIdx.x;
+  if (i == 0 && !ok) rejected[0] += 1;
+  if (ok && i < n) U[i] = Y[i];
+}
+
+__global__ void copy_scalar_kernel(const double *__restrict__ s, double *__restrict__ d) { d[0] = s[0]; }
+
+__global__ void clear_int_kernel(int *__restrict__ p) { p[0] = 0; }
+
+int grid_for(long n) { return (int)((n + 255) / 256); }
+
+}  // namespace
+
+struct sbp_corrector {
+  int H = 0, W = 0, nb = 0;  // image shape and batch capacity
+  hipStream_t stream = nullptr;
+  float *params = nullptr, *grads = nullptr, *m = nullptr, *v = nullptr;
+  bf16 *w1 = nullptr, *w2 = nullptr, *w3 = nullptr, *w2t = nullptr, *w3t = nullptr;
+  float *b1 = nullptr, *b2 = nullptr, *b3 = nullptr;
+  float *norm = nullptr;  // [0..4] shift, [5..9] scale, [10..12] output scale
+  bf16 *X0 = nullptr, *A1 = nullptr, *A2 = nullptr, *dY3 = nullptr, *dP2 = nullptr, *dP1 = nullptr;
+  float *Yo = nullptr, *Tg = nullptr, *dA = nullptr, *wpart = nullptr;
+  double *lpart = nullptr, *scalars = nullptr;  // scalars: [0] loss, [1] E_old
+  int *rejected = nullptr, *idx = nullptr;
+  float *ds_raw = nullptr, *ds_tgt = nullptr;
+  int ds_n = 0;
+  long adam_t = 0;
+};
+
+namespace {
+
+void release(sbp_corrector *c)
+{
+  if (!c) return;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  void *ptrs[] = {c->params, c->grads, c->m, c->v, c->w1, c->w2, c->w3, c->w2t, c->w3t, c->b1, c->b2, c->b3,
+                  c->norm, c->X0, c->A1, c->A2, c->dY3, c->dP2, c->dP1, c->Yo, c->Tg, c->dA, c->wpart,
+                  c->lpart, c->scalars, c->rejected, c->idx, c->ds_raw, c->ds_tgt};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+template <typename P> int dmalloc(P **p, size_t bytes)
+{
+  void *q = nullptr;
+  if (hipMalloc(&q, std::max<size_t>(bytes, 16)) != hipSuccess) return fail(SBP_ENOMEM, "corrector allocation failed");
+  *p = static_cast<P *>(q);
+  return SBP_OK;
+}
+
+int prepare(sbp_corrector *c)
+{
+  hipLaunchKernelGGL(prepare_weights_kernel, dim3(grid_for(C_H * KP2)), dim3(256), 0, c->stream, c->params, c->w1,
+                     c->w2, c->w3, c->w2t, c->w3t, c->b1, c->b2, c->b3);
+  HIP_OR_FAIL(hipGetLastError());
+  return SBP_OK;
+}
+
+int conv_grid(int nb, int H, int W) { return (int)(((long)nb * H * ((W + 15) / 16) + 3) / 4); }
+
+// Forward of nb images already packed in X0: A1, A2 (bf16) and Yo (fp32, 3 ch).
+int forward(sbp_corrector *c, int nb)
+{
+  const int g = conv_grid(nb, c->H, c->W);
+  hipLaunchKernelGGL((conv3x3_kernel<C_INP, C_H, KP1, 0>), dim3(g), dim3(256), 0, c->stream, c->X0, nb, c->H, c->W,
+                     c->w1, c->b1, c->A1, (float *)nullptr, C_H);
+  hipLaunchKernelGGL((conv3x3_kernel<C_H, C_H, KP2, 0>), dim3(g), dim3(256), 0, c->stream, c->A1, nb, c->H, c->W,
+                     c->w2, c->b2, c->A2, (float *)nullptr, C_H);
+  hipLaunchKernelGGL((conv3x3_kernel<C_H, C_OUTP, KP3, 1>), dim3(g), dim3(256), 0, c->stream, c->A2, nb, c->H, c->W,
+                     c->w3, c->b3, (bf16 *)nullptr, c->Yo, C_OUT);
+  HIP_OR_FAIL(hipGetLastError());
+  return SBP_OK;
+}
+
+// Loss against targets Tg and gradients of every parameter into c->grads.
+int backward(sbp_corrector *c, int nb)
+{
+  const long npix = (long)nb * c->H * c->W;
+  const int g = conv_grid(nb, c->H, c->W);
+  const int lblocks = (int)std::min<long>(grid_for(npix), 1024);
+  hipLaunchKernelGGL(loss_kernel, dim3(lblocks), dim3(256), 0, c->stream, c->Yo, c->Tg, npix, 1.0 / (double)(npix * C_OUT),
+                     c->dY3, c->lpart);
+  hipLaunchKernelGGL(sum_kernel, dim3(1), dim3(64), 0, c->stream, c->lpart, lblocks, 1.0 / (double)(npix * C_OUT),
+                     c->scalars);
+  // layer 3
+  hipLaunchKernelGGL((wgrad_kernel<C_H, C_OUTP, KP3>), dim3(WG_WAVES / 4), dim3(256), 0, c->stream, c->A2, c->dY3, nb,
+                     c->H, c->W, c->wpart);
+  hipLaunchKernelGGL(wgrad_reduce_kernel, dim3(grid_for(N_W3)), dim3(256), 0, c->stream, c->wpart, WG_WAVES, C_OUT,
+                     C_H, C_H, C_OUTP, KP3, c->grads + OFF_W3);
+  hipLaunchKernelGGL(bgrad_kernel, dim3(C_OUT), dim3(256), 0, c->stream, c->dY3, npix, C_OUT, C_OUTP,
+                     c->grads + OFF_B3);
+  hipLaunchKernelGGL((conv3x3_kernel<C_OUTP, C_H, KP3T, 2>), dim3(g), dim3(256), 0, c->stream, c->dY3, nb, c->H, c->W,
+                     c->w3t, (const float *)nullptr, (bf16 *)nullptr, c->dA, C_H);
+  hipLaunchKernelGGL(relu_back_kernel, dim3(grid_for(npix * C_H)), dim3(256), 0, c->stream, c->dA, c->A2, npix * C_H,
+                     c->dP2);
+  // layer 2
+  hipLaunchKernelGGL((wgrad_kernel<C_H, C_H, KP2>), dim3(WG_WAVES / 4), dim3(256), 0, c->stream, c->A1, c->dP2, nb,
+                     c->H, c->W, c->wpart);
+  hipLaunchKernelGGL(wgrad_reduce_kernel, dim3(grid_for(N_W2)), dim3(256), 0, c->stream, c->wpart, WG_WAVES, C_H, C_H,
+                     C_H, C_H, KP2, c->grads + OFF_W2);
+  hipLaunchKernelGGL(bgrad_kernel, dim3(C_H), dim3(256), 0, c->stream, c->dP2, npix, C_H, C_H, c->grads + OFF_B2);
+  hipLaunchKernelGGL((conv3x3_kernel<C_H, C_H, KP2, 2>), dim3(g), dim3(256), 0, c->stream, c->dP2, nb, c->H, c->W,
+                     c->w2t, (const float *)nullptr, (bf16 *)nullptr, c->dA, C_H);
+  hipLaunchKernelGGL(relu_back_kernel, dim3(grid_for(npix * C_H)), dim3(256), 0, c->stream, c->dA, c->A1, npix * C_H,
+                     c->dP1);
+  // layer 1
+  hipLaunchKernelGGL((wgrad_kernel<C_INP, C_H, KP1>), dim3(WG_WAVES / 4), dim3(256), 0, c->stream, c->X0, c->dP1, nb,
+                     c->H, c->W, c->wpart);
+  hipLaunchKernelGGL(wgrad_reduce_kernel, dim3(grid_for(N_W1)), dim3(256), 0, c->stream, c->wpart, WG_WAVES, C_H,
+                     C_IN, C_INP, C_H, KP1, c->grads + OFF_W1);
+  hipLaunchKernelGGL(bgrad_kernel, dim3(C_H), dim3(256), 0, c->stream, c->dP1, npix, C_H, C_H, c->grads + OFF_B1);
+  HIP_OR_FAIL(hipGetLastError());
+  return SBP_OK;
+}
+
+int check_nb(const sbp_corrector *c, int nb)
+{
+  if (!c) return fail(SBP_EINVAL, "null corrector");
+  if (nb < 1 || nb > c->nb) return fail(SBP_EINVAL, "batch size outside 1..capacity");
+  return SBP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sbp_corr_param_count(void) { return N_PARAMS; }
+
+int sbp_corr_create(int nx, int ny, int batch, sbp_corrector **out)
+{
+  if (!out) return fail(SBP_EINVAL, "null output pointer");
+  *out = nullptr;
+  if (nx < 3 || ny < 3 || batch < 1 || (long)nx * ny * batch > (1L << 28))
+    return fail(SBP_EINVAL, "corrector image or batch size out of range");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(SBP_ENODEV, "no HIP device");
+  sbp_corrector *c = new (std::nothrow) sbp_corrector();
+  if (!c) return fail(SBP_ENOMEM, "host allocation failed");
+  c->H = ny;
+  c->W = nx;
+  c->nb = batch;
+  const long npix = (long)nx * ny * batch;
+  int rc = SBP_OK;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) rc = fail(SBP_EHIP, "stream creation failed");
+  if (!rc) rc = dmalloc(&c->params, sizeof(float) * N_PARAMS);
+  if (!rc) rc = dmalloc(&c->grads, sizeof(float) * N_PARAMS);
+  if (!rc) rc = dmalloc(&c->m, sizeof(float) * N_PARAMS);
+  if (!rc) rc = dmalloc(&c->v, sizeof(float) * N_PARAMS);
+  if (!rc) rc = dmalloc(&c->w1, sizeof(bf16) * C_H * KP1);
+  if (!rc) rc = dmalloc(&c->w2, sizeof(bf16) * C_H * KP2);
+  if (!rc) rc = dmalloc(&c->w3, sizeof(bf16) * C_OUTP * KP3);
+  if (!rc) rc = dmalloc(&c->w2t, sizeof(bf16) * C_H * KP2);
+  if (!rc) rc = dmalloc(&c->w3t, sizeof(bf16) * C_H * KP3T);
+  if (!rc) rc = dmalloc(&c->b1, sizeof(float) * C_H);
+  if (!rc) rc = dmalloc(&c->b2, sizeof(float) * C_H);
+  if (!rc) rc = dmalloc(&c->b3, sizeof(float) * C_OUTP);
+  if (!rc) rc = dmalloc(&c->norm, sizeof(float) * 16);
+  if (!rc) rc = dmalloc(&c->X0, sizeof(bf16) * npix * C_INP);
+  if (!rc) rc = dmalloc(&c->A1, sizeof(bf16) * npix * C_H);
+  if (!rc) rc = dmalloc(&c->A2, sizeof(bf16) * npix * C_H);
+  if (!rc) rc = dmalloc(&c->dY3, sizeof(bf16) * npix * C_OUTP);
+  if (!rc) rc = dmalloc(&c->dP2, sizeof(bf16) * npix * C_H);
+  if (!rc) rc = dmalloc(&c->dP1, sizeof(bf16) * npix * C_H);
+  if (!rc) rc = dmalloc(&c->Yo, sizeof(float) * npix * C_OUT);
+  if (!rc) rc = dmalloc(&c->Tg, sizeof(float) * npix * C_OUT);
+  if (!rc) rc = dmalloc(&c->dA, sizeof(float) * npix * C_H);
+  if (!rc) rc = dmalloc(&c->wpart, sizeof(float) * WG_WAVES * C_H * KP2);
+  if (!rc) rc = dmalloc(&c->lpart, sizeof(double) * 1024);
+  if (!rc) rc = dmalloc(&c->scalars, sizeof(double) * 4);
+  if (!rc) rc = dmalloc(&c->rejected, sizeof(int));
+  if (!rc) rc = dmalloc(&c->idx, sizeof(int) * batch);
+  if (!rc) {
+    // Default: zero network (identity correction), unit normalisation.
+    std::vector<float> zero(N_PARAMS, 0.f);
+    std::vector<float> nrm(16, 0.f);
+    for (int k = 0; k < C_IN; k++) nrm[C_IN + k] = 1.f;
+    for (int k = 0; k < C_OUT; k++) nrm[10 + k] = 1.f;
+    if (hipMemcpy(c->params, zero.data(), sizeof(float) * N_PARAMS, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->m, zero.data(), sizeof(float) * N_PARAMS, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->v, zero.data(), sizeof(float) * N_PARAMS, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->norm, nrm.data(), sizeof(float) * 16, hipMemcpyHostToDevice) != hipSuccess)
+      rc = fail(SBP_EHIP, "corrector initialisation failed");
+  }
+  if (!rc) rc = prepare(c);
+  if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(SBP_EHIP, "corrector initialisation failed");
+  if (rc) {
+    std::string msg = sbp_last_error();
+    release(c);
+    fail(rc, msg);
+    return rc;
+  }
+  *out = c;
+  return SBP_OK;
+}
+
+int sbp_corr_destroy(sbp_corrector *c)
+{
+  release(c);
+  return SBP_OK;
+}
+
+int sbp_corr_set_params(sbp_corrector *c, const float *params, int n)
+{
+  if (!c || !params || n != N_PARAMS) return fail(SBP_EINVAL, "parameter vector has the wrong length");
+  HIP_OR_FAIL(hipMemcpyAsync(c->params, params, sizeof(float) * N_PARAMS, hipMemcpyHostToDevice, c->stream));
+  std::vector<float> zero(N_PARAMS, 0.f);
+  HIP_OR_FAIL(hipMemcpyAsync(c->m, zero.data(), sizeof(float) * N_PARAMS, hipMemcpyHostToDevice, c->stream));
+  HIP_OR_FAIL(hipMemcpyAsync(c->v, zero.data(), sizeof(float) * N_PARAMS, hipMemcpyHostToDevice, c->stream));
+  c->adam_t = 0;
+  int rc = prepare(c);
+  if (rc) return rc;
+  HIP_OR_FAIL(hipStreamSynchronize(c->stream));
+  return SBP_OK;
+}
+
+int sbp_corr_get_params(sbp_corrector *c, float *params, int n)
+{
+  if (!c || !params || n != N_PARAMS) return fail(SBP_EINVAL, "parameter vector has the wrong length");
+  HIP_OR_FAIL(hipMemcpyAsync(params, c->params, sizeof(float) * N_PARAMS, hipMemcpyDeviceToHost, c->stream));
+  HIP_OR_FAIL(hipStreamSynchronize(c->stream));
+  return SBP_OK;
+}
+
+int sbp_corr_set_normalization(sbp_corrector *c, const float *in_shift, const float *in_scale, const float *out_scale)
+{
+  if (!c || !in_shift || !in_scale || !out_scale) return fail(SBP_EINVAL, "null argument");
+  float nrm[16] = {0};
+  for (int k = 0; k < C_IN; k++) {
+    nrm[k] = in_shift[k];
+    nrm[C_IN + k] = in_scale[k];
+  }
+  for (int k = 0; k < C_OUT; k++) nrm[10 + k] = out_scale[k];
+  HIP_OR_FAIL(hipMemcpyAsync(c->norm, nrm, sizeof(nrm), hipMemcpyHostToDevice, c->stream));
+  HIP_OR_FAIL(hipStreamSynchronize(c->stream));
+  return SBP_OK;
+}
+
+// Network output for nb images of raw inputs [nb][ny][nx][5] (u, v, p, rho, K):
+// out [nb][ny][nx][3] in normalised units (multiply by out_scale for the correction).
+int sbp_corr_forward(sbp_corrector *c, const float *raw, int nb, float *out)
+{
+  int rc = check_nb(c, nb);
+  if (rc) return rc;
+  if (!raw || !out) return fail(SBP_EINVAL, "null argument");
+  const long npix = (long)nb * c->H * c->W;
+  float *draw = nullptr;
+  if ((rc = dmalloc(&draw, sizeof(float) * npix * C_IN))) return rc;
+  rc = SBP_OK;
+  if (hipMemcpyAsync(draw, raw, sizeof(float) * npix * C_IN, hipMemcpyHostToDevice, c->stream) != hipSuccess)
+    rc = fail(SBP_EHIP, "upload failed");
+  if (!rc) {
+    hipLaunchKernelGGL(pack_raw_kernel, dim3(grid_for(npix)), dim3(256), 0, c->stream, draw, npix, c->norm, c->X0);
+    rc = forward(c, nb);
+  }
+  if (!rc && (hipMemcpyAsync(out, c->Yo, sizeof(float) * npix * C_OUT, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+              hipStreamSynchronize(c->stream) != hipSuccess))
+    rc = fail(SBP_EHIP, "forward failed");
+  (void)hipFree(draw);
+  return rc;
+}
+
+// Loss and parameter gradients for nb images (raw inputs, targets in
+// physical units [nb][ny][nx][3]); grads has sbp_corr_param_count() entries.
+int sbp_corr_gradients(sbp_corrector *c, const float *raw, const float *target, int nb, float *grads, double *loss)
+{
+  int rc = check_nb(c, nb);
+  if (rc) return rc;
+  if (!raw || !target || !grads) return fail(SBP_EINVAL, "null argument");
+  const long npix = (long)nb * c->H * c->W;
+  float *draw = nullptr, *dtgt = nullptr;
+  int *didx = nullptr;
+  if ((rc = dmalloc(&draw, sizeof(float) * npix * C_IN))) return rc;
+  if ((rc = dmalloc(&dtgt, sizeof(float) * npix * C_OUT))) {
+    (void)hipFree(draw);
+    return rc;
+  }
+  std::vector<int> idx(nb);
+  for (int b = 0; b < nb; b++) idx[b] = b;
+  rc = SBP_OK;
+  if (hipMemcpyAsync(draw, raw, sizeof(float) * npix * C_IN, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+      hipMemcpyAsync(dtgt, target, sizeof(float) * npix * C_OUT, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+      hipMemcpyAsync(c->idx, idx.data(), sizeof(int) * nb, hipMemcpyHostToDevice, c->stream) != hipSuccess)
+    rc = fail(SBP_EHIP, "upload failed");
+  (void)didx;
+  if (!rc) {
+    hipLaunchKernelGGL(gather_kernel, dim3(grid_for(npix)), dim3(256), 0, c->stream, draw, dtgt, c->idx, nb,
+                       (long)c->H * c->W, c->norm, c->norm + 10, c->X0, c->Tg);
+    rc = forward(c, nb);
+  }
+  if (!rc) rc = backward(c, nb);
+  double l = 0;
+  if (!rc && (hipMemcpyAsync(grads, c->grads, sizeof(float) * N_PARAMS, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+              hipMemcpyAsync(&l, c->scalars, sizeof(double), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+              hipStreamSynchronize(c->stream) != hipSuccess))
+    rc = fail(SBP_EHIP, "gradient download failed");
+  if (!rc && loss) *loss = l;
+  (void)hipFree(draw);
+  (void)hipFree(dtgt);
+  return rc;
+}
+
+// Resident training set: raw inputs [n][ny][nx][5] and targets [n][ny][nx][3].
+int sbp_corr_load_dataset(sbp_corrector *c, const float *raw, const float *target, int n)
+{
+  if (!c || !raw || !target || n < 1) return fail(SBP_EINVAL, "bad dataset");
+  HIP_OR_FAIL(hipStreamSynchronize(c->stream));
+  if (c->ds_raw) (void)hipFree(c->ds_raw);
+  if (c->ds_tgt) (void)hipFree(c->ds_tgt);
+  c->ds_raw = nullptr;
+  c->ds_tgt = nullptr;
+  c->ds_n = 0;
+  const long npix = (long)n * c->H * c->W;
+  int rc;
+  if ((rc = dmalloc(&c->ds_raw, sizeof(float) * npix * C_IN)) || (rc = dmalloc(&c->ds_tgt, sizeof(float) * npix * C_OUT)))
+    return rc;
+  HIP_OR_FAIL(hipMemcpy(c->ds_raw, raw, sizeof(float) * npix * C_IN, hipMemcpyHostToDevice));
+  HIP_OR_FAIL(hipMemcpy(c->ds_tgt, target, sizeof(float) * npix * C_OUT, hipMemcpyHostToDevice));
+  c->ds_n = n;
+  return SBP_OK;
+}
+
+// One Adam step on the mini-batch idx[0..nb) of the resident dataset.
+int sbp_corr_train_step(sbp_corrector *c, const int *idx, int nb, float lr, double *loss)
+{
+  int rc = check_nb(c, nb);
+  if (rc) return rc;
+  if (!idx || c->ds_n == 0) return fail(SBP_ESTATE, "no dataset loaded");
+  for (int b = 0; b < nb; b++)
+    if (idx[b] < 0 || idx[b] >= c->ds_n) return fail(SBP_EINVAL, "sample index out of range");
+  const long npix = (long)nb * c->H * c->W;
+  HIP_OR_FAIL(hipMemcpyAsync(c->idx, idx, sizeof(int) * nb, hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(gather_kernel, dim3(grid_for(npix)), dim3(256), 0, c->stream, c->ds_raw, c->ds_tgt, c->idx, nb,
+                     (long)c->H * c->W, c->norm, c->norm + 10, c->X0, c->Tg);
+  if ((rc = forward(c, nb)) || (rc = backward(c, nb))) return rc;
+  c->adam_t++;
+  const float b1 = 0.9f, b2 = 0.999f;
+  const float c1 = 1.f - std::pow(b1, (float)c->adam_t), c2 = 1.f - std::pow(b2, (float)c->adam_t);
+  hipLaunchKernelGGL(adam_kernel, dim3(grid_for(N_PARAMS)), dim3(256), 0, c->stream, c->params, c->m, c->v, c->grads,
+                     N_PARAMS, lr, b1, b2, 1e-8f, c1, c2);
+  HIP_OR_FAIL(hipGetLastError());
+  if ((rc = prepare(c))) return rc;
+  if (loss) {
+    HIP_OR_FAIL(hipMemcpyAsync(loss, c->scalars, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_OR_FAIL(hipStreamSynchronize(c->stream));
+  }
+  return SBP_OK;
+}
+
+int sbp_corr_synchronize(sbp_corrector *c)
+{
+  if (!c) return fail(SBP_EINVAL, "null corrector");
+  HIP_OR_FAIL(hipStreamSynchronize(c->stream));
+  return SBP_OK;
+}
+
+// Correct the resident state of a 2D acoustic context in place:
+// U <- U + out_scale * net(U, rho, K), unless that raises the discrete energy
+// above E_old * (1 + bound), in which case the plain state is kept.
+static int apply_one(sbp_corrector *c, sbp_ctx *x, double bound)
+{
+  const long npix = (long)x->nx * x->ny;
+  const float *ss = c->norm;
+  if (x->esz == 8)
+    hipLaunchKernelGGL(pack_state_kernel<double>, dim3(grid_for(npix)), dim3(256), 0, x->stream,
+                       (const double *)x->U, (const double *)x->rinv, (const double *)x->bulk, x->nx, x->ny, x->pitch,
+                       x->xoff, x->plane, ss, c->X0);
+  else
+    hipLaunchKernelGGL(pack_state_kernel<float>, dim3(grid_for(npix)), dim3(256), 0, x->stream, (const float *)x->U,
+                       (const float *)x->rinv, (const float *)x->bulk, x->nx, x->ny, x->pitch, x->xoff, x->plane, ss,
+                       c->X0);
+  HIP_OR_FAIL(hipGetLastError());
+  // the network runs on the context's stream so steps and corrections stay ordered
+  hipStream_t keep = c->stream;
+  c->stream = x->stream;
+  int rc = forward(c, 1);
+  c->stream = keep;
+  if (rc) return rc;
+  // E_old -> scalars[1]; candidate into the Y0 scratch register; E_new in x->scalar
+  if ((rc = sbp::host::energy_of(x, x->U, x->KC, nullptr))) return rc;
+  hipLaunchKernelGGL(copy_scalar_kernel, dim3(1), dim3(1), 0, x->stream, x->scalar, c->scalars + 1);
+  if (x->esz == 8)
+    hipLaunchKernelGGL(candidate_kernel<double>, dim3(grid_for(npix)), dim3(256), 0, x->stream, (const double *)x->U,
+                       c->Yo, x->nx, x->ny, x->pitch, x->xoff, x->plane, c->norm + 10, (double *)x->Y0);
+  else
+    hipLaunchKernelGGL(candidate_kernel<float>, dim3(grid_for(npix)), dim3(256), 0, x->stream, (const float *)x->U,
+                       c->Yo, x->nx, x->ny, x->pitch, x->xoff, x->plane, c->norm + 10, (float *)x->Y0);
+  HIP_OR_FAIL(hipGetLastError());
+  if ((rc = sbp::host::energy_of(x, x->Y0, x->KC, nullptr))) return rc;
+  const long n = 3 * x->plane;
+  if (x->esz == 8)
+    hipLaunchKernelGGL(accept_kernel<double>, dim3(grid_for(n)), dim3(256), 0, x->stream, (const double *)x->Y0,
+                       (double *)x->U, n, c->scalars + 1, x->scalar, bound, c->rejected);
+  else
+    hipLaunchKernelGGL(accept_kernel<float>, dim3(grid_for(n)), dim3(256), 0, x->stream, (const float *)x->Y0,
+                       (float *)x->U, n, c->scalars + 1, x->scalar, bound, c->rejected);
+  HIP_OR_FAIL(hipGetLastError());
+  return SBP_OK;
+}
+
+// nsteps coarse RK4 steps of ctx, each followed by the guarded correction.
+// *rejected receives the number of corrections refused by the energy bound.
+int sbp_corr_advance(sbp_corrector *c, sbp_ctx *x, double t0, double dt, long nsteps, double bound, int *rejected)
+{
+  if (!c || !x) return fail(SBP_EINVAL, "null argument");
+  if (x->phys != SBP_ACOUSTIC_2D) return fail(SBP_EINVAL, "the corrector acts on 2D acoustic contexts");
+  if (x->prec == SBP_FP32_KAHAN) return fail(SBP_EINVAL, "the corrector acts on FP64 or FP32 contexts");
+  if (x->nx != c->W || x->ny != c->H) return fail(SBP_EINVAL, "context grid differs from the corrector's");
+  if (!x->have_state) return fail(SBP_ESTATE, "no state loaded");
+  if (!(bound >= 0)) return fail(SBP_EINVAL, "energy bound must be >= 0");
+  hipLaunchKernelGGL(clear_int_kernel, dim3(1), dim3(1), 0, x->stream, c->rejected);
+  double t = t0;
+  for (long s = 0; s < nsteps; s++) {
+    HIP_OR_FAIL(sbp::host::launch_step(x, t, dt));
+    int rc = apply_one(c, x, bound);
+    if (rc) return rc;
+    t = t + dt;
+  }
+  int rej = 0;
+  HIP_OR_FAIL(hipMemcpyAsync(&rej, c->rejected, sizeof(int), hipMemcpyDeviceToHost, x->stream));
+  HIP_OR_FAIL(hipStreamSynchronize(x->stream));
+  if (rejected) *rejected = rej;
+  return SBP_OK;
+}
+
+}  // extern "C"

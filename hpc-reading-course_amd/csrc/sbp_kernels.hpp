@@ -681,6 +681,24 @@ __global__ __launch_bounds__(256) void planar_to_aos_kernel(const T *__restrict_
   }
 }
 
+// Every stride-th point in both directions (injection onto a nested grid).
+template <typename T>
+__global__ __launch_bounds__(256) void planar_to_aos_strided_kernel(const T *__restrict__ U, const T *__restrict__ comp,
+                                                                    double *__restrict__ aos, int nx, int ny, int nc,
+                                                                    int pitch, int xoff, long plane, int stride)
+{
+  const int cx = (nx - 1) / stride + 1, cy = (ny - 1) / stride + 1;
+  const long k = blockIdx.x * 256L + threadIdx.x;
+  if (k >= (long)cx * cy) return;
+  const int cj = (int)(k / cx), ci = (int)(k - (long)cj * cx);
+  const long o = (long)cj * stride * pitch + xoff + (long)ci * stride;
+  for (int c = 0; c < nc; c++) {
+    double v = (double)U[c * plane + o];
+    if (comp) v -= (double)comp[c * plane + o];
+    aos[k * nc + c] = v;
+  }
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void matrix_to_plane_kernel(const double *__restrict__ m, T *__restrict__ P,
                                                               int nx, int ny, int pitch, int xoff)

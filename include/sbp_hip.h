@@ -114,6 +114,9 @@ int  sbp_rk4_advance(sbp_ctx *ctx, double *v, double t0, double dt, long nsteps)
 /* ---- device-resident state (the time loop stays on the GPU) -------------- */
 int  sbp_set_state(sbp_ctx *ctx, const double *v);  /* host AoS -> device state */
 int  sbp_get_state(sbp_ctx *ctx, double *v);        /* device state -> host AoS */
+/* Every stride-th point in x and y (injection onto the nested grid of
+ * ((nx-1)/stride+1) x ((ny-1)/stride+1) points), host AoS. */
+int  sbp_get_state_strided(sbp_ctx *ctx, int stride, double *v);
 /* Advance the resident state nsteps RK4 steps from time t0 (asynchronous). */
 int  sbp_step(sbp_ctx *ctx, double t0, double dt, long nsteps);
 int  sbp_synchronize(sbp_ctx *ctx);
@@ -152,6 +155,7 @@ int  sbp_batch_destroy(sbp_batch *b);
 int  sbp_batch_count(const sbp_batch *b);
 int  sbp_batch_set_state(sbp_batch *b, int k, const double *v);   /* host AoS of instance k */
 int  sbp_batch_get_state(sbp_batch *b, int k, double *v);
+int  sbp_batch_get_state_strided(sbp_batch *b, int k, int stride, double *v);
 /* nsteps RK4 steps of every instance, dt[k] per instance (asynchronous). */
 int  sbp_batch_step(sbp_batch *b, const double *dt, long nsteps);
 int  sbp_batch_synchronize(sbp_batch *b);
@@ -165,6 +169,37 @@ int  sbp_batch_get_config(const sbp_batch *b, int *config, int *tile_width, int 
 /* Time every candidate on this ensemble (resident states are saved and
  * restored) and keep the fastest; *choice receives its index. */
 int  sbp_batch_autotune(sbp_batch *b, const double *dt, int *choice);
+
+/* ---- learned coarse-grid corrector ----------------------------------------
+ * A 3-layer 3x3 convolutional network (5 -> 32 -> 32 -> 3 channels, ReLU;
+ * inputs u, v, p, rho, K; output the correction of u, v, p) applied after
+ * each coarse RK4 step of a 2D acoustic context; bf16 MFMA forward, backward
+ * and inference with FP32 accumulation, FP32 master weights and Adam.  No
+ * counterpart in the reference.  Parameters are a flat FP32 vector:
+ * W1[32][5][3][3], b1[32], W2[32][32][3][3], b2[32], W3[3][32][3][3], b3[3].
+ * Inputs are normalised as (x - shift) * scale; the network predicts the
+ * correction divided by out_scale. */
+typedef struct sbp_corrector sbp_corrector;
+int  sbp_corr_param_count(void);
+int  sbp_corr_create(int nx, int ny, int batch, sbp_corrector **out);   /* image size, batch capacity */
+int  sbp_corr_destroy(sbp_corrector *c);
+int  sbp_corr_set_params(sbp_corrector *c, const float *params, int n); /* also resets Adam */
+int  sbp_corr_get_params(sbp_corrector *c, float *params, int n);
+int  sbp_corr_set_normalization(sbp_corrector *c, const float *in_shift, const float *in_scale,
+                                const float *out_scale);
+/* raw [nb][ny][nx][5] -> out [nb][ny][nx][3] (normalised units). */
+int  sbp_corr_forward(sbp_corrector *c, const float *raw, int nb, float *out);
+/* MSE loss of (net - target/out_scale) and its parameter gradients. */
+int  sbp_corr_gradients(sbp_corrector *c, const float *raw, const float *target, int nb,
+                        float *grads, double *loss);
+int  sbp_corr_load_dataset(sbp_corrector *c, const float *raw, const float *target, int n);
+int  sbp_corr_train_step(sbp_corrector *c, const int *idx, int nb, float lr, double *loss);
+int  sbp_corr_synchronize(sbp_corrector *c);
+/* nsteps coarse RK4 steps of ctx, each followed by U += out_scale*net(U);
+ * a correction raising the discrete energy above E*(1+bound) is rejected
+ * (the plain step is kept); *rejected counts them. */
+int  sbp_corr_advance(sbp_corrector *c, sbp_ctx *ctx, double t0, double dt, long nsteps,
+                      double bound, int *rejected);
 
 /* ---- diagnostics ---------------------------------------------------------- */
 const char *sbp_last_error(void);

@@ -1,0 +1,119 @@
+"""Learned corrector kernels (bf16 MFMA) against the CPU network oracle.
+
+Bars: forward output within 2e-2 of the oracle's max norm (bf16 activations,
+FP32 accumulation); every parameter-gradient tensor within 3e-2 relative
+Frobenius error of the oracle run with the same bf16 rounding points, and
+within 6e-2 of the exact float64 gradients; loss agrees to 1e-3.
+"""
+import numpy as np
+import pytest
+
+from oracle import corr_oracle as CO
+from problems_helper import P
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def capi():
+    import importlib
+    m = importlib.import_module("hpc-reading-course_amd.capi")
+    m.load_library()
+    return m
+
+
+def setup(seed, nb=2, H=21, W=37):
+    rng = np.random.default_rng(seed)
+    raw = rng.standard_normal((nb, H, W, 5)).astype(np.float32)
+    raw[..., 3:] = rng.uniform(0.5, 2.0, (nb, H, W, 2))
+    tgt = (0.3 * rng.standard_normal((nb, H, W, 3))).astype(np.float32)
+    params = (rng.standard_normal(CO.n_params()) * 0.15).astype(np.float32)
+    shift = np.array([0, 0, 0, 1.25, 1.25], np.float32)
+    scale = np.array([1.5, 1.5, 2.0, 1.3, 1.3], np.float32)
+    osc = np.array([0.5, 0.5, 2.0], np.float32)
+    return raw, tgt, params, shift, scale, osc
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+
+
+@pytest.mark.parametrize("shape", [(1, 16, 16), (2, 21, 37), (3, 9, 70)])
+def test_forward_matches_oracle(capi, shape):
+    nb, H, W = shape
+    raw, tgt, params, shift, scale, osc = setup(1, nb, H, W)
+    with capi.Corrector(W, H, nb) as c:
+        c.params = params
+        c.set_normalization(shift, scale, osc)
+        y = c.forward(raw)
+    ref, _ = CO.forward(params, raw, shift, scale, exact=False)
+    assert np.max(np.abs(y - ref)) <= 2e-2 * np.max(np.abs(ref))
+    exact, _ = CO.forward(params, raw, shift, scale, exact=True)
+    assert np.max(np.abs(y - exact)) <= 5e-2 * np.max(np.abs(exact))
+
+
+def test_gradients_match_oracle(capi):
+    raw, tgt, params, shift, scale, osc = setup(2)
+    nb, H, W = raw.shape[:3]
+    with capi.Corrector(W, H, nb) as c:
+        c.params = params
+        c.set_normalization(shift, scale, osc)
+        loss, g = c.gradients(raw, tgt)
+    l_ref, g_ref = CO.loss_and_grads(params, raw, tgt, shift, scale, osc, exact=False)
+    l_ex, g_ex = CO.loss_and_grads(params, raw, tgt, shift, scale, osc, exact=True)
+    assert loss == pytest.approx(l_ref, rel=1e-3)
+    k = 0
+    for name, shp in CO.SHAPES:
+        n = int(np.prod(shp))
+        a, b, e = g[k:k + n], g_ref[k:k + n], g_ex[k:k + n]
+        print(name, "rel vs bf16 oracle %.2e, vs exact %.2e" % (rel(a, b), rel(a, e)), flush=True)
+        assert rel(a, b) <= 3e-2, name
+        assert rel(a, e) <= 6e-2, name
+        k += n
+
+
+def test_training_reduces_loss(capi):
+    rng = np.random.default_rng(3)
+    n, H, W = 16, 24, 24
+    raw = rng.standard_normal((n, H, W, 5)).astype(np.float32)
+    # a learnable target: a fixed local stencil of the inputs
+    tgt = np.zeros((n, H, W, 3), np.float32)
+    tgt[..., 0] = 0.5 * raw[..., 2] - 0.25 * np.roll(raw[..., 0], 1, axis=2)
+    tgt[..., 1] = 0.3 * raw[..., 1] * (raw[..., 3] > 0)
+    tgt[..., 2] = -0.2 * raw[..., 2] + 0.1 * np.roll(raw[..., 1], -1, axis=1)
+    with capi.Corrector(W, H, 4) as c:
+        c.params = (rng.standard_normal(CO.n_params()) * 0.05).astype(np.float32)
+        c.load_dataset(raw, tgt)
+        first = c.train_step([0, 1, 2, 3], 3e-3)
+        losses = []
+        for it in range(300):
+            losses.append(c.train_step(rng.choice(n, 4, replace=False), 3e-3))
+    assert np.mean(losses[-20:]) < 0.2 * first, (first, losses[-5:])
+
+
+def test_energy_guard_rejects_growth(capi):
+    """A corrector that pumps energy into the state is refused every step;
+    the result then equals the plain solve bit for bit."""
+    spec, v0 = P.acoustic_box(65, 4, materials=True)
+    dt = P.dt_from_cfl(spec, 0.25)
+    with capi.Context(spec) as ctx:
+        plain = ctx.rk4_advance(v0, 0.0, dt, 10)
+    with capi.Corrector(65, 65, 1) as c:
+        p = np.zeros(c.n_params, np.float32)
+        p[-3:] = 1.0  # b3: constant positive correction of u, v, p
+        c.params = p
+        c.set_normalization(np.zeros(5), np.ones(5), np.array([0.1, 0.1, 0.1]))
+        with capi.Context(spec) as ctx:
+            ctx.set_state(v0)
+            rej = c.advance(ctx, 0.0, dt, 10, 1e-6)
+            got = ctx.get_state()
+        assert rej == 10
+        assert np.array_equal(got, plain)
+        # the zero network is an exact identity and never rejected
+        c.params = np.zeros(c.n_params, np.float32)
+        with capi.Context(spec) as ctx:
+            ctx.set_state(v0)
+            rej = c.advance(ctx, 0.0, dt, 10, 0.0)
+            got = ctx.get_state()
+        assert rej == 0
+        assert np.array_equal(got, plain)
