@@ -4,13 +4,14 @@ Python host binding over the C-ABI in include/sbp_hip.h.  Importing the
 package does not touch the GPU; ``capi.load_library()`` loads libsbp_hip.so
 (raising if it is not built) on first use.
 """
+import importlib
+
 from . import problems  # noqa: F401
 
-__all__ = ["problems", "capi"]
+__all__ = ["problems", "capi", "corrector"]
 
 
 def __getattr__(name):
-    if name == "capi":
-        from . import capi
-        return capi
+    if name in ("capi", "corrector"):
+        return importlib.import_module(__name__ + "." + name)
     raise AttributeError(name)

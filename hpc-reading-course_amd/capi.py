@@ -90,6 +90,8 @@ EXPORTS = {
     "sbp_corr_synchronize": (C.c_int, [C.c_void_p]),
     "sbp_corr_advance": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_long, C.c_double,
                                    C.POINTER(C.c_int)]),
+    "sbp_corr_advance_batch": (C.c_int, [C.c_void_p, C.c_void_p, _DP, C.c_long, C.c_double,
+                                         C.POINTER(C.c_int)]),
     "sbp_last_error": (C.c_char_p, []),
     "sbp_device_arch": (C.c_char_p, []),
     "sbp_version": (C.c_int, []),
@@ -452,6 +454,13 @@ class Corrector:
         rej = C.c_int()
         _check(load_library().sbp_corr_advance(self._h, ctx._h, t0, dt, int(nsteps), float(bound), C.byref(rej)))
         return rej.value
+
+    def advance_batch(self, batch: "Batch", dt, nsteps: int, bound: float) -> np.ndarray:
+        d = batch._dt(dt)
+        rej = np.zeros(len(batch), dtype=np.int32)
+        _check(load_library().sbp_corr_advance_batch(self._h, batch._h, _ptr(d), int(nsteps), float(bound),
+                                                     rej.ctypes.data_as(C.POINTER(C.c_int))))
+        return rej
 
 
 def rhs(ctx: Context, t: float, v_src: np.ndarray, v_dst: np.ndarray) -> int:

@@ -1,0 +1,246 @@
+"""Learned coarse-grid corrector: training data from the solver itself, GPU
+training with the library's own bf16 MFMA kernels, corrected coarse solves,
+and the time-to-accuracy comparison against the plain FP64 path.
+
+Problem family: 2D acoustics on the unit square, wall SATs, order 4, smooth
+seeded rho and K in [0.5, 2], seeded Gaussian pressure pulse (sigma 0.03),
+v = 0.  Coarse grid 257^2 (dt = h/4); the fine grid 1025^2 nests 4:1, so the
+fine solution is compared on the coarse nodes by injection.
+
+A training pair comes from one fine FP64 trajectory: the fine state F_n at a
+coarse step n is injected onto the coarse grid, advanced by one coarse FP64
+RK4 step (C'), and the target is the injected F_{n+1} minus C'.  The network
+sees (C', rho, K) and predicts that residual.
+"""
+from __future__ import annotations
+
+import concurrent.futures as cf
+import os
+import time
+
+import numpy as np
+
+from . import capi
+from . import problems as P
+
+N_FINE, N_COARSE, RATIO = 1025, 257, 4
+SIGMA = 0.03
+ORDER = 4
+CFL = 0.25
+WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "corrector_weights.npz")
+
+
+def instance(seed: int, n: int, precision: int = P.FP64, sigma: float = None):
+    """Seeded instance on an n x n grid; the same seed gives the same fields
+    on every nested grid (exact injection)."""
+    return P.acoustic_box(n, ORDER, seed=seed, sigma=SIGMA if sigma is None else sigma, materials=True,
+                          material_seed=seed + 7, precision=precision)
+
+
+def _instances(seeds, n, precision, threads=16):
+    with cf.ThreadPoolExecutor(min(threads, len(seeds))) as ex:
+        return list(ex.map(lambda s: instance(s, n, precision), seeds))
+
+
+def raw_features(spec, state_aos):
+    """(u, v, p, rho, K) per point, float32 [ny][nx][5]."""
+    q = np.asarray(state_aos, dtype=np.float64).reshape(spec.ny, spec.nx, 3)
+    rho = 1.0 / spec.rho_inv if spec.rho_inv is not None else np.ones((spec.ny, spec.nx))
+    K = spec.bulk if spec.bulk is not None else np.ones((spec.ny, spec.nx))
+    return np.concatenate([q, rho[..., None], K[..., None]], axis=-1).astype(np.float32)
+
+
+def make_training_data(n_instances=2000, seed0=100000, t_train=0.25, chunk=64, log=print):
+    """Solver-generated (input, residual) pairs, one per instance, at coarse
+    steps spread over [0, t_train)."""
+    dt_c = P.dt_from_cfl(instance(seed0, N_COARSE)[0], CFL)
+    dt_f = dt_c / RATIO
+    M = int(round(t_train / dt_c))
+    seeds = np.arange(seed0, seed0 + n_instances)
+    snap = {int(s): int((i * 7919) % M) for i, s in enumerate(seeds)}
+    order = sorted(seeds.tolist(), key=lambda s: snap[s])  # similar snapshot times share a batch
+    raws, tgts = [], []
+    t0 = time.time()
+    for c0 in range(0, n_instances, chunk):
+        group = order[c0:c0 + chunk]
+        fine = _instances(group, N_FINE, P.FP64)
+        coarse = _instances(group, N_COARSE, P.FP64)
+        last = max(snap[s] for s in group)
+        F_n, F_n1 = {}, {}
+        with capi.Batch([f[0] for f in fine]) as fb:
+            for k, (_, v0) in enumerate(fine):
+                fb.set_state(k, v0)
+            for n in range(last + 1):
+                ks = [k for k, s in enumerate(group) if snap[s] == n]
+                for k in ks:
+                    F_n[k] = fb.get_state_strided(k, RATIO)
+                fb.step(dt_f, RATIO)
+                for k in ks:
+                    F_n1[k] = fb.get_state_strided(k, RATIO)
+        with capi.Batch([c[0] for c in coarse]) as cb:
+            for k in range(len(group)):
+                cb.set_state(k, F_n[k])
+            cb.step(dt_c, 1)
+            for k, (spec, _) in enumerate(coarse):
+                Cp = cb.get_state(k)
+                raws.append(raw_features(spec, Cp))
+                tgts.append((F_n1[k] - Cp).reshape(spec.ny, spec.nx, 3).astype(np.float32))
+        log("training data: %d/%d instances (%.1f s)" % (min(c0 + chunk, n_instances), n_instances, time.time() - t0))
+    return np.stack(raws), np.stack(tgts)
+
+
+def normalization(raw, tgt):
+    shift = np.zeros(5, np.float32)
+    scale = np.ones(5, np.float32)
+    for c in range(5):
+        x = raw[..., c]
+        if c >= 3:
+            shift[c] = float(x.mean())
+        scale[c] = 1.0 / max(float(x.std()), 1e-12)
+    out = np.array([max(float(tgt[..., c].std()), 1e-30) for c in range(3)], np.float32)
+    return shift, scale, out
+
+
+def init_params(seed=0):
+    rng = np.random.default_rng(seed)
+    parts = [rng.standard_normal(32 * 5 * 9) * np.sqrt(2.0 / 45), np.zeros(32),
+             rng.standard_normal(32 * 32 * 9) * np.sqrt(2.0 / 288), np.zeros(32),
+             rng.standard_normal(3 * 32 * 9) * 1e-3, np.zeros(3)]
+    return np.concatenate(parts).astype(np.float32)
+
+
+def train(raw, tgt, epochs=12, batch=16, lr=2e-3, seed=0, val_frac=0.1, log=print):
+    """Adam on the GPU through sbp_corr_train_step; returns (params,
+    normalisation, history).  A random val_frac of the samples is held out."""
+    n = raw.shape[0]
+    rng = np.random.default_rng(seed)
+    order = rng.permutation(n)
+    raw, tgt = raw[order], tgt[order]
+    nval = max(1, int(n * val_frac))
+    ntr = n - nval
+    shift, scale, out = normalization(raw[:ntr], tgt[:ntr])
+    hist = []
+    with capi.Corrector(raw.shape[2], raw.shape[1], batch) as c:
+        c.params = init_params(seed)
+        c.set_normalization(shift, scale, out)
+        c.load_dataset(raw, tgt)
+        steps_per_epoch = max(1, ntr // batch)
+        total = epochs * steps_per_epoch
+        it = 0
+        t0 = time.time()
+        for ep in range(epochs):
+            perm = rng.permutation(ntr)
+            losses = []
+            for s in range(steps_per_epoch):
+                lr_t = lr * 0.5 * (1 + np.cos(np.pi * it / total))  # cosine decay
+                losses.append(c.train_step(perm[s * batch:(s + 1) * batch], lr_t))
+                it += 1
+            val = [c.gradients(raw[ntr + k:ntr + k + 1], tgt[ntr + k:ntr + k + 1])[0] for k in range(min(nval, 32))]
+            hist.append((ep, float(np.mean(losses)), float(np.mean(val))))
+            log("epoch %d train %.4f val %.4f (%.1f s)" % (ep, hist[-1][1], hist[-1][2], time.time() - t0))
+        params = c.params
+    return params, (shift, scale, out), hist
+
+
+def save_weights(params, norm, hist, path=WEIGHTS):
+    shift, scale, out = norm
+    np.savez(path, params=params, shift=shift, scale=scale, out=out, history=np.array(hist))
+
+
+def load_weights(path=WEIGHTS):
+    d = np.load(path, allow_pickle=False)
+    return d["params"], (d["shift"], d["scale"], d["out"])
+
+
+def _l2(spec, v, ref):
+    return float(np.sqrt(spec.hx * spec.hy) * np.linalg.norm(v - ref))
+
+
+def time_to_accuracy(params, norm, n_eval=32, seed0=900000, t_end=0.5, plain_n=513, bound=1e-6, log=print):
+    """Wall time of three ways to solve n_eval held-out instances to t_end,
+    each judged by the L2 error against the FP64 1025^2 solution injected at
+    its own grid nodes:
+      plain  FP64 single-problem path, one instance after another, plain_n^2
+      batched FP32 ensemble, plain_n^2, one persistent kernel per stage
+      learned FP64 ensemble on 257^2 + the corrector after every step
+    plus the plain 257^2 solve (the method without the network)."""
+    seeds = list(range(seed0, seed0 + n_eval))
+    res = {"n_eval": n_eval, "t_end": t_end, "target_l2": 1e-4}
+    # fine FP64 reference, batched
+    fine = _instances(seeds, N_FINE, P.FP64)
+    dt_f = P.dt_from_cfl(fine[0][0], CFL)
+    nf = int(round(t_end / dt_f))
+    t = time.time()
+    with capi.Batch([f[0] for f in fine]) as b:
+        for k, (_, v0) in enumerate(fine):
+            b.set_state(k, v0)
+        b.step(dt_f, nf)
+        b.synchronize()
+        res["fine_ref_s"] = time.time() - t
+        stride_p = (N_FINE - 1) // (plain_n - 1)
+        ref_p = [b.get_state_strided(k, stride_p) for k in range(n_eval)]
+        ref_c = [b.get_state_strided(k, RATIO) for k in range(n_eval)]
+    log("reference: %d instances of %d^2, %d steps (%.2f s)" % (n_eval, N_FINE, nf, res["fine_ref_s"]))
+
+    def run_plain(n, prec, refs):
+        insts = _instances(seeds, n, prec)
+        dt = P.dt_from_cfl(insts[0][0], CFL)
+        ns = int(round(t_end / dt))
+        errs, tot = [], 0.0
+        for (spec, v0), ref in zip(insts, refs):
+            with capi.Context(spec) as c:
+                c.set_state(v0)
+                c.synchronize()
+                t = time.time()
+                c.step(0.0, dt, ns)
+                c.synchronize()
+                tot += time.time() - t
+                errs.append(_l2(spec, c.get_state(), ref))
+        return tot, errs
+
+    tp, ep = run_plain(plain_n, P.FP64, ref_p)
+    res["plain_fp64"] = {"grid": plain_n, "wall_s": tp, "max_l2": max(ep), "mean_l2": float(np.mean(ep))}
+    log("plain FP64 %d^2: %.3f s, max L2 %.2e" % (plain_n, tp, max(ep)))
+    tc, ec = run_plain(N_COARSE, P.FP64, ref_c)
+    res["plain_fp64_coarse"] = {"grid": N_COARSE, "wall_s": tc, "max_l2": max(ec), "mean_l2": float(np.mean(ec))}
+    log("plain FP64 %d^2 (no network): %.3f s, max L2 %.2e" % (N_COARSE, tc, max(ec)))
+
+    # FP32 batched ensemble at plain_n
+    insts = _instances(seeds, plain_n, P.FP32)
+    dt = P.dt_from_cfl(insts[0][0], CFL)
+    ns = int(round(t_end / dt))
+    with capi.Batch([s for s, _ in insts]) as b:
+        for k, (_, v0) in enumerate(insts):
+            b.set_state(k, v0)
+        b.autotune(dt)
+        b.synchronize()
+        t = time.time()
+        b.step(dt, ns)
+        b.synchronize()
+        tb = time.time() - t
+        eb = [_l2(insts[k][0], b.get_state(k), ref_p[k]) for k in range(n_eval)]
+    res["batched_fp32"] = {"grid": plain_n, "wall_s": tb, "max_l2": max(eb), "mean_l2": float(np.mean(eb))}
+    log("batched FP32 %d^2: %.3f s, max L2 %.2e" % (plain_n, tb, max(eb)))
+
+    # learned corrector on the coarse FP64 ensemble
+    insts = _instances(seeds, N_COARSE, P.FP64)
+    dt = P.dt_from_cfl(insts[0][0], CFL)
+    ns = int(round(t_end / dt))
+    shift, scale, out = norm
+    with capi.Batch([s for s, _ in insts]) as b, capi.Corrector(N_COARSE, N_COARSE, n_eval) as cor:
+        cor.params = params
+        cor.set_normalization(shift, scale, out)
+        for k, (_, v0) in enumerate(insts):
+            b.set_state(k, v0)
+        b.synchronize()
+        t = time.time()
+        rej = cor.advance_batch(b, dt, ns, bound)
+        tl = time.time() - t
+        el = [_l2(insts[k][0], b.get_state(k), ref_c[k]) for k in range(n_eval)]
+    res["learned"] = {"grid": N_COARSE, "wall_s": tl, "max_l2": max(el), "mean_l2": float(np.mean(el)),
+                      "rejected_steps": int(np.sum(rej)), "steps": ns * n_eval}
+    log("learned %d^2: %.3f s, max L2 %.2e, rejected %d of %d" % (N_COARSE, tl, max(el), int(np.sum(rej)), ns * n_eval))
+    res["speedup_batched_fp32"] = tp / tb
+    res["speedup_learned"] = tp / tl
+    res["gain_learned_vs_plain_coarse_l2"] = float(np.mean(ec) / np.mean(el))
+    return res

@@ -18,10 +18,12 @@
 
 #include "../../include/sbp_hip.h"
 #include "sbp_batch.hpp"
+#include "sbp_batch_impl.hpp"
 #include "sbp_host.hpp"
 
 using namespace sbp;
 using sbp::host::fail;
+using sbp::host::InstHost;
 #define HIP_OR_FAIL(expr) SBP_HIP_OR_FAIL(expr)
 
 namespace {
@@ -41,34 +43,9 @@ constexpr int kBatchGraphSteps = 10;
 
 int closure_of(int order) { return order == 2 ? 1 : (order == 4 ? 4 : 6); }
 
-struct InstHost {
-  int nx, ny, pitch, xoff;
-  long plane, off;  // element offset of the instance's region in every pool
-  double hix, hiy, dt;
-  int bc;
-};
 
 }  // namespace
 
-struct sbp_batch {
-  int count = 0, order = 0, cls = 0, prec = 0, mat = 0;
-  size_t esz = 4;
-  std::vector<InstHost> inst;
-  long pool = 0;       // elements per register pool
-  long max_pts = 0;    // largest nx*ny
-  void *U = nullptr, *A = nullptr, *Y0 = nullptr, *Y1 = nullptr, *KC = nullptr;
-  void *RI = nullptr, *BK = nullptr;
-  void *inst_dev = nullptr;
-  int2 *work_dev = nullptr;
-  int nwork = 0;
-  double *aos = nullptr, *partials = nullptr, *scalar = nullptr;
-  hipStream_t stream = nullptr;
-  hipGraphExec_t gexec = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  int config = 0, grid = 0, num_cus = 256;
-  BatchLaunchFn launch = nullptr;
-  std::vector<char> have_state;
-};
 
 namespace {
 
@@ -383,6 +360,9 @@ int norm_instance(sbp_batch *b, int k, double *out)
 }
 
 }  // namespace
+
+hipError_t sbp::host::batch_launch_step(sbp_batch *b) { return ::launch_step(b); }
+int sbp::host::batch_upload_dt(sbp_batch *b, const double *dt) { return ::upload_dt(b, dt); }
 
 extern "C" {
 

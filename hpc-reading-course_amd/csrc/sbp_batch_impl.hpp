@@ -1,0 +1,49 @@
+// sbp_batch_impl.hpp -- the batched-ensemble object shared by the host
+// translation units (batch C-ABI, learned corrector).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "../../include/sbp_hip.h"
+#include "sbp_batch.hpp"
+
+namespace sbp {
+namespace host {
+struct InstHost {
+  int nx, ny, pitch, xoff;
+  long plane, off;  // element offset of the instance's region in every pool
+  double hix, hiy, dt;
+  int bc;
+};
+}  // namespace host
+}  // namespace sbp
+
+struct sbp_batch {
+  int count = 0, order = 0, cls = 0, prec = 0, mat = 0;
+  size_t esz = 4;
+  std::vector<sbp::host::InstHost> inst;
+  long pool = 0;       // elements per register pool
+  long max_pts = 0;    // largest nx*ny
+  void *U = nullptr, *A = nullptr, *Y0 = nullptr, *Y1 = nullptr, *KC = nullptr;
+  void *RI = nullptr, *BK = nullptr;
+  void *inst_dev = nullptr;
+  int2 *work_dev = nullptr;
+  int nwork = 0;
+  double *aos = nullptr, *partials = nullptr, *scalar = nullptr;
+  hipStream_t stream = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int config = 0, grid = 0, num_cus = 256;
+  sbp::BatchLaunchFn launch = nullptr;
+  std::vector<char> have_state;
+};
+
+namespace sbp {
+namespace host {
+// One RK4 step of every instance (four persistent-kernel launches).
+hipError_t batch_launch_step(sbp_batch *b);
+// Upload per-instance time steps into the device descriptors.
+int batch_upload_dt(sbp_batch *b, const double *dt);
+}  // namespace host
+}  // namespace sbp

@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "../../include/sbp_hip.h"
+#include "sbp_batch_impl.hpp"
 #include "sbp_ctx.hpp"
 #include "sbp_host.hpp"
 
@@ -70,51 +71,59 @@ __global__ __launch_bounds__(256) void conv3x3_kernel(const bf16 *__restrict__ i
                                                       bf16 *__restrict__ out_bf, float *__restrict__ out_f, int out_ch)
 {
   static_assert(CIN_P % 4 == 0 && COUT_P % 16 == 0 && KP % 16 == 0, "MFMA tiling");
+  constexpr int NT = COUT_P / 16, NK = KP / 16;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int segs = (W + 15) / 16;
-  const long seg = (long)blockIdx.x * 4 + wv;
-  if (seg >= (long)nimg * H * segs) return;  // whole wavefront; no barriers below
-  const int sx = (int)(seg % segs);
-  const long row = seg / segs;
-  const int y = (int)(row % H), img = (int)(row / H);
-  const int x0 = sx * 16;
   const int m = lane & 15, kg = lane >> 4;
-  constexpr int NT = COUT_P / 16;
-  f32x4 acc[NT];
+  // The wavefront's B fragments (all weights it needs) stay in registers
+  // while it walks its 16-pixel segments.
+  bf16x4 bw[NK][NT];
 #pragma unroll
-  for (int t = 0; t < NT; t++) acc[t] = f32x4{0, 0, 0, 0};
-  const long img_base = (long)img * H * W;
-#pragma unroll 2
-  for (int ks = 0; ks < KP / 16; ks++) {
-    const int k0 = ks * 16 + kg * 4;
-    const int tap = k0 / CIN_P, ci = k0 - tap * CIN_P;
-    bf16x4 a = {0, 0, 0, 0};
-    if (tap < 9) {
-      const int yy = y + tap / 3 - 1, xx = x0 + m + tap % 3 - 1;
-      if (yy >= 0 && yy < H && xx >= 0 && xx < W)
-        a = *reinterpret_cast<const bf16x4 *>(in + (img_base + (long)yy * W + xx) * CIN_P + ci);
+  for (int ks = 0; ks < NK; ks++)
+#pragma unroll
+    for (int t = 0; t < NT; t++)
+      bw[ks][t] = *reinterpret_cast<const bf16x4 *>(wt + (long)(t * 16 + m) * KP + ks * 16 + kg * 4);
+  float bv[NT];
+#pragma unroll
+  for (int t = 0; t < NT; t++) bv[t] = MODE != 2 ? bias[t * 16 + m] : 0.f;
+  const int segs = (W + 15) / 16;
+  const long nseg = (long)nimg * H * segs;
+  for (long seg = (long)blockIdx.x * 4 + wv; seg < nseg; seg += (long)gridDim.x * 4) {
+    const int sx = (int)(seg % segs);
+    const long row = seg / segs;
+    const int y = (int)(row % H), img = (int)(row / H);
+    const int x0 = sx * 16;
+    f32x4 acc[NT];
+#pragma unroll
+    for (int t = 0; t < NT; t++) acc[t] = f32x4{0, 0, 0, 0};
+    const long img_base = (long)img * H * W;
+#pragma unroll
+    for (int ks = 0; ks < NK; ks++) {
+      const int k0 = ks * 16 + kg * 4;
+      const int tap = k0 / CIN_P, ci = k0 - tap * CIN_P;
+      bf16x4 a = {0, 0, 0, 0};
+      if (tap < 9) {
+        const int yy = y + tap / 3 - 1, xx = x0 + m + tap % 3 - 1;
+        if (yy >= 0 && yy < H && xx >= 0 && xx < W)
+          a = *reinterpret_cast<const bf16x4 *>(in + (img_base + (long)yy * W + xx) * CIN_P + ci);
+      }
+#pragma unroll
+      for (int t = 0; t < NT; t++) acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bw[ks][t], acc[t], 0, 0, 0);
     }
+    // acc[t][i] = C[row 4*kg+i][col m]: pixel x0 + 4*kg + i, output channel 16*t + m
 #pragma unroll
     for (int t = 0; t < NT; t++) {
-      const bf16x4 b = *reinterpret_cast<const bf16x4 *>(wt + (long)(t * 16 + m) * KP + k0);
-      acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
-    }
-  }
-  // acc[t][i] = C[row 4*kg+i][col m]: pixel x0 + 4*kg + i, output channel 16*t + m
+      const int co = t * 16 + m;
 #pragma unroll
-  for (int t = 0; t < NT; t++) {
-    const int co = t * 16 + m;
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-      const int px = x0 + 4 * kg + i;
-      if (px >= W) continue;
-      float v = acc[t][i];
-      if constexpr (MODE != 2) v += bias[co];
-      const long p = img_base + (long)y * W + px;
-      if constexpr (MODE == 0) {
-        out_bf[p * COUT_P + co] = f2bf(v > 0.f ? v : 0.f);
-      } else {
-        if (co < out_ch) out_f[p * out_ch + co] = v;
+      for (int i = 0; i < 4; i++) {
+        const int px = x0 + 4 * kg + i;
+        if (px >= W) continue;
+        const float v = acc[t][i] + bv[t];
+        const long p = img_base + (long)y * W + px;
+        if constexpr (MODE == 0) {
+          out_bf[p * COUT_P + co] = f2bf(v > 0.f ? v : 0.f);
+        } else {
+          if (co < out_ch) out_f[p * out_ch + co] = v;
+        }
       }
     }
   }
@@ -384,6 +393,109 @@ __global__ __launch_bounds__(256) void accept_kernel(const T *__restrict__ Y, T 
   if (ok && i < n) U[i] = Y[i];
 }
 
+// ---- batched ensemble (every instance on the corrector's grid) ------------
+template <typename T>
+__global__ __launch_bounds__(256) void pack_batch_kernel(const BatchInst<T> *__restrict__ inst, int count, int H, int W,
+                                                         const float *__restrict__ shift_scale, bf16 *__restrict__ X)
+{
+  const long p = blockIdx.x * 256L + threadIdx.x;
+  const long npi = (long)H * W;
+  if (p >= count * npi) return;
+  const int k = (int)(p / npi);
+  const long q = p - k * npi;
+  const int j = (int)(q / W), i = (int)(q - (long)j * W);
+  const BatchInst<T> &d = inst[k];
+  const long o = (long)j * d.pitch + d.xoff + i;
+  float raw[C_IN];
+  raw[0] = (float)d.U[o];
+  raw[1] = (float)d.U[d.plane + o];
+  raw[2] = (float)d.U[2 * d.plane + o];
+  raw[3] = d.rinv ? (float)(1.0 / (double)d.rinv[o]) : 1.f;
+  raw[4] = d.bulk ? (float)d.bulk[o] : 1.f;
+#pragma unroll
+  for (int c = 0; c < C_INP; c++)
+    X[p * C_INP + c] = c < C_IN ? f2bf((raw[c] - shift_scale[c]) * shift_scale[C_IN + c]) : (bf16)0;
+}
+
+// Per-instance H-weighted energy of U (which = 0) or of the Y0 scratch
+// register (which = 1): blocks [k*bpi, (k+1)*bpi) reduce instance k.
+template <typename T>
+__global__ __launch_bounds__(256) void batch_energy_kernel(const BatchInst<T> *__restrict__ inst, int bpi, int which,
+                                                           int order, double *__restrict__ partial)
+{
+  __shared__ double sh[4];
+  const int k = blockIdx.x / bpi, chunk = blockIdx.x % bpi;
+  const BatchInst<T> &d = inst[k];
+  const T *u = which ? d.Y0 : d.U;
+  const T *comp = which ? nullptr : d.comp;
+  const long npts = (long)d.nx * d.ny;
+  const double hx = 1.0 / (double)d.hix, hy = 1.0 / (double)d.hiy;
+  double acc = 0;
+  for (long q = chunk * 256L + threadIdx.x; q < npts; q += (long)bpi * 256) {
+    const int j = (int)(q / d.nx), i = (int)(q - (long)j * d.nx);
+    const long o = (long)j * d.pitch + d.xoff + i;
+    const double w = hweight(order, d.nx, i, hx) * hweight(order, d.ny, j, hy);
+    double v[3];
+    for (int c = 0; c < 3; c++) {
+      double x = (double)u[c * d.plane + o];
+      if (comp) x -= (double)comp[c * d.plane + o];
+      v[c] = x;
+    }
+    const double rho = d.rinv ? 1.0 / (double)d.rinv[o] : 1.0;
+    const double kinv = d.bulk ? 1.0 / (double)d.bulk[o] : 1.0;
+    acc += w * (rho * (v[0] * v[0] + v[1] * v[1]) + kinv * v[2] * v[2]);
+  }
+  const double s = block_sum_256(acc, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+__global__ void batch_energy_final_kernel(const double *__restrict__ partial, int count, int bpi, double *__restrict__ e)
+{
+  const int k = blockIdx.x * 64 + threadIdx.x;
+  if (k >= count) return;
+  double s = 0;
+  for (int b = 0; b < bpi; b++) s += partial[(long)k * bpi + b];
+  e[k] = s;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void batch_candidate_kernel(const BatchInst<T> *__restrict__ inst, int count, int H,
+                                                              int W, const float *__restrict__ net,
+                                                              const float *__restrict__ out_scale)
+{
+  const long p = blockIdx.x * 256L + threadIdx.x;
+  const long npi = (long)H * W;
+  if (p >= count * npi) return;
+  const int k = (int)(p / npi);
+  const long q = p - k * npi;
+  const int j = (int)(q / W), i = (int)(q - (long)j * W);
+  const BatchInst<T> &d = inst[k];
+  const long o = (long)j * d.pitch + d.xoff + i;
+#pragma unroll
+  for (int c = 0; c < 3; c++) d.Y0[c * d.plane + o] = d.U[c * d.plane + o] + (T)(net[p * C_OUT + c] * out_scale[c]);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void batch_accept_kernel(const BatchInst<T> *__restrict__ inst, int count, long reg,
+                                                           const double *__restrict__ e_old,
+                                                           const double *__restrict__ e_new, double bound,
+                                                           int *__restrict__ rejected)
+{
+  const long p = blockIdx.x * 256L + threadIdx.x;
+  if (p >= count * reg) return;
+  const int k = (int)(p / reg);
+  const long q = p - k * reg;
+  const bool ok = e_new[k] <= e_old[k] * (1.0 + bound) && e_new[k] == e_new[k];
+  if (q == 0 && !ok) rejected[k] += 1;
+  if (ok) inst[k].U[q] = inst[k].Y0[q];
+}
+
+__global__ void clear_ints_kernel(int *__restrict__ p, int n)
+{
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
 __global__ void copy_scalar_kernel(const double *__restrict__ s, double *__restrict__ d) { d[0] = s[0]; }
 
 __global__ void clear_int_kernel(int *__restrict__ p) { p[0] = 0; }
@@ -403,6 +515,8 @@ struct sbp_corrector {
   float *Yo = nullptr, *Tg = nullptr, *dA = nullptr, *wpart = nullptr;
   double *lpart = nullptr, *scalars = nullptr;  // scalars: [0] loss, [1] E_old
   int *rejected = nullptr, *idx = nullptr;
+  double *bpart = nullptr, *be_old = nullptr, *be_new = nullptr;  // batched energies
+  int *brej = nullptr;
   float *ds_raw = nullptr, *ds_tgt = nullptr;
   int ds_n = 0;
   long adam_t = 0;
@@ -416,7 +530,8 @@ void release(sbp_corrector *c)
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   void *ptrs[] = {c->params, c->grads, c->m, c->v, c->w1, c->w2, c->w3, c->w2t, c->w3t, c->b1, c->b2, c->b3,
                   c->norm, c->X0, c->A1, c->A2, c->dY3, c->dP2, c->dP1, c->Yo, c->Tg, c->dA, c->wpart,
-                  c->lpart, c->scalars, c->rejected, c->idx, c->ds_raw, c->ds_tgt};
+                  c->lpart, c->scalars, c->rejected, c->idx, c->ds_raw, c->ds_tgt, c->bpart, c->be_old, c->be_new,
+                  c->brej};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -439,7 +554,13 @@ int prepare(sbp_corrector *c)
   return SBP_OK;
 }
 
-int conv_grid(int nb, int H, int W) { return (int)(((long)nb * H * ((W + 15) / 16) + 3) / 4); }
+// Persistent conv grid: at most 2048 blocks (8 per CU), each wavefront walks
+// several 16-pixel segments with its weights held in registers.
+int conv_grid(int nb, int H, int W)
+{
+  const long blocks = ((long)nb * H * ((W + 15) / 16) + 3) / 4;
+  return (int)std::min<long>(blocks, 2048);
+}
 
 // Forward of nb images already packed in X0: A1, A2 (bf16) and Yo (fp32, 3 ch).
 int forward(sbp_corrector *c, int nb)
@@ -552,6 +673,10 @@ int sbp_corr_create(int nx, int ny, int batch, sbp_corrector **out)
   if (!rc) rc = dmalloc(&c->scalars, sizeof(double) * 4);
   if (!rc) rc = dmalloc(&c->rejected, sizeof(int));
   if (!rc) rc = dmalloc(&c->idx, sizeof(int) * batch);
+  if (!rc) rc = dmalloc(&c->bpart, sizeof(double) * batch * 64);
+  if (!rc) rc = dmalloc(&c->be_old, sizeof(double) * batch);
+  if (!rc) rc = dmalloc(&c->be_new, sizeof(double) * batch);
+  if (!rc) rc = dmalloc(&c->brej, sizeof(int) * batch);
   if (!rc) {
     // Default: zero network (identity correction), unit normalisation.
     std::vector<float> zero(N_PARAMS, 0.f);
@@ -736,9 +861,10 @@ int sbp_corr_synchronize(sbp_corrector *c)
   return SBP_OK;
 }
 
-// Correct the resident state of a 2D acoustic context in place:
-// U <- U + out_scale * net(U, rho, K), unless that raises the discrete energy
-// above E_old * (1 + bound), in which case the plain state is kept.
+// Correct the resident state of a 2D acoustic context in place after a step:
+// U <- U + out_scale * net(U, rho, K), unless the corrected state's discrete
+// energy exceeds the energy before the step times (1 + bound), in which case
+// the plain step is kept.
 static int apply_one(sbp_corrector *c, sbp_ctx *x, double bound)
 {
   const long npix = (long)x->nx * x->ny;
@@ -758,9 +884,8 @@ static int apply_one(sbp_corrector *c, sbp_ctx *x, double bound)
   int rc = forward(c, 1);
   c->stream = keep;
   if (rc) return rc;
-  // E_old -> scalars[1]; candidate into the Y0 scratch register; E_new in x->scalar
-  if ((rc = sbp::host::energy_of(x, x->U, x->KC, nullptr))) return rc;
-  hipLaunchKernelGGL(copy_scalar_kernel, dim3(1), dim3(1), 0, x->stream, x->scalar, c->scalars + 1);
+  // candidate into the Y0 scratch register; its energy lands in x->scalar and
+  // is compared with the energy before the step (c->scalars[1])
   if (x->esz == 8)
     hipLaunchKernelGGL(candidate_kernel<double>, dim3(grid_for(npix)), dim3(256), 0, x->stream, (const double *)x->U,
                        c->Yo, x->nx, x->ny, x->pitch, x->xoff, x->plane, c->norm + 10, (double *)x->Y0);
@@ -793,8 +918,12 @@ int sbp_corr_advance(sbp_corrector *c, sbp_ctx *x, double t0, double dt, long ns
   hipLaunchKernelGGL(clear_int_kernel, dim3(1), dim3(1), 0, x->stream, c->rejected);
   double t = t0;
   for (long s = 0; s < nsteps; s++) {
+    // energy before the step -> c->scalars[1]
+    int rc = sbp::host::energy_of(x, x->U, x->KC, nullptr);
+    if (rc) return rc;
+    hipLaunchKernelGGL(copy_scalar_kernel, dim3(1), dim3(1), 0, x->stream, x->scalar, c->scalars + 1);
     HIP_OR_FAIL(sbp::host::launch_step(x, t, dt));
-    int rc = apply_one(c, x, bound);
+    rc = apply_one(c, x, bound);
     if (rc) return rc;
     t = t + dt;
   }
@@ -802,6 +931,76 @@ int sbp_corr_advance(sbp_corrector *c, sbp_ctx *x, double t0, double dt, long ns
   HIP_OR_FAIL(hipMemcpyAsync(&rej, c->rejected, sizeof(int), hipMemcpyDeviceToHost, x->stream));
   HIP_OR_FAIL(hipStreamSynchronize(x->stream));
   if (rejected) *rejected = rej;
+  return SBP_OK;
+}
+
+// nsteps RK4 steps of every instance of a batch, each followed by the
+// guarded correction of every instance (one network pass over all of them).
+// Every instance must have the corrector's grid; rejected[k] (count entries,
+// optional) receives the corrections refused for instance k.
+int sbp_corr_advance_batch(sbp_corrector *c, sbp_batch *b, const double *dt, long nsteps, double bound,
+                           int *rejected)
+{
+  if (!c || !b || !dt) return fail(SBP_EINVAL, "null argument");
+  if (b->count > c->nb) return fail(SBP_EINVAL, "batch larger than the corrector's capacity");
+  if (b->prec == SBP_FP32_KAHAN) return fail(SBP_EINVAL, "the corrector acts on FP64 or FP32 batches");
+  for (int k = 0; k < b->count; k++) {
+    if (b->inst[k].nx != c->W || b->inst[k].ny != c->H)
+      return fail(SBP_EINVAL, "every instance must have the corrector's grid");
+    if (!b->have_state[k]) return fail(SBP_ESTATE, "every instance needs a state");
+  }
+  if (!(bound >= 0)) return fail(SBP_EINVAL, "energy bound must be >= 0");
+  int rc = sbp::host::batch_upload_dt(b, dt);
+  if (rc) return rc;
+  const int count = b->count, bpi = 64;
+  const long npix = (long)count * c->H * c->W;
+  const long reg = 3 * b->inst[0].plane;
+  hipStream_t s = b->stream;
+  hipLaunchKernelGGL(clear_ints_kernel, dim3((count + 255) / 256), dim3(256), 0, s, c->brej, count);
+  hipStream_t keep = c->stream;
+  for (long st = 0; st < nsteps; st++) {
+    if (b->esz == 8) {
+      const BatchInst<double> *inst = static_cast<const BatchInst<double> *>(b->inst_dev);
+      // energy of every instance before the step
+      hipLaunchKernelGGL(batch_energy_kernel<double>, dim3(count * bpi), dim3(256), 0, s, inst, bpi, 0, b->order, c->bpart);
+      hipLaunchKernelGGL(batch_energy_final_kernel, dim3((count + 63) / 64), dim3(64), 0, s, c->bpart, count, bpi, c->be_old);
+      HIP_OR_FAIL(sbp::host::batch_launch_step(b));
+      hipLaunchKernelGGL(pack_batch_kernel<double>, dim3(grid_for(npix)), dim3(256), 0, s, inst, count, c->H, c->W,
+                         c->norm, c->X0);
+      c->stream = s;
+      rc = forward(c, count);
+      c->stream = keep;
+      if (rc) return rc;
+      hipLaunchKernelGGL(batch_candidate_kernel<double>, dim3(grid_for(npix)), dim3(256), 0, s, inst, count, c->H, c->W,
+                         c->Yo, c->norm + 10);
+      hipLaunchKernelGGL(batch_energy_kernel<double>, dim3(count * bpi), dim3(256), 0, s, inst, bpi, 1, b->order, c->bpart);
+      hipLaunchKernelGGL(batch_energy_final_kernel, dim3((count + 63) / 64), dim3(64), 0, s, c->bpart, count, bpi, c->be_new);
+      hipLaunchKernelGGL(batch_accept_kernel<double>, dim3(grid_for(count * reg)), dim3(256), 0, s, inst, count, reg,
+                         c->be_old, c->be_new, bound, c->brej);
+    } else {
+      const BatchInst<float> *inst = static_cast<const BatchInst<float> *>(b->inst_dev);
+      hipLaunchKernelGGL(batch_energy_kernel<float>, dim3(count * bpi), dim3(256), 0, s, inst, bpi, 0, b->order, c->bpart);
+      hipLaunchKernelGGL(batch_energy_final_kernel, dim3((count + 63) / 64), dim3(64), 0, s, c->bpart, count, bpi, c->be_old);
+      HIP_OR_FAIL(sbp::host::batch_launch_step(b));
+      hipLaunchKernelGGL(pack_batch_kernel<float>, dim3(grid_for(npix)), dim3(256), 0, s, inst, count, c->H, c->W,
+                         c->norm, c->X0);
+      c->stream = s;
+      rc = forward(c, count);
+      c->stream = keep;
+      if (rc) return rc;
+      hipLaunchKernelGGL(batch_candidate_kernel<float>, dim3(grid_for(npix)), dim3(256), 0, s, inst, count, c->H, c->W,
+                         c->Yo, c->norm + 10);
+      hipLaunchKernelGGL(batch_energy_kernel<float>, dim3(count * bpi), dim3(256), 0, s, inst, bpi, 1, b->order, c->bpart);
+      hipLaunchKernelGGL(batch_energy_final_kernel, dim3((count + 63) / 64), dim3(64), 0, s, c->bpart, count, bpi, c->be_new);
+      hipLaunchKernelGGL(batch_accept_kernel<float>, dim3(grid_for(count * reg)), dim3(256), 0, s, inst, count, reg,
+                         c->be_old, c->be_new, bound, c->brej);
+    }
+    HIP_OR_FAIL(hipGetLastError());
+  }
+  std::vector<int> rej(count, 0);
+  HIP_OR_FAIL(hipMemcpyAsync(rej.data(), c->brej, sizeof(int) * count, hipMemcpyDeviceToHost, s));
+  HIP_OR_FAIL(hipStreamSynchronize(s));
+  if (rejected) std::memcpy(rejected, rej.data(), sizeof(int) * count);
   return SBP_OK;
 }
 

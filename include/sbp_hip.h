@@ -196,10 +196,15 @@ int  sbp_corr_load_dataset(sbp_corrector *c, const float *raw, const float *targ
 int  sbp_corr_train_step(sbp_corrector *c, const int *idx, int nb, float lr, double *loss);
 int  sbp_corr_synchronize(sbp_corrector *c);
 /* nsteps coarse RK4 steps of ctx, each followed by U += out_scale*net(U);
- * a correction raising the discrete energy above E*(1+bound) is rejected
- * (the plain step is kept); *rejected counts them. */
+ * a step whose corrected energy exceeds the energy before the step times
+ * (1+bound) keeps the plain RK4 result instead; *rejected counts them. */
 int  sbp_corr_advance(sbp_corrector *c, sbp_ctx *ctx, double t0, double dt, long nsteps,
                       double bound, int *rejected);
+
+/* The same over every instance of a batch in one network pass per step;
+ * all instances must have the corrector's grid; rejected[k] optional. */
+int  sbp_corr_advance_batch(sbp_corrector *c, sbp_batch *b, const double *dt, long nsteps,
+                            double bound, int *rejected);
 
 /* ---- diagnostics ---------------------------------------------------------- */
 const char *sbp_last_error(void);
