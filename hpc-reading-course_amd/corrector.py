@@ -10,7 +10,8 @@ fine solution is compared on the coarse nodes by injection.
 A training pair comes from one fine FP64 trajectory: the fine state F_n at a
 coarse step n is injected onto the coarse grid, advanced by one coarse FP64
 RK4 step (C'), and the target is the injected F_{n+1} minus C'.  The network
-sees (C', rho, K) and predicts that residual.
+sees the clamped fourth differences of C' (u, v, p) with rho and K, and
+predicts that residual.
 """
 from __future__ import annotations
 
@@ -42,12 +43,26 @@ def _instances(seeds, n, precision, threads=16):
         return list(ex.map(lambda s: instance(s, n, precision), seeds))
 
 
+def d4(f):
+    """Clamped fourth difference dxxxx + dyyyy of a 2D field (the device
+    packing kernels evaluate the same expression in the same order)."""
+    ny, nx = f.shape
+    ix = np.arange(nx)
+    iy = np.arange(ny)
+    X = lambda k: f[:, np.clip(ix + k, 0, nx - 1)]
+    Y = lambda k: f[np.clip(iy + k, 0, ny - 1), :]
+    dx = X(-2) - 4 * X(-1) + 6 * f - 4 * X(1) + X(2)
+    dy = Y(-2) - 4 * Y(-1) + 6 * f - 4 * Y(1) + Y(2)
+    return dx + dy
+
+
 def raw_features(spec, state_aos):
-    """(u, v, p, rho, K) per point, float32 [ny][nx][5]."""
+    """(d4 u, d4 v, d4 p, rho, K) per point, float32 [ny][nx][5]."""
     q = np.asarray(state_aos, dtype=np.float64).reshape(spec.ny, spec.nx, 3)
     rho = 1.0 / spec.rho_inv if spec.rho_inv is not None else np.ones((spec.ny, spec.nx))
     K = spec.bulk if spec.bulk is not None else np.ones((spec.ny, spec.nx))
-    return np.concatenate([q, rho[..., None], K[..., None]], axis=-1).astype(np.float32)
+    feats = [d4(q[..., c]) for c in range(3)] + [rho, K]
+    return np.stack(feats, axis=-1).astype(np.float32)
 
 
 def make_training_data(n_instances=2000, seed0=100000, t_train=0.25, chunk=64, log=print):

@@ -1,6 +1,7 @@
 // sbp_corr.hip -- learned coarse-grid corrector: a 3-layer 3x3 convolutional
 // network (5 -> 32 -> 32 -> 3 channels, ReLU) applied to the state after each
-// coarse RK4 step.  Forward, backward and inference run on bf16 MFMA
+// coarse RK4 step.  Its inputs at each point are the clamped fourth
+// differences d4 = dxxxx + dyyyy of u, v and p, rho and K.  Forward, backward and inference run on bf16 MFMA
 // (v_mfma_f32_16x16x16_bf16) with FP32 accumulation; master weights and Adam
 // moments are FP32.  No counterpart in the reference: the plain fine-grid
 // solve of the same path is its fallback and judge, and a correction that
@@ -345,6 +346,23 @@ __global__ __launch_bounds__(256) void gather_kernel(const float *__restrict__ r
   for (int c = 0; c < C_OUT; c++) T[p * C_OUT + c] = tgt_all[q * C_OUT + c] / out_scale[c];
 }
 
+// Fourth difference d4 = dxxxx + dyyyy of one planar field at (i, j), indices
+// clamped into the grid, evaluated in double.  The network's state inputs are
+// these high-pass values: the per-step residual it predicts is a combination
+// of high-order differences of the state, far below bf16's resolution of the
+// state itself.
+template <typename T>
+__device__ __forceinline__ double d4_at(const T *__restrict__ f, int nx, int ny, int pitch, int xoff, int i, int j)
+{
+  const long r = (long)j * pitch + xoff;
+  auto X = [&](int ii) { return (double)f[r + min(max(ii, 0), nx - 1)]; };
+  auto Y = [&](int jj) { return (double)f[(long)min(max(jj, 0), ny - 1) * pitch + xoff + i]; };
+  const double c = (double)f[r + i];
+  const double dx = X(i - 2) - 4 * X(i - 1) + 6 * c - 4 * X(i + 1) + X(i + 2);
+  const double dy = Y(j - 2) - 4 * Y(j - 1) + 6 * c - 4 * Y(j + 1) + Y(j + 2);
+  return dx + dy;
+}
+
 // Features of a solver context's resident state.
 template <typename T>
 __global__ __launch_bounds__(256) void pack_state_kernel(const T *__restrict__ U, const T *__restrict__ rinv,
@@ -357,9 +375,7 @@ __global__ __launch_bounds__(256) void pack_state_kernel(const T *__restrict__ U
   const int j = (int)(p / nx), i = (int)(p - (long)j * nx);
   const long o = (long)j * pitch + xoff + i;
   float raw[C_IN];
-  raw[0] = (float)U[o];
-  raw[1] = (float)U[plane + o];
-  raw[2] = (float)U[2 * plane + o];
+  for (int c = 0; c < 3; c++) raw[c] = (float)d4_at(U + c * plane, nx, ny, pitch, xoff, i, j);
   raw[3] = rinv ? (float)(1.0 / (double)rinv[o]) : 1.f;
   raw[4] = bulk ? (float)bulk[o] : 1.f;
 #pragma unroll
@@ -407,9 +423,7 @@ __global__ __launch_bounds__(256) void pack_batch_kernel(const BatchInst<T> *__r
   const BatchInst<T> &d = inst[k];
   const long o = (long)j * d.pitch + d.xoff + i;
   float raw[C_IN];
-  raw[0] = (float)d.U[o];
-  raw[1] = (float)d.U[d.plane + o];
-  raw[2] = (float)d.U[2 * d.plane + o];
+  for (int c = 0; c < 3; c++) raw[c] = (float)d4_at(d.U + c * d.plane, d.nx, d.ny, d.pitch, d.xoff, i, j);
   raw[3] = d.rinv ? (float)(1.0 / (double)d.rinv[o]) : 1.f;
   raw[4] = d.bulk ? (float)d.bulk[o] : 1.f;
 #pragma unroll

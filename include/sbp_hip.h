@@ -172,7 +172,8 @@ int  sbp_batch_autotune(sbp_batch *b, const double *dt, int *choice);
 
 /* ---- learned coarse-grid corrector ----------------------------------------
  * A 3-layer 3x3 convolutional network (5 -> 32 -> 32 -> 3 channels, ReLU;
- * inputs u, v, p, rho, K; output the correction of u, v, p) applied after
+ * inputs at each point: the clamped fourth differences dxxxx + dyyyy of u, v
+ * and p, rho and K; output the correction of u, v, p) applied after
  * each coarse RK4 step of a 2D acoustic context; bf16 MFMA forward, backward
  * and inference with FP32 accumulation, FP32 master weights and Adam.  No
  * counterpart in the reference.  Parameters are a flat FP32 vector:
@@ -187,7 +188,7 @@ int  sbp_corr_set_params(sbp_corrector *c, const float *params, int n); /* also 
 int  sbp_corr_get_params(sbp_corrector *c, float *params, int n);
 int  sbp_corr_set_normalization(sbp_corrector *c, const float *in_shift, const float *in_scale,
                                 const float *out_scale);
-/* raw [nb][ny][nx][5] -> out [nb][ny][nx][3] (normalised units). */
+/* raw inputs [nb][ny][nx][5] -> out [nb][ny][nx][3] (normalised units). */
 int  sbp_corr_forward(sbp_corrector *c, const float *raw, int nb, float *out);
 /* MSE loss of (net - target/out_scale) and its parameter gradients. */
 int  sbp_corr_gradients(sbp_corrector *c, const float *raw, const float *target, int nb,
