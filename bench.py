@@ -10,8 +10,9 @@ state resident in HBM.
 Prints one JSON line:
   value       DOF-updates/s (nx*ny*3 values advanced one RK4 step per update)
   roofline    HBM roofline of the four stage kernels: algorithmic bytes per
-              step (16 words per DOF: 9 reads + 7 writes) over the summed mean
-              stage-kernel durations (HIP events on the library's stream)
+              step (13 words per DOF: 9 reads + 4 writes over the four stages)
+              over the summed mean stage-kernel durations (HIP events on the
+              library's stream)
   cpu_baseline the oracle compiled with the reference's optimisation flags,
               pthreads over a near-square block decomposition, timed on a
               bounded number of steps of the same workload on this host
@@ -29,6 +30,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
+# HBM bytes per DOF-update of the four FP64 stage kernels of the default
+# workload (4097^2, order 4, rho = K = 1), from rocprofv3 FETCH_SIZE x 2.0 +
+# WRITE_SIZE x 1.0 (factors from the 1 GiB copy kernel in the same runs):
+# profiles/stage2d_acoustic_4097sq_order4_fp64_hbm_counters.md.
+MEASURED_TRAFFIC_PER_DOF = {(4097, 4, 0): 107.41}
 
 
 def pkg(sub):
@@ -123,6 +129,17 @@ def batched(count=256, steps=500, warmup=10, prec=1):
             "setup_s": setup_s, "energy_head": e_end}
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown CPU"
+
+
 def cpu_baseline(spec, v0, dt, budget_s=12.0):
     from oracle import pyoracle
     threads = min(16, os.cpu_count() or 1)
@@ -136,8 +153,8 @@ def cpu_baseline(spec, v0, dt, budget_s=12.0):
     dofs = spec.nx * spec.ny * spec.components
     return {"value": dofs * n / el, "unit": "DOF-updates/s", "cores": threads, "kind": "port",
             "sample": "%d RK4 steps of the same %dx%d order-%d workload (%.1f s), oracle built with "
-                      "-O3 -ffast-math, %d pthreads, near-square block decomposition"
-                      % (n, spec.nx, spec.ny, spec.order, el, threads)}
+                      "-O3 -ffast-math, %d pthreads, near-square block decomposition, on %s (%d logical CPUs)"
+                      % (n, spec.nx, spec.ny, spec.order, el, threads, cpu_model(), os.cpu_count() or 0)}
 
 
 def main():
@@ -200,6 +217,9 @@ def main():
         cpu = cpu_baseline(spec, v0, dt)
         log("cpu:", cpu)
 
+    traffic_dof = MEASURED_TRAFFIC_PER_DOF.get((spec.nx, spec.order, spec.precision))
+    dofs = spec.nx * spec.ny * spec.components
+    traffic = traffic_dof * dofs / 4 if traffic_dof else None  # bytes per stage launch
     out = {
         "metric": "DOF-updates/s & % HBM roofline; time to L2 err 1e-4: chosen vs plain FP64",
         "value": main_m["dof_updates_per_s"],
@@ -218,7 +238,9 @@ def main():
                    "cfl": 0.25, "bc": "wall SAT", "material": "rho=K=1",
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": main_m["achieved_gbs"], "peak": PEAK_HBM_GBS,
-                     "unit": "GB/s", "frac": main_m["frac"], "traffic": None,
+                     "unit": "GB/s", "frac": main_m["frac"], "traffic": traffic,
+                     "traffic_unit": "bytes per stage launch (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE)",
+                     "algorithmic_bytes_per_launch": main_m["bytes_per_dof"] * dofs / 4,
                      "kernel": "stage2d_kernel x4 (one fused RK4 step)",
                      "bytes_per_dof": main_m["bytes_per_dof"],
                      "bytes_per_dof_3reg_scheme": main_m["bytes_per_dof_3reg"],
