@@ -10,9 +10,11 @@ state resident in HBM.
 Prints one JSON line:
   value       DOF-updates/s (nx*ny*3 values advanced one RK4 step per update)
   roofline    HBM roofline of the four stage kernels: algorithmic bytes per
-              step (13 words per DOF: 9 reads + 4 writes over the four stages)
-              over the summed mean stage-kernel durations (HIP events on the
-              library's stream)
+              DOF-update of a fused 3-register RK4 (16 words: 9 reads + 7
+              writes, 128 B in FP64) over the summed mean stage-kernel
+              durations (HIP events on the library's stream); the kernels as
+              built need 13 words (kernel_bytes_per_dof, kernel_frac) and the
+              measured HBM traffic is `traffic`
   cpu_baseline the oracle compiled with the reference's optimisation flags,
               pthreads over a near-square block decomposition, timed on a
               bounded number of steps of the same workload on this host
@@ -47,20 +49,22 @@ def log(*a):
 
 
 def bytes_per_dof(precision, materials=False, C=3):
-    """Algorithmic HBM bytes per DOF-update of the stage kernels as built:
-    9 reads + 4 writes per value and step (U, Y2, Y3, Y4 registers); the Kahan
-    form adds the compensation read and write (15 words)."""
+    """Algorithmic HBM bytes per DOF-update of a fused classical RK4 with
+    three registers (U, Y, accumulator; one kernel per stage): 9 reads + 7
+    writes = 16 words; 19 words (11 + 8) with a Kahan compensation; stored
+    rho^-1 and K add 4 stages x 2 words / C.  This is the roofline basis of
+    `achieved` and `frac`."""
     w = 8 if precision == 0 else 4
-    words = 13 if precision != 2 else 15
-    extra = (4 * 2 / C) if materials else 0.0  # rho^-1 and K read in each of 4 stages
+    words = 16 if precision != 2 else 19
+    extra = (4 * 2 / C) if materials else 0.0
     return (words + extra) * w
 
 
-def bytes_per_dof_3reg(precision, materials=False, C=3):
-    """The 16-word accounting of a fused 3-register RK4 (U, Y, accumulator):
-    9 reads + 7 writes; 19 words with Kahan (11 reads + 8 writes)."""
+def kernel_bytes_per_dof(precision, materials=False, C=3):
+    """Bytes per DOF-update the stage kernels as built need: the 13-word
+    scheme (U, Y2, Y3, Y4: 9 reads + 4 writes; 15 words with Kahan)."""
     w = 8 if precision == 0 else 4
-    words = 16 if precision != 2 else 19
+    words = 13 if precision != 2 else 15
     extra = (4 * 2 / C) if materials else 0.0
     return (words + extra) * w
 
@@ -86,16 +90,17 @@ def gpu_run(spec, v0, dt, steps, warmup, stage_steps=None):
 def measure(spec, v0, dt, steps, warmup):
     wall, ev_ms, stage_ms, energy = gpu_run(spec, v0, dt, steps, warmup)
     dofs = spec.nx * spec.ny * spec.components
-    bpd = bytes_per_dof(spec.precision, spec.material == 2, spec.components)
+    mat = spec.material == 2
+    bpd = bytes_per_dof(spec.precision, mat, spec.components)
+    kbpd = kernel_bytes_per_dof(spec.precision, mat, spec.components)
     step_kernel_s = float(np.sum(stage_ms)) * 1e-3
     achieved = dofs * bpd / step_kernel_s / 1e9
-    bpd16 = bytes_per_dof_3reg(spec.precision, spec.material == 2, spec.components)
-    eq16 = dofs * bpd16 / step_kernel_s / 1e9
+    k_achieved = dofs * kbpd / step_kernel_s / 1e9
     return dict(
         wall_s=wall, event_ms=ev_ms, ms_per_step=wall / steps * 1e3,
         dof_updates_per_s=dofs * steps / wall, stage_ms=[float(x) for x in stage_ms],
         bytes_per_dof=bpd, achieved_gbs=achieved, frac=achieved / PEAK_HBM_GBS, energy=energy,
-        bytes_per_dof_3reg=bpd16, achieved_3reg_equiv_gbs=eq16, frac_3reg_equiv=eq16 / PEAK_HBM_GBS)
+        kernel_bytes_per_dof=kbpd, kernel_achieved_gbs=k_achieved, kernel_frac=k_achieved / PEAK_HBM_GBS)
 
 
 def batched(count=256, steps=500, warmup=10, prec=1):
@@ -122,11 +127,47 @@ def batched(count=256, steps=500, warmup=10, prec=1):
     dofs = sum(sp.nx * sp.ny * 3 for sp in specs)
     bpd = bytes_per_dof(prec, materials=True)
     achieved = dofs * bpd * steps / (ev_ms * 1e-3) / 1e9
+    kbpd = kernel_bytes_per_dof(prec, materials=True)
     return {"instances": count, "dofs": dofs, "steps": steps, "precision": ["fp64", "fp32", "fp32_kahan"][prec],
             "dof_updates_per_s": dofs * steps / wall, "ms_per_step": wall / steps * 1e3,
             "event_ms_per_step": ev_ms / steps, "bytes_per_dof": bpd, "achieved_gbs": achieved,
-            "frac": achieved / PEAK_HBM_GBS, "autotune_choice": choice, "config": cfg,
+            "frac": achieved / PEAK_HBM_GBS, "kernel_bytes_per_dof": kbpd,
+            "kernel_frac": achieved * kbpd / bpd / PEAK_HBM_GBS, "autotune_choice": choice, "config": cfg,
             "setup_s": setup_s, "energy_head": e_end}
+
+
+def reflection_2d(steps=2000, warmup=0):
+    """Variable-material 2D acoustics (rho, K from 8 seeded Gaussians in
+    [0.5, 2]), 4097^2, order 6, wall SATs, 2000 RK4 steps: the FP32 and
+    FP32+Kahan forms against the FP64 path (which matches the CPU oracle to
+    1e-12 in the parity tests), error stated after the full run."""
+    P = pkg("problems")
+    capi = pkg("capi")
+    out, finals = {}, {}
+    for prec, name in ((0, "fp64"), (1, "fp32"), (2, "fp32_kahan")):
+        spec, v0, dt = P.baseline_reflection(precision=prec)
+        with capi.Context(spec) as ctx:
+            ctx.set_state(v0)
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            ctx.time_steps(0.0, dt, steps)
+            wall = time.perf_counter() - t0
+            finals[name] = ctx.get_state()
+            stage_ms = ctx.time_stages(dt, 5)
+        dofs = spec.nx * spec.ny * 3
+        bpd = bytes_per_dof(prec, True)
+        out[name] = {"dof_updates_per_s": dofs * steps / wall, "ms_per_step": wall / steps * 1e3,
+                     "bytes_per_dof": bpd, "frac": dofs * bpd / (float(np.sum(stage_ms)) * 1e-3) / 1e9 / PEAK_HBM_GBS}
+        del v0
+    ref = finals["fp64"]
+    scale = float(np.max(np.abs(ref)))
+    h = 1.0 / (4097 - 1)
+    for name in ("fp32", "fp32_kahan"):
+        d = finals[name] - ref
+        out[name]["max_err_rel_vs_fp64"] = float(np.max(np.abs(d)) / scale)
+        out[name]["l2_err_vs_fp64"] = float(h * np.linalg.norm(d))
+    out["steps"] = steps
+    return out
 
 
 def cpu_model():
@@ -197,7 +238,8 @@ def main():
         m2 = measure(s2, v2, dt2, 1000, 50)
         secondary["acoustic2d_wall_1025sq_order4_fp64"] = {
             "dof_updates_per_s": m2["dof_updates_per_s"], "ms_per_step": m2["ms_per_step"],
-            "achieved_gbs": m2["achieved_gbs"], "stage_ms": m2["stage_ms"]}
+            "achieved_gbs": m2["achieved_gbs"], "frac": m2["frac"], "kernel_frac": m2["kernel_frac"],
+            "stage_ms": m2["stage_ms"]}
         log("1025^2:", secondary["acoustic2d_wall_1025sq_order4_fp64"])
         for prec, name in ((1, "fp32"), (2, "fp32_kahan")):
             s3, v3, dt3 = P.baseline_acoustic(args.n, args.order, precision=prec)
@@ -205,11 +247,15 @@ def main():
             secondary["acoustic2d_wall_%dsq_order%d_%s" % (args.n, args.order, name)] = {
                 "dof_updates_per_s": m3["dof_updates_per_s"], "ms_per_step": m3["ms_per_step"],
                 "bytes_per_dof": m3["bytes_per_dof"], "achieved_gbs": m3["achieved_gbs"],
-                "frac": m3["frac"], "stage_ms": m3["stage_ms"]}
+                "frac": m3["frac"], "kernel_bytes_per_dof": m3["kernel_bytes_per_dof"],
+                "kernel_frac": m3["kernel_frac"], "stage_ms": m3["stage_ms"]}
             log(name, secondary["acoustic2d_wall_%dsq_order%d_%s" % (args.n, args.order, name)])
         bm = batched()
         secondary["batched_256x_N129-1025_order4_fp32"] = bm
         log("batched:", bm)
+        rf = reflection_2d()
+        secondary["reflection2d_4097sq_order6_materials_2000steps"] = rf
+        log("reflection 2D:", rf)
 
     cpu = None
     if not args.no_cpu:
@@ -239,12 +285,13 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": main_m["achieved_gbs"], "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": main_m["frac"], "traffic": traffic,
-                     "traffic_unit": "bytes per stage launch (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE)",
-                     "algorithmic_bytes_per_launch": main_m["bytes_per_dof"] * dofs / 4,
-                     "kernel": "stage2d_kernel x4 (one fused RK4 step)",
+                     "kernel": "stage2d_kernel x4 (one fused RK4 step), mean over the four stages",
                      "bytes_per_dof": main_m["bytes_per_dof"],
-                     "bytes_per_dof_3reg_scheme": main_m["bytes_per_dof_3reg"],
-                     "frac_3reg_equiv": main_m["frac_3reg_equiv"],
+                     "algorithmic_bytes_per_launch": main_m["bytes_per_dof"] * dofs / 4,
+                     "traffic_unit": "HBM bytes per stage launch (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE)",
+                     "kernel_bytes_per_dof": main_m["kernel_bytes_per_dof"],
+                     "kernel_achieved": main_m["kernel_achieved_gbs"],
+                     "kernel_frac": main_m["kernel_frac"],
                      "stage_ms": main_m["stage_ms"],
                      "copy_ceiling_gbs": copy_gbs},
         "cpu_baseline": cpu,

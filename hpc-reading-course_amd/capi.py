@@ -23,6 +23,7 @@ from . import problems as P
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libsbp_hip.so")
+CORR_INPUTS, CORR_OUTPUTS = 8, 3  # SBP_CORR_INPUTS, SBP_CORR_OUTPUTS of sbp_hip.h
 
 _DP = C.POINTER(C.c_double)
 _FP = C.POINTER(C.c_float)
@@ -415,13 +416,13 @@ class Corrector:
         _check(load_library().sbp_corr_set_params(self._h, _fptr(x), self.n_params))
 
     def set_normalization(self, in_shift, in_scale, out_scale):
-        a, b, c = _f32(in_shift, (5,)), _f32(in_scale, (5,)), _f32(out_scale, (3,))
+        a, b, c = _f32(in_shift, (CORR_INPUTS,)), _f32(in_scale, (CORR_INPUTS,)), _f32(out_scale, (3,))
         _check(load_library().sbp_corr_set_normalization(self._h, _fptr(a), _fptr(b), _fptr(c)))
 
     def forward(self, raw) -> np.ndarray:
         raw = _f32(raw)
         nb = raw.shape[0]
-        _f32(raw, (nb, self.ny, self.nx, 5))
+        _f32(raw, (nb, self.ny, self.nx, CORR_INPUTS))
         out = np.empty((nb, self.ny, self.nx, 3), dtype=np.float32)
         _check(load_library().sbp_corr_forward(self._h, _fptr(raw), nb, _fptr(out)))
         return out
@@ -429,7 +430,7 @@ class Corrector:
     def gradients(self, raw, target):
         raw = _f32(raw)
         nb = raw.shape[0]
-        _f32(raw, (nb, self.ny, self.nx, 5))
+        _f32(raw, (nb, self.ny, self.nx, CORR_INPUTS))
         tgt = _f32(target, (nb, self.ny, self.nx, 3))
         g = np.empty(self.n_params, dtype=np.float32)
         loss = C.c_double()
@@ -439,7 +440,7 @@ class Corrector:
     def load_dataset(self, raw, target):
         raw = _f32(raw)
         n = raw.shape[0]
-        _f32(raw, (n, self.ny, self.nx, 5))
+        _f32(raw, (n, self.ny, self.nx, CORR_INPUTS))
         tgt = _f32(target, (n, self.ny, self.nx, 3))
         _check(load_library().sbp_corr_load_dataset(self._h, _fptr(raw), _fptr(tgt), n))
 

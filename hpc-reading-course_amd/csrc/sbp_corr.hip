@@ -1,11 +1,13 @@
 // sbp_corr.hip -- learned coarse-grid corrector: a 3-layer 3x3 convolutional
 // network (5 -> 32 -> 32 -> 3 channels, ReLU) applied to the state after each
 // coarse RK4 step.  Its inputs at each point are the clamped fourth
-// differences d4 = dxxxx + dyyyy of u, v and p, rho and K.  Forward, backward and inference run on bf16 MFMA
+// differences d4 = dxxxx + dyyyy of u, v and p, rho and K.  Forward, backward
+// and inference run on bf16 MFMA
 // (v_mfma_f32_16x16x16_bf16) with FP32 accumulation; master weights and Adam
 // moments are FP32.  No counterpart in the reference: the plain fine-grid
 // solve of the same path is its fallback and judge, and a correction that
-// increases the discrete energy by more than a set bound is rejected.
+// lifts the discrete energy above the energy the solve started from by more
+// than a set bound is rejected.
 //
 // Activations are channels-last [image][y][x][channel] in bf16, channels
 // padded to a multiple of 4 (inputs: 5 -> 8).  A convolution is a GEMM with
@@ -39,7 +41,9 @@ using f32x4 = __attribute__((ext_vector_type(4))) float;
 typedef unsigned short bf16;
 
 // Network shape.
-constexpr int C_IN = 5, C_INP = 8, C_H = 32, C_OUT = 3, C_OUTP = 16;
+constexpr int C_IN = 8, C_INP = 8, C_H = 32, C_OUT = 3, C_OUTP = 16;
+// normalisation block: shift[C_IN], scale[C_IN], out_scale[C_OUT]
+constexpr int NORM_OUT = 2 * C_IN, NORM_LEN = 2 * C_IN + C_OUT;
 constexpr int KP1 = 80;   // 9*8 = 72 padded to 80
 constexpr int KP2 = 288;  // 9*32
 constexpr int KP3 = 288;
@@ -318,7 +322,7 @@ __global__ __launch_bounds__(256) void prepare_weights_kernel(const float *__res
   if (i < C_OUTP) b3[i] = i < C_OUT ? p[OFF_B3 + i] : 0.f;
 }
 
-// Raw per-point quantities (u, v, p, rho, K) -> normalised bf16 features.
+// Raw per-point inputs (C_IN floats) -> normalised bf16 features.
 __global__ __launch_bounds__(256) void pack_raw_kernel(const float *__restrict__ raw, long npix,
                                                        const float *__restrict__ shift_scale, bf16 *__restrict__ X)
 {
@@ -346,21 +350,60 @@ __global__ __launch_bounds__(256) void gather_kernel(const float *__restrict__ r
   for (int c = 0; c < C_OUT; c++) T[p * C_OUT + c] = tgt_all[q * C_OUT + c] / out_scale[c];
 }
 
-// Fourth difference d4 = dxxxx + dyyyy of one planar field at (i, j), indices
-// clamped into the grid, evaluated in double.  The network's state inputs are
-// these high-pass values: the per-step residual it predicts is a combination
-// of high-order differences of the state, far below bf16's resolution of the
-// state itself.
+// Network inputs at grid point (i, j) of a planar acoustic state (u, v, p),
+// evaluated in double with every index clamped into the grid:
+//   0  rho^-1 * Dx(d4x p)          Dx f = (f[i+1] - f[i-1]) / 2,
+//   1  rho^-1 * Dy(d4y p)          d4x f = f[i-2] - 4f[i-1] + 6f[i] - 4f[i+1] + f[i+2]
+//   2  K * (Dx(d4x u) + Dy(d4y v))
+//   3..5  d4x + d4y of u, v, p
+//   6, 7  rho, K
+// Channels 0..2 are, up to a constant factor, the leading truncation error
+// h^4/30 * d^5/dx^5 of the fourth-order central stencil in each equation of
+// the coarse step, so the per-step residual the network predicts is close to
+// a linear function of its inputs; 3..7 carry what the next terms and the
+// boundary closures need.  All of them are high-pass quantities: the residual
+// (about 1e-6 of the state) is far below bf16's resolution of the state.
 template <typename T>
-__device__ __forceinline__ double d4_at(const T *__restrict__ f, int nx, int ny, int pitch, int xoff, int i, int j)
+struct FieldView {
+  const T *f;
+  int nx, ny, pitch, xoff;
+  __device__ __forceinline__ double at(int i, int j) const
+  {
+    i = min(max(i, 0), nx - 1);
+    j = min(max(j, 0), ny - 1);
+    return (double)f[(long)j * pitch + xoff + i];
+  }
+  __device__ __forceinline__ double d4x(int i, int j) const
+  {
+    i = min(max(i, 0), nx - 1);
+    return at(i - 2, j) - 4 * at(i - 1, j) + 6 * at(i, j) - 4 * at(i + 1, j) + at(i + 2, j);
+  }
+  __device__ __forceinline__ double d4y(int i, int j) const
+  {
+    j = min(max(j, 0), ny - 1);
+    return at(i, j - 2) - 4 * at(i, j - 1) + 6 * at(i, j) - 4 * at(i, j + 1) + at(i, j + 2);
+  }
+  __device__ __forceinline__ double dx_d4x(int i, int j) const { return 0.5 * (d4x(i + 1, j) - d4x(i - 1, j)); }
+  __device__ __forceinline__ double dy_d4y(int i, int j) const { return 0.5 * (d4y(i, j + 1) - d4y(i, j - 1)); }
+};
+
+template <typename T>
+__device__ __forceinline__ void state_features(const T *__restrict__ U, const T *__restrict__ rinv,
+                                               const T *__restrict__ bulk, int nx, int ny, int pitch, int xoff,
+                                               long plane, int i, int j, float raw[C_IN])
 {
-  const long r = (long)j * pitch + xoff;
-  auto X = [&](int ii) { return (double)f[r + min(max(ii, 0), nx - 1)]; };
-  auto Y = [&](int jj) { return (double)f[(long)min(max(jj, 0), ny - 1) * pitch + xoff + i]; };
-  const double c = (double)f[r + i];
-  const double dx = X(i - 2) - 4 * X(i - 1) + 6 * c - 4 * X(i + 1) + X(i + 2);
-  const double dy = Y(j - 2) - 4 * Y(j - 1) + 6 * c - 4 * Y(j + 1) + Y(j + 2);
-  return dx + dy;
+  const FieldView<T> u{U, nx, ny, pitch, xoff}, v{U + plane, nx, ny, pitch, xoff}, p{U + 2 * plane, nx, ny, pitch, xoff};
+  const long o = (long)j * pitch + xoff + i;
+  const double ri = rinv ? (double)rinv[o] : 1.0;
+  const double K = bulk ? (double)bulk[o] : 1.0;
+  raw[0] = (float)(ri * p.dx_d4x(i, j));
+  raw[1] = (float)(ri * p.dy_d4y(i, j));
+  raw[2] = (float)(K * (u.dx_d4x(i, j) + v.dy_d4y(i, j)));
+  raw[3] = (float)(u.d4x(i, j) + u.d4y(i, j));
+  raw[4] = (float)(v.d4x(i, j) + v.d4y(i, j));
+  raw[5] = (float)(p.d4x(i, j) + p.d4y(i, j));
+  raw[6] = (float)(1.0 / ri);
+  raw[7] = (float)K;
 }
 
 // Features of a solver context's resident state.
@@ -373,11 +416,8 @@ __global__ __launch_bounds__(256) void pack_state_kernel(const T *__restrict__ U
   const long p = blockIdx.x * 256L + threadIdx.x;
   if (p >= (long)nx * ny) return;
   const int j = (int)(p / nx), i = (int)(p - (long)j * nx);
-  const long o = (long)j * pitch + xoff + i;
   float raw[C_IN];
-  for (int c = 0; c < 3; c++) raw[c] = (float)d4_at(U + c * plane, nx, ny, pitch, xoff, i, j);
-  raw[3] = rinv ? (float)(1.0 / (double)rinv[o]) : 1.f;
-  raw[4] = bulk ? (float)bulk[o] : 1.f;
+  state_features(U, rinv, bulk, nx, ny, pitch, xoff, plane, i, j, raw);
 #pragma unroll
   for (int c = 0; c < C_INP; c++)
     X[p * C_INP + c] = c < C_IN ? f2bf((raw[c] - shift_scale[c]) * shift_scale[C_IN + c]) : (bf16)0;
@@ -421,11 +461,8 @@ __global__ __launch_bounds__(256) void pack_batch_kernel(const BatchInst<T> *__r
   const long q = p - k * npi;
   const int j = (int)(q / W), i = (int)(q - (long)j * W);
   const BatchInst<T> &d = inst[k];
-  const long o = (long)j * d.pitch + d.xoff + i;
   float raw[C_IN];
-  for (int c = 0; c < 3; c++) raw[c] = (float)d4_at(d.U + c * d.plane, d.nx, d.ny, d.pitch, d.xoff, i, j);
-  raw[3] = d.rinv ? (float)(1.0 / (double)d.rinv[o]) : 1.f;
-  raw[4] = d.bulk ? (float)d.bulk[o] : 1.f;
+  state_features(d.U, d.rinv, d.bulk, d.nx, d.ny, d.pitch, d.xoff, d.plane, i, j, raw);
 #pragma unroll
   for (int c = 0; c < C_INP; c++)
     X[p * C_INP + c] = c < C_IN ? f2bf((raw[c] - shift_scale[c]) * shift_scale[C_IN + c]) : (bf16)0;
@@ -524,7 +561,7 @@ struct sbp_corrector {
   float *params = nullptr, *grads = nullptr, *m = nullptr, *v = nullptr;
   bf16 *w1 = nullptr, *w2 = nullptr, *w3 = nullptr, *w2t = nullptr, *w3t = nullptr;
   float *b1 = nullptr, *b2 = nullptr, *b3 = nullptr;
-  float *norm = nullptr;  // [0..4] shift, [5..9] scale, [10..12] output scale
+  float *norm = nullptr;  // NORM_LEN floats: input shift, input scale, output scale
   bf16 *X0 = nullptr, *A1 = nullptr, *A2 = nullptr, *dY3 = nullptr, *dP2 = nullptr, *dP1 = nullptr;
   float *Yo = nullptr, *Tg = nullptr, *dA = nullptr, *wpart = nullptr;
   double *lpart = nullptr, *scalars = nullptr;  // scalars: [0] loss, [1] E_old
@@ -672,7 +709,7 @@ int sbp_corr_create(int nx, int ny, int batch, sbp_corrector **out)
   if (!rc) rc = dmalloc(&c->b1, sizeof(float) * C_H);
   if (!rc) rc = dmalloc(&c->b2, sizeof(float) * C_H);
   if (!rc) rc = dmalloc(&c->b3, sizeof(float) * C_OUTP);
-  if (!rc) rc = dmalloc(&c->norm, sizeof(float) * 16);
+  if (!rc) rc = dmalloc(&c->norm, sizeof(float) * NORM_LEN);
   if (!rc) rc = dmalloc(&c->X0, sizeof(bf16) * npix * C_INP);
   if (!rc) rc = dmalloc(&c->A1, sizeof(bf16) * npix * C_H);
   if (!rc) rc = dmalloc(&c->A2, sizeof(bf16) * npix * C_H);
@@ -694,13 +731,13 @@ int sbp_corr_create(int nx, int ny, int batch, sbp_corrector **out)
   if (!rc) {
     // Default: zero network (identity correction), unit normalisation.
     std::vector<float> zero(N_PARAMS, 0.f);
-    std::vector<float> nrm(16, 0.f);
+    std::vector<float> nrm(NORM_LEN, 0.f);
     for (int k = 0; k < C_IN; k++) nrm[C_IN + k] = 1.f;
-    for (int k = 0; k < C_OUT; k++) nrm[10 + k] = 1.f;
+    for (int k = 0; k < C_OUT; k++) nrm[NORM_OUT + k] = 1.f;
     if (hipMemcpy(c->params, zero.data(), sizeof(float) * N_PARAMS, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(c->m, zero.data(), sizeof(float) * N_PARAMS, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(c->v, zero.data(), sizeof(float) * N_PARAMS, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(c->norm, nrm.data(), sizeof(float) * 16, hipMemcpyHostToDevice) != hipSuccess)
+        hipMemcpy(c->norm, nrm.data(), sizeof(float) * NORM_LEN, hipMemcpyHostToDevice) != hipSuccess)
       rc = fail(SBP_EHIP, "corrector initialisation failed");
   }
   if (!rc) rc = prepare(c);
@@ -746,18 +783,18 @@ int sbp_corr_get_params(sbp_corrector *c, float *params, int n)
 int sbp_corr_set_normalization(sbp_corrector *c, const float *in_shift, const float *in_scale, const float *out_scale)
 {
   if (!c || !in_shift || !in_scale || !out_scale) return fail(SBP_EINVAL, "null argument");
-  float nrm[16] = {0};
+  float nrm[NORM_LEN] = {0};
   for (int k = 0; k < C_IN; k++) {
     nrm[k] = in_shift[k];
     nrm[C_IN + k] = in_scale[k];
   }
-  for (int k = 0; k < C_OUT; k++) nrm[10 + k] = out_scale[k];
+  for (int k = 0; k < C_OUT; k++) nrm[NORM_OUT + k] = out_scale[k];
   HIP_OR_FAIL(hipMemcpyAsync(c->norm, nrm, sizeof(nrm), hipMemcpyHostToDevice, c->stream));
   HIP_OR_FAIL(hipStreamSynchronize(c->stream));
   return SBP_OK;
 }
 
-// Network output for nb images of raw inputs [nb][ny][nx][5] (u, v, p, rho, K):
+// Network output for nb images of raw inputs [nb][ny][nx][C_IN]:
 // out [nb][ny][nx][3] in normalised units (multiply by out_scale for the correction).
 int sbp_corr_forward(sbp_corrector *c, const float *raw, int nb, float *out)
 {
@@ -806,7 +843,7 @@ int sbp_corr_gradients(sbp_corrector *c, const float *raw, const float *target, 
   (void)didx;
   if (!rc) {
     hipLaunchKernelGGL(gather_kernel, dim3(grid_for(npix)), dim3(256), 0, c->stream, draw, dtgt, c->idx, nb,
-                       (long)c->H * c->W, c->norm, c->norm + 10, c->X0, c->Tg);
+                       (long)c->H * c->W, c->norm, c->norm + NORM_OUT, c->X0, c->Tg);
     rc = forward(c, nb);
   }
   if (!rc) rc = backward(c, nb);
@@ -852,7 +889,7 @@ int sbp_corr_train_step(sbp_corrector *c, const int *idx, int nb, float lr, doub
   const long npix = (long)nb * c->H * c->W;
   HIP_OR_FAIL(hipMemcpyAsync(c->idx, idx, sizeof(int) * nb, hipMemcpyHostToDevice, c->stream));
   hipLaunchKernelGGL(gather_kernel, dim3(grid_for(npix)), dim3(256), 0, c->stream, c->ds_raw, c->ds_tgt, c->idx, nb,
-                     (long)c->H * c->W, c->norm, c->norm + 10, c->X0, c->Tg);
+                     (long)c->H * c->W, c->norm, c->norm + NORM_OUT, c->X0, c->Tg);
   if ((rc = forward(c, nb)) || (rc = backward(c, nb))) return rc;
   c->adam_t++;
   const float b1 = 0.9f, b2 = 0.999f;
@@ -876,9 +913,9 @@ int sbp_corr_synchronize(sbp_corrector *c)
 }
 
 // Correct the resident state of a 2D acoustic context in place after a step:
-// U <- U + out_scale * net(U, rho, K), unless the corrected state's discrete
-// energy exceeds the energy before the step times (1 + bound), in which case
-// the plain step is kept.
+// U <- U + out_scale * net(features of U, rho, K), unless the corrected
+// state's discrete energy exceeds the reference energy (c->scalars[1]) times
+// (1 + bound), in which case the plain step is kept.
 static int apply_one(sbp_corrector *c, sbp_ctx *x, double bound)
 {
   const long npix = (long)x->nx * x->ny;
@@ -899,13 +936,13 @@ static int apply_one(sbp_corrector *c, sbp_ctx *x, double bound)
   c->stream = keep;
   if (rc) return rc;
   // candidate into the Y0 scratch register; its energy lands in x->scalar and
-  // is compared with the energy before the step (c->scalars[1])
+  // is compared with the reference energy (c->scalars[1])
   if (x->esz == 8)
     hipLaunchKernelGGL(candidate_kernel<double>, dim3(grid_for(npix)), dim3(256), 0, x->stream, (const double *)x->U,
-                       c->Yo, x->nx, x->ny, x->pitch, x->xoff, x->plane, c->norm + 10, (double *)x->Y0);
+                       c->Yo, x->nx, x->ny, x->pitch, x->xoff, x->plane, c->norm + NORM_OUT, (double *)x->Y0);
   else
     hipLaunchKernelGGL(candidate_kernel<float>, dim3(grid_for(npix)), dim3(256), 0, x->stream, (const float *)x->U,
-                       c->Yo, x->nx, x->ny, x->pitch, x->xoff, x->plane, c->norm + 10, (float *)x->Y0);
+                       c->Yo, x->nx, x->ny, x->pitch, x->xoff, x->plane, c->norm + NORM_OUT, (float *)x->Y0);
   HIP_OR_FAIL(hipGetLastError());
   if ((rc = sbp::host::energy_of(x, x->Y0, x->KC, nullptr))) return rc;
   const long n = 3 * x->plane;
@@ -920,6 +957,10 @@ static int apply_one(sbp_corrector *c, sbp_ctx *x, double bound)
 }
 
 // nsteps coarse RK4 steps of ctx, each followed by the guarded correction.
+// The reference energy is the state's energy when the call starts: with
+// energy-stable boundary conditions and no forcing the exact solution keeps
+// it, so a correction that lifts the energy above it by more than the
+// relative bound is growth the scheme cannot have, and is refused.
 // *rejected receives the number of corrections refused by the energy bound.
 int sbp_corr_advance(sbp_corrector *c, sbp_ctx *x, double t0, double dt, long nsteps, double bound, int *rejected)
 {
@@ -930,12 +971,12 @@ int sbp_corr_advance(sbp_corrector *c, sbp_ctx *x, double t0, double dt, long ns
   if (!x->have_state) return fail(SBP_ESTATE, "no state loaded");
   if (!(bound >= 0)) return fail(SBP_EINVAL, "energy bound must be >= 0");
   hipLaunchKernelGGL(clear_int_kernel, dim3(1), dim3(1), 0, x->stream, c->rejected);
+  // reference energy: the state's energy when the call starts -> c->scalars[1]
+  int rc = sbp::host::energy_of(x, x->U, x->KC, nullptr);
+  if (rc) return rc;
+  hipLaunchKernelGGL(copy_scalar_kernel, dim3(1), dim3(1), 0, x->stream, x->scalar, c->scalars + 1);
   double t = t0;
   for (long s = 0; s < nsteps; s++) {
-    // energy before the step -> c->scalars[1]
-    int rc = sbp::host::energy_of(x, x->U, x->KC, nullptr);
-    if (rc) return rc;
-    hipLaunchKernelGGL(copy_scalar_kernel, dim3(1), dim3(1), 0, x->stream, x->scalar, c->scalars + 1);
     HIP_OR_FAIL(sbp::host::launch_step(x, t, dt));
     rc = apply_one(c, x, bound);
     if (rc) return rc;
@@ -971,13 +1012,19 @@ int sbp_corr_advance_batch(sbp_corrector *c, sbp_batch *b, const double *dt, lon
   const long reg = 3 * b->inst[0].plane;
   hipStream_t s = b->stream;
   hipLaunchKernelGGL(clear_ints_kernel, dim3((count + 255) / 256), dim3(256), 0, s, c->brej, count);
+  // reference energy of every instance: its energy when the call starts
+  if (b->esz == 8)
+    hipLaunchKernelGGL(batch_energy_kernel<double>, dim3(count * bpi), dim3(256), 0, s,
+                       static_cast<const BatchInst<double> *>(b->inst_dev), bpi, 0, b->order, c->bpart);
+  else
+    hipLaunchKernelGGL(batch_energy_kernel<float>, dim3(count * bpi), dim3(256), 0, s,
+                       static_cast<const BatchInst<float> *>(b->inst_dev), bpi, 0, b->order, c->bpart);
+  hipLaunchKernelGGL(batch_energy_final_kernel, dim3((count + 63) / 64), dim3(64), 0, s, c->bpart, count, bpi, c->be_old);
+  HIP_OR_FAIL(hipGetLastError());
   hipStream_t keep = c->stream;
   for (long st = 0; st < nsteps; st++) {
     if (b->esz == 8) {
       const BatchInst<double> *inst = static_cast<const BatchInst<double> *>(b->inst_dev);
-      // energy of every instance before the step
-      hipLaunchKernelGGL(batch_energy_kernel<double>, dim3(count * bpi), dim3(256), 0, s, inst, bpi, 0, b->order, c->bpart);
-      hipLaunchKernelGGL(batch_energy_final_kernel, dim3((count + 63) / 64), dim3(64), 0, s, c->bpart, count, bpi, c->be_old);
       HIP_OR_FAIL(sbp::host::batch_launch_step(b));
       hipLaunchKernelGGL(pack_batch_kernel<double>, dim3(grid_for(npix)), dim3(256), 0, s, inst, count, c->H, c->W,
                          c->norm, c->X0);
@@ -986,15 +1033,13 @@ int sbp_corr_advance_batch(sbp_corrector *c, sbp_batch *b, const double *dt, lon
       c->stream = keep;
       if (rc) return rc;
       hipLaunchKernelGGL(batch_candidate_kernel<double>, dim3(grid_for(npix)), dim3(256), 0, s, inst, count, c->H, c->W,
-                         c->Yo, c->norm + 10);
+                         c->Yo, c->norm + NORM_OUT);
       hipLaunchKernelGGL(batch_energy_kernel<double>, dim3(count * bpi), dim3(256), 0, s, inst, bpi, 1, b->order, c->bpart);
       hipLaunchKernelGGL(batch_energy_final_kernel, dim3((count + 63) / 64), dim3(64), 0, s, c->bpart, count, bpi, c->be_new);
       hipLaunchKernelGGL(batch_accept_kernel<double>, dim3(grid_for(count * reg)), dim3(256), 0, s, inst, count, reg,
                          c->be_old, c->be_new, bound, c->brej);
     } else {
       const BatchInst<float> *inst = static_cast<const BatchInst<float> *>(b->inst_dev);
-      hipLaunchKernelGGL(batch_energy_kernel<float>, dim3(count * bpi), dim3(256), 0, s, inst, bpi, 0, b->order, c->bpart);
-      hipLaunchKernelGGL(batch_energy_final_kernel, dim3((count + 63) / 64), dim3(64), 0, s, c->bpart, count, bpi, c->be_old);
       HIP_OR_FAIL(sbp::host::batch_launch_step(b));
       hipLaunchKernelGGL(pack_batch_kernel<float>, dim3(grid_for(npix)), dim3(256), 0, s, inst, count, c->H, c->W,
                          c->norm, c->X0);
@@ -1003,7 +1048,7 @@ int sbp_corr_advance_batch(sbp_corrector *c, sbp_batch *b, const double *dt, lon
       c->stream = keep;
       if (rc) return rc;
       hipLaunchKernelGGL(batch_candidate_kernel<float>, dim3(grid_for(npix)), dim3(256), 0, s, inst, count, c->H, c->W,
-                         c->Yo, c->norm + 10);
+                         c->Yo, c->norm + NORM_OUT);
       hipLaunchKernelGGL(batch_energy_kernel<float>, dim3(count * bpi), dim3(256), 0, s, inst, bpi, 1, b->order, c->bpart);
       hipLaunchKernelGGL(batch_energy_final_kernel, dim3((count + 63) / 64), dim3(64), 0, s, c->bpart, count, bpi, c->be_new);
       hipLaunchKernelGGL(batch_accept_kernel<float>, dim3(grid_for(count * reg)), dim3(256), 0, s, inst, count, reg,

@@ -171,15 +171,20 @@ int  sbp_batch_get_config(const sbp_batch *b, int *config, int *tile_width, int 
 int  sbp_batch_autotune(sbp_batch *b, const double *dt, int *choice);
 
 /* ---- learned coarse-grid corrector ----------------------------------------
- * A 3-layer 3x3 convolutional network (5 -> 32 -> 32 -> 3 channels, ReLU;
- * inputs at each point: the clamped fourth differences dxxxx + dyyyy of u, v
- * and p, rho and K; output the correction of u, v, p) applied after
- * each coarse RK4 step of a 2D acoustic context; bf16 MFMA forward, backward
- * and inference with FP32 accumulation, FP32 master weights and Adam.  No
- * counterpart in the reference.  Parameters are a flat FP32 vector:
- * W1[32][5][3][3], b1[32], W2[32][32][3][3], b2[32], W3[3][32][3][3], b3[3].
+ * A 3-layer 3x3 convolutional network (8 -> 32 -> 32 -> 3 channels, ReLU)
+ * applied after each coarse RK4 step of a 2D acoustic context; output: the
+ * correction of u, v, p.  Inputs at each point (indices clamped into the
+ * grid, d4x f = f[i-2]-4f[i-1]+6f[i]-4f[i+1]+f[i+2], Dx f = (f[i+1]-f[i-1])/2,
+ * likewise in y):
+ *   rho^-1 Dx(d4x p), rho^-1 Dy(d4y p), K (Dx(d4x u) + Dy(d4y v)),
+ *   d4x u + d4y u, d4x v + d4y v, d4x p + d4y p, rho, K.
+ * bf16 MFMA forward, backward and inference with FP32 accumulation, FP32
+ * master weights and Adam.  No counterpart in the reference.  Parameters are
+ * a flat FP32 vector:
+ * W1[32][8][3][3], b1[32], W2[32][32][3][3], b2[32], W3[3][32][3][3], b3[3].
  * Inputs are normalised as (x - shift) * scale; the network predicts the
  * correction divided by out_scale. */
+enum { SBP_CORR_INPUTS = 8, SBP_CORR_OUTPUTS = 3 };
 typedef struct sbp_corrector sbp_corrector;
 int  sbp_corr_param_count(void);
 int  sbp_corr_create(int nx, int ny, int batch, sbp_corrector **out);   /* image size, batch capacity */
@@ -187,8 +192,8 @@ int  sbp_corr_destroy(sbp_corrector *c);
 int  sbp_corr_set_params(sbp_corrector *c, const float *params, int n); /* also resets Adam */
 int  sbp_corr_get_params(sbp_corrector *c, float *params, int n);
 int  sbp_corr_set_normalization(sbp_corrector *c, const float *in_shift, const float *in_scale,
-                                const float *out_scale);
-/* raw inputs [nb][ny][nx][5] -> out [nb][ny][nx][3] (normalised units). */
+                                const float *out_scale);   /* 8, 8 and 3 values */
+/* raw inputs [nb][ny][nx][8] -> out [nb][ny][nx][3] (normalised units). */
 int  sbp_corr_forward(sbp_corrector *c, const float *raw, int nb, float *out);
 /* MSE loss of (net - target/out_scale) and its parameter gradients. */
 int  sbp_corr_gradients(sbp_corrector *c, const float *raw, const float *target, int nb,
@@ -197,8 +202,10 @@ int  sbp_corr_load_dataset(sbp_corrector *c, const float *raw, const float *targ
 int  sbp_corr_train_step(sbp_corrector *c, const int *idx, int nb, float lr, double *loss);
 int  sbp_corr_synchronize(sbp_corrector *c);
 /* nsteps coarse RK4 steps of ctx, each followed by U += out_scale*net(U);
- * a step whose corrected energy exceeds the energy before the step times
- * (1+bound) keeps the plain RK4 result instead; *rejected counts them. */
+ * a step whose corrected energy exceeds the state's energy at the start of
+ * the call times (1+bound) keeps the plain RK4 result instead (with wall or
+ * free-surface SATs and no forcing the exact energy is constant, so this is
+ * growth); *rejected counts them. */
 int  sbp_corr_advance(sbp_corrector *c, sbp_ctx *ctx, double t0, double dt, long nsteps,
                       double bound, int *rejected);
 

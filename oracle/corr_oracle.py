@@ -1,8 +1,8 @@
 """CPU oracle of the learned corrector network (test infrastructure only).
 
-Network: conv3x3(5->32) + ReLU, conv3x3(32->32) + ReLU, conv3x3(32->3), zero
+Network: conv3x3(8->32) + ReLU, conv3x3(32->32) + ReLU, conv3x3(32->3), zero
 padding, channels-last images [n][y][x][c]; flat parameter order
-W1[32][5][3][3], b1, W2[32][32][3][3], b2, W3[3][32][3][3], b3.
+W1[32][8][3][3], b1, W2[32][32][3][3], b2, W3[3][32][3][3], b3.
 
 Two arithmetic modes:
   exact=True   float64 throughout: the plain mathematics, pinned in the tests
@@ -15,7 +15,7 @@ from __future__ import annotations
 
 import numpy as np
 
-C_IN, C_H, C_OUT = 5, 32, 3
+C_IN, C_H, C_OUT = 8, 32, 3
 SHAPES = [("W1", (C_H, C_IN, 3, 3)), ("b1", (C_H,)), ("W2", (C_H, C_H, 3, 3)), ("b2", (C_H,)),
           ("W3", (C_OUT, C_H, 3, 3)), ("b3", (C_OUT,))]
 

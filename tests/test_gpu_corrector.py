@@ -24,12 +24,12 @@ def capi():
 
 def setup(seed, nb=2, H=21, W=37):
     rng = np.random.default_rng(seed)
-    raw = rng.standard_normal((nb, H, W, 5)).astype(np.float32)
-    raw[..., 3:] = rng.uniform(0.5, 2.0, (nb, H, W, 2))
+    raw = rng.standard_normal((nb, H, W, CO.C_IN)).astype(np.float32)
+    raw[..., 6:] = rng.uniform(0.5, 2.0, (nb, H, W, 2))  # rho, K
     tgt = (0.3 * rng.standard_normal((nb, H, W, 3))).astype(np.float32)
     params = (rng.standard_normal(CO.n_params()) * 0.15).astype(np.float32)
-    shift = np.array([0, 0, 0, 1.25, 1.25], np.float32)
-    scale = np.array([1.5, 1.5, 2.0, 1.3, 1.3], np.float32)
+    shift = np.array([0, 0, 0, 0, 0, 0, 1.25, 1.25], np.float32)
+    scale = np.array([1.5, 1.5, 2.0, 0.7, 1.1, 1.3, 1.3, 1.3], np.float32)
     osc = np.array([0.5, 0.5, 2.0], np.float32)
     return raw, tgt, params, shift, scale, osc
 
@@ -75,7 +75,7 @@ def test_gradients_match_oracle(capi):
 def test_training_reduces_loss(capi):
     rng = np.random.default_rng(3)
     n, H, W = 16, 24, 24
-    raw = rng.standard_normal((n, H, W, 5)).astype(np.float32)
+    raw = rng.standard_normal((n, H, W, CO.C_IN)).astype(np.float32)
     # a learnable target: a fixed local stencil of the inputs
     tgt = np.zeros((n, H, W, 3), np.float32)
     tgt[..., 0] = 0.5 * raw[..., 2] - 0.25 * np.roll(raw[..., 0], 1, axis=2)
@@ -102,7 +102,7 @@ def test_energy_guard_rejects_growth(capi):
         p = np.zeros(c.n_params, np.float32)
         p[-3:] = 1.0  # b3: constant positive correction of u, v, p
         c.params = p
-        c.set_normalization(np.zeros(5), np.ones(5), np.array([0.1, 0.1, 0.1]))
+        c.set_normalization(np.zeros(CO.C_IN), np.ones(CO.C_IN), np.array([0.1, 0.1, 0.1]))
         with capi.Context(spec) as ctx:
             ctx.set_state(v0)
             rej = c.advance(ctx, 0.0, dt, 10, 1e-6)
@@ -118,3 +118,34 @@ def test_energy_guard_rejects_growth(capi):
             got = ctx.get_state()
         assert rej == 0
         assert np.array_equal(got, plain)
+
+
+def test_corrected_step_uses_host_features(capi):
+    """One corrected step on the device equals: plain step, the host's
+    feature definition (corrector.raw_features), the network, and the update
+    U + out_scale * net.  Pins the device feature packing to the host's."""
+    import importlib
+    CR = importlib.import_module("hpc-reading-course_amd.corrector")
+    spec, v0 = P.acoustic_box(57, 4, sigma=0.08, materials=True, seed=5, material_seed=12)
+    dt = P.dt_from_cfl(spec, 0.25)
+    with capi.Context(spec) as ctx:
+        plain = ctx.rk4_advance(v0, 0.0, dt, 1)
+    raw = CR.raw_features(spec, plain)[None]
+    rng = np.random.default_rng(4)
+    params = (rng.standard_normal(CO.n_params()) * 0.1).astype(np.float32)
+    shift = raw.reshape(-1, CO.C_IN).mean(axis=0)
+    scale = 1.0 / (raw.reshape(-1, CO.C_IN).std(axis=0) + 1e-30)
+    osc = np.array([1e-3, 2e-3, 3e-3], np.float32)
+    with capi.Corrector(57, 57, 1) as c:
+        c.params = params
+        c.set_normalization(shift, scale, osc)
+        net = c.forward(raw)[0]
+        with capi.Context(spec) as ctx:
+            ctx.set_state(v0)
+            rej = c.advance(ctx, 0.0, dt, 1, 1.0)
+            got = ctx.get_state()
+    assert rej == 0
+    want = plain + (net * osc).reshape(-1)
+    corr = (net * osc).reshape(-1)
+    assert np.max(np.abs(got - want)) <= 1e-3 * np.max(np.abs(corr))
+    assert np.max(np.abs(got - plain)) > 0.5 * np.max(np.abs(corr))

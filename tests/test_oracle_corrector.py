@@ -48,11 +48,11 @@ def test_dgrad_is_adjoint_and_wgrad_is_derivative():
 def test_exact_gradients_match_finite_differences():
     rng = np.random.default_rng(2)
     H, W = 6, 5
-    raw = rng.standard_normal((2, H, W, 5))
+    raw = rng.standard_normal((2, H, W, C.C_IN))
     tgt = rng.standard_normal((2, H, W, 3)) * 0.1
     flat = rng.standard_normal(C.n_params()) * 0.2
-    shift = np.zeros(5)
-    scale = np.ones(5)
+    shift = np.zeros(C.C_IN)
+    scale = np.ones(C.C_IN)
     osc = np.array([0.5, 2.0, 1.0])
     loss, g = C.loss_and_grads(flat, raw, tgt, shift, scale, osc, exact=True)
     # features are computed in float32 then widened: perturb parameters only
