@@ -80,17 +80,18 @@ void compute_geometry(sbp_ctx *c)
   c->geom.block = kBX;
 }
 
-// Default tile height: the tallest of 16..64 rows that still gives at least
-// four tiles per CU (256 CUs); tall tiles amortise the 2R-row y halo
-// (measured on 4097^2: 64 rows beat 16/32/128).
+// Default tile height: the tallest of 4..64 rows that still gives about four
+// tiles per CU (>= 960 tiles on 256 CUs).  Tall tiles amortise the 2R-row y
+// halo; on grids too small for that, occupancy wins (measured, order 4:
+// 4097^2 -> 64 rows, 2049^2 -> 16, 1025^2 -> 4 rows at 1.25x the 16-row rate).
 int default_ty_rows(int nx, int ny, int cls)
 {
   const int wi = nx - 2 * cls, hi = ny - 2 * cls;
   if (wi <= 0 || hi <= 0) return 16;
   const int tx = (wi + kBX - 1) / kBX;
-  int best = 16;
-  for (int t = 16; t <= 64; t *= 2)
-    if ((long)tx * ((hi + t - 1) / t) >= 1024) best = t;
+  int best = 4;
+  for (int t = 4; t <= 64; t *= 2)
+    if ((long)tx * ((hi + t - 1) / t) >= 960) best = t;
   return best;
 }
 

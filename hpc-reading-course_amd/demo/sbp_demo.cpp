@@ -14,14 +14,18 @@
 //   reflection  demo/reflection/reflection_sim.cpp:56-74,140-157,177-226
 // use_custom_sc selects the reference's halo scatter implementation; one GPU
 // owns the whole grid, so it is validated and has no effect.  When the
-// environment variable SBP_DEMO_DATA_DIR is set, the run appends the
-// reference's TSV row (io_util.cpp:16-36) to $SBP_DEMO_DATA_DIR/data.tsv.
+// environment variable SBP_DEMO_DATA_DIR is set, the run does what the
+// reference's write_data switch does (e.g. wave_eq_sim.cpp:136,184-195): it
+// writes v_init, v and v_error = |v_analytic - v| (vec_util.h:63-70) as PETSc
+// binary vectors (io_util.cpp:4-14) and appends the TSV row of
+// io_util.cpp:16-36 to data.tsv, all in $SBP_DEMO_DATA_DIR.
 #include <sys/stat.h>
 
 #include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <string>
 #include <vector>
 
@@ -169,10 +173,47 @@ static void solution(const sbp_problem &p, double t, std::vector<double> &v)
   }
 }
 
+// PETSc's binary Vec layout (what VecView writes through a binary viewer, the
+// natural ordering of a DMDA vector being the AoS layout on one rank):
+// big-endian int32 class id 1211214, int32 length, then big-endian doubles;
+// plus the "<file>.info" companion holding -vecload_block_size for dof > 1.
+static void put_be32(FILE *f, unsigned v)
+{
+  const unsigned char b[4] = {(unsigned char)(v >> 24), (unsigned char)(v >> 16), (unsigned char)(v >> 8),
+                              (unsigned char)v};
+  std::fwrite(b, 1, 4, f);
+}
+
+static void write_petsc_vec(const char *dir, const char *name, const std::vector<double> &v, int dof)
+{
+  mkdir(dir, 0755);
+  const std::string path = std::string(dir) + "/" + name;
+  FILE *f = std::fopen(path.c_str(), "wb");
+  if (!f) {
+    std::printf("File '%s' failed to open.\n", path.c_str());
+    return;
+  }
+  put_be32(f, 1211214u);
+  put_be32(f, (unsigned)v.size());
+  for (double x : v) {
+    unsigned long long u;
+    std::memcpy(&u, &x, 8);
+    put_be32(f, (unsigned)(u >> 32));
+    put_be32(f, (unsigned)u);
+  }
+  std::fclose(f);
+  if (dof > 1) {
+    FILE *g = std::fopen((path + ".info").c_str(), "w");
+    if (g) {
+      std::fprintf(g, "-vecload_block_size %d\n", dof);
+      std::fclose(g);
+    }
+  }
+}
+
 static void append_tsv(double dt, const Inputs &in, double elapsed, double l2, double maxerr)
 {
   const char *dir = std::getenv("SBP_DEMO_DATA_DIR");
-  if (!dir || !*dir) return;
   mkdir(dir, 0755);
   const std::string path = std::string(dir) + "/data.tsv";
   FILE *f = std::fopen(path.c_str(), "a");
@@ -232,6 +273,9 @@ int main(int argc, char **argv)
     }
     std::vector<double> v, v_analytic;
     solution(p, 0.0, v);
+    const char *data_dir = std::getenv("SBP_DEMO_DATA_DIR");
+    const bool write_data = data_dir && *data_dir;
+    if (write_data) write_petsc_vec(data_dir, "v_init", v, ctx.dofs());
 
     const auto t0 = std::chrono::steady_clock::now();
     sbp::check(sbp::ts_rk4(in.tend, dt, v.data(), ctx));
@@ -242,7 +286,13 @@ int main(int argc, char **argv)
     double l2 = 0, maxerr = 0;
     sbp::check(sbp::error_norms(ctx, v.data(), v_analytic.data(), &l2, &maxerr));
     std::printf("The l2-error is: %g, and the maximum error is %g\n", l2, maxerr);
-    append_tsv(dt, in, elapsed, l2, maxerr);
+    if (write_data) {
+      write_petsc_vec(data_dir, "v", v, ctx.dofs());
+      std::vector<double> err(v.size());
+      for (size_t k = 0; k < v.size(); k++) err[k] = std::fabs(v_analytic[k] - v[k]);
+      write_petsc_vec(data_dir, "v_error", err, ctx.dofs());
+      append_tsv(dt, in, elapsed, l2, maxerr);
+    }
   } catch (const std::exception &e) {
     std::printf("%s: %s\n", kName, e.what());
     return 1;

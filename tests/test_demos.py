@@ -5,8 +5,9 @@ code (get_inputs, code/src/util/io_util.cpp:54-154) before the GPU is touched,
 and a run without a device fails with the library's error.
 GPU: each driver's printed l2 / max errors against the analytic solution
 equal the oracle's errors for the same MATCHSTEP run (6 significant digits,
-the precision of the reference's %g print), and the TSV row of
-io_util.cpp:16-36 is appended when requested.
+the precision of the reference's %g print); with SBP_DEMO_DATA_DIR set the
+drivers write v_init, v, v_error as PETSc binary vectors (io_util.cpp:4-14)
+and the TSV row of io_util.cpp:16-36.
 """
 import os
 import re
@@ -106,8 +107,20 @@ def test_demo_errors_match_oracle(built, orc, demo, order, n, tend, cfl):
         assert "Processor topology dimensions: [1,1]" in out
 
 
+def read_petsc_vec(path):
+    """PETSc binary Vec: big-endian int32 class id 1211214, int32 n, n doubles."""
+    raw = open(path, "rb").read()
+    head = np.frombuffer(raw[:8], dtype=">i4")
+    assert head[0] == 1211214
+    vals = np.frombuffer(raw[8:], dtype=">f8")
+    assert len(vals) == head[1]
+    return vals.astype(np.float64)
+
+
 @pytest.mark.gpu
-def test_demo_tsv_row(built, tmp_path):
+def test_demo_data_output(built, tmp_path):
+    """SBP_DEMO_DATA_DIR switches on the reference's write_data outputs:
+    v_init, v, v_error (PETSc binary) and the TSV row."""
     d = tmp_path / "data"
     for _ in range(2):
         rc, out = run(4, "wave_hom", 33, 33, 0.1, 0.1, 0, env={"SBP_DEMO_DATA_DIR": str(d)})
@@ -120,3 +133,17 @@ def test_demo_tsv_row(built, tmp_path):
     assert row[:3] == ["1", "33", "33"]
     assert float(row[3]) == pytest.approx(0.1 / 16)
     assert np.isfinite([float(x) for x in row[3:]]).all()
+
+    spec, v0 = P.wave_hom(33, 4)
+    v_init = read_petsc_vec(d / "v_init")
+    assert np.max(np.abs(v_init - v0)) <= 1e-14
+    v = read_petsc_vec(d / "v")
+    err = read_petsc_vec(d / "v_error")
+    assert np.max(np.abs(err - np.abs(P.standing_mode(spec, 0.1) - v))) <= 1e-14
+    m = re.search(r"The l2-error is: (\S+), and the maximum error is (\S+)", out)
+    assert float(m.group(1)) == pytest.approx(np.sqrt(spec.hx * spec.hy) * np.linalg.norm(err), rel=1e-5)
+    assert float(m.group(2)) == pytest.approx(err.max(), rel=1e-5)
+    assert (d / "v.info").read_text().strip() == "-vecload_block_size 3"
+    # without the variable nothing is written
+    rc, out = run(4, "wave_hom", 33, 33, 0.1, 0.1, 0, env={"SBP_DEMO_DATA_DIR": ""})
+    assert rc == 0

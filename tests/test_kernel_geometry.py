@@ -25,10 +25,10 @@ def default_ty(nx, ny, cls):
     if wi <= 0 or hi <= 0:
         return 16
     tx = (wi + BX - 1) // BX
-    best = 16
-    t = 16
+    best = 4
+    t = 4
     while t <= 64:
-        if tx * ((hi + t - 1) // t) >= 1024:
+        if tx * ((hi + t - 1) // t) >= 960:
             best = t
         t *= 2
     return best
@@ -152,6 +152,9 @@ def test_geometry_large_grid_parameters():
     g = geometry(4097, 4097, 4)
     assert g["tiles_x"] == 16 and g["n_edge_blocks"] % NXCD == 0
     assert g["n_tiles"] >= 1024 and g["ty"] == 64
+    assert geometry(2049, 2049, 4)["ty"] == 16
+    assert geometry(1025, 1025, 4)["ty"] == 4
+    check(1025, 1025, 4)
     # remap is a bijection on the big grid too
     nt = g["n_tiles"]
     assert sorted(xcd_tile(b, nt) for b in range(nt)) == list(range(nt))
