@@ -170,6 +170,19 @@ def reflection_2d(steps=2000, warmup=0):
     return out
 
 
+def time_to_l2():
+    """Time to L2 error 1e-4 on 16 held-out seeded variable-material acoustic
+    instances (T = 0.5, judged against the FP64 1025^2 solve injected on each
+    grid): plain FP64 single-problem path vs the FP32 batched ensemble vs the
+    learned corrector on 257^2 with the trained weights shipped in the package."""
+    C = pkg("corrector")
+    if not os.path.exists(C.WEIGHTS):
+        return {"skipped": "no trained corrector weights at %s" % C.WEIGHTS}
+    params, norm = C.load_weights()
+    res = C.time_to_accuracy(params, norm, n_eval=16, log=log)
+    return res
+
+
 def cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -256,7 +269,10 @@ def main():
         rf = reflection_2d()
         secondary["reflection2d_4097sq_order6_materials_2000steps"] = rf
         log("reflection 2D:", rf)
+        tta = time_to_l2()
 
+    if args.no_secondary:
+        tta = None
     cpu = None
     if not args.no_cpu:
         log("cpu baseline ...")
@@ -295,6 +311,7 @@ def main():
                      "stage_ms": main_m["stage_ms"],
                      "copy_ceiling_gbs": copy_gbs},
         "cpu_baseline": cpu,
+        "time_to_l2_1e-4": tta,
         "secondary": secondary,
     }
     print(json.dumps(out), flush=True)

@@ -333,6 +333,173 @@ __global__ __launch_bounds__(256) void pack_raw_kernel(const float *__restrict__
     X[p * C_INP + c] = c < C_IN ? f2bf((raw[p * C_IN + c] - shift_scale[c]) * shift_scale[C_IN + c]) : (bf16)0;
 }
 
+// --------------------------------------------------------------------------
+// Fused inference: the three convolutions of one image window in one kernel,
+// activations kept in LDS.  A block owns a window of WW = 64 columns
+// (image columns c0 .. c0+63, c0 = x0 - 3) and a chunk of output rows; it
+// marches down the rows keeping the last three rows of the input, of layer 1
+// and of layer 2 in LDS rings.  Every layer is evaluated on all 64 window
+// columns; a layer's value at window column w is exact when its inputs at
+// w-1..w+1 are, so the output is exact on w in [3, 60]: 58 output columns
+// per window.  Activations at image positions outside the image are stored
+// as zero (the zero padding of the layer-by-layer kernels).  The GEMMs run on
+// v_mfma_f32_16x16x32_bf16 (K = 32 per instruction: one tap of 32 channels,
+// or four taps of the 8 input channels), so the output equals forward()'s up
+// to the FP32 accumulation order inside an instruction.
+// --------------------------------------------------------------------------
+using bf16x8 = __attribute__((ext_vector_type(8))) short;
+using vbf8 = __attribute__((ext_vector_type(8))) __bf16;
+constexpr int FW = 64, FS = FW / 16, FOUT = FW - 6;  // window width, segments, output columns
+constexpr int FPX = FW + 2;                            // pixels per LDS row (one zero pad each side)
+constexpr int F1S = 40;                                // LDS channel stride of the 32-channel rows (80 B)
+
+__device__ __forceinline__ f32x4 mfma32(bf16x8 a, bf16x8 b, f32x4 c)
+{
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(vbf8, a), __builtin_bit_cast(vbf8, b), c, 0, 0, 0);
+}
+
+__global__ __launch_bounds__(256) void fused_net_kernel(const bf16 *__restrict__ X, int nimg, int H, int W, int rows,
+                                                        const bf16 *__restrict__ w1, const float *__restrict__ b1,
+                                                        const bf16 *__restrict__ w2, const float *__restrict__ b2,
+                                                        const bf16 *__restrict__ w3, const float *__restrict__ b3,
+                                                        float *__restrict__ out)
+{
+  __shared__ __attribute__((aligned(16))) bf16 s_in[3][FPX * C_INP];
+  __shared__ __attribute__((aligned(16))) bf16 s_a1[3][FPX * F1S];
+  __shared__ __attribute__((aligned(16))) bf16 s_a2[3][FPX * F1S];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int m = lane & 15, kg = lane >> 4;
+  const int strips = (W + FOUT - 1) / FOUT, chunks = (H + rows - 1) / rows;
+  const int b = blockIdx.x;
+  const int img = b / (strips * chunks), rem = b - img * strips * chunks;
+  const int strip = rem / chunks, chunk = rem - strip * chunks;
+  const int c0 = strip * FOUT - 3;  // image column of window column 0
+  const int y0 = chunk * rows, y1 = min(y0 + rows, H);
+  const long ibase = (long)img * H * W;
+
+  // zero every LDS row once: pads stay zero, rows outside the image too
+  for (int q = threadIdx.x; q < 3 * FPX * C_INP; q += 256) (&s_in[0][0])[q] = 0;
+  for (int q = threadIdx.x; q < 3 * FPX * F1S; q += 256) {
+    (&s_a1[0][0])[q] = 0;
+    (&s_a2[0][0])[q] = 0;
+  }
+
+  // this wave's weights: output-channel tile tc (layers 1, 2), all of layer 3.
+  // Fragment k range of lane (m, kg) in K-step ks: 32 * ks + 8 * kg .. + 7.
+  const int tc = wv & 1, sA = wv >> 1;  // layers 1-2: segments sA and sA+2
+  constexpr int NK1 = 3, NK2 = KP2 / 32, NK3 = KP3 / 32;
+  bf16x8 bw1[NK1], bw2[NK2], bw3[NK3];
+#pragma unroll
+  for (int ks = 0; ks < NK1; ks++) {
+    const int k0 = ks * 32 + kg * 8;  // w1 rows hold KP1 = 80 entries (k >= 72 are zero)
+    bw1[ks] = k0 < KP1 ? *reinterpret_cast<const bf16x8 *>(w1 + (long)(tc * 16 + m) * KP1 + k0) : bf16x8{};
+  }
+#pragma unroll
+  for (int ks = 0; ks < NK2; ks++)
+    bw2[ks] = *reinterpret_cast<const bf16x8 *>(w2 + (long)(tc * 16 + m) * KP2 + ks * 32 + kg * 8);
+#pragma unroll
+  for (int ks = 0; ks < NK3; ks++)
+    bw3[ks] = *reinterpret_cast<const bf16x8 *>(w3 + (long)m * KP3 + ks * 32 + kg * 8);
+  const float bv1 = b1[tc * 16 + m], bv2 = b2[tc * 16 + m], bv3 = b3[m];
+  __syncthreads();
+
+  const int nsteps = (y1 - y0) + 6;
+  for (int t = 0; t < nsteps; t++) {
+    // input row y0 - 3 + t -> ring slot t % 3 (pixels w = -1 .. FW)
+    {
+      const int ri = y0 - 3 + t;
+      bf16 *dst = s_in[t % 3];
+      if (threadIdx.x < FPX) {
+        const int col = c0 - 1 + threadIdx.x;
+        uint4 v = {0, 0, 0, 0};
+        if (ri >= 0 && ri < H && col >= 0 && col < W)
+          v = *reinterpret_cast<const uint4 *>(X + (ibase + (long)ri * W + col) * C_INP);
+        *reinterpret_cast<uint4 *>(dst + threadIdx.x * C_INP) = v;
+      }
+    }
+    __syncthreads();
+    // layer 1, row r1 = y0 - 4 + t, from input slots (t-2, t-1, t) % 3;
+    // K-step ks covers taps 4 ks + kg (all 8 channels of one input pixel)
+    if (t >= 2) {
+      const int r1 = y0 - 4 + t;
+      const bool rin = r1 >= 0 && r1 < H;
+#pragma unroll
+      for (int u = 0; u < 2; u++) {
+        const int sg = sA + 2 * u;
+        f32x4 acc = {0, 0, 0, 0};
+#pragma unroll
+        for (int ks = 0; ks < NK1; ks++) {
+          const int tap = ks * 4 + kg;
+          bf16x8 a = {};
+          if (tap < 9) {
+            const bf16 *row = s_in[(t + 1 + tap / 3) % 3];  // slot (t - 1 + dy) % 3, dy = tap/3 - 1
+            const int px = 16 * sg + m + tap % 3;       // window column m + dx plus the pad column
+            a = *reinterpret_cast<const bf16x8 *>(row + px * C_INP);
+          }
+          acc = mfma32(a, bw1[ks], acc);
+        }
+        bf16 *dst = s_a1[t % 3];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          const int w = 16 * sg + 4 * kg + i, col = c0 + w;
+          const float v = acc[i] + bv1;
+          const bool in = rin && col >= 0 && col < W;
+          dst[(w + 1) * F1S + tc * 16 + m] = in ? f2bf(v > 0.f ? v : 0.f) : (bf16)0;
+        }
+      }
+    }
+    __syncthreads();
+    // layer 2, row r2 = y0 - 5 + t, from layer-1 slots (t-2, t-1, t) % 3;
+    // K-step ks is tap ks, channels 8 kg .. 8 kg + 7
+    if (t >= 4) {
+      const int r2 = y0 - 5 + t;
+      const bool rin = r2 >= 0 && r2 < H;
+#pragma unroll
+      for (int u = 0; u < 2; u++) {
+        const int sg = sA + 2 * u;
+        f32x4 acc = {0, 0, 0, 0};
+#pragma unroll
+        for (int ks = 0; ks < NK2; ks++) {
+          const bf16 *row = s_a1[(t + 1 + ks / 3) % 3];
+          const int px = 16 * sg + m + ks % 3;
+          const bf16x8 a = *reinterpret_cast<const bf16x8 *>(row + px * F1S + kg * 8);
+          acc = mfma32(a, bw2[ks], acc);
+        }
+        bf16 *dst = s_a2[t % 3];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          const int w = 16 * sg + 4 * kg + i, col = c0 + w;
+          const float v = acc[i] + bv2;
+          const bool in = rin && col >= 0 && col < W;
+          dst[(w + 1) * F1S + tc * 16 + m] = in ? f2bf(v > 0.f ? v : 0.f) : (bf16)0;
+        }
+      }
+    }
+    __syncthreads();
+    // layer 3, row r3 = y0 - 6 + t, from layer-2 slots (t-2, t-1, t) % 3
+    if (t >= 6) {
+      const int r3 = y0 - 6 + t;
+      const int sg = wv;
+      f32x4 acc = {0, 0, 0, 0};
+#pragma unroll
+      for (int ks = 0; ks < NK3; ks++) {
+        const bf16 *row = s_a2[(t + 1 + ks / 3) % 3];
+        const int px = 16 * sg + m + ks % 3;
+        const bf16x8 a = *reinterpret_cast<const bf16x8 *>(row + px * F1S + kg * 8);
+        acc = mfma32(a, bw3[ks], acc);
+      }
+      if (m < C_OUT) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          const int w = 16 * sg + 4 * kg + i, col = c0 + w;
+          if (w >= 3 && w < FW - 3 && col < W)
+            out[(ibase + (long)r3 * W + col) * C_OUT + m] = acc[i] + bv3;
+        }
+      }
+    }
+  }
+}
+
 // Gather dataset samples (raw inputs and targets) into a contiguous batch.
 __global__ __launch_bounds__(256) void gather_kernel(const float *__restrict__ raw_all, const float *__restrict__ tgt_all,
                                                      const int *__restrict__ idx, int nb, long npix_img,
@@ -363,64 +530,83 @@ __global__ __launch_bounds__(256) void gather_kernel(const float *__restrict__ r
 // a linear function of its inputs; 3..7 carry what the next terms and the
 // boundary closures need.  All of them are high-pass quantities: the residual
 // (about 1e-6 of the state) is far below bf16's resolution of the state.
-template <typename T>
-struct FieldView {
-  const T *f;
-  int nx, ny, pitch, xoff;
-  __device__ __forceinline__ double at(int i, int j) const
-  {
-    i = min(max(i, 0), nx - 1);
-    j = min(max(j, 0), ny - 1);
-    return (double)f[(long)j * pitch + xoff + i];
-  }
-  __device__ __forceinline__ double d4x(int i, int j) const
-  {
-    i = min(max(i, 0), nx - 1);
-    return at(i - 2, j) - 4 * at(i - 1, j) + 6 * at(i, j) - 4 * at(i + 1, j) + at(i + 2, j);
-  }
-  __device__ __forceinline__ double d4y(int i, int j) const
-  {
-    j = min(max(j, 0), ny - 1);
-    return at(i, j - 2) - 4 * at(i, j - 1) + 6 * at(i, j) - 4 * at(i, j + 1) + at(i, j + 2);
-  }
-  __device__ __forceinline__ double dx_d4x(int i, int j) const { return 0.5 * (d4x(i + 1, j) - d4x(i - 1, j)); }
-  __device__ __forceinline__ double dy_d4y(int i, int j) const { return 0.5 * (d4y(i, j + 1) - d4y(i, j - 1)); }
-};
+// The host restatement is corrector.raw_features.
+//
+// A block packs a PTW x PTH tile of one image.  The three state planes of the
+// tile plus a 3-point halo are staged in LDS as f[clamp(j)][clamp(i)], so a
+// neighbour read with its index clamped is a plain LDS read; the centres of
+// the differenced fourth differences are clamped into the image first.
+constexpr int PTW = 64, PTH = 16, PTX = PTW + 6, PTY = PTH + 6;
 
 template <typename T>
-__device__ __forceinline__ void state_features(const T *__restrict__ U, const T *__restrict__ rinv,
-                                               const T *__restrict__ bulk, int nx, int ny, int pitch, int xoff,
-                                               long plane, int i, int j, float raw[C_IN])
+__device__ __forceinline__ void pack_tile(const T *__restrict__ U, const T *__restrict__ rinv, const T *__restrict__ bulk,
+                                          int nx, int ny, int pitch, int xoff, long plane, int tx, int ty,
+                                          const float *__restrict__ shift_scale, bf16 *__restrict__ X,
+                                          double (*s)[PTY][PTX])
 {
-  const FieldView<T> u{U, nx, ny, pitch, xoff}, v{U + plane, nx, ny, pitch, xoff}, p{U + 2 * plane, nx, ny, pitch, xoff};
-  const long o = (long)j * pitch + xoff + i;
-  const double ri = rinv ? (double)rinv[o] : 1.0;
-  const double K = bulk ? (double)bulk[o] : 1.0;
-  raw[0] = (float)(ri * p.dx_d4x(i, j));
-  raw[1] = (float)(ri * p.dy_d4y(i, j));
-  raw[2] = (float)(K * (u.dx_d4x(i, j) + v.dy_d4y(i, j)));
-  raw[3] = (float)(u.d4x(i, j) + u.d4y(i, j));
-  raw[4] = (float)(v.d4x(i, j) + v.d4y(i, j));
-  raw[5] = (float)(p.d4x(i, j) + p.d4y(i, j));
-  raw[6] = (float)(1.0 / ri);
-  raw[7] = (float)K;
+  const int x0 = tx * PTW, y0 = ty * PTH;
+  for (int q = threadIdx.x; q < 3 * PTY * PTX; q += 256) {
+    const int c = q / (PTY * PTX), r = q - c * (PTY * PTX);
+    const int yy = r / PTX, xx = r - yy * PTX;
+    const int gi = min(max(x0 - 3 + xx, 0), nx - 1), gj = min(max(y0 - 3 + yy, 0), ny - 1);
+    s[c][yy][xx] = (double)U[c * plane + (long)gj * pitch + xoff + gi];
+  }
+  __syncthreads();
+  // image (i, j) -> LDS slot; valid for |i - x0 - PTW/2| <= PTW/2 + 3 etc.
+  auto at = [&](int c, int i, int j) { return s[c][j - y0 + 3][i - x0 + 3]; };
+  auto d4x = [&](int c, int i, int j) {
+    i = min(max(i, 0), nx - 1);
+    return at(c, i - 2, j) - 4 * at(c, i - 1, j) + 6 * at(c, i, j) - 4 * at(c, i + 1, j) + at(c, i + 2, j);
+  };
+  auto d4y = [&](int c, int i, int j) {
+    j = min(max(j, 0), ny - 1);
+    return at(c, i, j - 2) - 4 * at(c, i, j - 1) + 6 * at(c, i, j) - 4 * at(c, i, j + 1) + at(c, i, j + 2);
+  };
+  for (int q = threadIdx.x; q < PTW * PTH; q += 256) {
+    const int i = x0 + (q % PTW), j = y0 + (q / PTW);
+    if (i >= nx || j >= ny) continue;
+    const long o = (long)j * pitch + xoff + i;
+    const double ri = rinv ? (double)rinv[o] : 1.0;
+    const double K = bulk ? (double)bulk[o] : 1.0;
+    float raw[C_IN];
+    raw[0] = (float)(ri * (0.5 * (d4x(2, i + 1, j) - d4x(2, i - 1, j))));
+    raw[1] = (float)(ri * (0.5 * (d4y(2, i, j + 1) - d4y(2, i, j - 1))));
+    raw[2] = (float)(K * (0.5 * (d4x(0, i + 1, j) - d4x(0, i - 1, j)) + 0.5 * (d4y(1, i, j + 1) - d4y(1, i, j - 1))));
+    raw[3] = (float)(d4x(0, i, j) + d4y(0, i, j));
+    raw[4] = (float)(d4x(1, i, j) + d4y(1, i, j));
+    raw[5] = (float)(d4x(2, i, j) + d4y(2, i, j));
+    raw[6] = (float)(1.0 / ri);
+    raw[7] = (float)K;
+    __attribute__((aligned(16))) bf16 px[C_INP];
+#pragma unroll
+    for (int c = 0; c < C_INP; c++) px[c] = f2bf((raw[c] - shift_scale[c]) * shift_scale[C_IN + c]);
+    *reinterpret_cast<uint4 *>(X + ((long)j * nx + i) * C_INP) = *reinterpret_cast<const uint4 *>(px);
+  }
 }
 
-// Features of a solver context's resident state.
 template <typename T>
-__global__ __launch_bounds__(256) void pack_state_kernel(const T *__restrict__ U, const T *__restrict__ rinv,
-                                                         const T *__restrict__ bulk, int nx, int ny, int pitch, int xoff,
-                                                         long plane, const float *__restrict__ shift_scale,
-                                                         bf16 *__restrict__ X)
+__global__ __launch_bounds__(256) void pack_state_tiled_kernel(const T *__restrict__ U, const T *__restrict__ rinv,
+                                                               const T *__restrict__ bulk, int nx, int ny, int pitch,
+                                                               int xoff, long plane,
+                                                               const float *__restrict__ shift_scale,
+                                                               bf16 *__restrict__ X)
 {
-  const long p = blockIdx.x * 256L + threadIdx.x;
-  if (p >= (long)nx * ny) return;
-  const int j = (int)(p / nx), i = (int)(p - (long)j * nx);
-  float raw[C_IN];
-  state_features(U, rinv, bulk, nx, ny, pitch, xoff, plane, i, j, raw);
-#pragma unroll
-  for (int c = 0; c < C_INP; c++)
-    X[p * C_INP + c] = c < C_IN ? f2bf((raw[c] - shift_scale[c]) * shift_scale[C_IN + c]) : (bf16)0;
+  __shared__ double s[3][PTY][PTX];
+  const int tiles_x = (nx + PTW - 1) / PTW;
+  pack_tile(U, rinv, bulk, nx, ny, pitch, xoff, plane, blockIdx.x % tiles_x, blockIdx.x / tiles_x, shift_scale, X, s);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void pack_batch_tiled_kernel(const BatchInst<T> *__restrict__ inst, int H, int W,
+                                                               const float *__restrict__ shift_scale,
+                                                               bf16 *__restrict__ X)
+{
+  __shared__ double s[3][PTY][PTX];
+  const int tiles_x = (W + PTW - 1) / PTW, tiles_y = (H + PTH - 1) / PTH;
+  const int k = blockIdx.x / (tiles_x * tiles_y), t = blockIdx.x - k * tiles_x * tiles_y;
+  const BatchInst<T> &d = inst[k];
+  pack_tile(d.U, d.rinv, d.bulk, W, H, d.pitch, d.xoff, d.plane, t % tiles_x, t / tiles_x, shift_scale,
+            X + (long)k * H * W * C_INP, s);
 }
 
 // Corrected candidate: Y = U + out_scale * net (into the scratch register).
@@ -450,24 +636,6 @@ __global__ __launch_bounds__(256) void accept_kernel(const T *__restrict__ Y, T 
 }
 
 // ---- batched ensemble (every instance on the corrector's grid) ------------
-template <typename T>
-__global__ __launch_bounds__(256) void pack_batch_kernel(const BatchInst<T> *__restrict__ inst, int count, int H, int W,
-                                                         const float *__restrict__ shift_scale, bf16 *__restrict__ X)
-{
-  const long p = blockIdx.x * 256L + threadIdx.x;
-  const long npi = (long)H * W;
-  if (p >= count * npi) return;
-  const int k = (int)(p / npi);
-  const long q = p - k * npi;
-  const int j = (int)(q / W), i = (int)(q - (long)j * W);
-  const BatchInst<T> &d = inst[k];
-  float raw[C_IN];
-  state_features(d.U, d.rinv, d.bulk, d.nx, d.ny, d.pitch, d.xoff, d.plane, i, j, raw);
-#pragma unroll
-  for (int c = 0; c < C_INP; c++)
-    X[p * C_INP + c] = c < C_IN ? f2bf((raw[c] - shift_scale[c]) * shift_scale[C_IN + c]) : (bf16)0;
-}
-
 // Per-instance H-weighted energy of U (which = 0) or of the Y0 scratch
 // register (which = 1): blocks [k*bpi, (k+1)*bpi) reduce instance k.
 template <typename T>
@@ -509,36 +677,60 @@ __global__ void batch_energy_final_kernel(const double *__restrict__ partial, in
   e[k] = s;
 }
 
+// Energy of the corrected candidate U + (T)(out_scale * net) of every
+// instance, without storing it: blocks [k*bpi, (k+1)*bpi) reduce instance k.
 template <typename T>
-__global__ __launch_bounds__(256) void batch_candidate_kernel(const BatchInst<T> *__restrict__ inst, int count, int H,
-                                                              int W, const float *__restrict__ net,
-                                                              const float *__restrict__ out_scale)
+__global__ __launch_bounds__(256) void batch_candidate_energy_kernel(const BatchInst<T> *__restrict__ inst, int bpi,
+                                                                     int order, int H, int W,
+                                                                     const float *__restrict__ net,
+                                                                     const float *__restrict__ out_scale,
+                                                                     double *__restrict__ partial)
+{
+  __shared__ double sh[4];
+  const int k = blockIdx.x / bpi, chunk = blockIdx.x % bpi;
+  const BatchInst<T> &d = inst[k];
+  const long npts = (long)H * W;
+  const float *nk = net + (long)k * npts * C_OUT;
+  const double hx = 1.0 / (double)d.hix, hy = 1.0 / (double)d.hiy;
+  double acc = 0;
+  for (long q = chunk * 256L + threadIdx.x; q < npts; q += (long)bpi * 256) {
+    const int j = (int)(q / W), i = (int)(q - (long)j * W);
+    const long o = (long)j * d.pitch + d.xoff + i;
+    const double w = hweight(order, W, i, hx) * hweight(order, H, j, hy);
+    double v[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) v[c] = (double)(T)(d.U[c * d.plane + o] + (T)(nk[q * C_OUT + c] * out_scale[c]));
+    const double rho = d.rinv ? 1.0 / (double)d.rinv[o] : 1.0;
+    const double kinv = d.bulk ? 1.0 / (double)d.bulk[o] : 1.0;
+    acc += w * (rho * (v[0] * v[0] + v[1] * v[1]) + kinv * v[2] * v[2]);
+  }
+  const double sm = block_sum_256(acc, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = sm;
+}
+
+// U += (T)(out_scale * net) where the candidate's energy passed the bound
+// (the same expression the energy was taken of); count the refusals.
+template <typename T>
+__global__ __launch_bounds__(256) void batch_apply_kernel(const BatchInst<T> *__restrict__ inst, int count, int H, int W,
+                                                          const float *__restrict__ net,
+                                                          const float *__restrict__ out_scale,
+                                                          const double *__restrict__ e_ref,
+                                                          const double *__restrict__ e_new, double bound,
+                                                          int *__restrict__ rejected)
 {
   const long p = blockIdx.x * 256L + threadIdx.x;
   const long npi = (long)H * W;
   if (p >= count * npi) return;
   const int k = (int)(p / npi);
   const long q = p - k * npi;
+  const bool ok = e_new[k] <= e_ref[k] * (1.0 + bound) && e_new[k] == e_new[k];
+  if (q == 0 && !ok) rejected[k] += 1;
+  if (!ok) return;
   const int j = (int)(q / W), i = (int)(q - (long)j * W);
   const BatchInst<T> &d = inst[k];
   const long o = (long)j * d.pitch + d.xoff + i;
 #pragma unroll
-  for (int c = 0; c < 3; c++) d.Y0[c * d.plane + o] = d.U[c * d.plane + o] + (T)(net[p * C_OUT + c] * out_scale[c]);
-}
-
-template <typename T>
-__global__ __launch_bounds__(256) void batch_accept_kernel(const BatchInst<T> *__restrict__ inst, int count, long reg,
-                                                           const double *__restrict__ e_old,
-                                                           const double *__restrict__ e_new, double bound,
-                                                           int *__restrict__ rejected)
-{
-  const long p = blockIdx.x * 256L + threadIdx.x;
-  if (p >= count * reg) return;
-  const int k = (int)(p / reg);
-  const long q = p - k * reg;
-  const bool ok = e_new[k] <= e_old[k] * (1.0 + bound) && e_new[k] == e_new[k];
-  if (q == 0 && !ok) rejected[k] += 1;
-  if (ok) inst[k].U[q] = inst[k].Y0[q];
+  for (int c = 0; c < 3; c++) d.U[c * d.plane + o] = d.U[c * d.plane + o] + (T)(net[p * C_OUT + c] * out_scale[c]);
 }
 
 __global__ void clear_ints_kernel(int *__restrict__ p, int n)
@@ -623,6 +815,22 @@ int forward(sbp_corrector *c, int nb)
                      c->w2, c->b2, c->A2, (float *)nullptr, C_H);
   hipLaunchKernelGGL((conv3x3_kernel<C_H, C_OUTP, KP3, 1>), dim3(g), dim3(256), 0, c->stream, c->A2, nb, c->H, c->W,
                      c->w3, c->b3, (bf16 *)nullptr, c->Yo, C_OUT);
+  HIP_OR_FAIL(hipGetLastError());
+  return SBP_OK;
+}
+
+// Network output Yo of the nb packed images X0 (inference: the fused kernel).
+void launch_infer(sbp_corrector *c, int nb, hipStream_t stream)
+{
+  const int rows = 32;
+  const long blocks = (long)nb * ((c->W + FOUT - 1) / FOUT) * ((c->H + rows - 1) / rows);
+  hipLaunchKernelGGL(fused_net_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, c->X0, nb, c->H, c->W, rows,
+                     c->w1, c->b1, c->w2, c->b2, c->w3, c->b3, c->Yo);
+}
+
+int infer(sbp_corrector *c, int nb)
+{
+  launch_infer(c, nb, c->stream);
   HIP_OR_FAIL(hipGetLastError());
   return SBP_OK;
 }
@@ -809,7 +1017,7 @@ int sbp_corr_forward(sbp_corrector *c, const float *raw, int nb, float *out)
     rc = fail(SBP_EHIP, "upload failed");
   if (!rc) {
     hipLaunchKernelGGL(pack_raw_kernel, dim3(grid_for(npix)), dim3(256), 0, c->stream, draw, npix, c->norm, c->X0);
-    rc = forward(c, nb);
+    rc = infer(c, nb);
   }
   if (!rc && (hipMemcpyAsync(out, c->Yo, sizeof(float) * npix * C_OUT, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
               hipStreamSynchronize(c->stream) != hipSuccess))
@@ -920,19 +1128,20 @@ static int apply_one(sbp_corrector *c, sbp_ctx *x, double bound)
 {
   const long npix = (long)x->nx * x->ny;
   const float *ss = c->norm;
+  const unsigned ptiles = (unsigned)(((x->nx + PTW - 1) / PTW) * ((x->ny + PTH - 1) / PTH));
   if (x->esz == 8)
-    hipLaunchKernelGGL(pack_state_kernel<double>, dim3(grid_for(npix)), dim3(256), 0, x->stream,
-                       (const double *)x->U, (const double *)x->rinv, (const double *)x->bulk, x->nx, x->ny, x->pitch,
-                       x->xoff, x->plane, ss, c->X0);
+    hipLaunchKernelGGL(pack_state_tiled_kernel<double>, dim3(ptiles), dim3(256), 0, x->stream, (const double *)x->U,
+                       (const double *)x->rinv, (const double *)x->bulk, x->nx, x->ny, x->pitch, x->xoff, x->plane, ss,
+                       c->X0);
   else
-    hipLaunchKernelGGL(pack_state_kernel<float>, dim3(grid_for(npix)), dim3(256), 0, x->stream, (const float *)x->U,
+    hipLaunchKernelGGL(pack_state_tiled_kernel<float>, dim3(ptiles), dim3(256), 0, x->stream, (const float *)x->U,
                        (const float *)x->rinv, (const float *)x->bulk, x->nx, x->ny, x->pitch, x->xoff, x->plane, ss,
                        c->X0);
   HIP_OR_FAIL(hipGetLastError());
   // the network runs on the context's stream so steps and corrections stay ordered
   hipStream_t keep = c->stream;
   c->stream = x->stream;
-  int rc = forward(c, 1);
+  int rc = infer(c, 1);
   c->stream = keep;
   if (rc) return rc;
   // candidate into the Y0 scratch register; its energy lands in x->scalar and
@@ -1008,8 +1217,8 @@ int sbp_corr_advance_batch(sbp_corrector *c, sbp_batch *b, const double *dt, lon
   int rc = sbp::host::batch_upload_dt(b, dt);
   if (rc) return rc;
   const int count = b->count, bpi = 64;
+  const int ptiles = ((c->W + PTW - 1) / PTW) * ((c->H + PTH - 1) / PTH);
   const long npix = (long)count * c->H * c->W;
-  const long reg = 3 * b->inst[0].plane;
   hipStream_t s = b->stream;
   hipLaunchKernelGGL(clear_ints_kernel, dim3((count + 255) / 256), dim3(256), 0, s, c->brej, count);
   // reference energy of every instance: its energy when the call starts
@@ -1021,41 +1230,62 @@ int sbp_corr_advance_batch(sbp_corrector *c, sbp_batch *b, const double *dt, lon
                        static_cast<const BatchInst<float> *>(b->inst_dev), bpi, 0, b->order, c->bpart);
   hipLaunchKernelGGL(batch_energy_final_kernel, dim3((count + 63) / 64), dim3(64), 0, s, c->bpart, count, bpi, c->be_old);
   HIP_OR_FAIL(hipGetLastError());
-  hipStream_t keep = c->stream;
-  for (long st = 0; st < nsteps; st++) {
+  // one corrected step: plain RK4 stages, features, network, candidate
+  // energy, guarded update -- kernel launches only, so it can be captured
+  auto enqueue_step = [&]() -> hipError_t {
+    hipError_t e = sbp::host::batch_launch_step(b);
+    if (e != hipSuccess) return e;
     if (b->esz == 8) {
       const BatchInst<double> *inst = static_cast<const BatchInst<double> *>(b->inst_dev);
-      HIP_OR_FAIL(sbp::host::batch_launch_step(b));
-      hipLaunchKernelGGL(pack_batch_kernel<double>, dim3(grid_for(npix)), dim3(256), 0, s, inst, count, c->H, c->W,
+      hipLaunchKernelGGL(pack_batch_tiled_kernel<double>, dim3(count * ptiles), dim3(256), 0, s, inst, c->H, c->W,
                          c->norm, c->X0);
-      c->stream = s;
-      rc = forward(c, count);
-      c->stream = keep;
-      if (rc) return rc;
-      hipLaunchKernelGGL(batch_candidate_kernel<double>, dim3(grid_for(npix)), dim3(256), 0, s, inst, count, c->H, c->W,
-                         c->Yo, c->norm + NORM_OUT);
-      hipLaunchKernelGGL(batch_energy_kernel<double>, dim3(count * bpi), dim3(256), 0, s, inst, bpi, 1, b->order, c->bpart);
-      hipLaunchKernelGGL(batch_energy_final_kernel, dim3((count + 63) / 64), dim3(64), 0, s, c->bpart, count, bpi, c->be_new);
-      hipLaunchKernelGGL(batch_accept_kernel<double>, dim3(grid_for(count * reg)), dim3(256), 0, s, inst, count, reg,
-                         c->be_old, c->be_new, bound, c->brej);
+      launch_infer(c, count, s);
+      hipLaunchKernelGGL(batch_candidate_energy_kernel<double>, dim3(count * bpi), dim3(256), 0, s, inst, bpi, b->order,
+                         c->H, c->W, c->Yo, c->norm + NORM_OUT, c->bpart);
+      hipLaunchKernelGGL(batch_energy_final_kernel, dim3((count + 63) / 64), dim3(64), 0, s, c->bpart, count, bpi,
+                         c->be_new);
+      hipLaunchKernelGGL(batch_apply_kernel<double>, dim3(grid_for(npix)), dim3(256), 0, s, inst, count, c->H, c->W,
+                         c->Yo, c->norm + NORM_OUT, c->be_old, c->be_new, bound, c->brej);
     } else {
       const BatchInst<float> *inst = static_cast<const BatchInst<float> *>(b->inst_dev);
-      HIP_OR_FAIL(sbp::host::batch_launch_step(b));
-      hipLaunchKernelGGL(pack_batch_kernel<float>, dim3(grid_for(npix)), dim3(256), 0, s, inst, count, c->H, c->W,
+      hipLaunchKernelGGL(pack_batch_tiled_kernel<float>, dim3(count * ptiles), dim3(256), 0, s, inst, c->H, c->W,
                          c->norm, c->X0);
-      c->stream = s;
-      rc = forward(c, count);
-      c->stream = keep;
-      if (rc) return rc;
-      hipLaunchKernelGGL(batch_candidate_kernel<float>, dim3(grid_for(npix)), dim3(256), 0, s, inst, count, c->H, c->W,
-                         c->Yo, c->norm + NORM_OUT);
-      hipLaunchKernelGGL(batch_energy_kernel<float>, dim3(count * bpi), dim3(256), 0, s, inst, bpi, 1, b->order, c->bpart);
-      hipLaunchKernelGGL(batch_energy_final_kernel, dim3((count + 63) / 64), dim3(64), 0, s, c->bpart, count, bpi, c->be_new);
-      hipLaunchKernelGGL(batch_accept_kernel<float>, dim3(grid_for(count * reg)), dim3(256), 0, s, inst, count, reg,
-                         c->be_old, c->be_new, bound, c->brej);
+      launch_infer(c, count, s);
+      hipLaunchKernelGGL(batch_candidate_energy_kernel<float>, dim3(count * bpi), dim3(256), 0, s, inst, bpi, b->order,
+                         c->H, c->W, c->Yo, c->norm + NORM_OUT, c->bpart);
+      hipLaunchKernelGGL(batch_energy_final_kernel, dim3((count + 63) / 64), dim3(64), 0, s, c->bpart, count, bpi,
+                         c->be_new);
+      hipLaunchKernelGGL(batch_apply_kernel<float>, dim3(grid_for(npix)), dim3(256), 0, s, inst, count, c->H, c->W,
+                         c->Yo, c->norm + NORM_OUT, c->be_old, c->be_new, bound, c->brej);
     }
-    HIP_OR_FAIL(hipGetLastError());
+    return hipGetLastError();
+  };
+  // Runs of kGraphSteps corrected steps are captured once per call and
+  // replayed (no memset nodes: counters are cleared above, outside it).
+  constexpr long kGraphSteps = 16;
+  long done = 0;
+  if (nsteps >= 2 * kGraphSteps) {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ge = nullptr;
+    HIP_OR_FAIL(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    hipError_t le = hipSuccess;
+    for (long k = 0; k < kGraphSteps && le == hipSuccess; k++) le = enqueue_step();
+    hipError_t ce = hipStreamEndCapture(s, &g);
+    if (le != hipSuccess || ce != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      return fail(SBP_EHIP, std::string("corrector graph capture: ") + hipGetErrorString(le != hipSuccess ? le : ce));
+    }
+    hipError_t ie = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ie != hipSuccess) return fail(SBP_EHIP, std::string("corrector graph instantiate: ") + hipGetErrorString(ie));
+    hipError_t re = hipSuccess;
+    for (; done + kGraphSteps <= nsteps && re == hipSuccess; done += kGraphSteps) re = hipGraphLaunch(ge, s);
+    hipError_t se = hipStreamSynchronize(s);
+    (void)hipGraphExecDestroy(ge);
+    if (re != hipSuccess || se != hipSuccess)
+      return fail(SBP_EHIP, std::string("corrector graph launch: ") + hipGetErrorString(re != hipSuccess ? re : se));
   }
+  for (; done < nsteps; done++) HIP_OR_FAIL(enqueue_step());
   std::vector<int> rej(count, 0);
   HIP_OR_FAIL(hipMemcpyAsync(rej.data(), c->brej, sizeof(int) * count, hipMemcpyDeviceToHost, s));
   HIP_OR_FAIL(hipStreamSynchronize(s));

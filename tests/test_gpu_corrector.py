@@ -38,7 +38,7 @@ def rel(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
 
 
-@pytest.mark.parametrize("shape", [(1, 16, 16), (2, 21, 37), (3, 9, 70)])
+@pytest.mark.parametrize("shape", [(1, 16, 16), (2, 21, 37), (3, 9, 70), (1, 75, 130)])
 def test_forward_matches_oracle(capi, shape):
     nb, H, W = shape
     raw, tgt, params, shift, scale, osc = setup(1, nb, H, W)
@@ -50,6 +50,24 @@ def test_forward_matches_oracle(capi, shape):
     assert np.max(np.abs(y - ref)) <= 2e-2 * np.max(np.abs(ref))
     exact, _ = CO.forward(params, raw, shift, scale, exact=True)
     assert np.max(np.abs(y - exact)) <= 5e-2 * np.max(np.abs(exact))
+
+
+@pytest.mark.parametrize("shape", [(2, 21, 37), (1, 75, 130), (2, 257, 257)])
+def test_fused_inference_equals_layer_kernels(capi, shape):
+    """sbp_corr_forward runs the fused three-layer kernel (K = 32 MFMA); the
+    gradient path runs the layer-by-layer kernels (K = 16 MFMA).  Same bf16
+    roundings, FP32 accumulation in a different grouping: the losses agree to
+    1e-6 relative."""
+    nb, H, W = shape
+    raw, tgt, params, shift, scale, osc = setup(5, nb, H, W)
+    with capi.Corrector(W, H, nb) as c:
+        c.params = params
+        c.set_normalization(shift, scale, osc)
+        y = c.forward(raw)
+        loss, _ = c.gradients(raw, tgt)
+    t = tgt.astype(np.float64) / osc.astype(np.float64)
+    lf = float(np.mean((y.astype(np.float64) - t) ** 2))
+    assert lf == pytest.approx(loss, rel=1e-6)
 
 
 def test_gradients_match_oracle(capi):
@@ -149,3 +167,49 @@ def test_corrected_step_uses_host_features(capi):
     corr = (net * osc).reshape(-1)
     assert np.max(np.abs(got - want)) <= 1e-3 * np.max(np.abs(corr))
     assert np.max(np.abs(got - plain)) > 0.5 * np.max(np.abs(corr))
+
+
+@pytest.mark.parametrize("prec", [P.FP64, P.FP32])
+def test_batch_advance_matches_single(capi, prec):
+    """sbp_corr_advance_batch (corrected steps captured in a graph and
+    replayed, then the remainder eagerly) equals the single-context
+    sbp_corr_advance on every instance, bit for bit, when no step is refused;
+    two calls in a row (20 + 40 steps) with the state read in between."""
+    specs = [P.acoustic_box(49, 4, sigma=0.08, materials=True, seed=s, material_seed=s + 3, precision=prec)
+             for s in (21, 22, 23)]
+    dt = P.dt_from_cfl(specs[0][0], 0.25)
+    rng = np.random.default_rng(9)
+    params = (rng.standard_normal(CO.n_params()) * 0.05).astype(np.float32)
+    CR = __import__("importlib").import_module("hpc-reading-course_amd.corrector")
+    raws = []
+    for sp, v0 in specs:  # normalise on stepped states (v = 0 initially)
+        with capi.Context(sp) as ctx:
+            raws.append(CR.raw_features(sp, ctx.rk4_advance(v0, 0.0, dt, 10)))
+    raw = np.stack(raws).reshape(-1, CO.C_IN)
+    shift = raw.mean(axis=0)
+    scale = 1.0 / np.maximum(raw.std(axis=0), 1e-6)
+    osc = np.array([1e-7, 1e-7, 2e-7], np.float32)
+    with capi.Corrector(49, 49, 3) as c:
+        c.params = params
+        c.set_normalization(shift, scale, osc)
+        with capi.Batch([sp for sp, _ in specs]) as b:
+            for k, (_, v0) in enumerate(specs):
+                b.set_state(k, v0)
+            r1 = c.advance_batch(b, dt, 20, 1.0)
+            mid = [b.get_state(k) for k in range(3)]
+            r2 = c.advance_batch(b, dt, 40, 1.0)
+            got = [b.get_state(k) for k in range(3)]
+        assert not r1.any() and not r2.any()
+        for k, (sp, v0) in enumerate(specs):
+            with capi.Context(sp) as ctx:
+                ctx.set_state(v0)
+                assert c.advance(ctx, 0.0, dt, 20, 1.0) == 0
+                assert np.array_equal(ctx.get_state(), mid[k]), k
+                assert c.advance(ctx, 20 * dt, dt, 40, 1.0) == 0
+                assert np.array_equal(ctx.get_state(), got[k]), k
+        # the corrections did change the state
+        with capi.Batch([sp for sp, _ in specs]) as b:
+            for k, (_, v0) in enumerate(specs):
+                b.set_state(k, v0)
+            b.step(dt, 60)
+            assert not np.array_equal(b.get_state(0), got[0])
