@@ -117,46 +117,6 @@ def solver_pairs(n_instances=2000, seed0=100000, t_train=0.5, chunk=64, log=prin
     return out
 
 
-def rollout_pairs(pairs, params, norm, chunk=64, bound=1e-4, log=print):
-    """Solver-in-the-loop pairs: for each pair, the coarse state C_n is the
-    corrected coarse solution (this network, from t = 0) instead of the
-    injected fine state; C' is its plain coarse step and the target is
-    inj(F_{n+1}) - C'.  The network then also learns what its own
-    trajectories look like, and to pull them back."""
-    out = []
-    t0 = time.time()
-    order = sorted(range(len(pairs)), key=lambda q: pairs[q][4])
-    shift, scale, osc = norm
-    for c0 in range(0, len(order), chunk):
-        idx = order[c0:c0 + chunk]
-        seeds = [pairs[q][0] for q in idx]
-        coarse = _instances(seeds, N_COARSE, P.FP64)
-        dt = P.dt_from_cfl(coarse[0][0], CFL)
-        last = max(pairs[q][4] for q in idx)
-        C_n = {}
-        with capi.Batch([c[0] for c in coarse]) as b, capi.Corrector(N_COARSE, N_COARSE, len(idx)) as cor:
-            cor.params = params
-            cor.set_normalization(shift, scale, osc)
-            for k, (_, v0) in enumerate(coarse):
-                b.set_state(k, v0)
-            for n in range(last + 1):
-                for k, q in enumerate(idx):
-                    if pairs[q][4] == n:
-                        C_n[k] = b.get_state(k)
-                if n < last:
-                    cor.advance_batch(b, dt, 1, bound)
-        with capi.Batch([c[0] for c in coarse]) as cb:
-            for k in range(len(idx)):
-                cb.set_state(k, C_n[k])
-            cb.step(dt, 1)
-            for k, q in enumerate(idx):
-                seed, spec, Cp, res, n = pairs[q]
-                Cr = cb.get_state(k).reshape(spec.ny, spec.nx, 3)
-                out.append((seed, spec, Cr, (Cp + res) - Cr, n))
-        log("rollout pairs: %d/%d (%.1f s)" % (min(c0 + chunk, len(order)), len(order), time.time() - t0))
-    return out
-
-
 def featurize(pairs, noise=1.0):
     """(raw inputs, targets) of solver pairs.
 

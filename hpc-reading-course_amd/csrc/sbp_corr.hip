@@ -404,19 +404,20 @@ __global__ __launch_bounds__(256) void fused_net_kernel(const bf16 *__restrict__
   __syncthreads();
 
   const int nsteps = (y1 - y0) + 6;
+  // input pixel (row ri, window column threadIdx.x - 1) of thread threadIdx.x < FPX
+  auto load_in = [&](int ri) {
+    const int col = c0 - 1 + (int)threadIdx.x;
+    uint4 v = {0, 0, 0, 0};
+    if (threadIdx.x < FPX && ri >= 0 && ri < H && col >= 0 && col < W)
+      v = *reinterpret_cast<const uint4 *>(X + (ibase + (long)ri * W + col) * C_INP);
+    return v;
+  };
+  uint4 in_next = load_in(y0 - 3);
   for (int t = 0; t < nsteps; t++) {
-    // input row y0 - 3 + t -> ring slot t % 3 (pixels w = -1 .. FW)
-    {
-      const int ri = y0 - 3 + t;
-      bf16 *dst = s_in[t % 3];
-      if (threadIdx.x < FPX) {
-        const int col = c0 - 1 + threadIdx.x;
-        uint4 v = {0, 0, 0, 0};
-        if (ri >= 0 && ri < H && col >= 0 && col < W)
-          v = *reinterpret_cast<const uint4 *>(X + (ibase + (long)ri * W + col) * C_INP);
-        *reinterpret_cast<uint4 *>(dst + threadIdx.x * C_INP) = v;
-      }
-    }
+    // input row y0 - 3 + t -> ring slot t % 3 (pixels w = -1 .. FW); the
+    // next row's load is issued now and lands while this row is computed
+    if (threadIdx.x < FPX) *reinterpret_cast<uint4 *>(s_in[t % 3] + threadIdx.x * C_INP) = in_next;
+    if (t + 1 < nsteps) in_next = load_in(y0 - 2 + t);
     __syncthreads();
     // layer 1, row r1 = y0 - 4 + t, from input slots (t-2, t-1, t) % 3;
     // K-step ks covers taps 4 ks + kg (all 8 channels of one input pixel)
