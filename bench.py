@@ -222,6 +222,8 @@ def main():
     ap.add_argument("--no-secondary", action="store_true")
     ap.add_argument("--copy-only", action="store_true",
                     help="only run the 1 GiB plain copy kernel (HBM counter calibration)")
+    ap.add_argument("--tta-only", action="store_true",
+                    help="only run the time-to-L2-1e-4 comparison (plain FP64, batched FP32, learned)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the SBP-SAT path here is built and timed on a single "
@@ -237,6 +239,9 @@ def main():
     log("plain copy of 1 GiB: %.0f GB/s" % copy_gbs)
     if args.copy_only:
         print(json.dumps({"copy_1GiB_gbs": copy_gbs}))
+        return
+    if args.tta_only:
+        print(json.dumps(time_to_l2()), flush=True)
         return
 
     spec, v0, dt = P.baseline_acoustic(args.n, args.order)
