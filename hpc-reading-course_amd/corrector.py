@@ -320,20 +320,47 @@ def time_to_accuracy(params, norm, n_eval=32, seed0=900000, t_end=0.5, plain_n=5
     return res
 
 
+def run_learned(nsteps=512, n_eval=16, seed0=900000, bound=1e-4, log=print):
+    """Advance the held-out instances nsteps corrected FP32 steps with the
+    shipped weights (the learned form of time_to_accuracy, without the
+    comparison runs); returns the wall time per step in seconds."""
+    params, norm = load_weights()
+    insts = _instances(list(range(seed0, seed0 + n_eval)), N_COARSE, P.FP32)
+    dt = P.dt_from_cfl(insts[0][0], CFL)
+    with capi.Batch([s for s, _ in insts]) as b, capi.Corrector(N_COARSE, N_COARSE, n_eval) as cor:
+        cor.params = params
+        cor.set_normalization(*norm)
+        for k, (_, v0) in enumerate(insts):
+            b.set_state(k, v0)
+        b.autotune(dt)
+        b.synchronize()
+        t = time.time()
+        rej = cor.advance_batch(b, dt, nsteps, bound)
+        w = time.time() - t
+    log("%d corrected steps of %d x %d^2 (FP32): %.1f us per step, %d refused" % (
+        nsteps, n_eval, N_COARSE, w / nsteps * 1e6, int(np.sum(rej))))
+    return w / nsteps
+
+
 def main(argv=None):
-    """Regenerate the shipped weights: solver pairs, GPU training, time to
-    accuracy.  python -m hpc-reading-course_amd.corrector [--instances N]
-    [--epochs E] [--lr LR] [--noise A] [--out PATH]"""
+    """Regenerate the shipped weights (solver pairs, GPU training, time to
+    accuracy), or with --run N only advance the held-out instances N
+    corrected steps.  python -m hpc-reading-course_amd.corrector [--instances N]
+    [--epochs E] [--lr LR] [--noise A] [--out PATH] [--run STEPS]"""
     import argparse
     import json
     ap = argparse.ArgumentParser(description=main.__doc__)
     ap.add_argument("--instances", type=int, default=2000)
-    ap.add_argument("--epochs", type=int, default=80)
+    ap.add_argument("--epochs", type=int, default=40)
     ap.add_argument("--lr", type=float, default=3e-3)
     ap.add_argument("--noise", type=float, default=2.0)
     ap.add_argument("--out", default=WEIGHTS)
+    ap.add_argument("--run", type=int, default=0)
     a = ap.parse_args(argv)
     say = lambda *x: print(*x, flush=True)
+    if a.run:
+        run_learned(a.run, log=say)
+        return
     raw, tgt = make_training_data(a.instances, noise=a.noise, log=say)
     params, norm, hist = train(raw, tgt, epochs=a.epochs, lr=a.lr, log=say)
     del raw, tgt

@@ -33,9 +33,12 @@ struct BatchConfig {
 };
 
 // Candidate launch shapes of the autotuner.
+// Short tiles (4, 8 rows) are for ensembles of small grids, where tall tiles
+// leave most of the persistent blocks without work.
 constexpr BatchConfig kConfigs[] = {
     {256, 32, 4}, {256, 64, 4}, {256, 16, 4}, {256, 32, 2}, {256, 32, 8}, {256, 64, 2},
     {128, 32, 8}, {128, 64, 8}, {128, 16, 8}, {128, 32, 4}, {128, 64, 4}, {128, 32, 16},
+    {256, 8, 4},  {256, 4, 4},  {128, 8, 8},  {128, 4, 8},
 };
 constexpr int kNumConfigs = sizeof(kConfigs) / sizeof(kConfigs[0]);
 constexpr int kAlignB = 64;

@@ -534,27 +534,28 @@ __global__ __launch_bounds__(256) void gather_kernel(const float *__restrict__ r
 // The host restatement is corrector.raw_features.
 //
 // A block packs a PTW x PTH tile of one image.  The three state planes of the
-// tile plus a 3-point halo are staged in LDS as f[clamp(j)][clamp(i)], so a
-// neighbour read with its index clamped is a plain LDS read; the centres of
-// the differenced fourth differences are clamped into the image first.
+// tile plus a 3-point halo are staged in LDS (in the state's precision) as
+// f[clamp(j)][clamp(i)], so a neighbour read with its index clamped is a
+// plain LDS read; the centres of the differenced fourth differences are
+// clamped into the image first.
 constexpr int PTW = 64, PTH = 16, PTX = PTW + 6, PTY = PTH + 6;
 
 template <typename T>
 __device__ __forceinline__ void pack_tile(const T *__restrict__ U, const T *__restrict__ rinv, const T *__restrict__ bulk,
                                           int nx, int ny, int pitch, int xoff, long plane, int tx, int ty,
                                           const float *__restrict__ shift_scale, bf16 *__restrict__ X,
-                                          double (*s)[PTY][PTX])
+                                          T (*s)[PTY][PTX])
 {
   const int x0 = tx * PTW, y0 = ty * PTH;
   for (int q = threadIdx.x; q < 3 * PTY * PTX; q += 256) {
     const int c = q / (PTY * PTX), r = q - c * (PTY * PTX);
     const int yy = r / PTX, xx = r - yy * PTX;
     const int gi = min(max(x0 - 3 + xx, 0), nx - 1), gj = min(max(y0 - 3 + yy, 0), ny - 1);
-    s[c][yy][xx] = (double)U[c * plane + (long)gj * pitch + xoff + gi];
+    s[c][yy][xx] = U[c * plane + (long)gj * pitch + xoff + gi];
   }
   __syncthreads();
   // image (i, j) -> LDS slot; valid for |i - x0 - PTW/2| <= PTW/2 + 3 etc.
-  auto at = [&](int c, int i, int j) { return s[c][j - y0 + 3][i - x0 + 3]; };
+  auto at = [&](int c, int i, int j) { return (double)s[c][j - y0 + 3][i - x0 + 3]; };
   auto d4x = [&](int c, int i, int j) {
     i = min(max(i, 0), nx - 1);
     return at(c, i - 2, j) - 4 * at(c, i - 1, j) + 6 * at(c, i, j) - 4 * at(c, i + 1, j) + at(c, i + 2, j);
@@ -592,7 +593,7 @@ __global__ __launch_bounds__(256) void pack_state_tiled_kernel(const T *__restri
                                                                const float *__restrict__ shift_scale,
                                                                bf16 *__restrict__ X)
 {
-  __shared__ double s[3][PTY][PTX];
+  __shared__ T s[3][PTY][PTX];  // the state in its own precision (widened exactly)
   const int tiles_x = (nx + PTW - 1) / PTW;
   pack_tile(U, rinv, bulk, nx, ny, pitch, xoff, plane, blockIdx.x % tiles_x, blockIdx.x / tiles_x, shift_scale, X, s);
 }
@@ -602,7 +603,7 @@ __global__ __launch_bounds__(256) void pack_batch_tiled_kernel(const BatchInst<T
                                                                const float *__restrict__ shift_scale,
                                                                bf16 *__restrict__ X)
 {
-  __shared__ double s[3][PTY][PTX];
+  __shared__ T s[3][PTY][PTX];  // the state in its own precision (widened exactly)
   const int tiles_x = (W + PTW - 1) / PTW, tiles_y = (H + PTH - 1) / PTH;
   const int k = blockIdx.x / (tiles_x * tiles_y), t = blockIdx.x - k * tiles_x * tiles_y;
   const BatchInst<T> &d = inst[k];
