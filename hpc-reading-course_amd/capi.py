@@ -55,6 +55,8 @@ EXPORTS = {
     "sbp_set_state": (C.c_int, [C.c_void_p, _DP]),
     "sbp_get_state": (C.c_int, [C.c_void_p, _DP]),
     "sbp_step": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_long]),
+    "sbp_rk45": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, C.c_long,
+                           C.POINTER(C.c_long), C.POINTER(C.c_long), _DP]),
     "sbp_synchronize": (C.c_int, [C.c_void_p]),
     "sbp_time_steps": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_long, _DP]),
     "sbp_time_stages": (C.c_int, [C.c_void_p, C.c_double, C.c_long, _DP]),
@@ -203,6 +205,14 @@ class Context:
 
     def synchronize(self):
         _check(load_library().sbp_synchronize(self._h))
+
+    def rk45(self, t0: float, t_end: float, dt0: float, atol: float = 1e-4, rtol: float = 1e-4,
+             max_steps: int = 1000000):
+        """Adaptive RKF45 of the resident state to t_end; (accepted, rejected, dt_next)."""
+        a, r, d = C.c_long(), C.c_long(), C.c_double()
+        _check(load_library().sbp_rk45(self._h, t0, t_end, dt0, atol, rtol, int(max_steps), C.byref(a),
+                                       C.byref(r), C.byref(d)))
+        return a.value, r.value, d.value
 
     def time_steps(self, t0: float, dt: float, nsteps: int) -> float:
         ms = C.c_double()
@@ -481,6 +491,16 @@ def ts_rk4(ctx: Context, t_end: float, dt: float, v: np.ndarray, t0: float = 0.0
         ctx.step(t0 + n * dt, last, 1)
     v.reshape(-1)[:] = ctx.get_state()
     return t_end if (n or last) else t0
+
+
+def ts_rk45(ctx: Context, t_end: float, dt: float, v: np.ndarray, t0: float = 0.0, atol: float = 1e-4,
+            rtol: float = 1e-4):
+    """ts_rk45 (ts_rk.cpp:41-52): adaptive RKF45 of v in place from t0 to
+    t_end, first step dt; returns (accepted, rejected)."""
+    ctx.set_state(v)
+    a, r, _ = ctx.rk45(t0, t_end, dt, atol, rtol)
+    v.reshape(-1)[:] = ctx.get_state()
+    return a, r
 
 
 def copy_bandwidth(nbytes: int = 1 << 30, reps: int = 5) -> float:

@@ -404,6 +404,8 @@ void release(sbp_ctx *c)
                   c->c4y, c->s4y, c->aos, c->partials, c->scalar};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
+  for (void *p : c->rkw)
+    if (p) (void)hipFree(p);
   for (hipEvent_t &e : c->ev)
     if (e) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -491,6 +493,11 @@ int build_ctx(const sbp_problem *desc, sbp_ctx *c)
 }  // namespace
 
 hipError_t sbp::host::launch_step(sbp_ctx *c, double t, double dt) { return ::launch_step(c, t, dt); }
+hipError_t sbp::host::launch_rhs(sbp_ctx *c, double t, const void *src, void *dst)
+{
+  return ::launch_stage(c, MODE_RHS, 0.0, t, src, dst);
+}
+void sbp::host::drop_step_graph(sbp_ctx *c) { ::drop_graph(c); }
 
 int sbp::host::energy_of(sbp_ctx *c, const void *u, const void *comp, double *out)
 {

@@ -28,6 +28,7 @@ struct sbp_ctx {
   double g_dt = 0;
   hipEvent_t ev[5] = {};
   int device = 0;
+  void *rkw[8] = {};  // adaptive RK workspace (K1..K6, stage input, candidate), allocated on first use
 };
 
 
@@ -39,5 +40,9 @@ hipError_t launch_step(sbp_ctx *c, double t, double dt);
 // (U and optional compensation); result left in c->scalar (device) and, when
 // out != nullptr, copied to the host.
 int energy_of(sbp_ctx *c, const void *u, const void *comp, double *out);
+// dst = F(t, src) on planar registers of the context (the RHS stage kernels).
+hipError_t launch_rhs(sbp_ctx *c, double t, const void *src, void *dst);
+// Forget the captured step graph (after the state register moved).
+void drop_step_graph(sbp_ctx *c);
 }  // namespace host
 }  // namespace sbp

@@ -85,6 +85,17 @@ inline int ts_rk4(double t_end, double dt, double *v, AppCtx &ctx, double t0 = 0
   return rc;
 }
 
+// ts_rk45 (ts_rk.cpp:41-52): adaptive Runge-Kutta-Fehlberg 4(5) of v from t0
+// to t_end, first step dt, PETSc's default tolerances unless given.
+inline int ts_rk45(double t_end, double dt, double *v, AppCtx &ctx, double t0 = 0.0, double atol = 1e-4,
+                   double rtol = 1e-4, long *accepted = nullptr, long *rejected = nullptr)
+{
+  int rc = sbp_set_state(ctx.get(), v);
+  if (!rc) rc = sbp_rk45(ctx.get(), t0, t_end, dt, atol, rtol, 100000000L, accepted, rejected, nullptr);
+  if (!rc) rc = sbp_get_state(ctx.get(), v);
+  return rc;
+}
+
 // error_l2 / error_max of code/include/util/vec_util.h:47-61.
 inline int error_norms(const AppCtx &ctx, const double *v, const double *v_ref, double *l2, double *maxerr)
 {

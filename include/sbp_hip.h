@@ -111,6 +111,20 @@ int  sbp_rhs(sbp_ctx *ctx, double t, const double *src, double *dst);
  * (code/include/time_stepping/time_stepping.h:127-142,267-269). */
 int  sbp_rk4_advance(sbp_ctx *ctx, double *v, double t0, double dt, long nsteps);
 
+/* Adaptive Runge-Kutta-Fehlberg 4(5) on the resident state (FP64 contexts)
+ * from t0 to t_end, first attempted step dt0.  Replaces ts_rk45
+ * (ts_rk.cpp:41-52: TSRK5F, the fifth-order solution propagated and the
+ * embedded fourth-order one estimating the error; TSADAPTBASIC;
+ * TS_EXACTFINALTIME_MATCHSTEP).  A step is accepted when
+ *   sqrt(mean(((y - y_hat) / (atol + rtol*max(|y|, |y_hat|)))^2)) <= 1
+ * and the next step is h * clip(0.9 * enorm^(-1/5) (halved after a
+ * rejection), 0.1, 10) -- PETSc's documented basic adaptor with its default
+ * safety factors (its defaults for the tolerances are atol = rtol = 1e-4).
+ * Fails with SBP_EINVAL after max_steps attempts.  accepted, rejected and
+ * dt_next (the step the controller would take next) are optional. */
+int  sbp_rk45(sbp_ctx *ctx, double t0, double t_end, double dt0, double atol, double rtol,
+              long max_steps, long *accepted, long *rejected, double *dt_next);
+
 /* ---- device-resident state (the time loop stays on the GPU) -------------- */
 int  sbp_set_state(sbp_ctx *ctx, const double *v);  /* host AoS -> device state */
 int  sbp_get_state(sbp_ctx *ctx, double *v);        /* device state -> host AoS */

@@ -56,6 +56,10 @@ def _load(name):
         lib.orc_hnorm.argtypes = [P, _DP]
         lib.orc_hnorm.restype = C.c_double
         lib.orc_error_norms.argtypes = [P, _DP, _DP, _DP, _DP]
+        _LP = C.POINTER(C.c_long)
+        lib.orc_rk45.argtypes = [P, _DP, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, C.c_long,
+                                 _LP, _LP, _DP]
+        lib.orc_rkf45_tableau.argtypes = [_DP, _DP, _DP, _DP]
         _libs[name] = lib
     return _libs[name]
 
@@ -126,6 +130,26 @@ def rk4(spec, v: np.ndarray, t0: float, dt: float, nsteps: int, threads: int = 1
     if rc:
         raise ValueError("orc_rk4 rejected the problem")
     return out
+
+
+def rk45(spec, v: np.ndarray, t0: float, t_end: float, dt0: float, atol: float = 1e-4,
+         rtol: float = 1e-4, max_steps: int = 1000000):
+    """Adaptive RKF45 (orc_rk45); returns (v, accepted, rejected, dt_next)."""
+    d = _Desc(spec)
+    out = np.array(v, dtype=np.float64, copy=True)
+    acc, rej, dtn = C.c_long(), C.c_long(), C.c_double()
+    rc = lib().orc_rk45(d.ref, _ptr(out), t0, t_end, dt0, atol, rtol, max_steps, C.byref(acc), C.byref(rej),
+                        C.byref(dtn))
+    if rc:
+        raise ValueError("orc_rk45 failed (%d)" % rc)
+    return out, acc.value, rej.value, dtn.value
+
+
+def rkf45_tableau():
+    """(c[6], a[6][5], b5[6], b4[6]) as compiled into the oracle."""
+    c, a, b5, b4 = np.zeros(6), np.zeros((6, 5)), np.zeros(6), np.zeros(6)
+    lib().orc_rkf45_tableau(_ptr(c), _ptr(a), _ptr(b5), _ptr(b4))
+    return c, a, b5, b4
 
 
 def energy(spec, v: np.ndarray) -> float:

@@ -623,3 +623,103 @@ void orc_error_norms(const sbp_problem *p, const double *v, const double *w,
   if (l2) *l2 = sqrt(hprod) * sqrt(ss);
   if (maxerr) *maxerr = mx;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Adaptive Runge-Kutta-Fehlberg 4(5): what ts_rk45 (ts_rk.cpp:41-52) asks   */
+/* PETSc for -- TSRK5F (fifth-order solution propagated, fourth-order        */
+/* embedded estimate), TSADAPTBASIC, TS_EXACTFINALTIME_MATCHSTEP.  PETSc's   */
+/* source is not part of the reference; the controller restates its          */
+/* documented behaviour and defaults (safety 0.9, reject safety 0.5, clip    */
+/* [0.1, 10], weighted 2-norm with tol = atol + rtol*max(|y|, |y_hat|)).     */
+/* ------------------------------------------------------------------------ */
+/* Fehlberg's coefficients (E. Fehlberg, NASA TR R-315, 1969). */
+static const double RKF_C[6] = {0.0, 1.0 / 4.0, 3.0 / 8.0, 12.0 / 13.0, 1.0, 1.0 / 2.0};
+static const double RKF_A[6][5] = {
+    {0, 0, 0, 0, 0},
+    {1.0 / 4.0, 0, 0, 0, 0},
+    {3.0 / 32.0, 9.0 / 32.0, 0, 0, 0},
+    {1932.0 / 2197.0, -7200.0 / 2197.0, 7296.0 / 2197.0, 0, 0},
+    {439.0 / 216.0, -8.0, 3680.0 / 513.0, -845.0 / 4104.0, 0},
+    {-8.0 / 27.0, 2.0, -3544.0 / 2565.0, 1859.0 / 4104.0, -11.0 / 40.0}};
+static const double RKF_B5[6] = {16.0 / 135.0, 0.0, 6656.0 / 12825.0, 28561.0 / 56430.0, -9.0 / 50.0, 2.0 / 55.0};
+static const double RKF_B4[6] = {25.0 / 216.0, 0.0, 1408.0 / 2565.0, 2197.0 / 4104.0, -1.0 / 5.0, 0.0};
+
+void orc_rkf45_tableau(double c[6], double a[6][5], double b5[6], double b4[6])
+{
+  for (int s = 0; s < 6; s++) {
+    c[s] = RKF_C[s];
+    b5[s] = RKF_B5[s];
+    b4[s] = RKF_B4[s];
+    for (int j = 0; j < 5; j++) a[s][j] = RKF_A[s][j];
+  }
+}
+
+int orc_rk45(const sbp_problem *p, double *v, double t0, double t_end, double dt0, double atol,
+             double rtol, long max_steps, long *accepted, long *rejected, double *dt_next)
+{
+  if (orc_validate(p) || !(dt0 > 0) || !(t_end >= t0)) return -1;
+  const long n = (long)orc_components(p) * p->nx * (p->ny > 0 ? p->ny : 1);
+  double *K[6], *Y = malloc(sizeof(double) * n), *U5 = malloc(sizeof(double) * n);
+  int ok = Y && U5;
+  for (int s = 0; s < 6; s++) ok = ok && (K[s] = malloc(sizeof(double) * n)) != NULL;
+  if (!ok) return -1;
+  double e[6];
+  for (int s = 0; s < 6; s++) e[s] = RKF_B5[s] - RKF_B4[s];
+  double t = t0, dt = dt0;
+  long acc = 0, rej = 0;
+  int rc = 0;
+  while (t < t_end) {
+    if (acc + rej >= max_steps) {
+      rc = -2;
+      break;
+    }
+    int last = 0;
+    double h = dt;
+    if (t + h >= t_end) {  /* MATCHSTEP: land on t_end */
+      h = t_end - t;
+      last = 1;
+    }
+    orc_rhs(p, t, v, K[0]);
+    for (int s = 1; s < 6; s++) {
+      for (long k = 0; k < n; k++) {
+        double sum = RKF_A[s][0] * K[0][k];
+        for (int j = 1; j < s; j++) sum = sum + RKF_A[s][j] * K[j][k];
+        Y[k] = v[k] + h * sum;
+      }
+      orc_rhs(p, t + RKF_C[s] * h, Y, K[s]);
+    }
+    double ss = 0;
+    for (long k = 0; k < n; k++) {
+      double sb = RKF_B5[0] * K[0][k], se = e[0] * K[0][k];
+      for (int j = 1; j < 6; j++) {
+        sb = sb + RKF_B5[j] * K[j][k];
+        se = se + e[j] * K[j][k];
+      }
+      const double u5 = v[k] + h * sb, err = h * se, y4 = u5 - err;
+      const double tol = atol + rtol * fmax(fabs(u5), fabs(y4));
+      const double w = err / tol;
+      ss += w * w;
+      U5[k] = u5;
+    }
+    const double enorm = sqrt(ss / (double)n);
+    const int accept = enorm <= 1.0;
+    double hfac = enorm > 0 ? 0.9 * pow(enorm, -1.0 / 5.0) : 10.0;
+    if (!accept) hfac *= 0.5;
+    hfac = fmin(fmax(hfac, 0.1), 10.0);
+    if (accept) {
+      memcpy(v, U5, sizeof(double) * n);
+      t = last ? t_end : t + h;
+      acc++;
+    } else {
+      rej++;
+    }
+    dt = h * hfac;
+  }
+  for (int s = 0; s < 6; s++) free(K[s]);
+  free(Y);
+  free(U5);
+  if (accepted) *accepted = acc;
+  if (rejected) *rejected = rej;
+  if (dt_next) *dt_next = dt;
+  return rc;
+}

@@ -44,6 +44,13 @@ int    orc_rk4(const sbp_problem *p, double *v, double t0, double dt, long nstep
 int    orc_rk4_par(const sbp_problem *p, double *v, double t0, double dt, long nsteps,
                    int nthreads);
 
+/* Adaptive Runge-Kutta-Fehlberg 4(5) from t0 to t_end (ts_rk45, ts_rk.cpp:41-52:
+ * TSRK5F + TSADAPTBASIC + MATCHSTEP).  Returns 0, -1 on bad input, -2 when
+ * max_steps attempts did not reach t_end. */
+int    orc_rk45(const sbp_problem *p, double *v, double t0, double t_end, double dt0, double atol,
+                double rtol, long max_steps, long *accepted, long *rejected, double *dt_next);
+void   orc_rkf45_tableau(double c[6], double a[6][5], double b5[6], double b4[6]);
+
 double orc_energy(const sbp_problem *p, const double *v);
 double orc_hnorm(const sbp_problem *p, const double *v);
 void   orc_error_norms(const sbp_problem *p, const double *v, const double *w,
