@@ -401,7 +401,7 @@ void release(sbp_ctx *c)
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   drop_graph(c);
   void *ptrs[] = {c->U, c->A, c->Y0, c->Y1, c->KC, c->TK, c->rinv, c->bulk, c->xs, c->ys, c->c3x, c->s3x,
-                  c->c4y, c->s4y, c->aos, c->partials, c->scalar};
+                  c->c4y, c->s4y, c->aos, c->aos2, c->partials, c->scalar};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   for (void *p : c->rkw)
@@ -463,7 +463,7 @@ int build_ctx(const sbp_problem *desc, sbp_ctx *c)
     if (p) HIP_OR_FAIL(hipMemsetAsync(p, 0, reg_bytes, c->stream));
   if ((rc = dalloc(&c->aos, sizeof(double) * (size_t)c->C * c->nx * c->ny))) return rc;
   c->npartials = 1024;
-  if ((rc = dalloc(&c->partials, sizeof(double) * c->npartials)) || (rc = dalloc(&c->scalar, sizeof(double))))
+  if ((rc = dalloc(&c->partials, sizeof(double) * c->npartials)) || (rc = dalloc(&c->scalar, 2 * sizeof(double))))
     return rc;
   if (c->phys == SBP_ACOUSTIC_2D && c->mat == SBP_MAT_ARRAYS) {
     if ((rc = upload_plane(c, desc->rho_inv, &c->rinv)) || (rc = upload_plane(c, desc->bulk, &c->bulk))) return rc;
@@ -491,6 +491,60 @@ int build_ctx(const sbp_problem *desc, sbp_ctx *c)
 }
 
 }  // namespace
+
+// Sum of squares and maximum of |v - w| over two FP64 vectors: per-block
+// partials (sum, max) reduced with wavefront shuffles, then one block.
+static __global__ __launch_bounds__(256) void diff_norms_kernel(const double *__restrict__ v,
+                                                                 const double *__restrict__ w, long n,
+                                                                 double *__restrict__ partial)
+{
+  __shared__ double ss[4], mm[4];
+  double s2 = 0, mx = 0;
+  for (long k = blockIdx.x * 256L + threadIdx.x; k < n; k += (long)gridDim.x * 256) {
+    const double d = fabs(v[k] - w[k]);
+    s2 += d * d;
+    mx = fmax(mx, d);
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    s2 += __shfl_down(s2, off, 64);
+    mx = fmax(mx, __shfl_down(mx, off, 64));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    ss[threadIdx.x >> 6] = s2;
+    mm[threadIdx.x >> 6] = mx;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    partial[2 * blockIdx.x] = (ss[0] + ss[1]) + (ss[2] + ss[3]);
+    partial[2 * blockIdx.x + 1] = fmax(fmax(mm[0], mm[1]), fmax(mm[2], mm[3]));
+  }
+}
+
+static __global__ __launch_bounds__(256) void diff_norms_final_kernel(const double *__restrict__ partial, int nb,
+                                                                       double *__restrict__ out)
+{
+  __shared__ double ss[4], mm[4];
+  double s2 = 0, mx = 0;
+  for (int k = threadIdx.x; k < nb; k += 256) {
+    s2 += partial[2 * k];
+    mx = fmax(mx, partial[2 * k + 1]);
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    s2 += __shfl_down(s2, off, 64);
+    mx = fmax(mx, __shfl_down(mx, off, 64));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    ss[threadIdx.x >> 6] = s2;
+    mm[threadIdx.x >> 6] = mx;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    out[0] = (ss[0] + ss[1]) + (ss[2] + ss[3]);
+    out[1] = fmax(fmax(mm[0], mm[1]), fmax(mm[2], mm[3]));
+  }
+}
 
 hipError_t sbp::host::launch_step(sbp_ctx *c, double t, double dt) { return ::launch_step(c, t, dt); }
 hipError_t sbp::host::launch_rhs(sbp_ctx *c, double t, const void *src, void *dst)
@@ -687,20 +741,27 @@ int sbp_hnorm(sbp_ctx *ctx, const double *v, double *norm)
   return SBP_OK;
 }
 
-int sbp_error_norms(const sbp_ctx *ctx, const double *v, const double *w, double *l2, double *maxerr)
+int sbp_error_norms(sbp_ctx *ctx, const double *v, const double *w, double *l2, double *maxerr)
 {
   if (!ctx || !v || !w) return fail(SBP_EINVAL, "null argument");
   const long n = (long)ctx->C * ctx->nx * ctx->ny;
-  double ss = 0, mx = 0;
-  for (long k = 0; k < n; k++) {
-    const double d = std::fabs(v[k] - w[k]);
-    ss += d * d;
-    if (d > mx) mx = d;
+  if (!ctx->aos2) {
+    int rc = dalloc(&ctx->aos2, sizeof(double) * (size_t)n);
+    if (rc) return rc;
   }
+  HIP_OR_FAIL(hipMemcpyAsync(ctx->aos, v, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream));
+  HIP_OR_FAIL(hipMemcpyAsync(ctx->aos2, w, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream));
+  const int grid = (int)std::max<long>(1, std::min<long>((n + 255) / 256, ctx->npartials / 2));
+  hipLaunchKernelGGL(diff_norms_kernel, dim3(grid), dim3(256), 0, ctx->stream, ctx->aos, ctx->aos2, n, ctx->partials);
+  hipLaunchKernelGGL(diff_norms_final_kernel, dim3(1), dim3(256), 0, ctx->stream, ctx->partials, grid, ctx->scalar);
+  HIP_OR_FAIL(hipGetLastError());
+  double r[2] = {0, 0};
+  HIP_OR_FAIL(hipMemcpyAsync(r, ctx->scalar, sizeof(r), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_OR_FAIL(hipStreamSynchronize(ctx->stream));
   double hprod = 1.0 / ctx->hix;
   if (!is1d(ctx->phys)) hprod *= 1.0 / ctx->hiy;
-  if (l2) *l2 = std::sqrt(hprod) * std::sqrt(ss);
-  if (maxerr) *maxerr = mx;
+  if (l2) *l2 = std::sqrt(hprod) * std::sqrt(r[0]);
+  if (maxerr) *maxerr = r[1];
   return SBP_OK;
 }
 

@@ -118,6 +118,9 @@ int ensure_workspace(sbp_ctx *c)
       p = nullptr;
       return fail(SBP_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
     }
+    // zeroed like the context's own registers (a workspace register can
+    // become the state register, padding included)
+    HIP_OR_FAIL(hipMemsetAsync(p, 0, bytes, c->stream));
   }
   return SBP_OK;
 }

@@ -152,8 +152,8 @@ int  sbp_state_energy(sbp_ctx *ctx, double *E);                  /* resident sta
 int  sbp_hnorm(sbp_ctx *ctx, const double *v, double *norm);
 
 /* Error norms of vec_util.h:47-61: l2 = sqrt(hx*hy)*||v-w||_2, max = ||v-w||_inf. */
-int  sbp_error_norms(const sbp_ctx *ctx, const double *v, const double *w,
-                     double *l2, double *maxerr);
+int  sbp_error_norms(sbp_ctx *ctx, const double *v, const double *w,
+                     double *l2, double *maxerr);  /* on the device, FP64 */
 
 /* ---- batched ensemble ---------------------------------------------------
  * Many independent 2D acoustic problems (uneven sizes, own materials, own dt)

@@ -252,3 +252,18 @@ def test_errors_are_reported(capi):
             ctx.get_state()
         assert e.value.code == -5
     assert capi.device_arch().startswith("gfx950")
+
+
+@pytest.mark.parametrize("prec", [P.FP64, P.FP32])
+def test_error_norms_on_device(capi, orc, prec):
+    """sbp_error_norms (vec_util.h:47-61) reduces on the device in FP64, for
+    FP32 contexts too: equal to the oracle's norms to summation order."""
+    spec, v0 = P.wave(37, 4)
+    spec = spec.with_(material=P.MAT_UNIT, precision=prec)
+    rng = np.random.default_rng(3)
+    w = v0 + 1e-3 * rng.standard_normal(v0.shape)
+    with capi.Context(spec) as ctx:
+        l2, mx = ctx.error_norms(v0, w)
+    rl2, rmx = orc.error_norms(spec, v0, w)
+    assert l2 == pytest.approx(rl2, rel=1e-12)
+    assert mx == rmx
