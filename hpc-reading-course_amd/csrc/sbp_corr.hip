@@ -533,19 +533,23 @@ __global__ __launch_bounds__(256) void gather_kernel(const float *__restrict__ r
 // (about 1e-6 of the state) is far below bf16's resolution of the state.
 // The host restatement is corrector.raw_features.
 //
-// A block packs a PTW x PTH tile of one image.  The three state planes of the
+// A block packs a PTW x pth<T> tile of one image (taller for FP32 states,
+// whose staging takes half the LDS).  The three state planes of the
 // tile plus a 3-point halo are staged in LDS (in the state's precision) as
 // f[clamp(j)][clamp(i)], so a neighbour read with its index clamped is a
 // plain LDS read; the centres of the differenced fourth differences are
 // clamped into the image first.
-constexpr int PTW = 64, PTH = 16, PTX = PTW + 6, PTY = PTH + 6;
+constexpr int PTW = 64, PTX = PTW + 6;
+template <typename T> constexpr int pth = sizeof(T) == 4 ? 32 : 16;
+template <typename T> constexpr int pty = pth<T> + 6;
 
 template <typename T>
 __device__ __forceinline__ void pack_tile(const T *__restrict__ U, const T *__restrict__ rinv, const T *__restrict__ bulk,
                                           int nx, int ny, int pitch, int xoff, long plane, int tx, int ty,
                                           const float *__restrict__ shift_scale, bf16 *__restrict__ X,
-                                          T (*s)[PTY][PTX])
+                                          T (*s)[pty<T>][PTX])
 {
+  constexpr int PTH = pth<T>, PTY = pty<T>;
   const int x0 = tx * PTW, y0 = ty * PTH;
   for (int q = threadIdx.x; q < 3 * PTY * PTX; q += 256) {
     const int c = q / (PTY * PTX), r = q - c * (PTY * PTX);
@@ -593,7 +597,7 @@ __global__ __launch_bounds__(256) void pack_state_tiled_kernel(const T *__restri
                                                                const float *__restrict__ shift_scale,
                                                                bf16 *__restrict__ X)
 {
-  __shared__ T s[3][PTY][PTX];  // the state in its own precision (widened exactly)
+  __shared__ T s[3][pty<T>][PTX];  // the state in its own precision (widened exactly)
   const int tiles_x = (nx + PTW - 1) / PTW;
   pack_tile(U, rinv, bulk, nx, ny, pitch, xoff, plane, blockIdx.x % tiles_x, blockIdx.x / tiles_x, shift_scale, X, s);
 }
@@ -603,8 +607,8 @@ __global__ __launch_bounds__(256) void pack_batch_tiled_kernel(const BatchInst<T
                                                                const float *__restrict__ shift_scale,
                                                                bf16 *__restrict__ X)
 {
-  __shared__ T s[3][PTY][PTX];  // the state in its own precision (widened exactly)
-  const int tiles_x = (W + PTW - 1) / PTW, tiles_y = (H + PTH - 1) / PTH;
+  __shared__ T s[3][pty<T>][PTX];  // the state in its own precision (widened exactly)
+  const int tiles_x = (W + PTW - 1) / PTW, tiles_y = (H + pth<T> - 1) / pth<T>;
   const int k = blockIdx.x / (tiles_x * tiles_y), t = blockIdx.x - k * tiles_x * tiles_y;
   const BatchInst<T> &d = inst[k];
   pack_tile(d.U, d.rinv, d.bulk, W, H, d.pitch, d.xoff, d.plane, t % tiles_x, t / tiles_x, shift_scale,
@@ -711,21 +715,37 @@ __global__ __launch_bounds__(256) void batch_candidate_energy_kernel(const Batch
 }
 
 // U += (T)(out_scale * net) where the candidate's energy passed the bound
-// (the same expression the energy was taken of); count the refusals.
+// (the same expression the energy was taken of); count the refusals.  The
+// candidate energies are summed here from their per-block partials (bpi per
+// instance; each wavefront takes every fourth instance this block touches,
+// at most 256 of them).
 template <typename T>
 __global__ __launch_bounds__(256) void batch_apply_kernel(const BatchInst<T> *__restrict__ inst, int count, int H, int W,
                                                           const float *__restrict__ net,
                                                           const float *__restrict__ out_scale,
                                                           const double *__restrict__ e_ref,
-                                                          const double *__restrict__ e_new, double bound,
-                                                          int *__restrict__ rejected)
+                                                          const double *__restrict__ cand_partial, int bpi,
+                                                          double bound, int *__restrict__ rejected)
 {
-  const long p = blockIdx.x * 256L + threadIdx.x;
+  __shared__ bool s_ok[256];
   const long npi = (long)H * W;
+  const long p0 = blockIdx.x * 256L;
+  const int k0 = (int)(p0 / npi);
+  const int k1 = (int)(min(p0 + 255, count * npi - 1) / npi);  // k1 - k0 <= 255
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int k = k0 + wv; k <= k1; k += 4) {
+    double e = 0;
+    for (int b = lane; b < bpi; b += 64) e += cand_partial[(long)k * bpi + b];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) e += __shfl_down(e, off, 64);
+    if (lane == 0) s_ok[k - k0] = e <= e_ref[k] * (1.0 + bound) && e == e;
+  }
+  __syncthreads();
+  const long p = p0 + threadIdx.x;
   if (p >= count * npi) return;
   const int k = (int)(p / npi);
   const long q = p - k * npi;
-  const bool ok = e_new[k] <= e_ref[k] * (1.0 + bound) && e_new[k] == e_new[k];
+  const bool ok = s_ok[k - k0];
   if (q == 0 && !ok) rejected[k] += 1;
   if (!ok) return;
   const int j = (int)(q / W), i = (int)(q - (long)j * W);
@@ -760,7 +780,7 @@ struct sbp_corrector {
   float *Yo = nullptr, *Tg = nullptr, *dA = nullptr, *wpart = nullptr;
   double *lpart = nullptr, *scalars = nullptr;  // scalars: [0] loss, [1] E_old
   int *rejected = nullptr, *idx = nullptr;
-  double *bpart = nullptr, *be_old = nullptr, *be_new = nullptr;  // batched energies
+  double *bpart = nullptr, *be_old = nullptr;  // batched energies: per-block partials, reference energies
   int *brej = nullptr;
   float *ds_raw = nullptr, *ds_tgt = nullptr;
   int ds_n = 0;
@@ -775,7 +795,7 @@ void release(sbp_corrector *c)
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   void *ptrs[] = {c->params, c->grads, c->m, c->v, c->w1, c->w2, c->w3, c->w2t, c->w3t, c->b1, c->b2, c->b3,
                   c->norm, c->X0, c->A1, c->A2, c->dY3, c->dP2, c->dP1, c->Yo, c->Tg, c->dA, c->wpart,
-                  c->lpart, c->scalars, c->rejected, c->idx, c->ds_raw, c->ds_tgt, c->bpart, c->be_old, c->be_new,
+                  c->lpart, c->scalars, c->rejected, c->idx, c->ds_raw, c->ds_tgt, c->bpart, c->be_old,
                   c->brej};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -936,7 +956,6 @@ int sbp_corr_create(int nx, int ny, int batch, sbp_corrector **out)
   if (!rc) rc = dmalloc(&c->idx, sizeof(int) * batch);
   if (!rc) rc = dmalloc(&c->bpart, sizeof(double) * batch * 64);
   if (!rc) rc = dmalloc(&c->be_old, sizeof(double) * batch);
-  if (!rc) rc = dmalloc(&c->be_new, sizeof(double) * batch);
   if (!rc) rc = dmalloc(&c->brej, sizeof(int) * batch);
   if (!rc) {
     // Default: zero network (identity correction), unit normalisation.
@@ -1130,7 +1149,8 @@ static int apply_one(sbp_corrector *c, sbp_ctx *x, double bound)
 {
   const long npix = (long)x->nx * x->ny;
   const float *ss = c->norm;
-  const unsigned ptiles = (unsigned)(((x->nx + PTW - 1) / PTW) * ((x->ny + PTH - 1) / PTH));
+  const int pth_ = x->esz == 8 ? pth<double> : pth<float>;
+  const unsigned ptiles = (unsigned)(((x->nx + PTW - 1) / PTW) * ((x->ny + pth_ - 1) / pth_));
   if (x->esz == 8)
     hipLaunchKernelGGL(pack_state_tiled_kernel<double>, dim3(ptiles), dim3(256), 0, x->stream, (const double *)x->U,
                        (const double *)x->rinv, (const double *)x->bulk, x->nx, x->ny, x->pitch, x->xoff, x->plane, ss,
@@ -1219,7 +1239,8 @@ int sbp_corr_advance_batch(sbp_corrector *c, sbp_batch *b, const double *dt, lon
   int rc = sbp::host::batch_upload_dt(b, dt);
   if (rc) return rc;
   const int count = b->count, bpi = 64;
-  const int ptiles = ((c->W + PTW - 1) / PTW) * ((c->H + PTH - 1) / PTH);
+  const int pth_ = b->esz == 8 ? pth<double> : pth<float>;
+  const int ptiles = ((c->W + PTW - 1) / PTW) * ((c->H + pth_ - 1) / pth_);
   const long npix = (long)count * c->H * c->W;
   hipStream_t s = b->stream;
   hipLaunchKernelGGL(clear_ints_kernel, dim3((count + 255) / 256), dim3(256), 0, s, c->brej, count);
@@ -1244,10 +1265,8 @@ int sbp_corr_advance_batch(sbp_corrector *c, sbp_batch *b, const double *dt, lon
       launch_infer(c, count, s);
       hipLaunchKernelGGL(batch_candidate_energy_kernel<double>, dim3(count * bpi), dim3(256), 0, s, inst, bpi, b->order,
                          c->H, c->W, c->Yo, c->norm + NORM_OUT, c->bpart);
-      hipLaunchKernelGGL(batch_energy_final_kernel, dim3((count + 63) / 64), dim3(64), 0, s, c->bpart, count, bpi,
-                         c->be_new);
       hipLaunchKernelGGL(batch_apply_kernel<double>, dim3(grid_for(npix)), dim3(256), 0, s, inst, count, c->H, c->W,
-                         c->Yo, c->norm + NORM_OUT, c->be_old, c->be_new, bound, c->brej);
+                         c->Yo, c->norm + NORM_OUT, c->be_old, c->bpart, bpi, bound, c->brej);
     } else {
       const BatchInst<float> *inst = static_cast<const BatchInst<float> *>(b->inst_dev);
       hipLaunchKernelGGL(pack_batch_tiled_kernel<float>, dim3(count * ptiles), dim3(256), 0, s, inst, c->H, c->W,
@@ -1255,10 +1274,8 @@ int sbp_corr_advance_batch(sbp_corrector *c, sbp_batch *b, const double *dt, lon
       launch_infer(c, count, s);
       hipLaunchKernelGGL(batch_candidate_energy_kernel<float>, dim3(count * bpi), dim3(256), 0, s, inst, bpi, b->order,
                          c->H, c->W, c->Yo, c->norm + NORM_OUT, c->bpart);
-      hipLaunchKernelGGL(batch_energy_final_kernel, dim3((count + 63) / 64), dim3(64), 0, s, c->bpart, count, bpi,
-                         c->be_new);
       hipLaunchKernelGGL(batch_apply_kernel<float>, dim3(grid_for(npix)), dim3(256), 0, s, inst, count, c->H, c->W,
-                         c->Yo, c->norm + NORM_OUT, c->be_old, c->be_new, bound, c->brej);
+                         c->Yo, c->norm + NORM_OUT, c->be_old, c->bpart, bpi, bound, c->brej);
     }
     return hipGetLastError();
   };
