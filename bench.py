@@ -178,8 +178,8 @@ def time_to_l2():
     C = pkg("corrector")
     if not os.path.exists(C.WEIGHTS):
         return {"skipped": "no trained corrector weights at %s" % C.WEIGHTS}
-    params, norm = C.load_weights()
-    res = C.time_to_accuracy(params, norm, n_eval=16, log=log)
+    params, norm, band = C.load_weights()
+    res = C.time_to_accuracy(params, norm, band=band, n_eval=16, log=log)
     return res
 
 

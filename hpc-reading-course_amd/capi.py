@@ -86,6 +86,8 @@ EXPORTS = {
     "sbp_corr_set_params": (C.c_int, [C.c_void_p, _FP, C.c_int]),
     "sbp_corr_get_params": (C.c_int, [C.c_void_p, _FP, C.c_int]),
     "sbp_corr_set_normalization": (C.c_int, [C.c_void_p, _FP, _FP, _FP]),
+    "sbp_corr_set_band": (C.c_int, [C.c_void_p, C.c_int]),
+    "sbp_corr_get_band": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "sbp_corr_forward": (C.c_int, [C.c_void_p, _FP, C.c_int, _FP]),
     "sbp_corr_gradients": (C.c_int, [C.c_void_p, _FP, _FP, C.c_int, _FP, _DP]),
     "sbp_corr_load_dataset": (C.c_int, [C.c_void_p, _FP, _FP, C.c_int]),
@@ -428,6 +430,17 @@ class Corrector:
     def set_normalization(self, in_shift, in_scale, out_scale):
         a, b, c = _f32(in_shift, (CORR_INPUTS,)), _f32(in_scale, (CORR_INPUTS,)), _f32(out_scale, (3,))
         _check(load_library().sbp_corr_set_normalization(self._h, _fptr(a), _fptr(b), _fptr(c)))
+
+    @property
+    def band(self) -> int:
+        """Points nearer a wall than this keep the plain scheme's values."""
+        v = C.c_int()
+        _check(load_library().sbp_corr_get_band(self._h, C.byref(v)))
+        return v.value
+
+    @band.setter
+    def band(self, band: int):
+        _check(load_library().sbp_corr_set_band(self._h, int(band)))
 
     def forward(self, raw) -> np.ndarray:
         raw = _f32(raw)

@@ -233,18 +233,32 @@ __global__ __launch_bounds__(256) void relu_back_kernel(const float *__restrict_
   if (i < n) dpre[i] = f2bf(bf2f(A[i]) > 0.f ? dA[i] : 0.f);
 }
 
+// Whether the correction acts at grid point (i, j) of an H x W image: the
+// points closer than `band` to a wall (the boundary-closure rows) keep the
+// plain scheme's values, and the training loss ignores them.
+__device__ __forceinline__ bool corrected_at(int i, int j, int W, int H, int band)
+{
+  return min(min(i, W - 1 - i), min(j, H - 1 - j)) >= band;
+}
+
 // MSE loss over the C_OUT real channels and its gradient 2(y - t)/N in bf16
-// (padded channels zero).  Per-block partial sums of squares in FP64.
+// (padded channels, and points outside the corrected region, zero; N counts
+// every point).  Per-block partial sums of squares in FP64.
 __global__ __launch_bounds__(256) void loss_kernel(const float *__restrict__ y, const float *__restrict__ t, long npix,
-                                                   double inv_n, bf16 *__restrict__ dY, double *__restrict__ partial)
+                                                   int H, int W, int band, double inv_n, bf16 *__restrict__ dY,
+                                                   double *__restrict__ partial)
 {
   __shared__ double sh[256];
   double s = 0;
+  const long npi = (long)H * W;
   for (long p = blockIdx.x * 256L + threadIdx.x; p < npix; p += (long)gridDim.x * 256) {
+    const long q = p % npi;
+    const int j = (int)(q / W), i = (int)(q - (long)j * W);
+    const bool on = corrected_at(i, j, W, H, band);
 #pragma unroll
     for (int c = 0; c < C_OUTP; c++) {
       float g = 0.f;
-      if (c < C_OUT) {
+      if (c < C_OUT && on) {
         const float d = y[p * C_OUT + c] - t[p * C_OUT + c];
         s += (double)d * d;
         g = (float)(2.0 * inv_n) * d;
@@ -550,12 +564,37 @@ __device__ __forceinline__ void pack_tile(const T *__restrict__ U, const T *__re
                                           T (*s)[pty<T>][PTX])
 {
   constexpr int PTH = pth<T>, PTY = pty<T>;
+  constexpr int NSTAGE = 3 * PTY * PTX, NLOAD = (NSTAGE + 255) / 256;
   const int x0 = tx * PTW, y0 = ty * PTH;
-  for (int q = threadIdx.x; q < 3 * PTY * PTX; q += 256) {
-    const int c = q / (PTY * PTX), r = q - c * (PTY * PTX);
-    const int yy = r / PTX, xx = r - yy * PTX;
-    const int gi = min(max(x0 - 3 + xx, 0), nx - 1), gj = min(max(y0 - 3 + yy, 0), ny - 1);
-    s[c][yy][xx] = U[c * plane + (long)gj * pitch + xoff + gi];
+  // every load of the staging is issued before the first LDS store, so the
+  // block waits for one memory latency instead of NLOAD of them
+  T v[NLOAD];
+#pragma unroll
+  for (int k = 0; k < NLOAD; k++) {
+    const int q = threadIdx.x + 256 * k;
+    if (q < NSTAGE) {
+      const int c = q / (PTY * PTX), r = q - c * (PTY * PTX);
+      const int yy = r / PTX, xx = r - yy * PTX;
+      const int gi = min(max(x0 - 3 + xx, 0), nx - 1), gj = min(max(y0 - 3 + yy, 0), ny - 1);
+      v[k] = U[c * plane + (long)gj * pitch + xoff + gi];
+    }
+  }
+  // the material values of this thread's points, loaded with the staging
+  static_assert(PTW * PTH % 256 == 0, "whole passes over the tile");
+  constexpr int NPT = PTW * PTH / 256;
+  T mri[NPT], mK[NPT];
+#pragma unroll
+  for (int k = 0; k < NPT; k++) {
+    const int q = threadIdx.x + 256 * k;
+    const int i = min(x0 + (q % PTW), nx - 1), j = min(y0 + (q / PTW), ny - 1);
+    const long o = (long)j * pitch + xoff + i;
+    mri[k] = rinv ? rinv[o] : T(1);
+    mK[k] = bulk ? bulk[o] : T(1);
+  }
+#pragma unroll
+  for (int k = 0; k < NLOAD; k++) {
+    const int q = threadIdx.x + 256 * k;
+    if (q < NSTAGE) (&s[0][0][0])[q] = v[k];
   }
   __syncthreads();
   // image (i, j) -> LDS slot; valid for |i - x0 - PTW/2| <= PTW/2 + 3 etc.
@@ -568,12 +607,13 @@ __device__ __forceinline__ void pack_tile(const T *__restrict__ U, const T *__re
     j = min(max(j, 0), ny - 1);
     return at(c, i, j - 2) - 4 * at(c, i, j - 1) + 6 * at(c, i, j) - 4 * at(c, i, j + 1) + at(c, i, j + 2);
   };
-  for (int q = threadIdx.x; q < PTW * PTH; q += 256) {
+#pragma unroll
+  for (int k = 0; k < NPT; k++) {
+    const int q = threadIdx.x + 256 * k;
     const int i = x0 + (q % PTW), j = y0 + (q / PTW);
     if (i >= nx || j >= ny) continue;
-    const long o = (long)j * pitch + xoff + i;
-    const double ri = rinv ? (double)rinv[o] : 1.0;
-    const double K = bulk ? (double)bulk[o] : 1.0;
+    const double ri = (double)mri[k];
+    const double K = (double)mK[k];
     float raw[C_IN];
     raw[0] = (float)(ri * (0.5 * (d4x(2, i + 1, j) - d4x(2, i - 1, j))));
     raw[1] = (float)(ri * (0.5 * (d4y(2, i, j + 1) - d4y(2, i, j - 1))));
@@ -618,15 +658,17 @@ __global__ __launch_bounds__(256) void pack_batch_tiled_kernel(const BatchInst<T
 // Corrected candidate: Y = U + out_scale * net (into the scratch register).
 template <typename T>
 __global__ __launch_bounds__(256) void candidate_kernel(const T *__restrict__ U, const float *__restrict__ net, int nx,
-                                                        int ny, int pitch, int xoff, long plane,
+                                                        int ny, int pitch, int xoff, long plane, int band,
                                                         const float *__restrict__ out_scale, T *__restrict__ Y)
 {
   const long p = blockIdx.x * 256L + threadIdx.x;
   if (p >= (long)nx * ny) return;
   const int j = (int)(p / nx), i = (int)(p - (long)j * nx);
   const long o = (long)j * pitch + xoff + i;
+  const bool on = corrected_at(i, j, nx, ny, band);
 #pragma unroll
-  for (int c = 0; c < 3; c++) Y[c * plane + o] = U[c * plane + o] + (T)(net[p * C_OUT + c] * out_scale[c]);
+  for (int c = 0; c < 3; c++)
+    Y[c * plane + o] = on ? U[c * plane + o] + (T)(net[p * C_OUT + c] * out_scale[c]) : U[c * plane + o];
 }
 
 // Accept the candidate when E_new <= E_old * (1 + bound); count rejections.
@@ -687,7 +729,7 @@ __global__ void batch_energy_final_kernel(const double *__restrict__ partial, in
 // instance, without storing it: blocks [k*bpi, (k+1)*bpi) reduce instance k.
 template <typename T>
 __global__ __launch_bounds__(256) void batch_candidate_energy_kernel(const BatchInst<T> *__restrict__ inst, int bpi,
-                                                                     int order, int H, int W,
+                                                                     int order, int H, int W, int band,
                                                                      const float *__restrict__ net,
                                                                      const float *__restrict__ out_scale,
                                                                      double *__restrict__ partial)
@@ -699,16 +741,38 @@ __global__ __launch_bounds__(256) void batch_candidate_energy_kernel(const Batch
   const float *nk = net + (long)k * npts * C_OUT;
   const double hx = 1.0 / (double)d.hix, hy = 1.0 / (double)d.hiy;
   double acc = 0;
-  for (long q = chunk * 256L + threadIdx.x; q < npts; q += (long)bpi * 256) {
-    const int j = (int)(q / W), i = (int)(q - (long)j * W);
-    const long o = (long)j * d.pitch + d.xoff + i;
-    const double w = hweight(order, W, i, hx) * hweight(order, H, j, hy);
-    double v[3];
+  const long stride = (long)bpi * 256;
+  constexpr int UN = 4;  // points per thread and pass, loaded together
+  for (long q0 = chunk * 256L + threadIdx.x; q0 < npts; q0 += UN * stride) {
+    T u[UN][3], ri[UN], K[UN];
+    float nv[UN][3];
 #pragma unroll
-    for (int c = 0; c < 3; c++) v[c] = (double)(T)(d.U[c * d.plane + o] + (T)(nk[q * C_OUT + c] * out_scale[c]));
-    const double rho = d.rinv ? 1.0 / (double)d.rinv[o] : 1.0;
-    const double kinv = d.bulk ? 1.0 / (double)d.bulk[o] : 1.0;
-    acc += w * (rho * (v[0] * v[0] + v[1] * v[1]) + kinv * v[2] * v[2]);
+    for (int m = 0; m < UN; m++) {
+      const long q = min(q0 + m * stride, npts - 1);
+      const int j = (int)(q / W), i = (int)(q - (long)j * W);
+      const long o = (long)j * d.pitch + d.xoff + i;
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        u[m][c] = d.U[c * d.plane + o];
+        nv[m][c] = nk[q * C_OUT + c];
+      }
+      ri[m] = d.rinv ? d.rinv[o] : T(1);
+      K[m] = d.bulk ? d.bulk[o] : T(1);
+    }
+#pragma unroll
+    for (int m = 0; m < UN; m++) {
+      const long q = q0 + m * stride;
+      if (q >= npts) break;
+      const int j = (int)(q / W), i = (int)(q - (long)j * W);
+      const double w = hweight(order, W, i, hx) * hweight(order, H, j, hy);
+      const bool on = corrected_at(i, j, W, H, band);
+      double v[3];
+#pragma unroll
+      for (int c = 0; c < 3; c++) v[c] = on ? (double)(T)(u[m][c] + (T)(nv[m][c] * out_scale[c])) : (double)u[m][c];
+      const double rho = d.rinv ? 1.0 / (double)ri[m] : 1.0;
+      const double kinv = d.bulk ? 1.0 / (double)K[m] : 1.0;
+      acc += w * (rho * (v[0] * v[0] + v[1] * v[1]) + kinv * v[2] * v[2]);
+    }
   }
   const double sm = block_sum_256(acc, sh);
   if (threadIdx.x == 0) partial[blockIdx.x] = sm;
@@ -721,6 +785,7 @@ __global__ __launch_bounds__(256) void batch_candidate_energy_kernel(const Batch
 // at most 256 of them).
 template <typename T>
 __global__ __launch_bounds__(256) void batch_apply_kernel(const BatchInst<T> *__restrict__ inst, int count, int H, int W,
+                                                          int band,
                                                           const float *__restrict__ net,
                                                           const float *__restrict__ out_scale,
                                                           const double *__restrict__ e_ref,
@@ -747,8 +812,8 @@ __global__ __launch_bounds__(256) void batch_apply_kernel(const BatchInst<T> *__
   const long q = p - k * npi;
   const bool ok = s_ok[k - k0];
   if (q == 0 && !ok) rejected[k] += 1;
-  if (!ok) return;
   const int j = (int)(q / W), i = (int)(q - (long)j * W);
+  if (!ok || !corrected_at(i, j, W, H, band)) return;
   const BatchInst<T> &d = inst[k];
   const long o = (long)j * d.pitch + d.xoff + i;
 #pragma unroll
@@ -771,6 +836,7 @@ int grid_for(long n) { return (int)((n + 255) / 256); }
 
 struct sbp_corrector {
   int H = 0, W = 0, nb = 0;  // image shape and batch capacity
+  int band = 0;              // corrections act on points at least this far from every wall
   hipStream_t stream = nullptr;
   float *params = nullptr, *grads = nullptr, *m = nullptr, *v = nullptr;
   bf16 *w1 = nullptr, *w2 = nullptr, *w3 = nullptr, *w2t = nullptr, *w3t = nullptr;
@@ -863,8 +929,8 @@ int backward(sbp_corrector *c, int nb)
   const long npix = (long)nb * c->H * c->W;
   const int g = conv_grid(nb, c->H, c->W);
   const int lblocks = (int)std::min<long>(grid_for(npix), 1024);
-  hipLaunchKernelGGL(loss_kernel, dim3(lblocks), dim3(256), 0, c->stream, c->Yo, c->Tg, npix, 1.0 / (double)(npix * C_OUT),
-                     c->dY3, c->lpart);
+  hipLaunchKernelGGL(loss_kernel, dim3(lblocks), dim3(256), 0, c->stream, c->Yo, c->Tg, npix, c->H, c->W, c->band,
+                     1.0 / (double)(npix * C_OUT), c->dY3, c->lpart);
   hipLaunchKernelGGL(sum_kernel, dim3(1), dim3(64), 0, c->stream, c->lpart, lblocks, 1.0 / (double)(npix * C_OUT),
                      c->scalars);
   // layer 3
@@ -1025,6 +1091,21 @@ int sbp_corr_set_normalization(sbp_corrector *c, const float *in_shift, const fl
 
 // Network output for nb images of raw inputs [nb][ny][nx][C_IN]:
 // out [nb][ny][nx][3] in normalised units (multiply by out_scale for the correction).
+int sbp_corr_set_band(sbp_corrector *c, int band)
+{
+  if (!c) return fail(SBP_EINVAL, "null corrector");
+  if (band < 0 || 2 * band >= std::min(c->H, c->W)) return fail(SBP_EINVAL, "band leaves no corrected point");
+  c->band = band;
+  return SBP_OK;
+}
+
+int sbp_corr_get_band(const sbp_corrector *c, int *band)
+{
+  if (!c || !band) return fail(SBP_EINVAL, "null argument");
+  *band = c->band;
+  return SBP_OK;
+}
+
 int sbp_corr_forward(sbp_corrector *c, const float *raw, int nb, float *out)
 {
   int rc = check_nb(c, nb);
@@ -1170,10 +1251,10 @@ static int apply_one(sbp_corrector *c, sbp_ctx *x, double bound)
   // is compared with the reference energy (c->scalars[1])
   if (x->esz == 8)
     hipLaunchKernelGGL(candidate_kernel<double>, dim3(grid_for(npix)), dim3(256), 0, x->stream, (const double *)x->U,
-                       c->Yo, x->nx, x->ny, x->pitch, x->xoff, x->plane, c->norm + NORM_OUT, (double *)x->Y0);
+                       c->Yo, x->nx, x->ny, x->pitch, x->xoff, x->plane, c->band, c->norm + NORM_OUT, (double *)x->Y0);
   else
     hipLaunchKernelGGL(candidate_kernel<float>, dim3(grid_for(npix)), dim3(256), 0, x->stream, (const float *)x->U,
-                       c->Yo, x->nx, x->ny, x->pitch, x->xoff, x->plane, c->norm + NORM_OUT, (float *)x->Y0);
+                       c->Yo, x->nx, x->ny, x->pitch, x->xoff, x->plane, c->band, c->norm + NORM_OUT, (float *)x->Y0);
   HIP_OR_FAIL(hipGetLastError());
   if ((rc = sbp::host::energy_of(x, x->Y0, x->KC, nullptr))) return rc;
   const long n = 3 * x->plane;
@@ -1264,18 +1345,18 @@ int sbp_corr_advance_batch(sbp_corrector *c, sbp_batch *b, const double *dt, lon
                          c->norm, c->X0);
       launch_infer(c, count, s);
       hipLaunchKernelGGL(batch_candidate_energy_kernel<double>, dim3(count * bpi), dim3(256), 0, s, inst, bpi, b->order,
-                         c->H, c->W, c->Yo, c->norm + NORM_OUT, c->bpart);
+                         c->H, c->W, c->band, c->Yo, c->norm + NORM_OUT, c->bpart);
       hipLaunchKernelGGL(batch_apply_kernel<double>, dim3(grid_for(npix)), dim3(256), 0, s, inst, count, c->H, c->W,
-                         c->Yo, c->norm + NORM_OUT, c->be_old, c->bpart, bpi, bound, c->brej);
+                         c->band, c->Yo, c->norm + NORM_OUT, c->be_old, c->bpart, bpi, bound, c->brej);
     } else {
       const BatchInst<float> *inst = static_cast<const BatchInst<float> *>(b->inst_dev);
       hipLaunchKernelGGL(pack_batch_tiled_kernel<float>, dim3(count * ptiles), dim3(256), 0, s, inst, c->H, c->W,
                          c->norm, c->X0);
       launch_infer(c, count, s);
       hipLaunchKernelGGL(batch_candidate_energy_kernel<float>, dim3(count * bpi), dim3(256), 0, s, inst, bpi, b->order,
-                         c->H, c->W, c->Yo, c->norm + NORM_OUT, c->bpart);
+                         c->H, c->W, c->band, c->Yo, c->norm + NORM_OUT, c->bpart);
       hipLaunchKernelGGL(batch_apply_kernel<float>, dim3(grid_for(npix)), dim3(256), 0, s, inst, count, c->H, c->W,
-                         c->Yo, c->norm + NORM_OUT, c->be_old, c->bpart, bpi, bound, c->brej);
+                         c->band, c->Yo, c->norm + NORM_OUT, c->be_old, c->bpart, bpi, bound, c->brej);
     }
     return hipGetLastError();
   };

@@ -207,6 +207,11 @@ int  sbp_corr_set_params(sbp_corrector *c, const float *params, int n); /* also 
 int  sbp_corr_get_params(sbp_corrector *c, float *params, int n);
 int  sbp_corr_set_normalization(sbp_corrector *c, const float *in_shift, const float *in_scale,
                                 const float *out_scale);   /* 8, 8 and 3 values */
+/* Restrict the correction (and the training loss) to the points at least
+ * `band` grid points from every wall; the points nearer a wall (the boundary
+ * closure rows) keep the plain RK4 values.  Default 0: every point. */
+int  sbp_corr_set_band(sbp_corrector *c, int band);
+int  sbp_corr_get_band(const sbp_corrector *c, int *band);
 /* raw inputs [nb][ny][nx][8] -> out [nb][ny][nx][3] (normalised units). */
 int  sbp_corr_forward(sbp_corrector *c, const float *raw, int nb, float *out);
 /* MSE loss of (net - target/out_scale) and its parameter gradients. */

@@ -94,9 +94,19 @@ def forward(flat, raw, shift, scale, exact=False):
     return y, (x0, a1, a2)
 
 
-def loss_and_grads(flat, raw, target, shift, scale, out_scale, exact=False):
-    """MSE of (net - target/out_scale) over all points and channels, and the
-    gradient of that loss with respect to every parameter (flat order)."""
+def band_mask(H, W, band):
+    """1 on the points at least `band` from every wall of an H x W image, else
+    0 (the corrected region)."""
+    dx = np.minimum(np.arange(W), W - 1 - np.arange(W))
+    dy = np.minimum(np.arange(H), H - 1 - np.arange(H))
+    return (np.minimum(dy[:, None], dx[None, :]) >= band).astype(np.float64)
+
+
+def loss_and_grads(flat, raw, target, shift, scale, out_scale, exact=False, band=0):
+    """MSE of (net - target/out_scale) over the points of the corrected
+    region (band_mask) and all channels, divided by the count of every point
+    and channel, and the gradient of that loss with respect to every
+    parameter (flat order)."""
     p = unpack(flat)
     r = (lambda a: a) if exact else bf16
     y, (x0, a1, a2) = forward(flat, raw, shift, scale, exact)
@@ -104,7 +114,7 @@ def loss_and_grads(flat, raw, target, shift, scale, out_scale, exact=False):
     if not exact:
         t = np.asarray(np.asarray(target, np.float32) / np.asarray(out_scale, np.float32), np.float64)
     N = y.size
-    d = y - t
+    d = (y - t) * band_mask(y.shape[1], y.shape[2], band)[None, :, :, None]
     loss = float(np.sum(d * d) / N)
     dy = r(2.0 * d / N)
     g = {"W3": conv3x3_wgrad(a2, dy), "b3": dy.sum(axis=(0, 1, 2))}

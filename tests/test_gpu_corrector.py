@@ -70,15 +70,24 @@ def test_fused_inference_equals_layer_kernels(capi, shape):
     assert lf == pytest.approx(loss, rel=1e-6)
 
 
-def test_gradients_match_oracle(capi):
+@pytest.mark.parametrize("band", [0, 3])
+def test_gradients_match_oracle(capi, band):
+    """band = 3: the loss and its gradients cover the points at least 3 from
+    every wall of the 21 x 37 images."""
     raw, tgt, params, shift, scale, osc = setup(2)
     nb, H, W = raw.shape[:3]
     with capi.Corrector(W, H, nb) as c:
         c.params = params
         c.set_normalization(shift, scale, osc)
+        with pytest.raises(capi.SbpError):
+            c.band = 11  # 2 * 11 >= 21 rows: no corrected point left
+        with pytest.raises(capi.SbpError):
+            c.band = -1
+        c.band = band
+        assert c.band == band
         loss, g = c.gradients(raw, tgt)
-    l_ref, g_ref = CO.loss_and_grads(params, raw, tgt, shift, scale, osc, exact=False)
-    l_ex, g_ex = CO.loss_and_grads(params, raw, tgt, shift, scale, osc, exact=True)
+    l_ref, g_ref = CO.loss_and_grads(params, raw, tgt, shift, scale, osc, exact=False, band=band)
+    l_ex, g_ex = CO.loss_and_grads(params, raw, tgt, shift, scale, osc, exact=True, band=band)
     assert loss == pytest.approx(l_ref, rel=1e-3)
     k = 0
     for name, shp in CO.SHAPES:
@@ -138,10 +147,13 @@ def test_energy_guard_rejects_growth(capi):
         assert np.array_equal(got, plain)
 
 
-def test_corrected_step_uses_host_features(capi):
+@pytest.mark.parametrize("band", [0, 4])
+def test_corrected_step_uses_host_features(capi, band):
     """One corrected step on the device equals: plain step, the host's
     feature definition (corrector.raw_features), the network, and the update
-    U + out_scale * net.  Pins the device feature packing to the host's."""
+    U + out_scale * net on the points at least `band` from every wall (the
+    others keep the plain step's values exactly).  Pins the device feature
+    packing to the host's."""
     import importlib
     CR = importlib.import_module("hpc-reading-course_amd.corrector")
     spec, v0 = P.acoustic_box(57, 4, sigma=0.08, materials=True, seed=5, material_seed=12)
@@ -157,20 +169,24 @@ def test_corrected_step_uses_host_features(capi):
     with capi.Corrector(57, 57, 1) as c:
         c.params = params
         c.set_normalization(shift, scale, osc)
+        c.band = band
         net = c.forward(raw)[0]
         with capi.Context(spec) as ctx:
             ctx.set_state(v0)
             rej = c.advance(ctx, 0.0, dt, 1, 1.0)
             got = ctx.get_state()
     assert rej == 0
-    want = plain + (net * osc).reshape(-1)
-    corr = (net * osc).reshape(-1)
+    inner = (CO.band_mask(57, 57, band) > 0)[:, :, None]
+    corr = (net * osc * inner).reshape(-1)
+    want = plain + corr
     assert np.max(np.abs(got - want)) <= 1e-3 * np.max(np.abs(corr))
     assert np.max(np.abs(got - plain)) > 0.5 * np.max(np.abs(corr))
+    outer = ~np.broadcast_to(inner, (57, 57, 3)).reshape(-1)
+    assert np.array_equal(got[outer], plain[outer])
 
 
-@pytest.mark.parametrize("prec", [P.FP64, P.FP32])
-def test_batch_advance_matches_single(capi, prec):
+@pytest.mark.parametrize("prec,band", [(P.FP64, 0), (P.FP32, 0), (P.FP32, 4)])
+def test_batch_advance_matches_single(capi, prec, band):
     """sbp_corr_advance_batch (corrected steps captured in a graph and
     replayed, then the remainder eagerly) equals the single-context
     sbp_corr_advance on every instance, bit for bit, when no step is refused;
@@ -192,6 +208,7 @@ def test_batch_advance_matches_single(capi, prec):
     with capi.Corrector(49, 49, 3) as c:
         c.params = params
         c.set_normalization(shift, scale, osc)
+        c.band = band
         with capi.Batch([sp for sp, _ in specs]) as b:
             for k, (_, v0) in enumerate(specs):
                 b.set_state(k, v0)
@@ -223,7 +240,7 @@ def test_trained_network_gain(capi):
     257^2 error by at least 2x on average, and refuse few steps."""
     import importlib
     CR = importlib.import_module("hpc-reading-course_amd.corrector")
-    params, norm = CR.load_weights()
+    params, norm, band = CR.load_weights()
     seeds = [950001, 950002, 950003, 950004]
     t_end = 0.25
     fine = [CR.instance(s, CR.N_FINE) for s in seeds]
@@ -242,6 +259,7 @@ def test_trained_network_gain(capi):
         with capi.Batch([c[0] for c in coarse]) as b, capi.Corrector(CR.N_COARSE, CR.N_COARSE, 4) as c:
             c.params = params
             c.set_normalization(*norm)
+            c.band = band
             for k, (_, v0) in enumerate(coarse):
                 b.set_state(k, v0)
             rej = c.advance_batch(b, dt, ns, 1e-4) if corrected else np.zeros(4, int)

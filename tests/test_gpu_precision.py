@@ -74,6 +74,37 @@ def test_kahan_long_run_accuracy(capi, orc):
     assert errs[P.FP32_KAHAN] <= 2e-5
 
 
+def test_kahan_stops_drift_over_1e5_steps(capi):
+    """10^5 RK4 steps of a 65^2 variable-material acoustic problem with wall
+    SATs (about 200 wall-to-wall transits).  The reference is the FP64 path
+    (itself within 1e-12 of the oracle).  Plain FP32 accumulates the rounding
+    of every update U += increment; the compensated form keeps it, so its
+    error stays at the level of the FP32 right-hand side's rounding."""
+    spec64, v0 = P.acoustic_box(65, 4, materials=True)
+    dt = P.dt_from_cfl(spec64, 0.25)
+    marks = (10_000, 100_000)
+    errs = {}
+    with capi.Context(spec64) as ref:
+        ref.set_state(v0)
+        refs, done = [], 0
+        for m in marks:
+            ref.step(done * dt, dt, m - done)
+            done = m
+            refs.append(ref.get_state())
+    for prec in (P.FP32, P.FP32_KAHAN):
+        with capi.Context(spec64.with_(precision=prec)) as ctx:
+            ctx.set_state(v0)
+            done, e = 0, []
+            for m, r in zip(marks, refs):
+                ctx.step(done * dt, dt, m - done)
+                done = m
+                e.append(relerr(ctx.get_state(), r))
+            errs[prec] = e
+    print("rel err at", marks, "fp32", errs[P.FP32], "kahan", errs[P.FP32_KAHAN], flush=True)
+    assert errs[P.FP32_KAHAN][1] <= 0.5 * errs[P.FP32][1]
+    assert errs[P.FP32_KAHAN][1] <= 1e-4
+
+
 def test_fp32_energy_non_increasing(capi):
     spec, v0 = P.acoustic_box(257, 6, materials=True, precision=P.FP32_KAHAN)
     dt = P.dt_from_cfl(spec, 0.25)

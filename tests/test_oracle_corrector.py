@@ -45,7 +45,8 @@ def test_dgrad_is_adjoint_and_wgrad_is_derivative():
     assert lhs == pytest.approx(np.sum(w * C.conv3x3_wgrad(x, dy)), rel=1e-12)
 
 
-def test_exact_gradients_match_finite_differences():
+@pytest.mark.parametrize("band", [0, 1])
+def test_exact_gradients_match_finite_differences(band):
     rng = np.random.default_rng(2)
     H, W = 6, 5
     raw = rng.standard_normal((2, H, W, C.C_IN))
@@ -54,7 +55,7 @@ def test_exact_gradients_match_finite_differences():
     shift = np.zeros(C.C_IN)
     scale = np.ones(C.C_IN)
     osc = np.array([0.5, 2.0, 1.0])
-    loss, g = C.loss_and_grads(flat, raw, tgt, shift, scale, osc, exact=True)
+    loss, g = C.loss_and_grads(flat, raw, tgt, shift, scale, osc, exact=True, band=band)
     # features are computed in float32 then widened: perturb parameters only
     idx = rng.choice(C.n_params(), 40, replace=False)
     eps = 1e-6
@@ -63,10 +64,33 @@ def test_exact_gradients_match_finite_differences():
         fm = flat.copy()
         fp[k] += eps
         fm[k] -= eps
-        lp, _ = C.loss_and_grads(fp, raw, tgt, shift, scale, osc, exact=True)
-        lm, _ = C.loss_and_grads(fm, raw, tgt, shift, scale, osc, exact=True)
+        lp, _ = C.loss_and_grads(fp, raw, tgt, shift, scale, osc, exact=True, band=band)
+        lm, _ = C.loss_and_grads(fm, raw, tgt, shift, scale, osc, exact=True, band=band)
         fd = (lp - lm) / (2 * eps)
         assert g[k] == pytest.approx(fd, rel=1e-4, abs=1e-9), k
+
+
+def test_band_loss_counts_only_the_corrected_region():
+    """band = 2 on a 7 x 9 image: the loss is the squared misfit summed over
+    rows 2..4 and columns 2..6 only, divided by every point and channel."""
+    rng = np.random.default_rng(5)
+    H, W = 7, 9
+    raw = rng.standard_normal((1, H, W, C.C_IN))
+    tgt = rng.standard_normal((1, H, W, 3))
+    flat = rng.standard_normal(C.n_params()) * 0.2
+    shift, scale, osc = np.zeros(C.C_IN), np.ones(C.C_IN), np.array([0.5, 2.0, 1.0])
+    y, _ = C.forward(flat, raw, shift, scale, exact=True)
+    d = y - tgt / osc
+    want = np.sum(d[0, 2:5, 2:7] ** 2) / d.size
+    loss, g = C.loss_and_grads(flat, raw, tgt, shift, scale, osc, exact=True, band=2)
+    assert loss == pytest.approx(want, rel=1e-12)
+    # a target change outside the region moves neither the loss nor a gradient
+    t2 = tgt.copy()
+    t2[0, 1, :] += 5.0
+    t2[0, :, 7] -= 3.0
+    loss2, g2 = C.loss_and_grads(flat, raw, t2, shift, scale, osc, exact=True, band=2)
+    assert loss2 == loss and np.array_equal(g, g2)
+    assert C.band_mask(H, W, 2).sum() == 3 * 5
 
 
 def test_bf16_rounding():
