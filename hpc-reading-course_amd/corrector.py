@@ -393,9 +393,9 @@ def main(argv=None):
     import json
     ap = argparse.ArgumentParser(description=main.__doc__)
     ap.add_argument("--instances", type=int, default=2000)
-    ap.add_argument("--epochs", type=int, default=40)
+    ap.add_argument("--epochs", type=int, default=60)
     ap.add_argument("--lr", type=float, default=3e-3)
-    ap.add_argument("--noise", type=float, default=2.0)
+    ap.add_argument("--noise", type=float, default=1.0)
     ap.add_argument("--snaps", type=int, default=1)
     ap.add_argument("--band", type=int, default=BAND)
     ap.add_argument("--out", default=WEIGHTS)

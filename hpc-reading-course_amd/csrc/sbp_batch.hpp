@@ -72,7 +72,10 @@ __device__ __forceinline__ StageArgs<T> batch_args(const BatchInst<T> &d)
   return a;
 }
 
-template <typename T, int ORDER, int MAT, int MODE, bool KAHAN, int BX>
+// PF: rows of loads an interior tile keeps in flight ahead of the row it
+// computes (TilePolicy); deep prefetch pays off for the short tiles of
+// ensembles of small grids, where a tile is a few latency-bound rows.
+template <typename T, int ORDER, int MAT, int MODE, bool KAHAN, int BX, int PF>
 __global__ __launch_bounds__(BX) void batch_stage_kernel(const BatchInst<T> *__restrict__ inst,
                                                          const int2 *__restrict__ work, int nwork)
 {
@@ -86,7 +89,8 @@ __global__ __launch_bounds__(BX) void batch_stage_kernel(const BatchInst<T> *__r
     if (it.y < a.n_edge_blocks)
       edge_point<T, ORDER, PH_ACOUSTIC, MAT, MODE, KAHAN>(a, it.y * BX + threadIdx.x);
     else
-      interior_tile<T, ORDER, PH_ACOUSTIC, MAT, MODE, KAHAN, BX, DefaultPolicy>(a, it.y - a.n_edge_blocks);
+      interior_tile<T, ORDER, PH_ACOUSTIC, MAT, MODE, KAHAN, BX, TilePolicy<PF, true, true, false>>(
+          a, it.y - a.n_edge_blocks);
     __syncthreads();  // the next item reuses the LDS row buffers
   }
 }
@@ -94,11 +98,11 @@ __global__ __launch_bounds__(BX) void batch_stage_kernel(const BatchInst<T> *__r
 using BatchLaunchFn = hipError_t (*)(int mode, const void *inst, const int2 *work, int nwork, int grid,
                                      hipStream_t s);
 
-template <typename T, int ORDER, int MAT, bool KAHAN, int BX> struct BatchLauncher {
+template <typename T, int ORDER, int MAT, bool KAHAN, int BX, int PF> struct BatchLauncher {
   template <int MODE>
   static hipError_t go(const void *inst, const int2 *work, int nwork, int grid, hipStream_t s)
   {
-    hipLaunchKernelGGL((batch_stage_kernel<T, ORDER, MAT, MODE, KAHAN, BX>), dim3(grid), dim3(BX), 0, s,
+    hipLaunchKernelGGL((batch_stage_kernel<T, ORDER, MAT, MODE, KAHAN, BX, PF>), dim3(grid), dim3(BX), 0, s,
                        static_cast<const BatchInst<T> *>(inst), work, nwork);
     return hipGetLastError();
   }
@@ -113,16 +117,18 @@ template <typename T, int ORDER, int MAT, bool KAHAN, int BX> struct BatchLaunch
   }
 };
 
-template <typename T, bool KAHAN, int BX> BatchLaunchFn pick_batch(int order, int mat)
+template <typename T, bool KAHAN, int BX, int PF> BatchLaunchFn pick_batch(int order, int mat)
 {
   const bool arr = mat == M_ARR;
-  if (order == 2) return arr ? &BatchLauncher<T, 2, M_ARR, KAHAN, BX>::launch : &BatchLauncher<T, 2, M_UNIT, KAHAN, BX>::launch;
-  if (order == 4) return arr ? &BatchLauncher<T, 4, M_ARR, KAHAN, BX>::launch : &BatchLauncher<T, 4, M_UNIT, KAHAN, BX>::launch;
-  return arr ? &BatchLauncher<T, 6, M_ARR, KAHAN, BX>::launch : &BatchLauncher<T, 6, M_UNIT, KAHAN, BX>::launch;
+  if (order == 2)
+    return arr ? &BatchLauncher<T, 2, M_ARR, KAHAN, BX, PF>::launch : &BatchLauncher<T, 2, M_UNIT, KAHAN, BX, PF>::launch;
+  if (order == 4)
+    return arr ? &BatchLauncher<T, 4, M_ARR, KAHAN, BX, PF>::launch : &BatchLauncher<T, 4, M_UNIT, KAHAN, BX, PF>::launch;
+  return arr ? &BatchLauncher<T, 6, M_ARR, KAHAN, BX, PF>::launch : &BatchLauncher<T, 6, M_UNIT, KAHAN, BX, PF>::launch;
 }
 
-// sbp_batch_f64.hip / sbp_batch_f32.hip
-BatchLaunchFn batch_launcher_f64(int order, int mat, int bx);
-BatchLaunchFn batch_launcher_f32(int order, int mat, bool kahan, int bx);
+// sbp_batch_f64.hip / sbp_batch_f32.hip; pf is 1 or 4
+BatchLaunchFn batch_launcher_f64(int order, int mat, int bx, int pf);
+BatchLaunchFn batch_launcher_f32(int order, int mat, bool kahan, int bx, int pf);
 
 }  // namespace sbp

@@ -29,16 +29,17 @@ using sbp::host::InstHost;
 namespace {
 
 struct BatchConfig {
-  int bx, ty, blocks_per_cu;
+  int bx, ty, blocks_per_cu, pf;  // pf: rows of loads in flight per tile
 };
 
 // Candidate launch shapes of the autotuner.
 // Short tiles (4, 8 rows) are for ensembles of small grids, where tall tiles
 // leave most of the persistent blocks without work.
 constexpr BatchConfig kConfigs[] = {
-    {256, 32, 4}, {256, 64, 4}, {256, 16, 4}, {256, 32, 2}, {256, 32, 8}, {256, 64, 2},
-    {128, 32, 8}, {128, 64, 8}, {128, 16, 8}, {128, 32, 4}, {128, 64, 4}, {128, 32, 16},
-    {256, 8, 4},  {256, 4, 4},  {128, 8, 8},  {128, 4, 8},
+    {256, 32, 4, 1}, {256, 64, 4, 1}, {256, 16, 4, 1}, {256, 32, 2, 1}, {256, 32, 8, 1}, {256, 64, 2, 1},
+    {128, 32, 8, 1}, {128, 64, 8, 1}, {128, 16, 8, 1}, {128, 32, 4, 1}, {128, 64, 4, 1}, {128, 32, 16, 1},
+    {256, 8, 4, 1},  {256, 4, 4, 1},  {128, 8, 8, 1},  {128, 4, 8, 1},
+    {256, 8, 4, 4},  {256, 4, 4, 4},  {128, 8, 8, 4},  {128, 4, 8, 4},
 };
 constexpr int kNumConfigs = sizeof(kConfigs) / sizeof(kConfigs[0]);
 constexpr int kAlignB = 64;
@@ -95,8 +96,8 @@ template <typename T> void fill_inst(const sbp_batch *b, std::vector<BatchInst<T
 int apply_config(sbp_batch *b, int cfg)
 {
   const BatchConfig c = kConfigs[cfg];
-  BatchLaunchFn fn = b->prec == SBP_FP64 ? batch_launcher_f64(b->order, b->mat, c.bx)
-                                         : batch_launcher_f32(b->order, b->mat, b->prec == SBP_FP32_KAHAN, c.bx);
+  BatchLaunchFn fn = b->prec == SBP_FP64 ? batch_launcher_f64(b->order, b->mat, c.bx, c.pf)
+                                         : batch_launcher_f32(b->order, b->mat, b->prec == SBP_FP32_KAHAN, c.bx, c.pf);
   if (!fn) return fail(SBP_EINVAL, "no batched kernel for this order/material/precision");
   std::vector<int2> work;
   auto build = [&](auto &desc) {

@@ -351,8 +351,11 @@ __global__ __launch_bounds__(256) void pack_raw_kernel(const float *__restrict__
 // Fused inference: the three convolutions of one image window in one kernel,
 // activations kept in LDS.  A block owns a window of WW = 64 columns
 // (image columns c0 .. c0+63, c0 = x0 - 3) and a chunk of output rows; it
-// marches down the rows keeping the last three rows of the input, of layer 1
-// and of layer 2 in LDS rings.  Every layer is evaluated on all 64 window
+// marches down the rows keeping the last four rows of the input, of layer 1
+// and of layer 2 in LDS rings.  In row step t every layer reads only rows
+// written in steps t-3 .. t-1 and writes its new row into slot t % 4, so one
+// barrier per step orders everything: layer 1 produces row y0 - 5 + t, layer
+// 2 row y0 - 7 + t and layer 3 (the output) row y0 - 9 + t.  Every layer is evaluated on all 64 window
 // columns; a layer's value at window column w is exact when its inputs at
 // w-1..w+1 are, so the output is exact on w in [3, 60]: 58 output columns
 // per window.  Activations at image positions outside the image are stored
@@ -378,9 +381,10 @@ __global__ __launch_bounds__(256) void fused_net_kernel(const bf16 *__restrict__
                                                         const bf16 *__restrict__ w3, const float *__restrict__ b3,
                                                         float *__restrict__ out)
 {
-  __shared__ __attribute__((aligned(16))) bf16 s_in[3][FPX * C_INP];
-  __shared__ __attribute__((aligned(16))) bf16 s_a1[3][FPX * F1S];
-  __shared__ __attribute__((aligned(16))) bf16 s_a2[3][FPX * F1S];
+  constexpr int NR = 4;  // ring depth
+  __shared__ __attribute__((aligned(16))) bf16 s_in[NR][FPX * C_INP];
+  __shared__ __attribute__((aligned(16))) bf16 s_a1[NR][FPX * F1S];
+  __shared__ __attribute__((aligned(16))) bf16 s_a2[NR][FPX * F1S];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int m = lane & 15, kg = lane >> 4;
   const int strips = (W + FOUT - 1) / FOUT, chunks = (H + rows - 1) / rows;
@@ -392,8 +396,8 @@ __global__ __launch_bounds__(256) void fused_net_kernel(const bf16 *__restrict__
   const long ibase = (long)img * H * W;
 
   // zero every LDS row once: pads stay zero, rows outside the image too
-  for (int q = threadIdx.x; q < 3 * FPX * C_INP; q += 256) (&s_in[0][0])[q] = 0;
-  for (int q = threadIdx.x; q < 3 * FPX * F1S; q += 256) {
+  for (int q = threadIdx.x; q < NR * FPX * C_INP; q += 256) (&s_in[0][0])[q] = 0;
+  for (int q = threadIdx.x; q < NR * FPX * F1S; q += 256) {
     (&s_a1[0][0])[q] = 0;
     (&s_a2[0][0])[q] = 0;
   }
@@ -417,7 +421,7 @@ __global__ __launch_bounds__(256) void fused_net_kernel(const bf16 *__restrict__
   const float bv1 = b1[tc * 16 + m], bv2 = b2[tc * 16 + m], bv3 = b3[m];
   __syncthreads();
 
-  const int nsteps = (y1 - y0) + 6;
+  const int nsteps = (y1 - y0) + 9;
   // input pixel (row ri, window column threadIdx.x - 1) of thread threadIdx.x < FPX
   auto load_in = [&](int ri) {
     const int col = c0 - 1 + (int)threadIdx.x;
@@ -428,15 +432,16 @@ __global__ __launch_bounds__(256) void fused_net_kernel(const bf16 *__restrict__
   };
   uint4 in_next = load_in(y0 - 3);
   for (int t = 0; t < nsteps; t++) {
-    // input row y0 - 3 + t -> ring slot t % 3 (pixels w = -1 .. FW); the
-    // next row's load is issued now and lands while this row is computed
-    if (threadIdx.x < FPX) *reinterpret_cast<uint4 *>(s_in[t % 3] + threadIdx.x * C_INP) = in_next;
+    // input row y0 - 3 + t -> ring slot t % 4 (pixels w = -1 .. FW), first
+    // read in step t + 1; the next row's load is issued now and lands while
+    // this step computes
+    if (threadIdx.x < FPX) *reinterpret_cast<uint4 *>(s_in[t % NR] + threadIdx.x * C_INP) = in_next;
     if (t + 1 < nsteps) in_next = load_in(y0 - 2 + t);
-    __syncthreads();
-    // layer 1, row r1 = y0 - 4 + t, from input slots (t-2, t-1, t) % 3;
-    // K-step ks covers taps 4 ks + kg (all 8 channels of one input pixel)
-    if (t >= 2) {
-      const int r1 = y0 - 4 + t;
+    // layer 1, row r1 = y0 - 5 + t, from the input rows of steps t-3 .. t-1
+    // (slot (t + 1 + dy) % 4); K-step ks covers taps 4 ks + kg (all 8
+    // channels of one input pixel)
+    if (t >= 3) {
+      const int r1 = y0 - 5 + t;
       const bool rin = r1 >= 0 && r1 < H;
 #pragma unroll
       for (int u = 0; u < 2; u++) {
@@ -447,13 +452,13 @@ __global__ __launch_bounds__(256) void fused_net_kernel(const bf16 *__restrict__
           const int tap = ks * 4 + kg;
           bf16x8 a = {};
           if (tap < 9) {
-            const bf16 *row = s_in[(t + 1 + tap / 3) % 3];  // slot (t - 1 + dy) % 3, dy = tap/3 - 1
+            const bf16 *row = s_in[(t + 1 + tap / 3) % NR];  // step t - 3 + tap/3
             const int px = 16 * sg + m + tap % 3;       // window column m + dx plus the pad column
             a = *reinterpret_cast<const bf16x8 *>(row + px * C_INP);
           }
           acc = mfma32(a, bw1[ks], acc);
         }
-        bf16 *dst = s_a1[t % 3];
+        bf16 *dst = s_a1[t % NR];
 #pragma unroll
         for (int i = 0; i < 4; i++) {
           const int w = 16 * sg + 4 * kg + i, col = c0 + w;
@@ -463,11 +468,10 @@ __global__ __launch_bounds__(256) void fused_net_kernel(const bf16 *__restrict__
         }
       }
     }
-    __syncthreads();
-    // layer 2, row r2 = y0 - 5 + t, from layer-1 slots (t-2, t-1, t) % 3;
+    // layer 2, row r2 = y0 - 7 + t, from the layer-1 rows of steps t-3 .. t-1;
     // K-step ks is tap ks, channels 8 kg .. 8 kg + 7
-    if (t >= 4) {
-      const int r2 = y0 - 5 + t;
+    if (t >= 6) {
+      const int r2 = y0 - 7 + t;
       const bool rin = r2 >= 0 && r2 < H;
 #pragma unroll
       for (int u = 0; u < 2; u++) {
@@ -475,12 +479,12 @@ __global__ __launch_bounds__(256) void fused_net_kernel(const bf16 *__restrict__
         f32x4 acc = {0, 0, 0, 0};
 #pragma unroll
         for (int ks = 0; ks < NK2; ks++) {
-          const bf16 *row = s_a1[(t + 1 + ks / 3) % 3];
+          const bf16 *row = s_a1[(t + 1 + ks / 3) % NR];
           const int px = 16 * sg + m + ks % 3;
           const bf16x8 a = *reinterpret_cast<const bf16x8 *>(row + px * F1S + kg * 8);
           acc = mfma32(a, bw2[ks], acc);
         }
-        bf16 *dst = s_a2[t % 3];
+        bf16 *dst = s_a2[t % NR];
 #pragma unroll
         for (int i = 0; i < 4; i++) {
           const int w = 16 * sg + 4 * kg + i, col = c0 + w;
@@ -490,15 +494,14 @@ __global__ __launch_bounds__(256) void fused_net_kernel(const bf16 *__restrict__
         }
       }
     }
-    __syncthreads();
-    // layer 3, row r3 = y0 - 6 + t, from layer-2 slots (t-2, t-1, t) % 3
-    if (t >= 6) {
-      const int r3 = y0 - 6 + t;
+    // layer 3, row r3 = y0 - 9 + t, from the layer-2 rows of steps t-3 .. t-1
+    if (t >= 9) {
+      const int r3 = y0 - 9 + t;
       const int sg = wv;
       f32x4 acc = {0, 0, 0, 0};
 #pragma unroll
       for (int ks = 0; ks < NK3; ks++) {
-        const bf16 *row = s_a2[(t + 1 + ks / 3) % 3];
+        const bf16 *row = s_a2[(t + 1 + ks / 3) % NR];
         const int px = 16 * sg + m + ks % 3;
         const bf16x8 a = *reinterpret_cast<const bf16x8 *>(row + px * F1S + kg * 8);
         acc = mfma32(a, bw3[ks], acc);
@@ -512,6 +515,9 @@ __global__ __launch_bounds__(256) void fused_net_kernel(const bf16 *__restrict__
         }
       }
     }
+    // this step's rows become readable, and the slots read in this step
+    // ((t + 1) % 4 of each ring) may be overwritten in the next
+    __syncthreads();
   }
 }
 
@@ -741,38 +747,21 @@ __global__ __launch_bounds__(256) void batch_candidate_energy_kernel(const Batch
   const float *nk = net + (long)k * npts * C_OUT;
   const double hx = 1.0 / (double)d.hix, hy = 1.0 / (double)d.hiy;
   double acc = 0;
-  const long stride = (long)bpi * 256;
-  constexpr int UN = 4;  // points per thread and pass, loaded together
-  for (long q0 = chunk * 256L + threadIdx.x; q0 < npts; q0 += UN * stride) {
-    T u[UN][3], ri[UN], K[UN];
-    float nv[UN][3];
+  const long q = chunk * 256L + threadIdx.x;  // one point per thread: bpi = ceil(H W / 256)
+  if (q < npts) {
+    const int j = (int)(q / W), i = (int)(q - (long)j * W);
+    const long o = (long)j * d.pitch + d.xoff + i;
+    const double w = hweight(order, W, i, hx) * hweight(order, H, j, hy);
+    const bool on = corrected_at(i, j, W, H, band);
+    double v[3];
 #pragma unroll
-    for (int m = 0; m < UN; m++) {
-      const long q = min(q0 + m * stride, npts - 1);
-      const int j = (int)(q / W), i = (int)(q - (long)j * W);
-      const long o = (long)j * d.pitch + d.xoff + i;
-#pragma unroll
-      for (int c = 0; c < 3; c++) {
-        u[m][c] = d.U[c * d.plane + o];
-        nv[m][c] = nk[q * C_OUT + c];
-      }
-      ri[m] = d.rinv ? d.rinv[o] : T(1);
-      K[m] = d.bulk ? d.bulk[o] : T(1);
+    for (int c = 0; c < 3; c++) {
+      const T u = d.U[c * d.plane + o];
+      v[c] = on ? (double)(T)(u + (T)(nk[q * C_OUT + c] * out_scale[c])) : (double)u;
     }
-#pragma unroll
-    for (int m = 0; m < UN; m++) {
-      const long q = q0 + m * stride;
-      if (q >= npts) break;
-      const int j = (int)(q / W), i = (int)(q - (long)j * W);
-      const double w = hweight(order, W, i, hx) * hweight(order, H, j, hy);
-      const bool on = corrected_at(i, j, W, H, band);
-      double v[3];
-#pragma unroll
-      for (int c = 0; c < 3; c++) v[c] = on ? (double)(T)(u[m][c] + (T)(nv[m][c] * out_scale[c])) : (double)u[m][c];
-      const double rho = d.rinv ? 1.0 / (double)ri[m] : 1.0;
-      const double kinv = d.bulk ? 1.0 / (double)K[m] : 1.0;
-      acc += w * (rho * (v[0] * v[0] + v[1] * v[1]) + kinv * v[2] * v[2]);
-    }
+    const double rho = d.rinv ? 1.0 / (double)d.rinv[o] : 1.0;
+    const double kinv = d.bulk ? 1.0 / (double)d.bulk[o] : 1.0;
+    acc = w * (rho * (v[0] * v[0] + v[1] * v[1]) + kinv * v[2] * v[2]);
   }
   const double sm = block_sum_256(acc, sh);
   if (threadIdx.x == 0) partial[blockIdx.x] = sm;
@@ -847,6 +836,7 @@ struct sbp_corrector {
   double *lpart = nullptr, *scalars = nullptr;  // scalars: [0] loss, [1] E_old
   int *rejected = nullptr, *idx = nullptr;
   double *bpart = nullptr, *be_old = nullptr;  // batched energies: per-block partials, reference energies
+  double *cpart = nullptr;                      // candidate energies: one partial per 256 points
   int *brej = nullptr;
   float *ds_raw = nullptr, *ds_tgt = nullptr;
   int ds_n = 0;
@@ -862,7 +852,7 @@ void release(sbp_corrector *c)
   void *ptrs[] = {c->params, c->grads, c->m, c->v, c->w1, c->w2, c->w3, c->w2t, c->w3t, c->b1, c->b2, c->b3,
                   c->norm, c->X0, c->A1, c->A2, c->dY3, c->dP2, c->dP1, c->Yo, c->Tg, c->dA, c->wpart,
                   c->lpart, c->scalars, c->rejected, c->idx, c->ds_raw, c->ds_tgt, c->bpart, c->be_old,
-                  c->brej};
+                  c->cpart, c->brej};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -910,7 +900,8 @@ int forward(sbp_corrector *c, int nb)
 // Network output Yo of the nb packed images X0 (inference: the fused kernel).
 void launch_infer(sbp_corrector *c, int nb, hipStream_t stream)
 {
-  const int rows = 32;
+  // at most 32 rows per chunk, the chunks of an image as even as possible
+  const int nchunks = (c->H + 31) / 32, rows = (c->H + nchunks - 1) / nchunks;
   const long blocks = (long)nb * ((c->W + FOUT - 1) / FOUT) * ((c->H + rows - 1) / rows);
   hipLaunchKernelGGL(fused_net_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, c->X0, nb, c->H, c->W, rows,
                      c->w1, c->b1, c->w2, c->b2, c->w3, c->b3, c->Yo);
@@ -1021,6 +1012,7 @@ int sbp_corr_create(int nx, int ny, int batch, sbp_corrector **out)
   if (!rc) rc = dmalloc(&c->rejected, sizeof(int));
   if (!rc) rc = dmalloc(&c->idx, sizeof(int) * batch);
   if (!rc) rc = dmalloc(&c->bpart, sizeof(double) * batch * 64);
+  if (!rc) rc = dmalloc(&c->cpart, sizeof(double) * batch * (((long)nx * ny + 255) / 256));
   if (!rc) rc = dmalloc(&c->be_old, sizeof(double) * batch);
   if (!rc) rc = dmalloc(&c->brej, sizeof(int) * batch);
   if (!rc) {
@@ -1320,6 +1312,7 @@ int sbp_corr_advance_batch(sbp_corrector *c, sbp_batch *b, const double *dt, lon
   int rc = sbp::host::batch_upload_dt(b, dt);
   if (rc) return rc;
   const int count = b->count, bpi = 64;
+  const int cbpi = (int)(((long)c->H * c->W + 255) / 256);  // candidate-energy blocks per instance
   const int pth_ = b->esz == 8 ? pth<double> : pth<float>;
   const int ptiles = ((c->W + PTW - 1) / PTW) * ((c->H + pth_ - 1) / pth_);
   const long npix = (long)count * c->H * c->W;
@@ -1344,19 +1337,19 @@ int sbp_corr_advance_batch(sbp_corrector *c, sbp_batch *b, const double *dt, lon
       hipLaunchKernelGGL(pack_batch_tiled_kernel<double>, dim3(count * ptiles), dim3(256), 0, s, inst, c->H, c->W,
                          c->norm, c->X0);
       launch_infer(c, count, s);
-      hipLaunchKernelGGL(batch_candidate_energy_kernel<double>, dim3(count * bpi), dim3(256), 0, s, inst, bpi, b->order,
-                         c->H, c->W, c->band, c->Yo, c->norm + NORM_OUT, c->bpart);
+      hipLaunchKernelGGL(batch_candidate_energy_kernel<double>, dim3(count * cbpi), dim3(256), 0, s, inst, cbpi,
+                         b->order, c->H, c->W, c->band, c->Yo, c->norm + NORM_OUT, c->cpart);
       hipLaunchKernelGGL(batch_apply_kernel<double>, dim3(grid_for(npix)), dim3(256), 0, s, inst, count, c->H, c->W,
-                         c->band, c->Yo, c->norm + NORM_OUT, c->be_old, c->bpart, bpi, bound, c->brej);
+                         c->band, c->Yo, c->norm + NORM_OUT, c->be_old, c->cpart, cbpi, bound, c->brej);
     } else {
       const BatchInst<float> *inst = static_cast<const BatchInst<float> *>(b->inst_dev);
       hipLaunchKernelGGL(pack_batch_tiled_kernel<float>, dim3(count * ptiles), dim3(256), 0, s, inst, c->H, c->W,
                          c->norm, c->X0);
       launch_infer(c, count, s);
-      hipLaunchKernelGGL(batch_candidate_energy_kernel<float>, dim3(count * bpi), dim3(256), 0, s, inst, bpi, b->order,
-                         c->H, c->W, c->band, c->Yo, c->norm + NORM_OUT, c->bpart);
+      hipLaunchKernelGGL(batch_candidate_energy_kernel<float>, dim3(count * cbpi), dim3(256), 0, s, inst, cbpi,
+                         b->order, c->H, c->W, c->band, c->Yo, c->norm + NORM_OUT, c->cpart);
       hipLaunchKernelGGL(batch_apply_kernel<float>, dim3(grid_for(npix)), dim3(256), 0, s, inst, count, c->H, c->W,
-                         c->band, c->Yo, c->norm + NORM_OUT, c->be_old, c->bpart, bpi, bound, c->brej);
+                         c->band, c->Yo, c->norm + NORM_OUT, c->be_old, c->cpart, cbpi, bound, c->brej);
     }
     return hipGetLastError();
   };

@@ -5,7 +5,9 @@ Bars (stated per test):
 * the Kahan state round trip (FP64 -> FP32 value + FP32 compensation -> FP64)
   keeps ~48 bits: relative error <= 1e-13 (plain FP32: <= 6e-8);
 * long runs: the Kahan form is at least as accurate as plain FP32 and within
-  2e-5 relative max-norm of the FP64 oracle after 2000 steps.
+  2e-5 relative max-norm of the FP64 oracle after 2000 steps; over 10^5
+  small steps (CFL 0.01) it is at least 10x more accurate than plain FP32
+  and within 3e-5.
 """
 import numpy as np
 import pytest
@@ -75,13 +77,18 @@ def test_kahan_long_run_accuracy(capi, orc):
 
 
 def test_kahan_stops_drift_over_1e5_steps(capi):
-    """10^5 RK4 steps of a 65^2 variable-material acoustic problem with wall
-    SATs (about 200 wall-to-wall transits).  The reference is the FP64 path
-    (itself within 1e-12 of the oracle).  Plain FP32 accumulates the rounding
-    of every update U += increment; the compensated form keeps it, so its
-    error stays at the level of the FP32 right-hand side's rounding."""
-    spec64, v0 = P.acoustic_box(65, 4, materials=True)
-    dt = P.dt_from_cfl(spec64, 0.25)
+    """10^5 RK4 steps at CFL 0.01 of a 65^2 variable-material acoustic
+    problem with wall SATs (T = 15.6, dozens of wall-to-wall transits).  The
+    reference is the FP64 path (itself within 1e-12 of the oracle).  With
+    small steps every increment is small against U, so plain FP32 loses most
+    of each increment's low bits in U += increment and the loss builds up
+    over the run; the compensated form carries it.  Measured on MI355X:
+    FP32 3.7e-5 / 3.9e-4 after 10^4 / 10^5 steps, compensated 2.1e-6 /
+    1.2e-5.  (At CFL 0.25 the two agree, about 3e-4 after 10^5 steps: there
+    the FP32 rounding of the right-hand side itself dominates, which no
+    compensation of the update removes.)"""
+    spec64, v0 = P.acoustic_box(65, 4, materials=True, sigma=0.1)
+    dt = P.dt_from_cfl(spec64, 0.01)
     marks = (10_000, 100_000)
     errs = {}
     with capi.Context(spec64) as ref:
@@ -101,8 +108,8 @@ def test_kahan_stops_drift_over_1e5_steps(capi):
                 e.append(relerr(ctx.get_state(), r))
             errs[prec] = e
     print("rel err at", marks, "fp32", errs[P.FP32], "kahan", errs[P.FP32_KAHAN], flush=True)
-    assert errs[P.FP32_KAHAN][1] <= 0.5 * errs[P.FP32][1]
-    assert errs[P.FP32_KAHAN][1] <= 1e-4
+    assert errs[P.FP32_KAHAN][1] <= 0.1 * errs[P.FP32][1]
+    assert errs[P.FP32_KAHAN][1] <= 3e-5
 
 
 def test_fp32_energy_non_increasing(capi):
