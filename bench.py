@@ -9,11 +9,13 @@ state resident in HBM.
 
 Prints one JSON line:
   value       DOF-updates/s (nx*ny*3 values advanced one RK4 step per update)
-  roofline    HBM roofline of the four stage kernels: algorithmic bytes per
-              DOF-update of a fused 3-register RK4 (16 words: 9 reads + 7
-              writes, 128 B in FP64) over the summed mean stage-kernel
-              durations (HIP events on the library's stream); the kernels as
-              built need 13 words (kernel_bytes_per_dof, kernel_frac) and the
+  roofline    HBM roofline of one RK4 step's four launches: algorithmic
+              bytes per DOF-update of a 3-register RK4 with one launch per
+              stage (16 words: 9 reads + 7 writes, 128 B in FP64) over the
+              summed mean launch durations (HIP events on the library's
+              stream).  At this size the library runs two RK4 stages per
+              interior launch, which needs 7 words (kernel_bytes_per_dof,
+              kernel_frac), so the 16-word rate can exceed the peak; the
               measured HBM traffic is `traffic`
   cpu_baseline the oracle compiled with the reference's optimisation flags,
               pthreads over a near-square block decomposition, timed on a
@@ -32,11 +34,14 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
-# HBM bytes per DOF-update of the four FP64 stage kernels of the default
-# workload (4097^2, order 4, rho = K = 1), from rocprofv3 FETCH_SIZE x 2.0 +
-# WRITE_SIZE x 1.0 (factors from the 1 GiB copy kernel in the same runs):
-# profiles/stage2d_acoustic_4097sq_order4_fp64_hbm_counters.md.
-MEASURED_TRAFFIC_PER_DOF = {(4097, 4, 0): 107.41}
+# HBM bytes per DOF-update of one FP64 RK4 step of the default workload
+# (4097^2, order 4, rho = K = 1), summed over the step's four launches, from
+# rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (factors from the 1 GiB copy
+# kernel in the same runs), per schedule:
+# profiles/stage2d_acoustic_4097sq_order4_fp64_hbm_counters.md (one stage per
+# launch) and profiles/pair2d_acoustic_4097sq_order4_fp64_hbm_counters.md
+# (two-stage launches).
+MEASURED_TRAFFIC_PER_DOF = {(4097, 4, 0, "one stage per launch"): 107.41}
 
 
 def pkg(sub):
@@ -60,19 +65,30 @@ def bytes_per_dof(precision, materials=False, C=3):
     return (words + extra) * w
 
 
-def kernel_bytes_per_dof(precision, materials=False, C=3):
-    """Bytes per DOF-update the stage kernels as built need: the 13-word
-    scheme (U, Y2, Y3, Y4: 9 reads + 4 writes; 15 words with Kahan)."""
+def kernel_bytes_per_dof(precision, materials=False, C=3, fused=False):
+    """Bytes per DOF-update the kernels as built need.  One launch per stage:
+    the 13-word scheme (U, Y2, Y3, Y4: 9 reads + 4 writes; 15 words with
+    Kahan), materials read by 4 launches.  Two-stage launches: 7 words
+    (stages 1-2 read U and write Y2, Y3; stages 3-4 read Y3, U, Y2 and write
+    U'; 9 with Kahan), materials read by 2 launches.  The frame's own
+    one-stage launches (a (CLS+R)-deep band, 0.6% of 4097^2) are not counted."""
     w = 8 if precision == 0 else 4
-    words = 13 if precision != 2 else 15
-    extra = (4 * 2 / C) if materials else 0.0
+    if fused:
+        words = 7 if precision != 2 else 9
+        extra = (2 * 2 / C) if materials else 0.0
+    else:
+        words = 13 if precision != 2 else 15
+        extra = (4 * 2 / C) if materials else 0.0
     return (words + extra) * w
 
 
-def gpu_run(spec, v0, dt, steps, warmup, stage_steps=None):
+def gpu_run(spec, v0, dt, steps, warmup, stage_steps=None, fused=None):
     capi = pkg("capi")
     ctx = capi.Context(spec)
     try:
+        if fused is not None:
+            ctx.set_fused(fused)
+        fused = ctx.fused
         ctx.set_state(v0)
         if warmup:
             ctx.step(0.0, dt, warmup)
@@ -84,15 +100,15 @@ def gpu_run(spec, v0, dt, steps, warmup, stage_steps=None):
         energy = ctx.state_energy()
     finally:
         ctx.close()
-    return wall, ev_ms, stage_ms, energy
+    return wall, ev_ms, stage_ms, energy, fused
 
 
-def measure(spec, v0, dt, steps, warmup):
-    wall, ev_ms, stage_ms, energy = gpu_run(spec, v0, dt, steps, warmup)
+def measure(spec, v0, dt, steps, warmup, fused=None):
+    wall, ev_ms, stage_ms, energy, fused = gpu_run(spec, v0, dt, steps, warmup, fused=fused)
     dofs = spec.nx * spec.ny * spec.components
     mat = spec.material == 2
     bpd = bytes_per_dof(spec.precision, mat, spec.components)
-    kbpd = kernel_bytes_per_dof(spec.precision, mat, spec.components)
+    kbpd = kernel_bytes_per_dof(spec.precision, mat, spec.components, fused)
     step_kernel_s = float(np.sum(stage_ms)) * 1e-3
     achieved = dofs * bpd / step_kernel_s / 1e9
     k_achieved = dofs * kbpd / step_kernel_s / 1e9
@@ -100,7 +116,8 @@ def measure(spec, v0, dt, steps, warmup):
         wall_s=wall, event_ms=ev_ms, ms_per_step=wall / steps * 1e3,
         dof_updates_per_s=dofs * steps / wall, stage_ms=[float(x) for x in stage_ms],
         bytes_per_dof=bpd, achieved_gbs=achieved, frac=achieved / PEAK_HBM_GBS, energy=energy,
-        kernel_bytes_per_dof=kbpd, kernel_achieved_gbs=k_achieved, kernel_frac=k_achieved / PEAK_HBM_GBS)
+        kernel_bytes_per_dof=kbpd, kernel_achieved_gbs=k_achieved, kernel_frac=k_achieved / PEAK_HBM_GBS,
+        schedule="two-stage" if fused else "one stage per launch")
 
 
 def batched(count=256, steps=500, warmup=10, prec=1):
@@ -154,10 +171,15 @@ def reflection_2d(steps=2000, warmup=0):
             wall = time.perf_counter() - t0
             finals[name] = ctx.get_state()
             stage_ms = ctx.time_stages(dt, 5)
+            fused = ctx.fused
         dofs = spec.nx * spec.ny * 3
         bpd = bytes_per_dof(prec, True)
+        kbpd = kernel_bytes_per_dof(prec, True, 3, fused)
+        step_s = float(np.sum(stage_ms)) * 1e-3
         out[name] = {"dof_updates_per_s": dofs * steps / wall, "ms_per_step": wall / steps * 1e3,
-                     "bytes_per_dof": bpd, "frac": dofs * bpd / (float(np.sum(stage_ms)) * 1e-3) / 1e9 / PEAK_HBM_GBS}
+                     "bytes_per_dof": bpd, "frac": dofs * bpd / step_s / 1e9 / PEAK_HBM_GBS,
+                     "kernel_bytes_per_dof": kbpd, "kernel_frac": dofs * kbpd / step_s / 1e9 / PEAK_HBM_GBS,
+                     "schedule": "two-stage" if fused else "one stage per launch"}
         del v0
     ref = finals["fp64"]
     scale = float(np.max(np.abs(ref)))
@@ -257,7 +279,7 @@ def main():
         secondary["acoustic2d_wall_1025sq_order4_fp64"] = {
             "dof_updates_per_s": m2["dof_updates_per_s"], "ms_per_step": m2["ms_per_step"],
             "achieved_gbs": m2["achieved_gbs"], "frac": m2["frac"], "kernel_frac": m2["kernel_frac"],
-            "stage_ms": m2["stage_ms"]}
+            "stage_ms": m2["stage_ms"], "schedule": m2["schedule"]}
         log("1025^2:", secondary["acoustic2d_wall_1025sq_order4_fp64"])
         for prec, name in ((1, "fp32"), (2, "fp32_kahan")):
             s3, v3, dt3 = P.baseline_acoustic(args.n, args.order, precision=prec)
@@ -266,8 +288,15 @@ def main():
                 "dof_updates_per_s": m3["dof_updates_per_s"], "ms_per_step": m3["ms_per_step"],
                 "bytes_per_dof": m3["bytes_per_dof"], "achieved_gbs": m3["achieved_gbs"],
                 "frac": m3["frac"], "kernel_bytes_per_dof": m3["kernel_bytes_per_dof"],
-                "kernel_frac": m3["kernel_frac"], "stage_ms": m3["stage_ms"]}
+                "kernel_frac": m3["kernel_frac"], "stage_ms": m3["stage_ms"], "schedule": m3["schedule"]}
             log(name, secondary["acoustic2d_wall_%dsq_order%d_%s" % (args.n, args.order, name)])
+        s4, v4, dt4 = P.baseline_acoustic(args.n, args.order)
+        m4 = measure(s4, v4, dt4, args.steps, args.warmup, fused=False)
+        secondary["acoustic2d_wall_%dsq_order%d_fp64_one_stage_per_launch" % (args.n, args.order)] = {
+            k: m4[k] for k in ("dof_updates_per_s", "ms_per_step", "achieved_gbs", "frac", "kernel_bytes_per_dof",
+                               "kernel_achieved_gbs", "kernel_frac", "stage_ms", "schedule")}
+        log("one stage per launch:", m4)
+        del v4
         bm = batched()
         secondary["batched_256x_N129-1025_order4_fp32"] = bm
         log("batched:", bm)
@@ -284,9 +313,14 @@ def main():
         cpu = cpu_baseline(spec, v0, dt)
         log("cpu:", cpu)
 
-    traffic_dof = MEASURED_TRAFFIC_PER_DOF.get((spec.nx, spec.order, spec.precision))
+    traffic_dof = MEASURED_TRAFFIC_PER_DOF.get((spec.nx, spec.order, spec.precision, main_m["schedule"]))
     dofs = spec.nx * spec.ny * spec.components
-    traffic = traffic_dof * dofs / 4 if traffic_dof else None  # bytes per stage launch
+    traffic = traffic_dof * dofs / 4 if traffic_dof else None  # mean HBM bytes per launch of a step
+    if main_m["schedule"] == "two-stage":
+        kernel = ("pair2d_kernel x2 (interior stages 1-2 and 3-4, each launch with the frame's stage 1 or 3) "
+                  "+ stage2d_kernel x2 (frame stages 2 and 4): one RK4 step, mean over its four launches")
+    else:
+        kernel = "stage2d_kernel x4 (one RK4 step), mean over the four stages"
     out = {
         "metric": "DOF-updates/s & % HBM roofline; time to L2 err 1e-4: chosen vs plain FP64",
         "value": main_m["dof_updates_per_s"],
@@ -306,10 +340,14 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": main_m["achieved_gbs"], "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": main_m["frac"], "traffic": traffic,
-                     "kernel": "stage2d_kernel x4 (one fused RK4 step), mean over the four stages",
+                     "kernel": kernel, "schedule": main_m["schedule"],
+                     "bytes_basis": "16 words per DOF-update (3-register RK4, one launch per stage); "
+                                    "a two-stage schedule needs fewer (kernel_bytes_per_dof), so this "
+                                    "rate can exceed the HBM peak",
                      "bytes_per_dof": main_m["bytes_per_dof"],
                      "algorithmic_bytes_per_launch": main_m["bytes_per_dof"] * dofs / 4,
-                     "traffic_unit": "HBM bytes per stage launch (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE)",
+                     "traffic_unit": "mean HBM bytes per launch of one RK4 step "
+                                     "(rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE)",
                      "kernel_bytes_per_dof": main_m["kernel_bytes_per_dof"],
                      "kernel_achieved": main_m["kernel_achieved_gbs"],
                      "kernel_frac": main_m["kernel_frac"],

@@ -50,6 +50,8 @@ EXPORTS = {
     "sbp_components": (C.c_int, [C.c_void_p]),
     "sbp_vector_length": (C.c_long, [C.c_void_p]),
     "sbp_set_tile_rows": (C.c_int, [C.c_void_p, C.c_int]),
+    "sbp_set_fused": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    "sbp_get_fused": (C.c_int, [C.c_void_p]),
     "sbp_rhs": (C.c_int, [C.c_void_p, C.c_double, _DP, _DP]),
     "sbp_rk4_advance": (C.c_int, [C.c_void_p, _DP, C.c_double, C.c_double, C.c_long]),
     "sbp_set_state": (C.c_int, [C.c_void_p, _DP]),
@@ -236,6 +238,16 @@ class Context:
 
     def set_tile_rows(self, rows: int):
         _check(load_library().sbp_set_tile_rows(self._h, int(rows)))
+
+    def set_fused(self, on: bool = True, rows: int = 0):
+        """Two-stage stepping: interior stages 1-2 and 3-4 in one launch each
+        (2D acoustics with unit or array materials); rows = tile height, 0 =
+        chosen from the grid."""
+        _check(load_library().sbp_set_fused(self._h, int(bool(on)), int(rows)))
+
+    @property
+    def fused(self) -> bool:
+        return bool(load_library().sbp_get_fused(self._h))
 
     # -- norms -------------------------------------------------------------
     def energy(self, v: np.ndarray) -> float:

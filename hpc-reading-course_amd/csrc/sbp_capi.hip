@@ -84,6 +84,55 @@ void compute_geometry(sbp_ctx *c)
 // tiles per CU (>= 960 tiles on 256 CUs).  Tall tiles amortise the 2R-row y
 // halo; on grids too small for that, occupancy wins (measured, order 4:
 // 4097^2 -> 64 rows, 2049^2 -> 16, 1025^2 -> 4 rows at 1.25x the 16-row rate).
+// Two-stage interior tiles: kPairNT mid columns, kPairNT - 2R output
+// columns, inside the D = CLS + R deep frame.
+
+void compute_pair_geometry(sbp_ctx *c)
+{
+  const int R = c->order / 2;
+  const int D = c->cls + R;
+  c->p_depth = D;
+  const int wi = c->nx - 2 * D, hi = c->ny - 2 * D;
+  const int bxo = kPairNT - 2 * R;
+  c->p_tiles_x = wi > 0 ? (wi + bxo - 1) / bxo : 0;
+  c->p_n_tiles = (wi > 0 && hi > 0) ? c->p_tiles_x * ((hi + c->p_ty_rows - 1) / c->p_ty_rows) : 0;
+  c->p_n_edge = 2 * D * c->nx + 2 * D * (c->ny - 2 * D);
+  const int eb = (c->p_n_edge + kPairNT - 1) / kPairNT;
+  c->p_n_edge_blocks = (eb + kNumXcd - 1) / kNumXcd * kNumXcd;
+}
+
+// Default pair-tile height (measured on gfx950 at 2049^2 and 4097^2): 16
+// rows for orders 2 and 4, 32 for order 6 (its 4R = 12 halo rows per tile
+// favour taller tiles), halved down to 8 while the grid has fewer than 768
+// tiles (two 68 KB FP64 blocks per CU on 256 CUs, 1.5 rounds).
+int default_pair_rows(const sbp_ctx *c)
+{
+  const int R = c->order / 2, D = c->cls + R;
+  const int wi = c->nx - 2 * D, hi = c->ny - 2 * D;
+  if (wi <= 0 || hi <= 0) return 16;
+  const int tx = (wi + kPairNT - 2 * R - 1) / (kPairNT - 2 * R);
+  int t = c->order == 6 ? 32 : 16;
+  while (t > 8 && (long)tx * ((hi + t - 1) / t) < 768) t /= 2;
+  return t;
+}
+
+bool pair_supported(const sbp_ctx *c)
+{
+  const int D = c->cls + c->order / 2;
+  return c->pair_launch && c->phys == SBP_ACOUSTIC_2D && (c->mat == SBP_MAT_UNIT || c->mat == SBP_MAT_ARRAYS) &&
+         c->nx > 2 * D && c->ny > 2 * D;
+}
+
+// Two-stage stepping is the default where it was measured faster: grids of
+// at least 1500^2 interior points (2049^2 order 4 FP64: 64.6 G DOF-updates/s
+// against 51 for one launch per stage; at 1025^2 the latency of its longer
+// per-row pipeline loses, 34 G against 39).
+bool pair_default(const sbp_ctx *c)
+{
+  const int D = c->cls + c->order / 2;
+  return pair_supported(c) && (long)(c->nx - 2 * D) * (c->ny - 2 * D) >= 1500L * 1500L;
+}
+
 int default_ty_rows(int nx, int ny, int cls)
 {
   const int wi = nx - 2 * cls, hi = ny - 2 * cls;
@@ -174,8 +223,82 @@ hipError_t launch_stage(sbp_ctx *c, int mode, double dt, double t, const void *y
   return c->launch(mode, &a, c->geom, c->stream);
 }
 
+template <typename T> PairArgs<T> make_pair_args(const sbp_ctx *c, int pair, double dt)
+{
+  PairArgs<T> a{};
+  a.rinv = static_cast<const T *>(c->rinv);
+  a.bulk = static_cast<const T *>(c->bulk);
+  a.plane = c->plane;
+  a.pitch = c->pitch;
+  a.xoff = c->xoff;
+  a.nx = c->nx;
+  a.ny = c->ny;
+  a.hix = T(c->hix);
+  a.hiy = T(c->hiy);
+  a.third = T(1.0 / 3.0);
+  a.tiles_x = c->p_tiles_x;
+  a.ty_rows = c->p_ty_rows;
+  a.n_tiles = c->p_n_tiles;
+  a.n_edge_blocks = c->p_n_edge_blocks;
+  if (pair == 1) {  // stages 1-2: U -> Y2 (Y0), Y3 (Y1)
+    a.yin = static_cast<const T *>(c->U);
+    a.o1 = static_cast<T *>(c->Y0);
+    a.o2 = static_cast<T *>(c->Y1);
+    a.c1 = T(0.5 * dt);
+    a.c2 = T(0.5 * dt);
+  } else {  // stages 3-4: Y3, U, Y2 -> U' (into Y0), Y4 (A) next to the frame
+    a.yin = static_cast<const T *>(c->Y1);
+    a.u = static_cast<const T *>(c->U);
+    a.y2 = static_cast<const T *>(c->Y0);
+    a.o1 = static_cast<T *>(c->Y0);
+    a.o2 = static_cast<T *>(c->A);
+    a.comp = static_cast<T *>(c->KC);
+    a.c1 = T(dt);
+    a.c2 = T(dt / 6);
+  }
+  return a;
+}
+
+// Launch `part` (1..4) of a two-stage step: 1 = frame stage 1 + interior
+// stages 1-2, 2 = frame stage 2, 3 = frame stage 3 + interior stages 3-4,
+// 4 = frame stage 4, after which U' sits in Y0 and the two arrays swap roles.
+template <typename T> hipError_t launch_pair_part_t(sbp_ctx *c, int part, double t, double dt)
+{
+  static const int mode[4] = {MODE_S1, MODE_S2, MODE_S3, MODE_S4};
+  const double ts[4] = {t, t + 0.5 * dt, t + 0.5 * dt, t + dt};
+  const void *yin[4] = {c->U, c->Y0, c->Y1, c->A};
+  void *yout[4] = {c->Y0, c->Y1, c->A, nullptr};
+  StageArgs<T> e = make_args<T>(c, mode[part - 1], yin[part - 1], dt, ts[part - 1]);
+  e.yout = static_cast<T *>(yout[part - 1]);
+  e.depth = c->p_depth;
+  e.n_edge = c->p_n_edge;
+  e.n_edge_blocks = c->p_n_edge_blocks;
+  e.n_tiles = 0;
+  if (part == 4) e.uout = static_cast<T *>(c->Y0);  // U' next to the interior's; U read from uin
+  if (part == 1 || part == 3) {
+    const PairArgs<T> p = make_pair_args<T>(c, part == 1 ? 1 : 2, dt);
+    return c->pair_launch(part == 1 ? 1 : 2, &e, &p, c->p_n_edge_blocks + c->p_n_tiles, c->stream);
+  }
+  // frame-only launch of the one-stage kernel: blocks of kBX threads
+  const int eb = (c->p_n_edge + kBX - 1) / kBX;
+  e.n_edge_blocks = (eb + kNumXcd - 1) / kNumXcd * kNumXcd;
+  const hipError_t err = c->launch(mode[part - 1], &e, LaunchGeom{e.n_edge_blocks, kBX}, c->stream);
+  if (part == 4) std::swap(c->U, c->Y0);
+  return err;
+}
+
+hipError_t launch_pair_part(sbp_ctx *c, int part, double t, double dt)
+{
+  return c->esz == 8 ? launch_pair_part_t<double>(c, part, t, dt) : launch_pair_part_t<float>(c, part, t, dt);
+}
+
 hipError_t launch_step(sbp_ctx *c, double t, double dt)
 {
+  if (c->fused) {
+    hipError_t e = hipSuccess;
+    for (int p = 1; p <= 4 && e == hipSuccess; p++) e = launch_pair_part(c, p, t, dt);
+    return e;
+  }
   const double h = 0.5 * dt;
   const double ts[4] = {t, t + h, t + h, t + dt};  // stage times (RK4_custom)
   hipError_t e = hipSuccess;
@@ -192,15 +315,28 @@ bool time_dependent(const sbp_ctx *c) { return c->phys == SBP_ACOUSTIC_2D && c->
 
 void drop_graph(sbp_ctx *c)
 {
-  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
-  c->gexec = nullptr;
-  c->g_dt = 0;
+  for (auto &g : c->gslot) {
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    g = sbp_ctx::GraphSlot{};
+  }
+  c->g_next = 0;
 }
 
-int ensure_graph(sbp_ctx *c, double dt)
+// The graph of kGraphSteps steps for this dt and the current U array
+// (captured on first use; kGraphSteps is even, so a two-stage capture ends
+// with the arrays back in their starting roles).
+int ensure_graph(sbp_ctx *c, double dt, hipGraphExec_t *out)
 {
-  if (c->gexec && c->g_dt == dt) return SBP_OK;
-  drop_graph(c);
+  for (auto &g : c->gslot)
+    if (g.exec && g.dt == dt && g.u == c->U) {
+      *out = g.exec;
+      return SBP_OK;
+    }
+  sbp_ctx::GraphSlot &slot = c->gslot[c->g_next];
+  c->g_next ^= 1;
+  if (slot.exec) (void)hipGraphExecDestroy(slot.exec);
+  slot = sbp_ctx::GraphSlot{};
+  void *u0 = c->U;
   hipGraph_t g = nullptr;
   HIP_OR_FAIL(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   hipError_t le = hipSuccess;
@@ -210,13 +346,16 @@ int ensure_graph(sbp_ctx *c, double dt)
     if (g) (void)hipGraphDestroy(g);
     return fail(SBP_EHIP, std::string("graph capture: ") + hipGetErrorString(le != hipSuccess ? le : ce));
   }
-  hipError_t ie = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+  hipError_t ie = hipGraphInstantiate(&slot.exec, g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
   if (ie != hipSuccess) {
-    c->gexec = nullptr;
+    slot.exec = nullptr;
     return fail(SBP_EHIP, std::string("graph instantiate: ") + hipGetErrorString(ie));
   }
-  c->g_dt = dt;
+  if (c->U != u0) return fail(SBP_EHIP, "graph capture left the state in another array");
+  slot.dt = dt;
+  slot.u = u0;
+  *out = slot.exec;
   return SBP_OK;
 }
 
@@ -224,9 +363,10 @@ int run_steps(sbp_ctx *c, double t0, double dt, long nsteps)
 {
   long n = 0;
   if (!time_dependent(c) && nsteps >= kGraphSteps) {
-    int rc = ensure_graph(c, dt);
+    hipGraphExec_t ge = nullptr;
+    int rc = ensure_graph(c, dt, &ge);
     if (rc) return rc;
-    for (; n + kGraphSteps <= nsteps; n += kGraphSteps) HIP_OR_FAIL(hipGraphLaunch(c->gexec, c->stream));
+    for (; n + kGraphSteps <= nsteps; n += kGraphSteps) HIP_OR_FAIL(hipGraphLaunch(ge, c->stream));
   }
   // Step times accumulate as t = t + dt (RK4_custom, time_stepping.h:141).
   double t = t0;
@@ -437,6 +577,8 @@ int build_ctx(const sbp_problem *desc, sbp_ctx *c)
   c->plane = (long)c->pitch * c->ny;
   c->ty_rows = default_ty_rows(c->nx, c->ny, c->cls);
   compute_geometry(c);
+  c->p_ty_rows = default_pair_rows(c);
+  compute_pair_geometry(c);
 
   if (c->prec == SBP_FP64) {
     StageLaunchFn (*reg)(int, int, int) =
@@ -448,6 +590,16 @@ int build_ctx(const sbp_problem *desc, sbp_ctx *c)
     c->launch = reg(c->phys, c->mat, c->prec == SBP_FP32_KAHAN, kBX);
   }
   if (!c->launch) return fail(SBP_EINVAL, "no kernel for this physics/material/precision");
+  if (c->phys == SBP_ACOUSTIC_2D) {
+    if (c->prec == SBP_FP64)
+      c->pair_launch = (c->order == 2 ? pair_launcher_o2_f64
+                                      : (c->order == 4 ? pair_launcher_o4_f64 : pair_launcher_o6_f64))(c->mat);
+    else
+      c->pair_launch = (c->order == 2 ? pair_launcher_o2_f32
+                                      : (c->order == 4 ? pair_launcher_o4_f32 : pair_launcher_o6_f32))(
+          c->mat, c->prec == SBP_FP32_KAHAN);
+  }
+  c->fused = pair_default(c);
 
   HIP_OR_FAIL(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   for (hipEvent_t &e : c->ev) HIP_OR_FAIL(hipEventCreate(&e));
@@ -599,6 +751,22 @@ int sbp_set_tile_rows(sbp_ctx *ctx, int rows)
   return SBP_OK;
 }
 
+int sbp_set_fused(sbp_ctx *ctx, int on, int rows)
+{
+  if (!ctx || rows < 0 || rows > 4096) return fail(SBP_EINVAL, "bad argument");
+  if (on && !pair_supported(ctx))
+    return fail(SBP_EINVAL, "two-stage stepping needs a 2D acoustic context with unit or array materials "
+                            "and more than 2(CLS+R) points per direction");
+  HIP_OR_FAIL(hipStreamSynchronize(ctx->stream));
+  drop_graph(ctx);
+  ctx->fused = on != 0;
+  ctx->p_ty_rows = rows > 0 ? rows : default_pair_rows(ctx);
+  compute_pair_geometry(ctx);
+  return SBP_OK;
+}
+
+int sbp_get_fused(const sbp_ctx *ctx) { return ctx && ctx->fused ? 1 : 0; }
+
 int sbp_rhs(sbp_ctx *ctx, double t, const double *src, double *dst)
 {
   if (!ctx || !src || !dst) return fail(SBP_EINVAL, "null argument");
@@ -675,7 +843,8 @@ int sbp_time_steps(sbp_ctx *ctx, double t0, double dt, long nsteps, double *ms)
   if (!ctx || !ms) return fail(SBP_EINVAL, "null argument");
   if (!ctx->have_state) return fail(SBP_ESTATE, "no state loaded");
   if (!time_dependent(ctx) && nsteps >= kGraphSteps) {
-    int rc = ensure_graph(ctx, dt);  // capture outside the timed region
+    hipGraphExec_t ge = nullptr;
+    int rc = ensure_graph(ctx, dt, &ge);  // capture outside the timed region
     if (rc) return rc;
   }
   HIP_OR_FAIL(hipEventRecord(ctx->ev[0], ctx->stream));
@@ -698,10 +867,14 @@ int sbp_time_stages(sbp_ctx *ctx, double dt, long nsteps, double ms_stage[4])
   for (long n = 0; n < nsteps; n++) {
     HIP_OR_FAIL(hipEventRecord(ctx->ev[0], ctx->stream));
     for (int s = 0; s < 4; s++) {
-      const void *yin;
-      void *yout;
-      stage_io(ctx, s + 1, &yin, &yout);
-      HIP_OR_FAIL(launch_stage(ctx, MODE_S1 + s, dt, n * dt + ts[s], yin, yout));
+      if (ctx->fused) {  // the four launches of a two-stage step
+        HIP_OR_FAIL(launch_pair_part(ctx, s + 1, n * dt, dt));
+      } else {
+        const void *yin;
+        void *yout;
+        stage_io(ctx, s + 1, &yin, &yout);
+        HIP_OR_FAIL(launch_stage(ctx, MODE_S1 + s, dt, n * dt + ts[s], yin, yout));
+      }
       HIP_OR_FAIL(hipEventRecord(ctx->ev[s + 1], ctx->stream));
     }
     HIP_OR_FAIL(hipEventSynchronize(ctx->ev[4]));

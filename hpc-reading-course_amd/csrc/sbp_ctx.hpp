@@ -24,8 +24,20 @@ struct sbp_ctx {
   sbp::LaunchGeom geom{};
   int ty_rows = 32, tiles_x = 0, tiles_y = 0, n_tiles = 0, n_edge = 0, n_edge_blocks = 0;
   bool have_state = false;
-  hipGraphExec_t gexec = nullptr;
-  double g_dt = 0;
+  // Two-stage interior path (2D acoustics, unit or array materials): pair
+  // launcher, D = CLS + R deep frame, tile geometry of the pair tiles.
+  sbp::PairLaunchFn pair_launch = nullptr;
+  bool fused = false;
+  int p_depth = 0, p_ty_rows = 0, p_tiles_x = 0, p_n_tiles = 0, p_n_edge = 0, p_n_edge_blocks = 0;
+  // Captured multi-step graphs, keyed by dt and by the array holding U when
+  // the capture started (the two-stage path swaps U and Y2 every step).
+  struct GraphSlot {
+    hipGraphExec_t exec = nullptr;
+    double dt = 0;
+    void *u = nullptr;
+  };
+  GraphSlot gslot[2];
+  int g_next = 0;
   hipEvent_t ev[5] = {};
   int device = 0;
   void *rkw[8] = {};  // adaptive RK workspace (K1..K6, stage input, candidate), allocated on first use

@@ -22,6 +22,24 @@ StageLaunchFn pick_launcher(int phys, int mat)
   return nullptr;
 }
 
+// Two-stage interior tiles: kPairNT threads; prefetch depths (rows of loads in
+// flight per wave) from a sweep on gfx950.  LDS holds these kernels at two
+// blocks per CU, so deep register queues are what keeps HBM busy.
+#ifndef SBP_PAIR_PF1
+#define SBP_PAIR_PF1 3
+#endif
+#ifndef SBP_PAIR_PF2
+#define SBP_PAIR_PF2 1
+#endif
+template <typename T, int ORDER, bool KAHAN>
+PairLaunchFn pick_pair(int mat)
+{
+  constexpr int PF1 = SBP_PAIR_PF1, PF2 = SBP_PAIR_PF2;
+  if (mat == M_UNIT) return &PairLauncher<T, ORDER, M_UNIT, KAHAN, kPairNT, PF1, PF2>::launch;
+  if (mat == M_ARR) return &PairLauncher<T, ORDER, M_ARR, KAHAN, kPairNT, PF1, PF2>::launch;
+  return nullptr;
+}
+
 }  // namespace sbp
 
 #define SBP_DEFINE_LAUNCHER_F64(ORDER)                                          \
@@ -30,6 +48,10 @@ StageLaunchFn pick_launcher(int phys, int mat)
   {                                                                             \
     if (bx == 256) return pick_launcher<double, ORDER, false, 256>(phys, mat);  \
     return nullptr;                                                             \
+  }                                                                             \
+  PairLaunchFn pair_launcher_o##ORDER##_f64(int mat)                            \
+  {                                                                             \
+    return pick_pair<double, ORDER, false>(mat);                                \
   }                                                                             \
   }
 
@@ -40,5 +62,10 @@ StageLaunchFn pick_launcher(int phys, int mat)
     if (bx != 256) return nullptr;                                              \
     return kahan ? pick_launcher<float, ORDER, true, 256>(phys, mat)            \
                  : pick_launcher<float, ORDER, false, 256>(phys, mat);          \
+  }                                                                             \
+  PairLaunchFn pair_launcher_o##ORDER##_f32(int mat, bool kahan)                \
+  {                                                                             \
+    return kahan ? pick_pair<float, ORDER, true>(mat)                           \
+                 : pick_pair<float, ORDER, false>(mat);                         \
   }                                                                             \
   }

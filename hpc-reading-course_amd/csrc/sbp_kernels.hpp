@@ -74,6 +74,7 @@ template <typename T> struct StageArgs {
   T third;                   // 1/3 (stage 4)
   int bc;
   int tiles_x, ty_rows, n_tiles, n_edge_blocks, n_edge;
+  int depth;                 // frame depth of the edge blocks (0: CLS)
 };
 
 template <typename T, int ORDER> __constant__ EdgeTable<T, ORDER> c_edge = make_edge_table<T, ORDER>();
@@ -166,7 +167,7 @@ __device__ __forceinline__ void rk_point(const StageArgs<T> &a, long o, const T 
     } else if constexpr (MODE == MODE_S2 || MODE == MODE_S3) {
       a.yout[q] = a.uin[q] + a.cy * K[c];
     } else {
-      const T U = a.uout[q];
+      const T U = a.uin[q];
       const T inc = stage4_increment<T, KAHAN>(a, U, a.y2[q], a.y3[q], Ycen[c], K[c]);
       if constexpr (!KAHAN) {
         a.uout[q] = U + inc;
@@ -220,7 +221,9 @@ __device__ __forceinline__ void rk_point_exact(const StageArgs<T> &a, long o, co
 }
 
 // --------------------------------------------------------------------------
-// Boundary frame: one thread per point of the CLS-deep frame.
+// Boundary frame: one thread per point of the frame, CLS deep (or a.depth
+// deep: the two-stage interior tiles below leave a frame of CLS + R).  Points
+// of the frame beyond the closure rows take the table's interior row.
 // --------------------------------------------------------------------------
 template <typename T, int ORDER, int PHYS, int MAT, int MODE, bool KAHAN>
 __device__ __forceinline__ void edge_point(const StageArgs<T> &a, int e)
@@ -230,24 +233,25 @@ __device__ __forceinline__ void edge_point(const StageArgs<T> &a, int e)
   constexpr int NC = AC ? 3 : 1;
   if (e >= a.n_edge) return;
   const int nx = a.nx, ny = a.ny;
-  const int nstrip = CLS * nx;
-  const int nside = CLS * (ny - 2 * CLS);
+  const int dep = a.depth > 0 ? a.depth : CLS;
+  const int nstrip = dep * nx;
+  const int nside = dep * (ny - 2 * dep);
   int i, j;
-  if (e < nstrip) {  // bottom strip, rows 0..CLS-1
+  if (e < nstrip) {  // bottom strip, rows 0..dep-1
     j = e / nx;
     i = e - j * nx;
   } else if (e < 2 * nstrip) {  // top strip
     e -= nstrip;
-    j = ny - CLS + e / nx;
+    j = ny - dep + e / nx;
     i = e - (e / nx) * nx;
   } else if (e < 2 * nstrip + nside) {  // left strip between them
     e -= 2 * nstrip;
-    j = CLS + e / CLS;
-    i = e - (e / CLS) * CLS;
+    j = dep + e / dep;
+    i = e - (e / dep) * dep;
   } else {  // right strip
     e -= 2 * nstrip + nside;
-    j = CLS + e / CLS;
-    i = nx - CLS + (e - (e / CLS) * CLS);
+    j = dep + e / dep;
+    i = nx - dep + (e - (e / dep) * dep);
   }
   const long pitch = a.pitch;
   const long o = j * pitch + a.xoff + i;
@@ -523,6 +527,259 @@ __global__ __launch_bounds__(BX) void stage2d_kernel(StageArgs<T> a)
 }
 
 // --------------------------------------------------------------------------
+// Two RK4 stages per launch on the interior (temporal blocking, acoustics).
+// --------------------------------------------------------------------------
+//   pair 1: K1 = F(U),  Y2 = U + dt/2 K1 on the tile grown by R (LDS only),
+//           K2 = F(Y2), Y3 = U + dt/2 K2 on the tile.        (reads U; writes Y2, Y3)
+//   pair 2: K3 = F(Y3), Y4 = U + dt K3 on the tile grown by R (LDS only),
+//           K4 = F(Y4), U' = U + ((Y2-U) + 2(Y3-U) + (Y4-U))/3 + dt/6 K4
+//                                              (reads Y3, U, Y2; writes U')
+// 3 + 4 = 7 words per value and step instead of the 13 of one launch per
+// stage.  Tiles cover [D, nx-D) x [D, ny-D) with D = CLS + R, so every point
+// of a grown tile is an interior point of the operator.  The D-deep frame runs
+// the one-stage edge path: stage 1 (pair-1 launch), stage 2 (own launch),
+// stage 3 (pair-2 launch), stage 4 (own launch).  Pair 2 therefore also stores
+// Y4 on the interior points next to the frame, which the frame's stage 4
+// differentiates, and writes U' into the Y2 array (Y2 is read only at the
+// point itself there, while U is read R points around it by neighbouring
+// tiles); the host swaps the roles of the U and Y2 arrays after each step.
+// Per point the arithmetic is that of the one-stage kernels, term for term.
+//
+// A block of NT threads owns NT mid columns (x0-R .. x0-R+NT-1) and the
+// NT-2R output columns inside them, and marches down the rows: iteration k
+// stages input row r = j0-2R+k in an LDS ring, computes mid row r-R into a
+// second ring, then output row r-2R.  Two barriers per row; ring depths
+// 2R+2 (input) and 2R+1 (mid) keep every slot a row reads alive until the
+// barrier that precedes its overwrite.
+// Threads (mid columns) per two-stage tile; the host sizes the grid with it.
+#ifndef SBP_PAIR_NT
+#define SBP_PAIR_NT 256
+#endif
+constexpr int kPairNT = SBP_PAIR_NT;
+
+template <typename T> struct PairArgs {
+  const T *yin;  // stencil operand of the outer stage: U (pair 1), Y3 (pair 2)
+  const T *u;    // U (pair 2)
+  const T *y2;   // Y2 (pair 2)
+  T *o1;         // pair 1: Y2;  pair 2: U' (the y2 array, read first at each point)
+  T *o2;         // pair 1: Y3;  pair 2: Y4 on the points next to the frame
+  T *comp;       // Kahan compensation of U (pair 2)
+  const T *rinv, *bulk;
+  long plane;
+  int pitch, xoff, nx, ny;
+  T hix, hiy;
+  T c1, c2, third;  // pair 1: dt/2, dt/2;  pair 2: dt, dt/6
+  int tiles_x, ty_rows, n_tiles, n_edge_blocks;
+};
+
+template <typename T> struct PairRow {
+  T in[3];        // stencil operand (u, v, p) at input row r, mid column
+  T h[2];         // u, p at input row r, halo column
+  T U[3];         // pair 2: U at mid row r-R
+  T ri, K;        // materials at mid row r-R
+  T Y2[3], C[3];  // pair 2: Y2 and compensation at output row r-2R
+};
+
+template <typename T, int ORDER, int MAT, int PAIR, bool KAHAN, int NT, int PF>
+__device__ __forceinline__ void pair_tile(const PairArgs<T> &a, int tile)
+{
+  using O = Ops<ORDER>;
+  constexpr int R = O::R, D = O::CLS + O::R;
+  constexpr int DI = 2 * R + 2, DM = 2 * R + 1;
+  constexpr int LW = NT + 2 * R;
+  constexpr int BXO = NT - 2 * R;
+  constexpr bool LD_M = MAT == M_ARR;
+  // Y4 band next to the frame: the frame's table rows read TAPS consecutive
+  // points from i-R (zero padded past i+R), so up to TAPS-R-1 into the tiles.
+  constexpr int RW = Geom<ORDER>::TAPS - R - 1;
+
+  __shared__ T s_in[3][DI][LW];
+  __shared__ T s_mid[3][DM][NT];
+
+  const int t = threadIdx.x;
+  // row-major tiles (measured 4% faster than column-major, which shares the
+  // vertical halos in L2 but splits rows across XCDs)
+  const int tx = tile % a.tiles_x, tyb = tile / a.tiles_x;
+  const int x0 = D + tx * BXO;               // first output column of the tile
+  const int mc = x0 - R + t;                 // this thread's mid column
+  const int mcc = min(mc, a.nx - 1);
+  const bool hl = t < R, hr = t >= NT - R, halo = hl || hr;
+  const int hc = hl ? mc - R : min(mc + R, a.nx - 1);
+  const int hslot = hl ? t : t + 2 * R;
+  const bool out_col = t >= R && t < NT - R && mc < a.nx - D;
+  const int j0 = D + tyb * a.ty_rows;
+  const int j1 = min(j0 + a.ty_rows, a.ny - D);
+  const int nk = (j1 - j0) + 4 * R;
+  const long plane = a.plane, pitch = a.pitch;
+  const long col = a.xoff + mcc, hcol = a.xoff + hc;
+  const int jmax = a.ny - 1;
+  const T *__restrict__ yin = a.yin;
+
+  auto fetch = [&](PairRow<T> &q, int k) {
+    const int r = min(j0 - 2 * R + k, jmax);  // j0 - 2R >= CLS - R >= 0
+    const long orow = (long)r * pitch;
+#pragma unroll
+    for (int c = 0; c < 3; c++) q.in[c] = yin[c * plane + orow + col];
+    if (halo) {
+      q.h[0] = yin[orow + hcol];
+      q.h[1] = yin[2 * plane + orow + hcol];
+    }
+    const long om = (long)max(r - R, 0) * pitch + col;
+    if constexpr (PAIR == 2) {
+#pragma unroll
+      for (int c = 0; c < 3; c++) q.U[c] = a.u[c * plane + om];
+    }
+    if constexpr (LD_M) {
+      q.ri = a.rinv[om];
+      q.K = a.bulk[om];
+    }
+    if constexpr (PAIR == 2) {
+      const long oo = (long)max(r - 2 * R, 0) * pitch + col;
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        q.Y2[c] = ldg<true>(a.y2 + c * plane + oo);
+        if constexpr (KAHAN) q.C[c] = ldg<true>(a.comp + c * plane + oo);
+      }
+    }
+  };
+
+  // Prefetch queue of PF+1 slots: iteration k consumes slot k % (PF+1) and
+  // loads row k+PF into slot (k+PF) % (PF+1), which iteration k-1 emptied, so
+  // no slot is overwritten while its old value is live and every slot keeps
+  // one register set (no copies of in-flight loads at the loop latch, which
+  // would have to wait for them).
+  constexpr int S = PF + 1;
+  PairRow<T> q[S];
+#pragma unroll
+  for (int k = 0; k < S; k++) {
+    q[k] = PairRow<T>{};
+    if (k < PF) fetch(q[k], k);
+  }
+  // pointwise values of the last R+1 mid rows, oldest first: U (pair 2), rho^-1, K
+  T fU[R + 1][3], fri[R + 1], fK[R + 1];
+#pragma unroll
+  for (int p = 0; p <= R; p++) {
+    fri[p] = fK[p] = T(1);
+#pragma unroll
+    for (int c = 0; c < 3; c++) fU[p][c] = T(0);
+  }
+
+  int si = 0;  // input-ring slot of row r (k % DI)
+  int sm = 0;  // mid-ring slot of mid row r-R ((k - 2R) % DM)
+  // The row loop is unrolled by S so that the queue is indexed statically.
+  // It runs whole groups of S iterations (no exit inside a group, which
+  // would also make the compiler drain every load at the loop latch);
+  // iterations k >= nk stage clamped rows and store nothing.
+  for (int kb = 0; kb < nk; kb += S) {
+#pragma unroll
+    for (int pq = 0; pq < S; pq++) {
+      const int k = kb + pq;
+      const PairRow<T> &cur = q[pq];
+      fetch(q[(pq + PF) % S], k + PF);
+
+#pragma unroll
+      for (int c = 0; c < 3; c++) s_in[c][si][t + R] = cur.in[c];
+      if (halo) {
+        s_in[0][si][hslot] = cur.h[0];
+        s_in[2][si][hslot] = cur.h[1];
+      }
+      __syncthreads();
+
+      if (k >= 2 * R) {  // mid row m = r - R: outer stage on the grown tile
+        int sy[2 * R + 1];  // input slots of rows m-R .. m+R
+#pragma unroll
+        for (int s = 0; s <= 2 * R; s++) sy[s] = (si + DI - 2 * R + s) % DI;
+        const int sr = sy[R];
+        const T dxp = dint<T, ORDER>([&](int s) { return s_in[2][sr][t + s]; }, a.hix);
+        const T dyp = dint<T, ORDER>([&](int s) { return s_in[2][sy[s]][t + R]; }, a.hiy);
+        const T dxu = dint<T, ORDER>([&](int s) { return s_in[0][sr][t + s]; }, a.hix);
+        const T dyv = dint<T, ORDER>([&](int s) { return s_in[1][sy[s]][t + R]; }, a.hiy);
+        T Um[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) Um[c] = PAIR == 1 ? s_in[c][sr][t + R] : cur.U[c];
+        const T ri = LD_M ? cur.ri : T(1), Kb = LD_M ? cur.K : T(1);
+        T F[3];
+        acoustic_F<T, MAT>(dxp, dyp, dxu, dyv, ri, Kb, T(0), T(0), F[0], F[1], F[2]);
+#pragma unroll
+        for (int c = 0; c < 3; c++) s_mid[c][sm][t] = Um[c] + a.c1 * F[c];
+#pragma unroll
+        for (int p = 0; p < R; p++) {
+          fri[p] = fri[p + 1];
+          fK[p] = fK[p + 1];
+#pragma unroll
+          for (int c = 0; c < 3; c++) fU[p][c] = fU[p + 1][c];
+        }
+        fri[R] = ri;
+        fK[R] = Kb;
+#pragma unroll
+        for (int c = 0; c < 3; c++) fU[R][c] = Um[c];
+      }
+      __syncthreads();
+
+      if (k >= 4 * R && k < nk && out_col) {  // output row o = r - 2R: inner stage on the tile
+        const int o = j0 - 4 * R + k;
+        int sz[2 * R + 1];  // mid slots of rows o-R .. o+R
+#pragma unroll
+        for (int s = 0; s <= 2 * R; s++) sz[s] = (sm + DM - 2 * R + s) % DM;
+        const int so = sz[R];
+        const T dxp = dint<T, ORDER>([&](int s) { return s_mid[2][so][t - R + s]; }, a.hix);
+        const T dyp = dint<T, ORDER>([&](int s) { return s_mid[2][sz[s]][t]; }, a.hiy);
+        const T dxu = dint<T, ORDER>([&](int s) { return s_mid[0][so][t - R + s]; }, a.hix);
+        const T dyv = dint<T, ORDER>([&](int s) { return s_mid[1][sz[s]][t]; }, a.hiy);
+        T F[3];
+        acoustic_F<T, MAT>(dxp, dyp, dxu, dyv, fri[0], fK[0], T(0), T(0), F[0], F[1], F[2]);
+        const int sio = (si + DI - 2 * R) % DI;  // input slot of row o
+        const long qo = (long)o * pitch + a.xoff + mc;
+        if constexpr (PAIR == 1) {
+#pragma unroll
+          for (int c = 0; c < 3; c++) {
+            stg<true>(a.o1 + c * plane + qo, s_mid[c][so][t]);
+            stg<true>(a.o2 + c * plane + qo, s_in[c][sio][t + R] + a.c2 * F[c]);
+          }
+        } else {
+          const bool ring = mc - D < RW || a.nx - D - 1 - mc < RW || o - D < RW || a.ny - D - 1 - o < RW;
+#pragma unroll
+          for (int c = 0; c < 3; c++) {
+            const T U = fU[0][c];
+            const T Y4 = s_mid[c][so][t];
+            const T inc = ((cur.Y2[c] - U) + 2 * (s_in[c][sio][t + R] - U) + (Y4 - U)) * a.third + a.c2 * F[c];
+            if constexpr (!KAHAN) {
+              stg<true>(a.o1 + c * plane + qo, U + inc);
+            } else {
+              const T y = inc - cur.C[c];
+              const T tt = U + y;
+              stg<true>(a.comp + c * plane + qo, (tt - U) - y);
+              stg<true>(a.o1 + c * plane + qo, tt);
+            }
+            if (ring) a.o2[c * plane + qo] = Y4;
+          }
+        }
+      }
+      si = si + 1 == DI ? 0 : si + 1;
+      if (k >= 2 * R) sm = sm + 1 == DM ? 0 : sm + 1;
+    }
+  }
+}
+
+// Blocks [0, n_edge_blocks): stage 1 (pair 1) or stage 3 (pair 2) of the
+// D-deep frame (ea); the rest: two-stage interior tiles (a), XCD-remapped.
+template <typename T, int ORDER, int MAT, int PAIR, bool KAHAN, int NT, int PF>
+__global__ __launch_bounds__(NT) void pair2d_kernel(StageArgs<T> ea, PairArgs<T> a)
+{
+  const int b = blockIdx.x;
+  if (b < a.n_edge_blocks) {
+    edge_point<T, ORDER, PH_ACOUSTIC, MAT, PAIR == 1 ? MODE_S1 : MODE_S3, KAHAN>(ea, b * NT + threadIdx.x);
+    return;
+  }
+  const int bi = b - a.n_edge_blocks;
+  const int nt = a.n_tiles;
+  const int xcd = bi % kNumXcd, local = bi / kNumXcd;
+  const int q = nt / kNumXcd, r = nt % kNumXcd;
+  const int tile = xcd * q + min(xcd, r) + local;
+  pair_tile<T, ORDER, MAT, PAIR, KAHAN, NT, PF>(a, tile);
+}
+
+// --------------------------------------------------------------------------
 // 1D problems: one thread per point, table-driven rows.
 // --------------------------------------------------------------------------
 template <typename T, int ORDER, int PHYS, int MODE, bool KAHAN>
@@ -759,6 +1016,24 @@ struct Launcher1D {
   }
 };
 
+using PairLaunchFn = hipError_t (*)(int pair, const void *edge_args, const void *pair_args, int grid,
+                                    hipStream_t s);
+
+// PF1 / PF2: rows of loads in flight ahead of the row being computed in the
+// pair-1 (one stencil operand per row) and pair-2 (operand + U + Y2) launches.
+template <typename T, int ORDER, int MAT, bool KAHAN, int NT, int PF1, int PF2> struct PairLauncher {
+  static hipError_t launch(int pair, const void *ea, const void *pa, int grid, hipStream_t s)
+  {
+    const StageArgs<T> &e = *static_cast<const StageArgs<T> *>(ea);
+    const PairArgs<T> &p = *static_cast<const PairArgs<T> *>(pa);
+    if (pair == 1)
+      hipLaunchKernelGGL((pair2d_kernel<T, ORDER, MAT, 1, KAHAN, NT, PF1>), dim3(grid), dim3(NT), 0, s, e, p);
+    else
+      hipLaunchKernelGGL((pair2d_kernel<T, ORDER, MAT, 2, KAHAN, NT, PF2>), dim3(grid), dim3(NT), 0, s, e, p);
+    return hipGetLastError();
+  }
+};
+
 // Per-order, per-precision registries (sbp_inst_o{2,4,6}_{f64,f32}.hip).
 // The FP64 units are compiled without FMA contraction so that every
 // right-hand side evaluates in the same operation order, and with the same
@@ -769,5 +1044,13 @@ StageLaunchFn launcher_o6_f64(int phys, int mat, int bx);
 StageLaunchFn launcher_o2_f32(int phys, int mat, bool kahan, int bx);
 StageLaunchFn launcher_o4_f32(int phys, int mat, bool kahan, int bx);
 StageLaunchFn launcher_o6_f32(int phys, int mat, bool kahan, int bx);
+// Two-stage interior launchers of the acoustic system (material M_UNIT or
+// M_ARR); nullptr where none is built.
+PairLaunchFn pair_launcher_o2_f64(int mat);
+PairLaunchFn pair_launcher_o4_f64(int mat);
+PairLaunchFn pair_launcher_o6_f64(int mat);
+PairLaunchFn pair_launcher_o2_f32(int mat, bool kahan);
+PairLaunchFn pair_launcher_o4_f32(int mat, bool kahan);
+PairLaunchFn pair_launcher_o6_f32(int mat, bool kahan);
 
 }  // namespace sbp

@@ -95,6 +95,18 @@ long sbp_vector_length(const sbp_ctx *ctx);
  * tuning knob with no reference counterpart; results do not depend on it. */
 int  sbp_set_tile_rows(sbp_ctx *ctx, int rows);
 
+/* Two-stage stepping (on = 1): each RK4 step runs stages 1-2 and stages 3-4
+ * of the interior in one launch each, the inner stage kept on chip, and the
+ * (CLS+R)-deep boundary frame in the one-stage kernels; 7 array passes per
+ * step instead of 13, same arithmetic per point.  2D acoustic contexts with
+ * SBP_MAT_UNIT or SBP_MAT_ARRAYS only (SBP_EINVAL otherwise); on by default
+ * for those with at least 1500^2 interior points.  rows: tile height of the
+ * two-stage tiles (0 = chosen from the grid).  A performance knob with no
+ * reference counterpart: it changes how ts_rk4's steps are scheduled on the
+ * device (ts_rk.cpp:54-65), not what they compute. */
+int  sbp_set_fused(sbp_ctx *ctx, int on, int rows);
+int  sbp_get_fused(const sbp_ctx *ctx);
+
 /* ---- right-hand side ----------------------------------------------------
  * dst = F(t, src) on host AoS arrays.  Replaces the rhs_serial callbacks
  * (advection_1D_sim.cpp:212-228, advection_2D_sim.cpp:232-249,

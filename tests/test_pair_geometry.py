@@ -1,0 +1,210 @@
+"""CPU model of the two-stage interior launches (sbp_capi.hip
+compute_pair_geometry / default_pair_rows, sbp_kernels.hpp pair_tile /
+pair2d_kernel, edge_point with a D = CLS + R deep frame).
+
+Checks, for many grid shapes and all orders, that
+* every global load of a pair tile (operand rows with prefetch, halo
+  columns, the mid-row and output-row pointwise loads) and of the D-deep
+  frame's table rows stays inside [0,nx) x [0,ny);
+* the pair tiles' output points and the frame partition the grid;
+* every mid point an output point differentiates is computed by the same
+  tile, at an unclamped column, from input columns the tile staged, and is an
+  interior point of the operator (so the interior stencil is the right row);
+* every interior point the frame's stage 4 reads (zero-padded taps
+  included) is one of the points pair 2 stores Y4 at;
+* the LDS ring slots hold the rows each phase reads, and no slot a phase
+  reads is overwritten before a barrier separates the two.
+"""
+import numpy as np
+import pytest
+
+from test_kernel_geometry import CLS, R, TAPS, NXCD, dline_range, edge_decode, xcd_tile
+
+NT = 256
+
+
+def default_pair_rows(nx, ny, order):
+    r, d = R[order], CLS[order] + R[order]
+    wi, hi = nx - 2 * d, ny - 2 * d
+    if wi <= 0 or hi <= 0:
+        return 16
+    tx = (wi + NT - 2 * r - 1) // (NT - 2 * r)
+    t = 32 if order == 6 else 16
+    while t > 8 and tx * ((hi + t - 1) // t) < 768:
+        t //= 2
+    return t
+
+
+def pair_geometry(nx, ny, order, rows=None):
+    r, d = R[order], CLS[order] + R[order]
+    rows = rows or default_pair_rows(nx, ny, order)
+    wi, hi = nx - 2 * d, ny - 2 * d
+    bxo = NT - 2 * r
+    tiles_x = (wi + bxo - 1) // bxo if wi > 0 else 0
+    n_tiles = tiles_x * ((hi + rows - 1) // rows) if (wi > 0 and hi > 0) else 0
+    n_edge = 2 * d * nx + 2 * d * (ny - 2 * d)
+    eb = (n_edge + NT - 1) // NT
+    neb = (eb + NXCD - 1) // NXCD * NXCD
+    return dict(d=d, rows=rows, tiles_x=tiles_x, n_tiles=n_tiles, n_edge=n_edge, n_edge_blocks=neb)
+
+
+def simulate_rings(order, nrows):
+    """Replays the slot arithmetic of pair_tile for a tile of nrows output
+    rows (rows numbered from j0 = 0) and checks contents and hazards."""
+    r = R[order]
+    DI, DM = 2 * r + 2, 2 * r + 1
+    s_in = [None] * DI
+    s_mid = [None] * DM
+    # (slot, iteration) of the last read of each ring slot in the output phase
+    last_out_read_in = {}
+    si = sm = 0
+    nk = nrows + 4 * r
+    for k in range(nk):
+        row = -2 * r + k
+        # phase 0: store input row; must not overwrite a slot read by out(k-1)
+        assert last_out_read_in.get(si) != k - 1, ("input slot race", order, k)
+        s_in[si] = row
+        # barrier
+        if k >= 2 * r:
+            m = row - r
+            sy = [(si + DI - 2 * r + s) % DI for s in range(2 * r + 1)]
+            assert [s_in[x] for x in sy] == list(range(m - r, m + r + 1))
+            s_mid[sm] = m
+        # barrier
+        if k >= 4 * r:
+            o = -4 * r + k
+            sz = [(sm + DM - 2 * r + s) % DM for s in range(2 * r + 1)]
+            assert [s_mid[x] for x in sz] == list(range(o - r, o + r + 1)), (order, k)
+            sio = (si + DI - 2 * r) % DI
+            assert s_in[sio] == o
+            last_out_read_in[sio] = k
+            # the mid slots read here are overwritten at mid(k+1) at the
+            # earliest, which follows the first barrier of iteration k+1
+        si = 0 if si + 1 == DI else si + 1
+        if k >= 2 * r:
+            sm = 0 if sm + 1 == DM else sm + 1
+
+
+def check(nx, ny, order, rows=None, pf=3):
+    g = pair_geometry(nx, ny, order, rows)
+    r, d, cls = R[order], g["d"], CLS[order]
+    LW = NT + 2 * r
+    count = np.zeros((ny, nx), dtype=np.int32)
+    # frame: stage kernels with depth d
+    frame_reads_interior = set()
+    for e in range(g["n_edge"]):
+        i, j = edge_decode(e, nx, ny, d)
+        assert 0 <= i < nx and 0 <= j < ny
+        count[j, i] += 1
+        # every tap read, the zero-padded ones included: a stale non-finite
+        # value times zero would still poison the sum
+        lo, hi = dline_range(nx, i, order)
+        assert 0 <= lo and hi < nx
+        for x in range(lo, hi + 1):
+            if d <= x < nx - d and d <= j < ny - d:
+                frame_reads_interior.add((j, x))
+        lo, hi = dline_range(ny, j, order)
+        assert 0 <= lo and hi < ny
+        for y in range(lo, hi + 1):
+            if d <= i < nx - d and d <= y < ny - d:
+                frame_reads_interior.add((y, i))
+    assert g["n_edge_blocks"] * NT >= g["n_edge"] and g["n_edge_blocks"] % NXCD == 0
+    ring = set()
+    tiles = [xcd_tile(bi, g["n_tiles"]) for bi in range(g["n_tiles"])]
+    assert sorted(tiles) == list(range(g["n_tiles"]))
+    t = np.arange(NT)
+    for tile in range(g["n_tiles"]):
+        tx, tyb = tile % g["tiles_x"], tile // g["tiles_x"]
+        x0 = d + tx * (NT - 2 * r)
+        mc = x0 - r + t
+        mcc = np.minimum(mc, nx - 1)
+        hl, hr = t < r, t >= NT - r
+        hc = np.where(hl, mc - r, np.minimum(mc + r, nx - 1))
+        assert mcc.min() >= 0 and mcc.max() < nx
+        assert hc[hl | hr].min() >= 0 and hc[hl | hr].max() < nx
+        hslot = np.where(hl, t, t + 2 * r)
+        assert hslot[hl | hr].min() >= 0 and hslot[hl | hr].max() < LW
+        out = (t >= r) & (t < NT - r) & (mc < nx - d)
+        j0 = d + tyb * g["rows"]
+        j1 = min(j0 + g["rows"], ny - d)
+        assert j1 > j0
+        nk = (j1 - j0) + 4 * r
+        for k in range(nk + pf):
+            row = min(j0 - 2 * r + k, ny - 1)
+            assert 0 <= row < ny and 0 <= max(row - r, 0) and 0 <= max(row - 2 * r, 0)
+        # staged input columns of this tile (LDS position p <-> column x0 - 2R + p)
+        staged = set(range(x0 - 2 * r, x0 - 2 * r + LW))
+        for tt in t[out]:
+            c = mc[tt]
+            assert c >= d
+            for s in range(2 * r + 1):
+                mcol = c - r + s
+                # mid column computed by thread tt - r + s at an unclamped column
+                assert mc[tt - r + s] == mcol and mcol < nx
+                assert cls <= mcol < nx - cls
+                for s2 in range(2 * r + 1):
+                    assert mcol - r + s2 in staged and 0 <= mcol - r + s2 < nx
+        cols = mc[out]
+        for o in range(j0, j1):
+            for s in range(2 * r + 1):
+                assert cls <= o - r + s < ny - cls
+            count[o, cols] += 1
+            rw = TAPS[order] - r - 1
+            for c in cols:
+                if c - d < rw or nx - d - 1 - c < rw or o - d < rw or ny - d - 1 - o < rw:
+                    ring.add((o, c))
+    assert (count == 1).all(), (nx, ny, order, np.argwhere(count != 1)[:5])
+    assert frame_reads_interior <= ring, sorted(frame_reads_interior - ring)[:5]
+    return g
+
+
+SHAPES = [(13, 14), (19, 19), (37, 29), (256, 40), (257, 257), (270, 33), (300, 45), (520, 521), (531, 270)]
+
+
+@pytest.mark.parametrize("order", [2, 4, 6])
+@pytest.mark.parametrize("shape", SHAPES)
+def test_pair_geometry_covers_grid_once(order, shape):
+    nx, ny = shape
+    d = CLS[order] + R[order]
+    if nx <= 2 * d or ny <= 2 * d:
+        pytest.skip("no interior for the two-stage tiles")
+    check(nx, ny, order)
+
+
+@pytest.mark.parametrize("rows", [1, 3, 8, 64, 200])
+def test_pair_geometry_tile_heights(rows):
+    check(300, 300, 6, rows)
+    check(270, 70, 4, rows)
+    check(40, 300, 2, rows)
+
+
+@pytest.mark.parametrize("order", [2, 4, 6])
+@pytest.mark.parametrize("nrows", [1, 2, 5, 8, 64])
+def test_pair_ring_slots(order, nrows):
+    simulate_rings(order, nrows)
+
+
+def test_pair_ring_race_is_detected():
+    """The hazard check fails for an input ring one row shorter (2R+1 deep),
+    where the store of row r would overwrite the row the previous output
+    phase read without a barrier in between."""
+    r = 2
+    DI = 2 * r + 1
+    last = {}
+    si, clash = 0, False
+    for k in range(20):
+        if last.get(si) == k - 1:
+            clash = True
+        if k >= 4 * r:
+            last[(si + DI - 2 * r) % DI] = k
+        si = (si + 1) % DI
+    assert clash
+
+
+def test_pair_geometry_large_grid_parameters():
+    g = pair_geometry(4097, 4097, 4)
+    assert g["d"] == 6 and g["tiles_x"] == 17 and g["rows"] == 16 and g["n_tiles"] == 17 * 256
+    assert pair_geometry(4097, 4097, 6)["rows"] == 32
+    assert pair_geometry(1025, 1025, 4)["rows"] == 8
+    check(4097, 300, 4)
+    check(1025, 1025, 4)
