@@ -84,7 +84,7 @@ void compute_geometry(sbp_ctx *c)
 // tiles per CU (>= 960 tiles on 256 CUs).  Tall tiles amortise the 2R-row y
 // halo; on grids too small for that, occupancy wins (measured, order 4:
 // 4097^2 -> 64 rows, 2049^2 -> 16, 1025^2 -> 4 rows at 1.25x the 16-row rate).
-// Two-stage interior tiles: kPairNT mid columns, kPairNT - 2R output
+// Two-stage interior tiles: pair_nt mid columns, pair_nt - 2R output
 // columns, inside the D = CLS + R deep frame.
 
 void compute_pair_geometry(sbp_ctx *c)
@@ -93,11 +93,12 @@ void compute_pair_geometry(sbp_ctx *c)
   const int D = c->cls + R;
   c->p_depth = D;
   const int wi = c->nx - 2 * D, hi = c->ny - 2 * D;
-  const int bxo = kPairNT - 2 * R;
+  const int nt = pair_nt((int)c->esz, c->order);
+  const int bxo = nt - 2 * R;
   c->p_tiles_x = wi > 0 ? (wi + bxo - 1) / bxo : 0;
   c->p_n_tiles = (wi > 0 && hi > 0) ? c->p_tiles_x * ((hi + c->p_ty_rows - 1) / c->p_ty_rows) : 0;
   c->p_n_edge = 2 * D * c->nx + 2 * D * (c->ny - 2 * D);
-  const int eb = (c->p_n_edge + kPairNT - 1) / kPairNT;
+  const int eb = (c->p_n_edge + nt - 1) / nt;
   c->p_n_edge_blocks = (eb + kNumXcd - 1) / kNumXcd * kNumXcd;
 }
 
@@ -110,7 +111,8 @@ int default_pair_rows(const sbp_ctx *c)
   const int R = c->order / 2, D = c->cls + R;
   const int wi = c->nx - 2 * D, hi = c->ny - 2 * D;
   if (wi <= 0 || hi <= 0) return 16;
-  const int tx = (wi + kPairNT - 2 * R - 1) / (kPairNT - 2 * R);
+  const int nt = pair_nt((int)c->esz, c->order);
+  const int tx = (wi + nt - 2 * R - 1) / (nt - 2 * R);
   int t = c->order == 6 ? 32 : 16;
   while (t > 8 && (long)tx * ((hi + t - 1) / t) < 768) t /= 2;
   return t;

@@ -22,7 +22,7 @@ StageLaunchFn pick_launcher(int phys, int mat)
   return nullptr;
 }
 
-// Two-stage interior tiles: kPairNT threads; prefetch depths (rows of loads in
+// Two-stage interior tiles: pair_nt threads; prefetch depths (rows of loads in
 // flight per wave) from a sweep on gfx950.  LDS holds these kernels at two
 // blocks per CU, so deep register queues are what keeps HBM busy.
 #ifndef SBP_PAIR_PF1
@@ -35,8 +35,9 @@ template <typename T, int ORDER, bool KAHAN>
 PairLaunchFn pick_pair(int mat)
 {
   constexpr int PF1 = SBP_PAIR_PF1, PF2 = SBP_PAIR_PF2;
-  if (mat == M_UNIT) return &PairLauncher<T, ORDER, M_UNIT, KAHAN, kPairNT, PF1, PF2>::launch;
-  if (mat == M_ARR) return &PairLauncher<T, ORDER, M_ARR, KAHAN, kPairNT, PF1, PF2>::launch;
+  constexpr int NT = pair_nt(sizeof(T), ORDER);
+  if (mat == M_UNIT) return &PairLauncher<T, ORDER, M_UNIT, KAHAN, NT, PF1, PF2>::launch;
+  if (mat == M_ARR) return &PairLauncher<T, ORDER, M_ARR, KAHAN, NT, PF1, PF2>::launch;
   return nullptr;
 }
 

@@ -552,10 +552,9 @@ __global__ __launch_bounds__(BX) void stage2d_kernel(StageArgs<T> a)
 // 2R+2 (input) and 2R+1 (mid) keep every slot a row reads alive until the
 // barrier that precedes its overwrite.
 // Threads (mid columns) per two-stage tile; the host sizes the grid with it.
-#ifndef SBP_PAIR_NT
-#define SBP_PAIR_NT 256
-#endif
-constexpr int kPairNT = SBP_PAIR_NT;
+// FP64 order 6 takes 128: its 256-wide rings (93 KB of LDS) would leave one
+// block per CU (measured at 4097^2: 0.92 ms per step against 0.99).
+constexpr int pair_nt(int esz, int order) { return esz == 8 && order == 6 ? 128 : 256; }
 
 template <typename T> struct PairArgs {
   const T *yin;  // stencil operand of the outer stage: U (pair 1), Y3 (pair 2)

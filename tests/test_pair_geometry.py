@@ -20,10 +20,13 @@ import pytest
 
 from test_kernel_geometry import CLS, R, TAPS, NXCD, dline_range, edge_decode, xcd_tile
 
-NT = 256
+def pair_nt(esz, order):
+    """Threads per two-stage tile: 128 for FP64 order 6, else 256."""
+    return 128 if (esz == 8 and order == 6) else 256
 
 
-def default_pair_rows(nx, ny, order):
+def default_pair_rows(nx, ny, order, esz=8):
+    NT = pair_nt(esz, order)
     r, d = R[order], CLS[order] + R[order]
     wi, hi = nx - 2 * d, ny - 2 * d
     if wi <= 0 or hi <= 0:
@@ -35,9 +38,10 @@ def default_pair_rows(nx, ny, order):
     return t
 
 
-def pair_geometry(nx, ny, order, rows=None):
+def pair_geometry(nx, ny, order, rows=None, esz=8):
+    NT = pair_nt(esz, order)
     r, d = R[order], CLS[order] + R[order]
-    rows = rows or default_pair_rows(nx, ny, order)
+    rows = rows or default_pair_rows(nx, ny, order, esz)
     wi, hi = nx - 2 * d, ny - 2 * d
     bxo = NT - 2 * r
     tiles_x = (wi + bxo - 1) // bxo if wi > 0 else 0
@@ -45,7 +49,7 @@ def pair_geometry(nx, ny, order, rows=None):
     n_edge = 2 * d * nx + 2 * d * (ny - 2 * d)
     eb = (n_edge + NT - 1) // NT
     neb = (eb + NXCD - 1) // NXCD * NXCD
-    return dict(d=d, rows=rows, tiles_x=tiles_x, n_tiles=n_tiles, n_edge=n_edge, n_edge_blocks=neb)
+    return dict(d=d, rows=rows, tiles_x=tiles_x, n_tiles=n_tiles, n_edge=n_edge, n_edge_blocks=neb, nt=NT)
 
 
 def simulate_rings(order, nrows):
@@ -85,8 +89,9 @@ def simulate_rings(order, nrows):
             sm = 0 if sm + 1 == DM else sm + 1
 
 
-def check(nx, ny, order, rows=None, pf=3):
-    g = pair_geometry(nx, ny, order, rows)
+def check(nx, ny, order, rows=None, pf=3, esz=8):
+    g = pair_geometry(nx, ny, order, rows, esz)
+    NT = g["nt"]
     r, d, cls = R[order], g["d"], CLS[order]
     LW = NT + 2 * r
     count = np.zeros((ny, nx), dtype=np.int32)
@@ -161,19 +166,21 @@ def check(nx, ny, order, rows=None, pf=3):
 SHAPES = [(13, 14), (19, 19), (37, 29), (256, 40), (257, 257), (270, 33), (300, 45), (520, 521), (531, 270)]
 
 
+@pytest.mark.parametrize("esz", [8, 4])
 @pytest.mark.parametrize("order", [2, 4, 6])
 @pytest.mark.parametrize("shape", SHAPES)
-def test_pair_geometry_covers_grid_once(order, shape):
+def test_pair_geometry_covers_grid_once(order, shape, esz):
     nx, ny = shape
     d = CLS[order] + R[order]
     if nx <= 2 * d or ny <= 2 * d:
         pytest.skip("no interior for the two-stage tiles")
-    check(nx, ny, order)
+    check(nx, ny, order, esz=esz)
 
 
 @pytest.mark.parametrize("rows", [1, 3, 8, 64, 200])
 def test_pair_geometry_tile_heights(rows):
     check(300, 300, 6, rows)
+    check(300, 300, 6, rows, esz=4)
     check(270, 70, 4, rows)
     check(40, 300, 2, rows)
 
@@ -204,7 +211,8 @@ def test_pair_ring_race_is_detected():
 def test_pair_geometry_large_grid_parameters():
     g = pair_geometry(4097, 4097, 4)
     assert g["d"] == 6 and g["tiles_x"] == 17 and g["rows"] == 16 and g["n_tiles"] == 17 * 256
-    assert pair_geometry(4097, 4097, 6)["rows"] == 32
+    assert pair_geometry(4097, 4097, 6)["rows"] == 32 and pair_geometry(4097, 4097, 6)["nt"] == 128
+    assert pair_geometry(4097, 4097, 6, esz=4)["nt"] == 256
     assert pair_geometry(1025, 1025, 4)["rows"] == 8
     check(4097, 300, 4)
     check(1025, 1025, 4)
