@@ -144,7 +144,7 @@ def batched(count=256, steps=500, warmup=10, prec=1):
     dofs = sum(sp.nx * sp.ny * 3 for sp in specs)
     bpd = bytes_per_dof(prec, materials=True)
     achieved = dofs * bpd * steps / (ev_ms * 1e-3) / 1e9
-    kbpd = kernel_bytes_per_dof(prec, materials=True)
+    kbpd = kernel_bytes_per_dof(prec, materials=True, fused=cfg.get("two_stage", False))
     return {"instances": count, "dofs": dofs, "steps": steps, "precision": ["fp64", "fp32", "fp32_kahan"][prec],
             "dof_updates_per_s": dofs * steps / wall, "ms_per_step": wall / steps * 1e3,
             "event_ms_per_step": ev_ms / steps, "bytes_per_dof": bpd, "achieved_gbs": achieved,

@@ -78,7 +78,7 @@ EXPORTS = {
     "sbp_batch_num_configs": (C.c_int, []),
     "sbp_batch_set_config": (C.c_int, [C.c_void_p, C.c_int]),
     "sbp_batch_get_config": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int),
-                                       C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+                                       C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "sbp_batch_autotune": (C.c_int, [C.c_void_p, _DP, C.POINTER(C.c_int)]),
     "sbp_get_state_strided": (C.c_int, [C.c_void_p, C.c_int, _DP]),
     "sbp_batch_get_state_strided": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _DP]),
@@ -375,10 +375,10 @@ class Batch:
         _check(load_library().sbp_batch_set_config(self._h, int(cfg)))
 
     def get_config(self):
-        v = [C.c_int() for _ in range(4)]
+        v = [C.c_int() for _ in range(5)]
         _check(load_library().sbp_batch_get_config(self._h, *[C.byref(x) for x in v]))
         return dict(config=v[0].value, tile_width=v[1].value, tile_rows=v[2].value,
-                    blocks_per_cu=v[3].value)
+                    blocks_per_cu=v[3].value, two_stage=bool(v[4].value))
 
     def autotune(self, dt) -> int:
         d = self._dt(dt)

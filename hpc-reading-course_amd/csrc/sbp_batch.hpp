@@ -20,6 +20,8 @@ template <typename T> struct BatchInst {
   T hix, hiy;
   double dt;
   int bc, tiles_x, ty_rows, n_tiles, n_edge_blocks, n_edge;
+  // two-stage configurations: the (CLS+R)-deep frame and the pair tiles
+  int p_depth, p_tiles_x, p_ty_rows, p_n_tiles, p_n_edge, p_n_edge_blocks;
 };
 
 template <typename T, int MODE>
@@ -98,6 +100,115 @@ __global__ __launch_bounds__(BX) void batch_stage_kernel(const BatchInst<T> *__r
 using BatchLaunchFn = hipError_t (*)(int mode, const void *inst, const int2 *work, int nwork, int grid,
                                      hipStream_t s);
 
+// --------------------------------------------------------------------------
+// Two-stage configurations (the schedule of sbp_kernels.hpp pair2d_kernel):
+// four persistent launches per step over the same kind of work list,
+//   part 1: frame stage 1 blocks + pair-1 tiles      part 2: frame stage 2
+//   part 3: frame stage 3 blocks + pair-2 tiles      part 4: frame stage 4
+// A step leaves U' in each instance's Y0 array; the host then passes the
+// second descriptor set, in which U and Y0 trade places.
+// --------------------------------------------------------------------------
+template <typename T, int MODE>
+__device__ __forceinline__ StageArgs<T> batch_frame_args(const BatchInst<T> &d)
+{
+  StageArgs<T> a = batch_args<T, MODE>(d);
+  a.depth = d.p_depth;
+  a.n_edge = d.p_n_edge;
+  a.n_edge_blocks = d.p_n_edge_blocks;
+  a.n_tiles = 0;
+  if constexpr (MODE == MODE_S4) a.uout = d.Y0;  // U' beside the interior's (U is read through uin)
+  return a;
+}
+
+template <typename T, int PAIR>
+__device__ __forceinline__ PairArgs<T> batch_pair_args(const BatchInst<T> &d)
+{
+  PairArgs<T> a{};
+  a.rinv = d.rinv;
+  a.bulk = d.bulk;
+  a.plane = d.plane;
+  a.pitch = d.pitch;
+  a.xoff = d.xoff;
+  a.nx = d.nx;
+  a.ny = d.ny;
+  a.hix = d.hix;
+  a.hiy = d.hiy;
+  a.third = T(1.0 / 3.0);
+  a.tiles_x = d.p_tiles_x;
+  a.ty_rows = d.p_ty_rows;
+  a.n_tiles = d.p_n_tiles;
+  a.n_edge_blocks = d.p_n_edge_blocks;
+  const double dt = d.dt;
+  if constexpr (PAIR == 1) {
+    a.yin = d.U;
+    a.o1 = d.Y0;
+    a.o2 = d.Y1;
+    a.c1 = T(0.5 * dt);
+    a.c2 = T(0.5 * dt);
+  } else {
+    a.yin = d.Y1;
+    a.u = d.U;
+    a.y2 = d.Y0;
+    a.o1 = d.Y0;
+    a.o2 = d.A;
+    a.comp = d.comp;
+    a.c1 = T(dt);
+    a.c2 = T(dt / 6);
+  }
+  return a;
+}
+
+template <typename T, int ORDER, int MAT, int PART, bool KAHAN, int NT, int PF>
+__global__ __launch_bounds__(NT) void batch_pair_kernel(const BatchInst<T> *__restrict__ inst,
+                                                        const int2 *__restrict__ work, int nwork)
+{
+  constexpr int MODE = PART == 1 ? MODE_S1 : (PART == 2 ? MODE_S2 : (PART == 3 ? MODE_S3 : MODE_S4));
+  const int q = nwork / kNumXcd, r = nwork % kNumXcd;
+  for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+    const int xcd = w % kNumXcd, local = w / kNumXcd;
+    const int2 it = work[xcd * q + min(xcd, r) + local];
+    const BatchInst<T> &d = inst[it.x];
+    if (it.y < d.p_n_edge_blocks) {
+      edge_point<T, ORDER, PH_ACOUSTIC, MAT, MODE, KAHAN>(batch_frame_args<T, MODE>(d), it.y * NT + threadIdx.x);
+    } else if constexpr (PART == 1 || PART == 3) {
+      pair_tile<T, ORDER, MAT, PART == 1 ? 1 : 2, KAHAN, NT, PF>(batch_pair_args<T, PART == 1 ? 1 : 2>(d),
+                                                                 it.y - d.p_n_edge_blocks);
+    }
+    __syncthreads();  // the next item reuses the LDS rings
+  }
+}
+
+// part: 1..4; the pair launches take prefetch depths 3 (part 1) and 1 (part 3)
+template <typename T, int ORDER, int MAT, bool KAHAN> struct BatchPairLauncher {
+  static constexpr int NT = pair_nt(sizeof(T), ORDER);
+  template <int PART, int PF>
+  static hipError_t go(const void *inst, const int2 *work, int nwork, int grid, hipStream_t s)
+  {
+    hipLaunchKernelGGL((batch_pair_kernel<T, ORDER, MAT, PART, KAHAN, NT, PF>), dim3(grid), dim3(NT), 0, s,
+                       static_cast<const BatchInst<T> *>(inst), work, nwork);
+    return hipGetLastError();
+  }
+  static hipError_t launch(int part, const void *inst, const int2 *work, int nwork, int grid, hipStream_t s)
+  {
+    switch (part) {
+      case 1: return go<1, 3>(inst, work, nwork, grid, s);
+      case 2: return go<2, 1>(inst, work, nwork, grid, s);
+      case 3: return go<3, 1>(inst, work, nwork, grid, s);
+      default: return go<4, 1>(inst, work, nwork, grid, s);
+    }
+  }
+};
+
+template <typename T, bool KAHAN> BatchLaunchFn pick_batch_pair(int order, int mat)
+{
+  const bool arr = mat == M_ARR;
+  if (order == 2)
+    return arr ? &BatchPairLauncher<T, 2, M_ARR, KAHAN>::launch : &BatchPairLauncher<T, 2, M_UNIT, KAHAN>::launch;
+  if (order == 4)
+    return arr ? &BatchPairLauncher<T, 4, M_ARR, KAHAN>::launch : &BatchPairLauncher<T, 4, M_UNIT, KAHAN>::launch;
+  return arr ? &BatchPairLauncher<T, 6, M_ARR, KAHAN>::launch : &BatchPairLauncher<T, 6, M_UNIT, KAHAN>::launch;
+}
+
 template <typename T, int ORDER, int MAT, bool KAHAN, int BX, int PF> struct BatchLauncher {
   template <int MODE>
   static hipError_t go(const void *inst, const int2 *work, int nwork, int grid, hipStream_t s)
@@ -130,5 +241,8 @@ template <typename T, bool KAHAN, int BX, int PF> BatchLaunchFn pick_batch(int o
 // sbp_batch_f64.hip / sbp_batch_f32.hip; pf is 1 or 4
 BatchLaunchFn batch_launcher_f64(int order, int mat, int bx, int pf);
 BatchLaunchFn batch_launcher_f32(int order, int mat, bool kahan, int bx, int pf);
+// two-stage configurations (the launch's `mode` argument is the part, 1..4)
+BatchLaunchFn batch_pair_launcher_f64(int order, int mat);
+BatchLaunchFn batch_pair_launcher_f32(int order, int mat, bool kahan);
 
 }  // namespace sbp

@@ -30,16 +30,21 @@ namespace {
 
 struct BatchConfig {
   int bx, ty, blocks_per_cu, pf;  // pf: rows of loads in flight per tile
+  int pair;                        // 1: two RK4 stages per interior launch (bx from pair_nt)
 };
 
 // Candidate launch shapes of the autotuner.
 // Short tiles (4, 8 rows) are for ensembles of small grids, where tall tiles
 // leave most of the persistent blocks without work.
+// The last four are two-stage configurations (sbp_kernels.hpp pair2d_kernel);
+// they need every instance to have an interior beyond its (CLS+R)-deep frame.
+// Their tiles are pair_nt wide; bx only sizes the unused one-stage fields.
 constexpr BatchConfig kConfigs[] = {
-    {256, 32, 4, 1}, {256, 64, 4, 1}, {256, 16, 4, 1}, {256, 32, 2, 1}, {256, 32, 8, 1}, {256, 64, 2, 1},
-    {128, 32, 8, 1}, {128, 64, 8, 1}, {128, 16, 8, 1}, {128, 32, 4, 1}, {128, 64, 4, 1}, {128, 32, 16, 1},
-    {256, 8, 4, 1},  {256, 4, 4, 1},  {128, 8, 8, 1},  {128, 4, 8, 1},
-    {256, 8, 4, 4},  {256, 4, 4, 4},  {128, 8, 8, 4},  {128, 4, 8, 4},
+    {256, 32, 4, 1, 0}, {256, 64, 4, 1, 0}, {256, 16, 4, 1, 0}, {256, 32, 2, 1, 0}, {256, 32, 8, 1, 0},
+    {256, 64, 2, 1, 0}, {128, 32, 8, 1, 0}, {128, 64, 8, 1, 0}, {128, 16, 8, 1, 0}, {128, 32, 4, 1, 0},
+    {128, 64, 4, 1, 0}, {128, 32, 16, 1, 0}, {256, 8, 4, 1, 0},  {256, 4, 4, 1, 0},  {128, 8, 8, 1, 0},
+    {128, 4, 8, 1, 0},  {256, 8, 4, 4, 0},  {256, 4, 4, 4, 0},  {128, 8, 8, 4, 0},  {128, 4, 8, 4, 0},
+    {256, 16, 4, 0, 1}, {256, 8, 4, 0, 1},  {256, 32, 4, 0, 1}, {256, 16, 2, 0, 1},
 };
 constexpr int kNumConfigs = sizeof(kConfigs) / sizeof(kConfigs[0]);
 constexpr int kAlignB = 64;
@@ -55,12 +60,26 @@ namespace {
 
 void drop_graph(sbp_batch *b)
 {
-  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
-  b->gexec = nullptr;
+  for (hipGraphExec_t &g : b->gexec) {
+    if (g) (void)hipGraphExecDestroy(g);
+    g = nullptr;
+  }
 }
 
-template <typename T> void fill_inst(const sbp_batch *b, std::vector<BatchInst<T>> &out, int bx, int ty)
+// Bring the state back to the U pool (after an odd number of two-stage steps
+// it sits in Y0); the one-stage kernels, tuning and configuration changes
+// start from there.
+int normalize_parity(sbp_batch *b)
 {
+  if (!b->par) return SBP_OK;
+  HIP_OR_FAIL(hipMemcpyAsync(b->U, b->Y0, b->esz * 3 * (size_t)b->pool, hipMemcpyDeviceToDevice, b->stream));
+  b->par = 0;
+  return SBP_OK;
+}
+
+template <typename T> bool fill_inst(const sbp_batch *b, std::vector<BatchInst<T>> &out, int bx, int ty, bool pair)
+{
+  bool ok = true;
   out.resize(b->count);
   for (int k = 0; k < b->count; k++) {
     const InstHost &h = b->inst[k];
@@ -89,17 +108,34 @@ template <typename T> void fill_inst(const sbp_batch *b, std::vector<BatchInst<T
     d.n_tiles = d.tiles_x * (hi > 0 ? (hi + ty - 1) / ty : 0);
     d.n_edge = 2 * b->cls * h.nx + 2 * b->cls * (h.ny - 2 * b->cls);
     d.n_edge_blocks = (d.n_edge + bx - 1) / bx;
+    // two-stage geometry: pair_nt-thread tiles inside a (CLS+R)-deep frame
+    const int nt = pair_nt((int)b->esz, b->order), R = b->order / 2, D = b->cls + R;
+    const int pw = h.nx - 2 * D, ph = h.ny - 2 * D;
+    if (pair && (pw <= 0 || ph <= 0)) ok = false;
+    d.p_depth = D;
+    d.p_tiles_x = pw > 0 ? (pw + nt - 2 * R - 1) / (nt - 2 * R) : 0;
+    d.p_ty_rows = ty;
+    d.p_n_tiles = (pw > 0 && ph > 0) ? d.p_tiles_x * ((ph + ty - 1) / ty) : 0;
+    d.p_n_edge = 2 * D * h.nx + 2 * D * (h.ny - 2 * D);
+    d.p_n_edge_blocks = (d.p_n_edge + nt - 1) / nt;
   }
+  return ok;
 }
 
 // Upload descriptors and the work list for configuration cfg.
 int apply_config(sbp_batch *b, int cfg)
 {
   const BatchConfig c = kConfigs[cfg];
-  BatchLaunchFn fn = b->prec == SBP_FP64 ? batch_launcher_f64(b->order, b->mat, c.bx, c.pf)
-                                         : batch_launcher_f32(b->order, b->mat, b->prec == SBP_FP32_KAHAN, c.bx, c.pf);
+  const bool kahan = b->prec == SBP_FP32_KAHAN;
+  BatchLaunchFn fn;
+  if (c.pair)
+    fn = b->prec == SBP_FP64 ? batch_pair_launcher_f64(b->order, b->mat)
+                             : batch_pair_launcher_f32(b->order, b->mat, kahan);
+  else
+    fn = b->prec == SBP_FP64 ? batch_launcher_f64(b->order, b->mat, c.bx, c.pf)
+                             : batch_launcher_f32(b->order, b->mat, kahan, c.bx, c.pf);
   if (!fn) return fail(SBP_EINVAL, "no batched kernel for this order/material/precision");
-  std::vector<int2> work;
+  std::vector<int2> work, frame;
   auto build = [&](auto &desc) {
     // Interior tiles of every instance first (largest instances first), then
     // the boundary frames: long items start early, short ones fill the tail.
@@ -108,23 +144,49 @@ int apply_config(sbp_batch *b, int cfg)
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
       return (long)b->inst[x].nx * b->inst[x].ny > (long)b->inst[y].nx * b->inst[y].ny;
     });
+    if (c.pair) {
+      // parts 1 and 3: pair tiles, then the frame's blocks; parts 2 and 4:
+      // the frame's blocks only
+      for (int k : order)
+        for (int t = 0; t < desc[k].p_n_tiles; t++) work.push_back(make_int2(k, desc[k].p_n_edge_blocks + t));
+      for (int k : order)
+        for (int e = 0; e < desc[k].p_n_edge_blocks; e++) {
+          work.push_back(make_int2(k, e));
+          frame.push_back(make_int2(k, e));
+        }
+      return;
+    }
     for (int k : order)
       for (int t = 0; t < desc[k].n_tiles; t++) work.push_back(make_int2(k, desc[k].n_edge_blocks + t));
     for (int k : order)
       for (int e = 0; e < desc[k].n_edge_blocks; e++) work.push_back(make_int2(k, e));
   };
+  // the second descriptor set trades U and Y0 (state of an odd two-stage step)
+  auto swapped = [](auto v) {
+    for (auto &d : v) std::swap(d.U, d.Y0);
+    return v;
+  };
   size_t desc_bytes;
   std::vector<BatchInst<double>> dd;
   std::vector<BatchInst<float>> df;
+  bool fits;
   if (b->esz == 8) {
-    fill_inst(b, dd, c.bx, c.ty);
+    fits = fill_inst(b, dd, c.bx, c.ty, c.pair);
     build(dd);
     desc_bytes = dd.size() * sizeof(BatchInst<double>);
+    const auto sw = swapped(dd);
+    dd.insert(dd.end(), sw.begin(), sw.end());
   } else {
-    fill_inst(b, df, c.bx, c.ty);
+    fits = fill_inst(b, df, c.bx, c.ty, c.pair);
     build(df);
     desc_bytes = df.size() * sizeof(BatchInst<float>);
+    const auto sw = swapped(df);
+    df.insert(df.end(), sw.begin(), sw.end());
   }
+  if (!fits) return fail(SBP_EINVAL, "two-stage configurations need every instance larger than 2(CLS+R) points");
+  work.insert(work.end(), frame.begin(), frame.end());
+  int rc = normalize_parity(b);
+  if (rc) return rc;
   HIP_OR_FAIL(hipStreamSynchronize(b->stream));
   drop_graph(b);
   if (b->work_dev) (void)hipFree(b->work_dev);
@@ -132,11 +194,14 @@ int apply_config(sbp_batch *b, int cfg)
   b->work_dev = nullptr;
   b->inst_dev = nullptr;
   HIP_OR_FAIL(hipMalloc(&b->work_dev, sizeof(int2) * std::max<size_t>(1, work.size())));
-  HIP_OR_FAIL(hipMalloc(&b->inst_dev, desc_bytes));
+  HIP_OR_FAIL(hipMalloc(&b->inst_dev, 2 * desc_bytes));
   HIP_OR_FAIL(hipMemcpy(b->work_dev, work.data(), sizeof(int2) * work.size(), hipMemcpyHostToDevice));
-  HIP_OR_FAIL(hipMemcpy(b->inst_dev, b->esz == 8 ? (const void *)dd.data() : (const void *)df.data(), desc_bytes,
+  HIP_OR_FAIL(hipMemcpy(b->inst_dev, b->esz == 8 ? (const void *)dd.data() : (const void *)df.data(), 2 * desc_bytes,
                         hipMemcpyHostToDevice));
-  b->nwork = (int)work.size();
+  b->desc_bytes = desc_bytes;
+  b->nwork_frame = (int)frame.size();
+  b->nwork = (int)(work.size() - frame.size());
+  b->pair = c.pair != 0;
   b->grid = std::max(kNumXcd, (b->num_cus * c.blocks_per_cu + kNumXcd - 1) / kNumXcd * kNumXcd);
   b->launch = fn;
   b->config = cfg;
@@ -158,23 +223,35 @@ int upload_dt(sbp_batch *b, const double *dt)
   const size_t stride = b->esz == 8 ? sizeof(BatchInst<double>) : sizeof(BatchInst<float>);
   const size_t off = b->esz == 8 ? offsetof(BatchInst<double>, dt) : offsetof(BatchInst<float>, dt);
   HIP_OR_FAIL(hipStreamSynchronize(b->stream));
-  for (int k = 0; k < b->count; k++)
-    HIP_OR_FAIL(hipMemcpy(static_cast<char *>(b->inst_dev) + k * stride + off, &b->inst[k].dt, sizeof(double),
-                          hipMemcpyHostToDevice));
+  for (int set = 0; set < 2; set++)
+    for (int k = 0; k < b->count; k++)
+      HIP_OR_FAIL(hipMemcpy(static_cast<char *>(b->inst_dev) + set * b->desc_bytes + k * stride + off,
+                            &b->inst[k].dt, sizeof(double), hipMemcpyHostToDevice));
   return SBP_OK;
 }
 
 hipError_t launch_step(sbp_batch *b)
 {
   hipError_t e = hipSuccess;
+  if (b->pair) {  // parts 1..4 on the descriptors of the current parity
+    const void *inst = sbp::host::batch_inst(b);
+    for (int p = 1; p <= 4 && e == hipSuccess; p++) {
+      const bool frame_only = p == 2 || p == 4;
+      e = b->launch(p, inst, frame_only ? b->work_dev + b->nwork : b->work_dev,
+                    frame_only ? b->nwork_frame : b->nwork, b->grid, b->stream);
+    }
+    b->par ^= 1;
+    return e;
+  }
   for (int m = MODE_S1; m <= MODE_S4 && e == hipSuccess; m++)
     e = b->launch(m, b->inst_dev, b->work_dev, b->nwork, b->grid, b->stream);
   return e;
 }
 
+// kBatchGraphSteps is even: a two-stage capture ends at the parity it began.
 int ensure_graph(sbp_batch *b)
 {
-  if (b->gexec) return SBP_OK;
+  if (b->gexec[b->par]) return SBP_OK;
   hipGraph_t g = nullptr;
   HIP_OR_FAIL(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
   hipError_t le = hipSuccess;
@@ -184,10 +261,10 @@ int ensure_graph(sbp_batch *b)
     if (g) (void)hipGraphDestroy(g);
     return fail(SBP_EHIP, std::string("batch graph capture: ") + hipGetErrorString(le != hipSuccess ? le : ce));
   }
-  hipError_t ie = hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0);
+  hipError_t ie = hipGraphInstantiate(&b->gexec[b->par], g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
   if (ie != hipSuccess) {
-    b->gexec = nullptr;
+    b->gexec[b->par] = nullptr;
     return fail(SBP_EHIP, std::string("batch graph instantiate: ") + hipGetErrorString(ie));
   }
   return SBP_OK;
@@ -199,7 +276,7 @@ int run(sbp_batch *b, long nsteps)
   if (nsteps >= kBatchGraphSteps) {
     int rc = ensure_graph(b);
     if (rc) return rc;
-    for (; n + kBatchGraphSteps <= nsteps; n += kBatchGraphSteps) HIP_OR_FAIL(hipGraphLaunch(b->gexec, b->stream));
+    for (; n + kBatchGraphSteps <= nsteps; n += kBatchGraphSteps) HIP_OR_FAIL(hipGraphLaunch(b->gexec[b->par], b->stream));
   }
   for (; n < nsteps; n++) HIP_OR_FAIL(launch_step(b));
   return SBP_OK;
@@ -335,7 +412,7 @@ int norm_instance(sbp_batch *b, int k, double *out)
     using T = std::remove_pointer_t<decltype(tag)>;
     NormArgs<T> a{};
     a.aos = nullptr;
-    a.u = static_cast<const T *>(b->U) + 3 * h.off;
+    a.u = static_cast<const T *>(sbp::host::batch_state(b)) + 3 * h.off;
     a.comp = b->KC ? static_cast<const T *>(b->KC) + 3 * h.off : nullptr;
     a.rinv = b->RI ? static_cast<const T *>(b->RI) + h.off : nullptr;
     a.bulk = b->BK ? static_cast<const T *>(b->BK) + h.off : nullptr;
@@ -408,11 +485,11 @@ int sbp_batch_set_state(sbp_batch *b, int k, const double *v)
   const int grid = (int)((n + 255) / 256);
   if (b->esz == 8)
     hipLaunchKernelGGL(aos_to_planar_kernel<double>, dim3(grid), dim3(256), 0, b->stream, b->aos,
-                       static_cast<double *>(b->U) + 3 * h.off, (double *)nullptr, h.nx, h.ny, 3, h.pitch, h.xoff,
-                       h.plane);
+                       static_cast<double *>(sbp::host::batch_state(b)) + 3 * h.off, (double *)nullptr, h.nx, h.ny, 3,
+                       h.pitch, h.xoff, h.plane);
   else
     hipLaunchKernelGGL(aos_to_planar_kernel<float>, dim3(grid), dim3(256), 0, b->stream, b->aos,
-                       static_cast<float *>(b->U) + 3 * h.off,
+                       static_cast<float *>(sbp::host::batch_state(b)) + 3 * h.off,
                        b->KC ? static_cast<float *>(b->KC) + 3 * h.off : nullptr, h.nx, h.ny, 3, h.pitch, h.xoff,
                        h.plane);
   HIP_OR_FAIL(hipGetLastError());
@@ -432,11 +509,11 @@ int sbp_batch_get_state(sbp_batch *b, int k, double *v)
   const int grid = (int)((n + 255) / 256);
   if (b->esz == 8)
     hipLaunchKernelGGL(planar_to_aos_kernel<double>, dim3(grid), dim3(256), 0, b->stream,
-                       static_cast<const double *>(b->U) + 3 * h.off, (const double *)nullptr, b->aos, h.nx, h.ny,
+                       static_cast<const double *>(sbp::host::batch_state(b)) + 3 * h.off, (const double *)nullptr, b->aos, h.nx, h.ny,
                        3, h.pitch, h.xoff, h.plane);
   else
     hipLaunchKernelGGL(planar_to_aos_kernel<float>, dim3(grid), dim3(256), 0, b->stream,
-                       static_cast<const float *>(b->U) + 3 * h.off,
+                       static_cast<const float *>(sbp::host::batch_state(b)) + 3 * h.off,
                        b->KC ? static_cast<const float *>(b->KC) + 3 * h.off : nullptr, b->aos, h.nx, h.ny, 3,
                        h.pitch, h.xoff, h.plane);
   HIP_OR_FAIL(hipGetLastError());
@@ -457,11 +534,11 @@ int sbp_batch_get_state_strided(sbp_batch *b, int k, int stride, double *v)
   const int grid = (int)((n + 255) / 256);
   if (b->esz == 8)
     hipLaunchKernelGGL(planar_to_aos_strided_kernel<double>, dim3(grid), dim3(256), 0, b->stream,
-                       static_cast<const double *>(b->U) + 3 * h.off, (const double *)nullptr, b->aos, h.nx, h.ny, 3,
+                       static_cast<const double *>(sbp::host::batch_state(b)) + 3 * h.off, (const double *)nullptr, b->aos, h.nx, h.ny, 3,
                        h.pitch, h.xoff, h.plane, stride);
   else
     hipLaunchKernelGGL(planar_to_aos_strided_kernel<float>, dim3(grid), dim3(256), 0, b->stream,
-                       static_cast<const float *>(b->U) + 3 * h.off,
+                       static_cast<const float *>(sbp::host::batch_state(b)) + 3 * h.off,
                        b->KC ? static_cast<const float *>(b->KC) + 3 * h.off : nullptr, b->aos, h.nx, h.ny, 3, h.pitch,
                        h.xoff, h.plane, stride);
   HIP_OR_FAIL(hipGetLastError());
@@ -514,14 +591,16 @@ int sbp_batch_set_config(sbp_batch *b, int config)
   return apply_config(b, config);
 }
 
-int sbp_batch_get_config(const sbp_batch *b, int *config, int *tile_width, int *tile_rows, int *blocks_per_cu)
+int sbp_batch_get_config(const sbp_batch *b, int *config, int *tile_width, int *tile_rows, int *blocks_per_cu,
+                         int *two_stage)
 {
   if (!b) return fail(SBP_EINVAL, "null batch");
   const BatchConfig c = kConfigs[b->config];
   if (config) *config = b->config;
-  if (tile_width) *tile_width = c.bx;
+  if (tile_width) *tile_width = c.pair ? pair_nt((int)b->esz, b->order) : c.bx;
   if (tile_rows) *tile_rows = c.ty;
   if (blocks_per_cu) *blocks_per_cu = c.blocks_per_cu;
+  if (two_stage) *two_stage = c.pair;
   return SBP_OK;
 }
 
@@ -533,6 +612,7 @@ int sbp_batch_autotune(sbp_batch *b, const double *dt, int *choice)
   // Time every candidate on this ensemble; the resident state is saved first
   // and restored afterwards, so tuning leaves the instances untouched.
   const size_t reg = b->esz * 3 * (size_t)b->pool;
+  if ((rc = normalize_parity(b))) return rc;
   void *save_u = nullptr, *save_c = nullptr;
   HIP_OR_FAIL(hipMalloc(&save_u, reg));
   if (b->KC && hipMalloc(&save_c, reg) != hipSuccess) {
@@ -544,7 +624,9 @@ int sbp_batch_autotune(sbp_batch *b, const double *dt, int *choice)
   int best = b->config;
   float best_ms = 1e30f;
   for (int cfg = 0; cfg < kNumConfigs && rc == SBP_OK; cfg++) {
-    if ((rc = apply_config(b, cfg))) break;
+    const int arc = apply_config(b, cfg);
+    if (arc == SBP_EINVAL && kConfigs[cfg].pair) continue;  // instances too small for a two-stage tile
+    if ((rc = arc)) break;
     for (int s = 0; s < 2 && rc == SBP_OK; s++)
       if (launch_step(b) != hipSuccess) rc = fail(SBP_EHIP, "autotune warm-up launch failed");
     if (rc) break;
@@ -561,6 +643,7 @@ int sbp_batch_autotune(sbp_batch *b, const double *dt, int *choice)
       best = cfg;
     }
   }
+  b->par = 0;  // the trial steps' state is discarded: U comes back from the copy
   hipError_t e1 = hipMemcpyAsync(b->U, save_u, reg, hipMemcpyDeviceToDevice, b->stream);
   hipError_t e2 = save_c ? hipMemcpyAsync(b->KC, save_c, reg, hipMemcpyDeviceToDevice, b->stream) : hipSuccess;
   hipError_t e3 = hipStreamSynchronize(b->stream);

@@ -27,12 +27,16 @@ struct sbp_batch {
   long max_pts = 0;    // largest nx*ny
   void *U = nullptr, *A = nullptr, *Y0 = nullptr, *Y1 = nullptr, *KC = nullptr;
   void *RI = nullptr, *BK = nullptr;
-  void *inst_dev = nullptr;
+  void *inst_dev = nullptr;  // two descriptor sets: as allocated, and U / Y0 traded
+  size_t desc_bytes = 0;     // bytes of one set
   int2 *work_dev = nullptr;
-  int nwork = 0;
+  int nwork = 0;             // items of the stage launches (two-stage: frame + pair tiles)
+  int nwork_frame = 0;       // two-stage: frame-only items stored after the nwork items
+  bool pair = false;         // the configuration runs two RK4 stages per interior launch
+  int par = 0;               // 1: a two-stage step left the state in the Y0 pool
   double *aos = nullptr, *partials = nullptr, *scalar = nullptr;
   hipStream_t stream = nullptr;
-  hipGraphExec_t gexec = nullptr;
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};  // captured steps, by starting parity
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int config = 0, grid = 0, num_cus = 256;
   sbp::BatchLaunchFn launch = nullptr;
@@ -41,6 +45,13 @@ struct sbp_batch {
 
 namespace sbp {
 namespace host {
+// Device descriptors of the current step parity (in the second set U and Y0
+// trade places), and the pool that holds the state now.
+inline const void *batch_inst(const sbp_batch *b)
+{
+  return static_cast<const char *>(b->inst_dev) + (b->par ? b->desc_bytes : 0);
+}
+inline void *batch_state(const sbp_batch *b) { return b->par ? b->Y0 : b->U; }
 // One RK4 step of every instance (four persistent-kernel launches).
 hipError_t batch_launch_step(sbp_batch *b);
 // Upload per-instance time steps into the device descriptors.

@@ -1321,19 +1321,21 @@ int sbp_corr_advance_batch(sbp_corrector *c, sbp_batch *b, const double *dt, lon
   // reference energy of every instance: its energy when the call starts
   if (b->esz == 8)
     hipLaunchKernelGGL(batch_energy_kernel<double>, dim3(count * bpi), dim3(256), 0, s,
-                       static_cast<const BatchInst<double> *>(b->inst_dev), bpi, 0, b->order, c->bpart);
+                       static_cast<const BatchInst<double> *>(sbp::host::batch_inst(b)), bpi, 0, b->order, c->bpart);
   else
     hipLaunchKernelGGL(batch_energy_kernel<float>, dim3(count * bpi), dim3(256), 0, s,
-                       static_cast<const BatchInst<float> *>(b->inst_dev), bpi, 0, b->order, c->bpart);
+                       static_cast<const BatchInst<float> *>(sbp::host::batch_inst(b)), bpi, 0, b->order, c->bpart);
   hipLaunchKernelGGL(batch_energy_final_kernel, dim3((count + 63) / 64), dim3(64), 0, s, c->bpart, count, bpi, c->be_old);
   HIP_OR_FAIL(hipGetLastError());
   // one corrected step: plain RK4 stages, features, network, candidate
   // energy, guarded update -- kernel launches only, so it can be captured
+  // (descriptors of the parity after the step: a two-stage step moves the
+  // state into the Y0 pool)
   auto enqueue_step = [&]() -> hipError_t {
     hipError_t e = sbp::host::batch_launch_step(b);
     if (e != hipSuccess) return e;
     if (b->esz == 8) {
-      const BatchInst<double> *inst = static_cast<const BatchInst<double> *>(b->inst_dev);
+      const BatchInst<double> *inst = static_cast<const BatchInst<double> *>(sbp::host::batch_inst(b));
       hipLaunchKernelGGL(pack_batch_tiled_kernel<double>, dim3(count * ptiles), dim3(256), 0, s, inst, c->H, c->W,
                          c->norm, c->X0);
       launch_infer(c, count, s);
@@ -1342,7 +1344,7 @@ int sbp_corr_advance_batch(sbp_corrector *c, sbp_batch *b, const double *dt, lon
       hipLaunchKernelGGL(batch_apply_kernel<double>, dim3(grid_for(npix)), dim3(256), 0, s, inst, count, c->H, c->W,
                          c->band, c->Yo, c->norm + NORM_OUT, c->be_old, c->cpart, cbpi, bound, c->brej);
     } else {
-      const BatchInst<float> *inst = static_cast<const BatchInst<float> *>(b->inst_dev);
+      const BatchInst<float> *inst = static_cast<const BatchInst<float> *>(sbp::host::batch_inst(b));
       hipLaunchKernelGGL(pack_batch_tiled_kernel<float>, dim3(count * ptiles), dim3(256), 0, s, inst, c->H, c->W,
                          c->norm, c->X0);
       launch_infer(c, count, s);

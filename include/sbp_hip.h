@@ -187,11 +187,14 @@ int  sbp_batch_step(sbp_batch *b, const double *dt, long nsteps);
 int  sbp_batch_synchronize(sbp_batch *b);
 int  sbp_batch_time_steps(sbp_batch *b, const double *dt, long nsteps, double *ms);
 int  sbp_batch_energy(sbp_batch *b, int k, double *E);
-/* Launch-shape candidates (tile width, tile rows, persistent blocks per CU). */
+/* Launch-shape candidates (tile width, tile rows, persistent blocks per CU,
+ * and whether the step runs two RK4 stages per interior launch: the schedule
+ * of sbp_set_fused, which needs every instance larger than 2(CLS+R) points;
+ * sbp_batch_set_config fails with SBP_EINVAL otherwise). */
 int  sbp_batch_num_configs(void);
 int  sbp_batch_set_config(sbp_batch *b, int config);
 int  sbp_batch_get_config(const sbp_batch *b, int *config, int *tile_width, int *tile_rows,
-                          int *blocks_per_cu);
+                          int *blocks_per_cu, int *two_stage);
 /* Time every candidate on this ensemble (resident states are saved and
  * restored) and keep the fastest; *choice receives its index. */
 int  sbp_batch_autotune(sbp_batch *b, const double *dt, int *choice);

@@ -1,8 +1,9 @@
 """Batched ensemble form: every instance must equal the single-problem path.
 
 Bars: FP32/FP32+Kahan instances are bit-identical to the same problem run
-alone through sbp_rk4_advance (same kernels, same arithmetic, any tiling);
-FP64 instances are within 1e-12 relative max norm of the CPU oracle; the
+alone through sbp_rk4_advance with the same schedule (one launch per stage,
+or two stages per launch: same kernels, same arithmetic, any tiling); FP64
+instances are within 1e-12 relative max norm of the CPU oracle; the
 autotuner leaves the states untouched.
 """
 import numpy as np
@@ -58,20 +59,94 @@ def test_batch_fp64_matches_oracle(capi, orc):
             assert b.energy(k) == pytest.approx(orc.energy(spec, ref), rel=1e-12)
 
 
+def two_stage_configs(capi):
+    out = []
+    with capi.Batch([P.acoustic_box(40, 4)[0]]) as b:
+        for cfg in range(capi.Batch.num_configs()):
+            b.set_config(cfg)
+            if b.get_config()["two_stage"]:
+                out.append(cfg)
+    return out
+
+
 def test_batch_unit_material_and_configs(capi):
+    """Every launch shape gives the same result: bit for bit within each
+    schedule; one launch per stage vs two stages per launch within FP32
+    rounding (the two kernels may contract multiply-adds differently)."""
     ens = ensemble(P.FP32, sizes=(33, 200, 90), materials=False)
     dts = [e[2] for e in ens]
-    results = []
+    results = {False: [], True: []}
     for cfg in range(capi.Batch.num_configs()):
         with capi.Batch([e[0] for e in ens]) as b:
             b.set_config(cfg)
+            two = b.get_config()["two_stage"]
             for k, (_, v0, _) in enumerate(ens):
                 b.set_state(k, v0)
             b.step(dts, 11)
-            results.append([b.get_state(k) for k in range(len(ens))])
-    for r in results[1:]:
-        for a, c in zip(r, results[0]):
-            assert np.array_equal(a, c)
+            results[two].append([b.get_state(k) for k in range(len(ens))])
+    assert results[True], "no two-stage configuration"
+    for group in results.values():
+        for r in group[1:]:
+            for a, c in zip(r, group[0]):
+                assert np.array_equal(a, c)
+    for a, c in zip(results[True][0], results[False][0]):
+        assert np.max(np.abs(a - c)) <= 2e-6 * np.max(np.abs(c))
+    # the two-stage batch equals the two-stage single-problem path
+    for (spec, v0, dt), g in zip(ens, results[True][0]):
+        with capi.Context(spec) as ctx:
+            ctx.set_fused(True)
+            assert np.array_equal(g, ctx.rk4_advance(v0, 0.0, dt, 11))
+
+
+@pytest.mark.parametrize("prec", [P.FP64, P.FP32_KAHAN])
+def test_batch_two_stage_parity_and_switching(capi, orc, prec):
+    """Two-stage configurations: graph replays (10 steps), odd remainders and
+    a switch back to a one-stage configuration mid-run keep every instance
+    on the single-problem path; FP64 within 1e-12 of the oracle."""
+    ens = ensemble(prec, sizes=(20, 77, 130, 41), order=6)
+    dts = [e[2] for e in ens]
+    cfgs = two_stage_configs(capi)
+    with capi.Batch([e[0] for e in ens]) as b:
+        for k, (_, v0, _) in enumerate(ens):
+            b.set_state(k, v0)
+        b.set_config(cfgs[0])
+        b.step(dts, 13)        # 10 from a graph + 3: state in the Y0 pool
+        mid = [b.get_state(k) for k in range(len(ens))]
+        e_mid = [b.energy(k) for k in range(len(ens))]
+        b.step(dts, 21)        # graph of the other parity + 1: back in U
+        b.set_config(cfgs[-1])
+        b.step(dts, 5)         # odd again
+        b.set_config(0)        # one launch per stage: the state moves back first
+        b.step(dts, 4)
+        got = [b.get_state(k) for k in range(len(ens))]
+    for (spec, v0, dt), m, em, g in zip(ens, mid, e_mid, got):
+        with capi.Context(spec) as ctx:   # (no state reload: a Kahan state's split would change)
+            ctx.set_fused(True)
+            ctx.set_state(v0)
+            ctx.step(0.0, dt, 13)
+            ref_mid = ctx.get_state()
+            ctx.step(0.0, dt, 26)
+            ctx.set_fused(False)
+            ctx.step(0.0, dt, 4)
+            ref = ctx.get_state()
+            assert np.array_equal(m, ref_mid)
+            assert np.array_equal(g, ref)
+            assert em == pytest.approx(ctx.energy(ref_mid), rel=1e-12)
+        if prec == P.FP64:
+            assert np.max(np.abs(g - orc.rk4(spec, v0, 0.0, dt, 43))) <= 1e-12 * np.max(np.abs(g))
+
+
+def test_batch_two_stage_rejects_small_instances(capi):
+    ens = ensemble(P.FP32, sizes=(40, 18), order=6)   # 18 <= 2(CLS+R) = 18
+    with capi.Batch([e[0] for e in ens]) as b:
+        with pytest.raises(capi.SbpError) as e:
+            b.set_config(two_stage_configs(capi)[0])
+        assert e.value.code == -1
+        assert not b.get_config()["two_stage"]
+        for k, (_, v0, _) in enumerate(ens):
+            b.set_state(k, v0)
+        choice = b.autotune([e[2] for e in ens])   # skips the two-stage shapes
+        assert not b.get_config()["two_stage"] and b.get_config()["config"] == choice
 
 
 def test_batch_autotune_keeps_state(capi):
@@ -82,9 +157,11 @@ def test_batch_autotune_keeps_state(capi):
             b.set_state(k, v0)
         b.step(dts, 5)
         before = [b.get_state(k) for k in range(len(ens))]
+        b.step(dts, 0)
         choice = b.autotune(dts)
         assert 0 <= choice < capi.Batch.num_configs()
         assert b.get_config()["config"] == choice
+        b_two_stage = b.get_config()["two_stage"]
         after = [b.get_state(k) for k in range(len(ens))]
         for x, y in zip(before, after):
             assert np.array_equal(x, y)
@@ -92,6 +169,7 @@ def test_batch_autotune_keeps_state(capi):
         final = [b.get_state(k) for k in range(len(ens))]
     for (spec, v0, dt), f in zip(ens, final):
         with capi.Context(spec) as ctx:
+            ctx.set_fused(b_two_stage)
             assert np.array_equal(f, ctx.rk4_advance(v0, 0.0, dt, 10))
 
 

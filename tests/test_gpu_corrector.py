@@ -185,12 +185,14 @@ def test_corrected_step_uses_host_features(capi, band):
     assert np.array_equal(got[outer], plain[outer])
 
 
-@pytest.mark.parametrize("prec,band", [(P.FP64, 0), (P.FP32, 0), (P.FP32, 4)])
-def test_batch_advance_matches_single(capi, prec, band):
+@pytest.mark.parametrize("prec,band,two_stage", [(P.FP64, 0, False), (P.FP32, 0, False), (P.FP32, 4, False),
+                                                  (P.FP64, 0, True), (P.FP32, 4, True)])
+def test_batch_advance_matches_single(capi, prec, band, two_stage):
     """sbp_corr_advance_batch (corrected steps captured in a graph and
     replayed, then the remainder eagerly) equals the single-context
     sbp_corr_advance on every instance, bit for bit, when no step is refused;
-    two calls in a row (20 + 40 steps) with the state read in between."""
+    two calls in a row (20 + 40 steps) with the state read in between; with
+    the plain steps on either schedule (one stage or two stages per launch)."""
     specs = [P.acoustic_box(49, 4, sigma=0.08, materials=True, seed=s, material_seed=s + 3, precision=prec)
              for s in (21, 22, 23)]
     dt = P.dt_from_cfl(specs[0][0], 0.25)
@@ -210,6 +212,9 @@ def test_batch_advance_matches_single(capi, prec, band):
         c.set_normalization(shift, scale, osc)
         c.band = band
         with capi.Batch([sp for sp, _ in specs]) as b:
+            if two_stage:
+                b.set_config(next(k for k in range(capi.Batch.num_configs())
+                                  if (b.set_config(k) or b.get_config()["two_stage"])))
             for k, (_, v0) in enumerate(specs):
                 b.set_state(k, v0)
             r1 = c.advance_batch(b, dt, 20, 1.0)
@@ -219,6 +224,7 @@ def test_batch_advance_matches_single(capi, prec, band):
         assert not r1.any() and not r2.any()
         for k, (sp, v0) in enumerate(specs):
             with capi.Context(sp) as ctx:
+                ctx.set_fused(two_stage)
                 ctx.set_state(v0)
                 assert c.advance(ctx, 0.0, dt, 20, 1.0) == 0
                 assert np.array_equal(ctx.get_state(), mid[k]), k
