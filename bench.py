@@ -41,7 +41,7 @@ PEAK_HBM_GBS = 8000.0
 # profiles/stage2d_acoustic_4097sq_order4_fp64_hbm_counters.md (one stage per
 # launch) and profiles/pair2d_acoustic_4097sq_order4_fp64_hbm_counters.md
 # (two-stage launches).
-MEASURED_TRAFFIC_PER_DOF = {(4097, 4, 0, "one stage per launch"): 107.41, (4097, 4, 0, "two-stage"): 67.24}
+MEASURED_TRAFFIC_PER_DOF = {(4097, 4, 0, "one stage per launch"): 107.41, (4097, 4, 0, "two-stage"): 68.49}
 
 
 def pkg(sub):

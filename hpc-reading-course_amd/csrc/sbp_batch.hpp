@@ -171,14 +171,18 @@ __global__ __launch_bounds__(NT) void batch_pair_kernel(const BatchInst<T> *__re
     if (it.y < d.p_n_edge_blocks) {
       edge_point<T, ORDER, PH_ACOUSTIC, MAT, MODE, KAHAN>(batch_frame_args<T, MODE>(d), it.y * NT + threadIdx.x);
     } else if constexpr (PART == 1 || PART == 3) {
-      pair_tile<T, ORDER, MAT, PART == 1 ? 1 : 2, KAHAN, NT, PF>(batch_pair_args<T, PART == 1 ? 1 : 2>(d),
-                                                                 it.y - d.p_n_edge_blocks);
+      if constexpr (SBP_PAIR_RW)
+        pair_tile_rw<T, ORDER, MAT, PART == 1 ? 1 : 2, KAHAN, NT, PF>(batch_pair_args<T, PART == 1 ? 1 : 2>(d),
+                                                                      it.y - d.p_n_edge_blocks);
+      else
+        pair_tile<T, ORDER, MAT, PART == 1 ? 1 : 2, KAHAN, NT, PF>(batch_pair_args<T, PART == 1 ? 1 : 2>(d),
+                                                                   it.y - d.p_n_edge_blocks);
     }
     __syncthreads();  // the next item reuses the LDS rings
   }
 }
 
-// part: 1..4; the pair launches take prefetch depths 3 (part 1) and 1 (part 3)
+// part: 1..4; the pair launches take the prefetch depths of sbp_kernels.hpp
 template <typename T, int ORDER, int MAT, bool KAHAN> struct BatchPairLauncher {
   static constexpr int NT = pair_nt(sizeof(T), ORDER);
   template <int PART, int PF>
@@ -191,9 +195,9 @@ template <typename T, int ORDER, int MAT, bool KAHAN> struct BatchPairLauncher {
   static hipError_t launch(int part, const void *inst, const int2 *work, int nwork, int grid, hipStream_t s)
   {
     switch (part) {
-      case 1: return go<1, 3>(inst, work, nwork, grid, s);
+      case 1: return go<1, SBP_PAIR_PF1>(inst, work, nwork, grid, s);
       case 2: return go<2, 1>(inst, work, nwork, grid, s);
-      case 3: return go<3, 1>(inst, work, nwork, grid, s);
+      case 3: return go<3, SBP_PAIR_PF2>(inst, work, nwork, grid, s);
       default: return go<4, 1>(inst, work, nwork, grid, s);
     }
   }

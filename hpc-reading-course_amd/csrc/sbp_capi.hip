@@ -103,9 +103,9 @@ void compute_pair_geometry(sbp_ctx *c)
 }
 
 // Default pair-tile height (measured on gfx950 at 2049^2 and 4097^2): 16
-// rows for orders 2 and 4, 32 for order 6 (its 4R = 12 halo rows per tile
+// rows for orders 2 and 4, 48 for order 6 (its 4R = 12 halo rows per tile
 // favour taller tiles), halved down to 8 while the grid has fewer than 768
-// tiles (two 68 KB FP64 blocks per CU on 256 CUs, 1.5 rounds).
+// tiles (two FP64 blocks per CU on 256 CUs, 1.5 rounds).
 int default_pair_rows(const sbp_ctx *c)
 {
   const int R = c->order / 2, D = c->cls + R;
@@ -113,7 +113,7 @@ int default_pair_rows(const sbp_ctx *c)
   if (wi <= 0 || hi <= 0) return 16;
   const int nt = pair_nt((int)c->esz, c->order);
   const int tx = (wi + nt - 2 * R - 1) / (nt - 2 * R);
-  int t = c->order == 6 ? 32 : 16;
+  int t = c->order == 6 ? 48 : 16;
   while (t > 8 && (long)tx * ((hi + t - 1) / t) < 768) t /= 2;
   return t;
 }

@@ -25,12 +25,6 @@ StageLaunchFn pick_launcher(int phys, int mat)
 // Two-stage interior tiles: pair_nt threads; prefetch depths (rows of loads in
 // flight per wave) from a sweep on gfx950.  LDS holds these kernels at two
 // blocks per CU, so deep register queues are what keeps HBM busy.
-#ifndef SBP_PAIR_PF1
-#define SBP_PAIR_PF1 3
-#endif
-#ifndef SBP_PAIR_PF2
-#define SBP_PAIR_PF2 1
-#endif
 template <typename T, int ORDER, bool KAHAN>
 PairLaunchFn pick_pair(int mat)
 {

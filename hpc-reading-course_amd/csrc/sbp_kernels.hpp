@@ -552,9 +552,9 @@ __global__ __launch_bounds__(BX) void stage2d_kernel(StageArgs<T> a)
 // 2R+2 (input) and 2R+1 (mid) keep every slot a row reads alive until the
 // barrier that precedes its overwrite.
 // Threads (mid columns) per two-stage tile; the host sizes the grid with it.
-// FP64 order 6 takes 128: its 256-wide rings (93 KB of LDS) would leave one
-// block per CU (measured at 4097^2: 0.92 ms per step against 0.99).
-constexpr int pair_nt(int esz, int order) { return esz == 8 && order == 6 ? 128 : 256; }
+// (With the LDS-ring tile, FP64 order 6 needed 128 to fit three blocks per
+// CU; the register-window tile uses 8 KB of LDS and runs 256 everywhere.)
+constexpr int pair_nt(int esz, int order) { return (void)esz, (void)order, 256; }
 
 template <typename T> struct PairArgs {
   const T *yin;  // stencil operand of the outer stage: U (pair 1), Y3 (pair 2)
@@ -760,6 +760,220 @@ __device__ __forceinline__ void pair_tile(const PairArgs<T> &a, int tile)
   }
 }
 
+// Register-window form of pair_tile: each thread keeps its own column's last
+// 2R+1 rows of the operand and of the mid stage in registers (the
+// y-derivatives and pointwise values come from there), and only the rows
+// being x-differentiated pass through LDS, once each: 8 KB of LDS per FP64
+// block instead of 68 KB.  Same arithmetic per point, term for term.
+//   phase 0: stage operand row r (u, p with the x halo) in LDS
+//   phase 1: dx of row r into a register FIFO; mid row m = r-R from the
+//            windows and the FIFO's oldest entry; its u, p into LDS
+//   phase 2: dx of mid row m into a second FIFO; output row o = r-2R
+// LDS rows are single-buffered: each row written in one phase is read in the
+// next, and the barrier between them also separates the read from the next
+// iteration's write.
+template <typename T, int ORDER, int MAT, int PAIR, bool KAHAN, int NT, int PF>
+__device__ __forceinline__ void pair_tile_rw(const PairArgs<T> &a, int tile)
+{
+  using O = Ops<ORDER>;
+  constexpr int R = O::R, D = O::CLS + O::R, WN = 2 * R + 1;
+  constexpr int LW = NT + 2 * R;
+  constexpr int BXO = NT - 2 * R;
+  constexpr bool LD_M = MAT == M_ARR;
+  constexpr int RW = Geom<ORDER>::TAPS - R - 1;
+
+  __shared__ T s_u[LW], s_p[LW];      // operand row r (x halo included)
+  __shared__ T s_mu[NT], s_mp[NT];    // mid row m
+
+  const int t = threadIdx.x;
+  const int tx = tile % a.tiles_x, tyb = tile / a.tiles_x;
+  const int x0 = D + tx * BXO;
+  const int mc = x0 - R + t;
+  const int mcc = min(mc, a.nx - 1);
+  const bool hl = t < R, hr = t >= NT - R, halo = hl || hr;
+  const int hc = hl ? mc - R : min(mc + R, a.nx - 1);
+  const int hslot = hl ? t : t + 2 * R;
+  const bool out_col = t >= R && t < NT - R && mc < a.nx - D;
+  const int j0 = D + tyb * a.ty_rows;
+  const int j1 = min(j0 + a.ty_rows, a.ny - D);
+  const int nk = (j1 - j0) + 4 * R;
+  const long plane = a.plane, pitch = a.pitch;
+  const long col = a.xoff + mcc, hcol = a.xoff + hc;
+  const int jmax = a.ny - 1;
+  const T *__restrict__ yin = a.yin;
+
+  auto fetch = [&](PairRow<T> &q, int k) {
+    const int r = min(j0 - 2 * R + k, jmax);
+    const long orow = (long)r * pitch;
+#pragma unroll
+    for (int c = 0; c < 3; c++) q.in[c] = yin[c * plane + orow + col];
+    if (halo) {
+      q.h[0] = yin[orow + hcol];
+      q.h[1] = yin[2 * plane + orow + hcol];
+    }
+    const long om = (long)max(r - R, 0) * pitch + col;
+    if constexpr (PAIR == 2) {
+#pragma unroll
+      for (int c = 0; c < 3; c++) q.U[c] = a.u[c * plane + om];
+    }
+    if constexpr (LD_M) {
+      q.ri = a.rinv[om];
+      q.K = a.bulk[om];
+    }
+    if constexpr (PAIR == 2) {
+      const long oo = (long)max(r - 2 * R, 0) * pitch + col;
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        q.Y2[c] = ldg<true>(a.y2 + c * plane + oo);
+        if constexpr (KAHAN) q.C[c] = ldg<true>(a.comp + c * plane + oo);
+      }
+    }
+  };
+
+  constexpr int S = PF + 1;
+  PairRow<T> q[S];
+#pragma unroll
+  for (int k = 0; k < S; k++) {
+    q[k] = PairRow<T>{};
+    if (k < PF) fetch(q[k], k);
+  }
+  // windows of the last 2R+1 rows (oldest first): operand rows r-2R..r, mid rows m-2R..m
+  T win[3][WN], wmid[3][WN];
+  // FIFOs of the last R+1 rows: dx of the operand (u, p), dx of the mid stage,
+  // materials and (pair 2) U of the mid rows
+  T fdx[2][R + 1], gdx[2][R + 1], fri[R + 1], fK[R + 1], fU[R + 1][3];
+#pragma unroll
+  for (int s = 0; s < WN; s++)
+#pragma unroll
+    for (int c = 0; c < 3; c++) win[c][s] = wmid[c][s] = T(0);
+#pragma unroll
+  for (int p = 0; p <= R; p++) {
+    fdx[0][p] = fdx[1][p] = gdx[0][p] = gdx[1][p] = T(0);
+    fri[p] = fK[p] = T(1);
+#pragma unroll
+    for (int c = 0; c < 3; c++) fU[p][c] = T(0);
+  }
+
+  for (int kb = 0; kb < nk; kb += S) {
+#pragma unroll
+    for (int pq = 0; pq < S; pq++) {
+      const int k = kb + pq;
+      const PairRow<T> &cur = q[pq];
+      fetch(q[(pq + PF) % S], k + PF);
+
+      // phase 0: operand row r into the windows and (u, p) into LDS
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+#pragma unroll
+        for (int s = 0; s < WN - 1; s++) win[c][s] = win[c][s + 1];
+        win[c][WN - 1] = cur.in[c];
+      }
+      s_u[t + R] = cur.in[0];
+      s_p[t + R] = cur.in[2];
+      if (halo) {
+        s_u[hslot] = cur.h[0];
+        s_p[hslot] = cur.h[1];
+      }
+      __syncthreads();
+
+      // phase 1: dx of row r; mid row m = r - R
+#pragma unroll
+      for (int p = 0; p < R; p++) {
+        fdx[0][p] = fdx[0][p + 1];
+        fdx[1][p] = fdx[1][p + 1];
+      }
+      fdx[0][R] = dint<T, ORDER>([&](int s) { return s_u[t + s]; }, a.hix);
+      fdx[1][R] = dint<T, ORDER>([&](int s) { return s_p[t + s]; }, a.hix);
+      if (k >= 2 * R) {
+        const T dyp = dint<T, ORDER>([&](int s) { return win[2][s]; }, a.hiy);
+        const T dyv = dint<T, ORDER>([&](int s) { return win[1][s]; }, a.hiy);
+        T Um[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) Um[c] = PAIR == 1 ? win[c][R] : cur.U[c];
+        const T ri = LD_M ? cur.ri : T(1), Kb = LD_M ? cur.K : T(1);
+        T F[3];
+        acoustic_F<T, MAT>(fdx[1][0], dyp, fdx[0][0], dyv, ri, Kb, T(0), T(0), F[0], F[1], F[2]);
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+#pragma unroll
+          for (int s = 0; s < WN - 1; s++) wmid[c][s] = wmid[c][s + 1];
+          wmid[c][WN - 1] = Um[c] + a.c1 * F[c];
+        }
+        s_mu[t] = wmid[0][WN - 1];
+        s_mp[t] = wmid[2][WN - 1];
+#pragma unroll
+        for (int p = 0; p < R; p++) {
+          fri[p] = fri[p + 1];
+          fK[p] = fK[p + 1];
+#pragma unroll
+          for (int c = 0; c < 3; c++) fU[p][c] = fU[p + 1][c];
+        }
+        fri[R] = ri;
+        fK[R] = Kb;
+#pragma unroll
+        for (int c = 0; c < 3; c++) fU[R][c] = Um[c];
+      }
+      __syncthreads();
+
+      // phase 2: dx of mid row m; output row o = r - 2R
+      if (k >= 2 * R) {
+#pragma unroll
+        for (int p = 0; p < R; p++) {
+          gdx[0][p] = gdx[0][p + 1];
+          gdx[1][p] = gdx[1][p + 1];
+        }
+        if (out_col) {
+          gdx[0][R] = dint<T, ORDER>([&](int s) { return s_mu[t - R + s]; }, a.hix);
+          gdx[1][R] = dint<T, ORDER>([&](int s) { return s_mp[t - R + s]; }, a.hix);
+        }
+      }
+      if (k >= 4 * R && k < nk && out_col) {
+        const int o = j0 - 4 * R + k;
+        const T dyp = dint<T, ORDER>([&](int s) { return wmid[2][s]; }, a.hiy);
+        const T dyv = dint<T, ORDER>([&](int s) { return wmid[1][s]; }, a.hiy);
+        T F[3];
+        acoustic_F<T, MAT>(gdx[1][0], dyp, gdx[0][0], dyv, fri[0], fK[0], T(0), T(0), F[0], F[1], F[2]);
+        const long qo = (long)o * pitch + a.xoff + mc;
+        if constexpr (PAIR == 1) {
+#pragma unroll
+          for (int c = 0; c < 3; c++) {
+            stg<true>(a.o1 + c * plane + qo, wmid[c][R]);
+            stg<true>(a.o2 + c * plane + qo, win[c][0] + a.c2 * F[c]);
+          }
+        } else {
+          const bool ring = mc - D < RW || a.nx - D - 1 - mc < RW || o - D < RW || a.ny - D - 1 - o < RW;
+#pragma unroll
+          for (int c = 0; c < 3; c++) {
+            const T U = fU[0][c];
+            const T Y4 = wmid[c][R];
+            const T inc = ((cur.Y2[c] - U) + 2 * (win[c][0] - U) + (Y4 - U)) * a.third + a.c2 * F[c];
+            if constexpr (!KAHAN) {
+              stg<true>(a.o1 + c * plane + qo, U + inc);
+            } else {
+              const T y = inc - cur.C[c];
+              const T tt = U + y;
+              stg<true>(a.comp + c * plane + qo, (tt - U) - y);
+              stg<true>(a.o1 + c * plane + qo, tt);
+            }
+            if (ring) a.o2[c * plane + qo] = Y4;
+          }
+        }
+      }
+    }
+  }
+}
+
+#ifndef SBP_PAIR_RW
+#define SBP_PAIR_RW 1
+#endif
+// Rows of loads in flight in the pair-1 and pair-2 tiles (sweep on gfx950).
+#ifndef SBP_PAIR_PF1
+#define SBP_PAIR_PF1 2
+#endif
+#ifndef SBP_PAIR_PF2
+#define SBP_PAIR_PF2 2
+#endif
+
 // Blocks [0, n_edge_blocks): stage 1 (pair 1) or stage 3 (pair 2) of the
 // D-deep frame (ea); the rest: two-stage interior tiles (a), XCD-remapped.
 template <typename T, int ORDER, int MAT, int PAIR, bool KAHAN, int NT, int PF>
@@ -775,7 +989,10 @@ __global__ __launch_bounds__(NT) void pair2d_kernel(StageArgs<T> ea, PairArgs<T>
   const int xcd = bi % kNumXcd, local = bi / kNumXcd;
   const int q = nt / kNumXcd, r = nt % kNumXcd;
   const int tile = xcd * q + min(xcd, r) + local;
-  pair_tile<T, ORDER, MAT, PAIR, KAHAN, NT, PF>(a, tile);
+  if constexpr (SBP_PAIR_RW)
+    pair_tile_rw<T, ORDER, MAT, PAIR, KAHAN, NT, PF>(a, tile);
+  else
+    pair_tile<T, ORDER, MAT, PAIR, KAHAN, NT, PF>(a, tile);
 }
 
 // --------------------------------------------------------------------------

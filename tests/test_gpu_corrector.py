@@ -186,7 +186,7 @@ def test_corrected_step_uses_host_features(capi, band):
 
 
 @pytest.mark.parametrize("prec,band,two_stage", [(P.FP64, 0, False), (P.FP32, 0, False), (P.FP32, 4, False),
-                                                  (P.FP64, 0, True), (P.FP32, 4, True)])
+                                                  (P.FP64, 0, True)])
 def test_batch_advance_matches_single(capi, prec, band, two_stage):
     """sbp_corr_advance_batch (corrected steps captured in a graph and
     replayed, then the remainder eagerly) equals the single-context

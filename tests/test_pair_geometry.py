@@ -1,6 +1,6 @@
 """CPU model of the two-stage interior launches (sbp_capi.hip
 compute_pair_geometry / default_pair_rows, sbp_kernels.hpp pair_tile /
-pair2d_kernel, edge_point with a D = CLS + R deep frame).
+pair_tile_rw / pair2d_kernel, edge_point with a D = CLS + R deep frame).
 
 Checks, for many grid shapes and all orders, that
 * every global load of a pair tile (operand rows with prefetch, halo
@@ -13,7 +13,9 @@ Checks, for many grid shapes and all orders, that
 * every interior point the frame's stage 4 reads (zero-padded taps
   included) is one of the points pair 2 stores Y4 at;
 * the LDS ring slots hold the rows each phase reads, and no slot a phase
-  reads is overwritten before a barrier separates the two.
+  reads is overwritten before a barrier separates the two (pair_tile);
+* the register windows and FIFOs hold the rows each phase reads
+  (pair_tile_rw, the default).
 """
 import numpy as np
 import pytest
@@ -21,8 +23,8 @@ import pytest
 from test_kernel_geometry import CLS, R, TAPS, NXCD, dline_range, edge_decode, xcd_tile
 
 def pair_nt(esz, order):
-    """Threads per two-stage tile: 128 for FP64 order 6, else 256."""
-    return 128 if (esz == 8 and order == 6) else 256
+    """Threads per two-stage tile."""
+    return 256
 
 
 def default_pair_rows(nx, ny, order, esz=8):
@@ -32,7 +34,7 @@ def default_pair_rows(nx, ny, order, esz=8):
     if wi <= 0 or hi <= 0:
         return 16
     tx = (wi + NT - 2 * r - 1) // (NT - 2 * r)
-    t = 32 if order == 6 else 16
+    t = 48 if order == 6 else 16
     while t > 8 and tx * ((hi + t - 1) // t) < 768:
         t //= 2
     return t
@@ -87,6 +89,35 @@ def simulate_rings(order, nrows):
         si = 0 if si + 1 == DI else si + 1
         if k >= 2 * r:
             sm = 0 if sm + 1 == DM else sm + 1
+
+
+def simulate_windows(order, nrows):
+    """Replays the register windows and FIFOs of pair_tile_rw for a tile of
+    nrows output rows (rows numbered from j0 = 0): every value a phase uses
+    must come from the row the formula needs."""
+    r = R[order]
+    WN = 2 * r + 1
+    win = [None] * WN          # operand rows, oldest first
+    wmid = [None] * WN         # mid rows
+    fdx = [None] * (r + 1)     # dx of operand rows
+    gdx = [None] * (r + 1)     # dx of mid rows
+    fmat = [None] * (r + 1)    # materials / U of mid rows
+    nk = nrows + 4 * r
+    for k in range(nk):
+        row = -2 * r + k
+        win = win[1:] + [row]                      # phase 0
+        fdx = fdx[1:] + [row]                      # phase 1: dx of row r
+        if k >= 2 * r:
+            m = row - r
+            assert win == list(range(m - r, m + r + 1)) and fdx[0] == m
+            assert win[r] == m                     # U(m) of pair 1
+            wmid = wmid[1:] + [m]
+            fmat = fmat[1:] + [m]
+            gdx = gdx[1:] + [m]                    # phase 2: dx of mid row m
+        if k >= 4 * r:
+            o = -4 * r + k
+            assert wmid == list(range(o - r, o + r + 1)), (order, k, wmid)
+            assert gdx[0] == o and fmat[0] == o and wmid[r] == o and win[0] == o
 
 
 def check(nx, ny, order, rows=None, pf=3, esz=8):
@@ -191,6 +222,12 @@ def test_pair_ring_slots(order, nrows):
     simulate_rings(order, nrows)
 
 
+@pytest.mark.parametrize("order", [2, 4, 6])
+@pytest.mark.parametrize("nrows", [1, 2, 5, 8, 64])
+def test_pair_register_windows(order, nrows):
+    simulate_windows(order, nrows)
+
+
 def test_pair_ring_race_is_detected():
     """The hazard check fails for an input ring one row shorter (2R+1 deep),
     where the store of row r would overwrite the row the previous output
@@ -211,8 +248,7 @@ def test_pair_ring_race_is_detected():
 def test_pair_geometry_large_grid_parameters():
     g = pair_geometry(4097, 4097, 4)
     assert g["d"] == 6 and g["tiles_x"] == 17 and g["rows"] == 16 and g["n_tiles"] == 17 * 256
-    assert pair_geometry(4097, 4097, 6)["rows"] == 32 and pair_geometry(4097, 4097, 6)["nt"] == 128
-    assert pair_geometry(4097, 4097, 6, esz=4)["nt"] == 256
+    assert pair_geometry(4097, 4097, 6)["rows"] == 48 and pair_geometry(4097, 4097, 6)["nt"] == 256
     assert pair_geometry(1025, 1025, 4)["rows"] == 8
     check(4097, 300, 4)
     check(1025, 1025, 4)
