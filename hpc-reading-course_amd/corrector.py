@@ -37,16 +37,16 @@ BAND = 4
 WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "corrector_weights.npz")
 
 
-def instance(seed: int, n: int, precision: int = P.FP64, sigma: float = None):
+def instance(seed: int, n: int, precision: int = P.FP64, sigma: float = None, order: int = ORDER):
     """Seeded instance on an n x n grid; the same seed gives the same fields
-    on every nested grid (exact injection)."""
-    return P.acoustic_box(n, ORDER, seed=seed, sigma=SIGMA if sigma is None else sigma, materials=True,
+    on every nested grid (exact injection) and for every operator order."""
+    return P.acoustic_box(n, order, seed=seed, sigma=SIGMA if sigma is None else sigma, materials=True,
                           material_seed=seed + 7, precision=precision)
 
 
-def _instances(seeds, n, precision, threads=16):
+def _instances(seeds, n, precision, threads=16, order=ORDER):
     with cf.ThreadPoolExecutor(min(threads, len(seeds))) as ex:
-        return list(ex.map(lambda s: instance(s, n, precision), seeds))
+        return list(ex.map(lambda s: instance(s, n, precision, order=order), seeds))
 
 
 def _shift(f, axis, k):
@@ -268,8 +268,9 @@ def time_to_accuracy(params, norm, band=BAND, n_eval=32, seed0=900000, t_end=0.5
       batched_fp32  FP32 ensemble, plain_n^2, one persistent kernel per stage
       learned       FP32 ensemble on 257^2 + the corrector after every step
     and, for comparison, the FP64 ensemble at plain_n^2 (batched_fp64), the
-    learned form with FP64 stages (learned_fp64) and the plain 257^2 solve
-    (the method without the network)."""
+    learned form with FP64 stages (learned_fp64), the plain 257^2 solve
+    (the method without the network), and the plain FP64 path and the FP32
+    ensemble with the order-6 operators on 257^2."""
     seeds = list(range(seed0, seed0 + n_eval))
     res = {"n_eval": n_eval, "t_end": t_end, "target_l2": 1e-4}
     # fine FP64 reference, batched
@@ -288,8 +289,8 @@ def time_to_accuracy(params, norm, band=BAND, n_eval=32, seed0=900000, t_end=0.5
         ref_c = [b.get_state_strided(k, RATIO) for k in range(n_eval)]
     log("reference: %d instances of %d^2, %d steps (%.2f s)" % (n_eval, N_FINE, nf, res["fine_ref_s"]))
 
-    def run_plain(n, prec, refs):
-        insts = _instances(seeds, n, prec)
+    def run_plain(n, prec, refs, order=ORDER):
+        insts = _instances(seeds, n, prec, order=order)
         dt = P.dt_from_cfl(insts[0][0], CFL)
         ns = int(round(t_end / dt))
         errs, tot = [], 0.0
@@ -358,6 +359,31 @@ def time_to_accuracy(params, norm, band=BAND, n_eval=32, seed0=900000, t_end=0.5
     res["speedup_batched_fp32"] = tp / tb
     res["speedup_learned"] = tp / tl
     res["gain_learned_vs_plain_coarse_l2"] = float(np.mean(ec) / res["learned"]["mean_l2"])
+
+    # the same comparison with the order-6 operators on the coarse grid: the
+    # plain FP64 path one instance after another and the FP32 ensemble
+    t6, e6 = run_plain(N_COARSE, P.FP64, ref_c, order=6)
+    res["plain_fp64_order6"] = {"grid": N_COARSE, "order": 6, "wall_s": t6, "max_l2": max(e6),
+                                "mean_l2": float(np.mean(e6))}
+    log("plain FP64 order 6 %d^2: %.3f s, max L2 %.2e, mean %.2e" % (N_COARSE, t6, max(e6), float(np.mean(e6))))
+    insts = _instances(seeds, N_COARSE, P.FP32, order=6)
+    dt = P.dt_from_cfl(insts[0][0], CFL)
+    ns = int(round(t_end / dt))
+    with capi.Batch([s for s, _ in insts]) as b:
+        for k, (_, v0) in enumerate(insts):
+            b.set_state(k, v0)
+        b.autotune(dt)
+        b.synchronize()
+        t = time.time()
+        b.step(dt, ns)
+        b.synchronize()
+        tb6 = time.time() - t
+        eb6 = [_l2(insts[k][0], b.get_state(k), ref_c[k]) for k in range(n_eval)]
+    res["batched_fp32_order6"] = {"grid": N_COARSE, "order": 6, "wall_s": tb6, "max_l2": max(eb6),
+                                  "mean_l2": float(np.mean(eb6))}
+    log("batched FP32 order 6 %d^2: %.3f s, max L2 %.2e, mean %.2e" % (N_COARSE, tb6, max(eb6), float(np.mean(eb6))))
+    res["speedup_batched_fp32_order6"] = tp / tb6
+    res["speedup_batched_fp32_order6_vs_plain_order6"] = t6 / tb6
     return res
 
 

@@ -126,13 +126,13 @@ bool pair_supported(const sbp_ctx *c)
 }
 
 // Two-stage stepping is the default where it was measured faster: grids of
-// at least 1500^2 interior points (2049^2 order 4 FP64: 64.6 G DOF-updates/s
-// against 51 for one launch per stage; at 1025^2 the latency of its longer
-// per-row pipeline loses, 34 G against 39).
+// at least 1000^2 interior points (order 4 FP64, ms per step against one
+// launch per stage: 1025^2 0.072 / 0.079, 1537^2 0.120 / 0.157, 2049^2
+// 0.176 / 0.238, 4097^2 0.667 / 0.980).
 bool pair_default(const sbp_ctx *c)
 {
   const int D = c->cls + c->order / 2;
-  return pair_supported(c) && (long)(c->nx - 2 * D) * (c->ny - 2 * D) >= 1500L * 1500L;
+  return pair_supported(c) && (long)(c->nx - 2 * D) * (c->ny - 2 * D) >= 1000L * 1000L;
 }
 
 int default_ty_rows(int nx, int ny, int cls)

@@ -100,7 +100,7 @@ int  sbp_set_tile_rows(sbp_ctx *ctx, int rows);
  * (CLS+R)-deep boundary frame in the one-stage kernels; 7 array passes per
  * step instead of 13, same arithmetic per point.  2D acoustic contexts with
  * SBP_MAT_UNIT or SBP_MAT_ARRAYS only (SBP_EINVAL otherwise); on by default
- * for those with at least 1500^2 interior points.  rows: tile height of the
+ * for those with at least 1000^2 interior points.  rows: tile height of the
  * two-stage tiles (0 = chosen from the grid).  A performance knob with no
  * reference counterpart: it changes how ts_rk4's steps are scheduled on the
  * device (ts_rk.cpp:54-65), not what they compute. */

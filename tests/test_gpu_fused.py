@@ -190,14 +190,14 @@ def test_fused_then_rk45_and_time_stages(capi, orc):
 
 
 def test_fused_default_by_grid_size(capi):
-    """On by default for 2D acoustics with >= 1500^2 interior points (unit or
+    """On by default for 2D acoustics with >= 1000^2 interior points (unit or
     array materials), off below and for every other problem."""
-    big, _, _ = P.baseline_acoustic(1600, 4)
+    big, _, _ = P.baseline_acoustic(1025, 4)
     with capi.Context(big) as ctx:
         assert ctx.fused
     with capi.Context(big.with_(precision=P.FP32_KAHAN)) as ctx:
         assert ctx.fused
-    small, _, _ = P.baseline_acoustic(1025, 4)
+    small, _, _ = P.baseline_acoustic(1001, 4)   # 989^2 interior points
     with capi.Context(small) as ctx:
         assert not ctx.fused
 
