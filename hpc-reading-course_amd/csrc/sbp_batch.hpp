@@ -4,9 +4,11 @@
 // per RK4 stage.  The host flattens the work of every instance into one list
 // of (instance, item) pairs -- the boundary-frame blocks and the interior
 // tiles of sbp_kernels.hpp -- and a fixed grid of persistent blocks walks
-// that list.  Each XCD takes a contiguous range of the list (items of one
-// instance stay together in one L2).  The per-instance descriptors live in
-// device memory, so a captured graph stays valid when the time steps change.
+// that list.  The list is cut into one chunk per XCD: whole instances,
+// bin-packed by work, so every XCD gets the same share of the heavy tiles and
+// an instance's halo rows stay in one L2.  The per-instance descriptors live
+// in device memory, so a captured graph stays valid when the time steps
+// change.
 #pragma once
 #include "sbp_kernels.hpp"
 
@@ -81,12 +83,13 @@ template <typename T, int ORDER, int MAT, int MODE, bool KAHAN, int BX, int PF>
 __global__ __launch_bounds__(BX) void batch_stage_kernel(const BatchInst<T> *__restrict__ inst,
                                                          const int2 *__restrict__ work, int nwork)
 {
-  // gridDim.x is a multiple of kNumXcd: block b (on XCD b % 8) visits list
-  // positions w = b, b + G, ...; position w maps into XCD (w % 8)'s chunk.
-  const int q = nwork / kNumXcd, r = nwork % kNumXcd;
-  for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
-    const int xcd = w % kNumXcd, local = w / kNumXcd;
-    const int2 it = work[xcd * q + min(xcd, r) + local];
+  // The list opens with one {offset, count} per XCD; gridDim.x is a
+  // multiple of kNumXcd and block b runs on XCD b % 8, whose blocks stride
+  // through that XCD's chunk.
+  const int2 chunk = work[blockIdx.x % kNumXcd];
+  const int2 *items = work + kNumXcd + chunk.x;
+  for (int w = blockIdx.x / kNumXcd; w < chunk.y; w += gridDim.x / kNumXcd) {
+    const int2 it = items[w];
     const StageArgs<T> a = batch_args<T, MODE>(inst[it.x]);
     if (it.y < a.n_edge_blocks)
       edge_point<T, ORDER, PH_ACOUSTIC, MAT, MODE, KAHAN>(a, it.y * BX + threadIdx.x);
@@ -163,10 +166,10 @@ __global__ __launch_bounds__(NT) void batch_pair_kernel(const BatchInst<T> *__re
                                                         const int2 *__restrict__ work, int nwork)
 {
   constexpr int MODE = PART == 1 ? MODE_S1 : (PART == 2 ? MODE_S2 : (PART == 3 ? MODE_S3 : MODE_S4));
-  const int q = nwork / kNumXcd, r = nwork % kNumXcd;
-  for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
-    const int xcd = w % kNumXcd, local = w / kNumXcd;
-    const int2 it = work[xcd * q + min(xcd, r) + local];
+  const int2 chunk = work[blockIdx.x % kNumXcd];  // as in batch_stage_kernel
+  const int2 *items = work + kNumXcd + chunk.x;
+  for (int w = blockIdx.x / kNumXcd; w < chunk.y; w += gridDim.x / kNumXcd) {
+    const int2 it = items[w];
     const BatchInst<T> &d = inst[it.x];
     if (it.y < d.p_n_edge_blocks) {
       edge_point<T, ORDER, PH_ACOUSTIC, MAT, MODE, KAHAN>(batch_frame_args<T, MODE>(d), it.y * NT + threadIdx.x);

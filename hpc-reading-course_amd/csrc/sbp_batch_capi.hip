@@ -135,31 +135,46 @@ int apply_config(sbp_batch *b, int cfg)
     fn = b->prec == SBP_FP64 ? batch_launcher_f64(b->order, b->mat, c.bx, c.pf)
                              : batch_launcher_f32(b->order, b->mat, kahan, c.bx, c.pf);
   if (!fn) return fail(SBP_EINVAL, "no batched kernel for this order/material/precision");
+  // Work lists: kNumXcd {offset, count} headers, then the items of each
+  // XCD's chunk.  Whole instances are bin-packed onto the XCDs by work
+  // (heaviest first onto the least loaded XCD); a chunk holds the tiles of
+  // its instances (largest first), then their frame blocks, so long items
+  // start early and short ones fill the tail on every XCD alike.
   std::vector<int2> work, frame;
   auto build = [&](auto &desc) {
-    // Interior tiles of every instance first (largest instances first), then
-    // the boundary frames: long items start early, short ones fill the tail.
+    const int R = b->order / 2;
+    auto tiles = [&](int k) { return c.pair ? desc[k].p_n_tiles : desc[k].n_tiles; };
+    auto eblocks = [&](int k) { return c.pair ? desc[k].p_n_edge_blocks : desc[k].n_edge_blocks; };
+    std::vector<double> weight(b->count);  // in 256-point rows
+    for (int k = 0; k < b->count; k++)
+      weight[k] = c.pair ? (double)tiles(k) * (c.ty + 4 * R) + eblocks(k)
+                         : (double)tiles(k) * (c.ty + 2 * R) * c.bx / 256.0 + eblocks(k);
     std::vector<int> order(b->count);
     for (int k = 0; k < b->count; k++) order[k] = k;
-    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
-      return (long)b->inst[x].nx * b->inst[x].ny > (long)b->inst[y].nx * b->inst[y].ny;
-    });
-    if (c.pair) {
-      // parts 1 and 3: pair tiles, then the frame's blocks; parts 2 and 4:
-      // the frame's blocks only
-      for (int k : order)
-        for (int t = 0; t < desc[k].p_n_tiles; t++) work.push_back(make_int2(k, desc[k].p_n_edge_blocks + t));
-      for (int k : order)
-        for (int e = 0; e < desc[k].p_n_edge_blocks; e++) {
-          work.push_back(make_int2(k, e));
-          frame.push_back(make_int2(k, e));
-        }
-      return;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return weight[x] > weight[y]; });
+    std::vector<std::vector<int>> on(kNumXcd);
+    std::vector<double> load(kNumXcd, 0.0);
+    for (int k : order) {
+      const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+      on[x].push_back(k);
+      load[x] += weight[k];
     }
-    for (int k : order)
-      for (int t = 0; t < desc[k].n_tiles; t++) work.push_back(make_int2(k, desc[k].n_edge_blocks + t));
-    for (int k : order)
-      for (int e = 0; e < desc[k].n_edge_blocks; e++) work.push_back(make_int2(k, e));
+    auto emit = [&](std::vector<int2> &list, bool with_tiles) {
+      std::vector<int2> items;
+      list.assign(kNumXcd, make_int2(0, 0));
+      for (int x = 0; x < kNumXcd; x++) {
+        const int start = (int)items.size();
+        if (with_tiles)
+          for (int k : on[x])
+            for (int t = 0; t < tiles(k); t++) items.push_back(make_int2(k, eblocks(k) + t));
+        for (int k : on[x])
+          for (int e = 0; e < eblocks(k); e++) items.push_back(make_int2(k, e));
+        list[x] = make_int2(start, (int)items.size() - start);
+      }
+      list.insert(list.end(), items.begin(), items.end());
+    };
+    emit(work, true);                 // every stage (one-stage), parts 1 and 3 (two-stage)
+    if (c.pair) emit(frame, false);   // parts 2 and 4: the frame's blocks only
   };
   // the second descriptor set trades U and Y0 (state of an odd two-stage step)
   auto swapped = [](auto v) {
@@ -200,7 +215,7 @@ int apply_config(sbp_batch *b, int cfg)
                         hipMemcpyHostToDevice));
   b->desc_bytes = desc_bytes;
   b->nwork_frame = (int)frame.size();
-  b->nwork = (int)(work.size() - frame.size());
+  b->nwork = (int)(work.size() - frame.size());  // the frame list starts at work_dev + nwork
   b->pair = c.pair != 0;
   b->grid = std::max(kNumXcd, (b->num_cus * c.blocks_per_cu + kNumXcd - 1) / kNumXcd * kNumXcd);
   b->launch = fn;
