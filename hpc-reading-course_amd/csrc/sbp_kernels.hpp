@@ -547,10 +547,13 @@ __global__ __launch_bounds__(BX) void stage2d_kernel(StageArgs<T> a)
 //
 // A block of NT threads owns NT mid columns (x0-R .. x0-R+NT-1) and the
 // NT-2R output columns inside them, and marches down the rows: iteration k
-// stages input row r = j0-2R+k in an LDS ring, computes mid row r-R into a
-// second ring, then output row r-2R.  Two barriers per row; ring depths
-// 2R+2 (input) and 2R+1 (mid) keep every slot a row reads alive until the
-// barrier that precedes its overwrite.
+// takes input row r = j0-2R+k, computes mid row r-R, then output row r-2R.
+// Two forms: pair_tile keeps the rows in LDS rings (ring depths 2R+2 for the
+// input and 2R+1 for the mid stage keep every slot a row reads alive until
+// the barrier that precedes its overwrite); pair_tile_rw (the default,
+// SBP_PAIR_RW) keeps each thread's column in register windows and passes
+// only the rows being x-differentiated through LDS.  Two barriers per row.
+//
 // Threads (mid columns) per two-stage tile; the host sizes the grid with it.
 // (With the LDS-ring tile, FP64 order 6 needed 128 to fit three blocks per
 // CU; the register-window tile uses 8 KB of LDS and runs 256 everywhere.)
