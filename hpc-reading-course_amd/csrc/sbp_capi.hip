@@ -279,12 +279,10 @@ template <typename T> hipError_t launch_pair_part_t(sbp_ctx *c, int part, double
   if (part == 4) e.uout = static_cast<T *>(c->Y0);  // U' next to the interior's; U read from uin
   if (part == 1 || part == 3) {
     const PairArgs<T> p = make_pair_args<T>(c, part == 1 ? 1 : 2, dt);
-    return c->pair_launch(part == 1 ? 1 : 2, &e, &p, c->p_n_edge_blocks + c->p_n_tiles, c->stream);
+    return c->pair_launch(part, &e, &p, c->p_n_edge_blocks + c->p_n_tiles, c->stream);
   }
-  // frame-only launch of the one-stage kernel: blocks of kBX threads
-  const int eb = (c->p_n_edge + kBX - 1) / kBX;
-  e.n_edge_blocks = (eb + kNumXcd - 1) / kNumXcd * kNumXcd;
-  const hipError_t err = c->launch(mode[part - 1], &e, LaunchGeom{e.n_edge_blocks, kBX}, c->stream);
+  // frame-only launch: one wave per block
+  const hipError_t err = c->pair_launch(part, &e, nullptr, (c->p_n_edge + kFrameBX - 1) / kFrameBX, c->stream);
   if (part == 4) std::swap(c->U, c->Y0);
   return err;
 }

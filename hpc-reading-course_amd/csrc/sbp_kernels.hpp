@@ -1235,20 +1235,42 @@ struct Launcher1D {
   }
 };
 
-using PairLaunchFn = hipError_t (*)(int pair, const void *edge_args, const void *pair_args, int grid,
+// Frame-only stages 2 and 4 of the two-stage schedule: one wave per block,
+// so the few thousand frame points spread over every CU.
+constexpr int kFrameBX = 64;
+template <typename T, int ORDER, int MAT, int MODE, bool KAHAN>
+__global__ __launch_bounds__(kFrameBX) void frame2d_kernel(StageArgs<T> a)
+{
+  edge_point<T, ORDER, PH_ACOUSTIC, MAT, MODE, KAHAN>(a, blockIdx.x * kFrameBX + threadIdx.x);
+}
+
+// part 1: pair2d_kernel pair 1 (+ frame stage 1); part 2: frame stage 2;
+// part 3: pair2d_kernel pair 2 (+ frame stage 3); part 4: frame stage 4.
+// pair_args is ignored for parts 2 and 4.
+using PairLaunchFn = hipError_t (*)(int part, const void *edge_args, const void *pair_args, int grid,
                                     hipStream_t s);
 
 // PF1 / PF2: rows of loads in flight ahead of the row being computed in the
 // pair-1 (one stencil operand per row) and pair-2 (operand + U + Y2) launches.
 template <typename T, int ORDER, int MAT, bool KAHAN, int NT, int PF1, int PF2> struct PairLauncher {
-  static hipError_t launch(int pair, const void *ea, const void *pa, int grid, hipStream_t s)
+  static hipError_t launch(int part, const void *ea, const void *pa, int grid, hipStream_t s)
   {
     const StageArgs<T> &e = *static_cast<const StageArgs<T> *>(ea);
-    const PairArgs<T> &p = *static_cast<const PairArgs<T> *>(pa);
-    if (pair == 1)
-      hipLaunchKernelGGL((pair2d_kernel<T, ORDER, MAT, 1, KAHAN, NT, PF1>), dim3(grid), dim3(NT), 0, s, e, p);
-    else
-      hipLaunchKernelGGL((pair2d_kernel<T, ORDER, MAT, 2, KAHAN, NT, PF2>), dim3(grid), dim3(NT), 0, s, e, p);
+    switch (part) {
+      case 1:
+        hipLaunchKernelGGL((pair2d_kernel<T, ORDER, MAT, 1, KAHAN, NT, PF1>), dim3(grid), dim3(NT), 0, s, e,
+                           *static_cast<const PairArgs<T> *>(pa));
+        break;
+      case 2:
+        hipLaunchKernelGGL((frame2d_kernel<T, ORDER, MAT, MODE_S2, KAHAN>), dim3(grid), dim3(kFrameBX), 0, s, e);
+        break;
+      case 3:
+        hipLaunchKernelGGL((pair2d_kernel<T, ORDER, MAT, 2, KAHAN, NT, PF2>), dim3(grid), dim3(NT), 0, s, e,
+                           *static_cast<const PairArgs<T> *>(pa));
+        break;
+      default:
+        hipLaunchKernelGGL((frame2d_kernel<T, ORDER, MAT, MODE_S4, KAHAN>), dim3(grid), dim3(kFrameBX), 0, s, e);
+    }
     return hipGetLastError();
   }
 };
