@@ -318,7 +318,7 @@ def main():
     traffic = traffic_dof * dofs / 4 if traffic_dof else None  # mean HBM bytes per launch of a step
     if main_m["schedule"] == "two-stage":
         kernel = ("pair2d_kernel x2 (interior stages 1-2 and 3-4, each launch with the frame's stage 1 or 3) "
-                  "+ stage2d_kernel x2 (frame stages 2 and 4): one RK4 step, mean over its four launches")
+                  "+ frame2d_kernel x2 (frame stages 2 and 4): one RK4 step, mean over its four launches")
     else:
         kernel = "stage2d_kernel x4 (one RK4 step), mean over the four stages"
     out = {
