@@ -185,7 +185,29 @@ __global__ __launch_bounds__(NT) void batch_pair_kernel(const BatchInst<T> *__re
   }
 }
 
-// part: 1..4; the pair launches take the prefetch depths of sbp_kernels.hpp
+// Frame-only parts 2 and 4 (stages 2 and 4 on the D-deep frames): each
+// frame work item holds NT points, taken by NT / kFrameBX one-wave blocks.
+template <typename T, int ORDER, int MAT, int PART, bool KAHAN, int NT>
+__global__ __launch_bounds__(kFrameBX) void batch_frame_kernel(const BatchInst<T> *__restrict__ inst,
+                                                               const int2 *__restrict__ work, int nwork)
+{
+  constexpr int MODE = PART == 2 ? MODE_S2 : MODE_S4;
+  constexpr int SUB = NT / kFrameBX;
+  static_assert(NT % kFrameBX == 0, "frame items split into whole waves");
+  const int2 chunk = work[blockIdx.x % kNumXcd];
+  const int2 *items = work + kNumXcd + chunk.x;
+  const int n = chunk.y * SUB;
+  for (int v = blockIdx.x / kNumXcd; v < n; v += gridDim.x / kNumXcd) {
+    const int2 it = items[v / SUB];
+    const BatchInst<T> &d = inst[it.x];
+    edge_point<T, ORDER, PH_ACOUSTIC, MAT, MODE, KAHAN>(batch_frame_args<T, MODE>(d),
+                                                         it.y * NT + (v % SUB) * kFrameBX + threadIdx.x);
+  }
+}
+
+// part: 1..4; the pair launches take the prefetch depths of sbp_kernels.hpp;
+// the frame-only parts run as many threads as a pair launch, in one-wave
+// blocks (grid * NT / kFrameBX of them, still a multiple of kNumXcd)
 template <typename T, int ORDER, int MAT, bool KAHAN> struct BatchPairLauncher {
   static constexpr int NT = pair_nt(sizeof(T), ORDER);
   template <int PART, int PF>
@@ -195,13 +217,20 @@ template <typename T, int ORDER, int MAT, bool KAHAN> struct BatchPairLauncher {
                        static_cast<const BatchInst<T> *>(inst), work, nwork);
     return hipGetLastError();
   }
+  template <int PART>
+  static hipError_t frame(const void *inst, const int2 *work, int nwork, int grid, hipStream_t s)
+  {
+    hipLaunchKernelGGL((batch_frame_kernel<T, ORDER, MAT, PART, KAHAN, NT>), dim3(grid * (NT / kFrameBX)),
+                       dim3(kFrameBX), 0, s, static_cast<const BatchInst<T> *>(inst), work, nwork);
+    return hipGetLastError();
+  }
   static hipError_t launch(int part, const void *inst, const int2 *work, int nwork, int grid, hipStream_t s)
   {
     switch (part) {
-      case 1: return go<1, SBP_PAIR_PF1>(inst, work, nwork, grid, s);
-      case 2: return go<2, 1>(inst, work, nwork, grid, s);
-      case 3: return go<3, SBP_PAIR_PF2>(inst, work, nwork, grid, s);
-      default: return go<4, 1>(inst, work, nwork, grid, s);
+      case 1: return go<1, pair_pf(sizeof(T), SBP_PAIR_PF1)>(inst, work, nwork, grid, s);
+      case 2: return frame<2>(inst, work, nwork, grid, s);
+      case 3: return go<3, pair_pf(sizeof(T), SBP_PAIR_PF2)>(inst, work, nwork, grid, s);
+      default: return frame<4>(inst, work, nwork, grid, s);
     }
   }
 };

@@ -23,12 +23,11 @@ StageLaunchFn pick_launcher(int phys, int mat)
 }
 
 // Two-stage interior tiles: pair_nt threads; prefetch depths (rows of loads in
-// flight per wave) from a sweep on gfx950.  LDS holds these kernels at two
-// blocks per CU, so deep register queues are what keeps HBM busy.
+// flight per wave) from a sweep on gfx950 (pair_pf).
 template <typename T, int ORDER, bool KAHAN>
 PairLaunchFn pick_pair(int mat)
 {
-  constexpr int PF1 = SBP_PAIR_PF1, PF2 = SBP_PAIR_PF2;
+  constexpr int PF1 = pair_pf(sizeof(T), SBP_PAIR_PF1), PF2 = pair_pf(sizeof(T), SBP_PAIR_PF2);
   constexpr int NT = pair_nt(sizeof(T), ORDER);
   if (mat == M_UNIT) return &PairLauncher<T, ORDER, M_UNIT, KAHAN, NT, PF1, PF2>::launch;
   if (mat == M_ARR) return &PairLauncher<T, ORDER, M_ARR, KAHAN, NT, PF1, PF2>::launch;
