@@ -401,12 +401,13 @@ def run_learned(nsteps=512, n_eval=16, seed0=900000, bound=1e-4, log=print):
         for k, (_, v0) in enumerate(insts):
             b.set_state(k, v0)
         b.autotune(dt)
+        cfg = b.get_config()
         b.synchronize()
         t = time.time()
         rej = cor.advance_batch(b, dt, nsteps, bound)
         w = time.time() - t
-    log("%d corrected steps of %d x %d^2 (FP32): %.1f us per step, %d refused" % (
-        nsteps, n_eval, N_COARSE, w / nsteps * 1e6, int(np.sum(rej))))
+    log("%d corrected steps of %d x %d^2 (FP32): %.1f us per step, %d refused, stage config %s" % (
+        nsteps, n_eval, N_COARSE, w / nsteps * 1e6, int(np.sum(rej)), cfg))
     return w / nsteps
 
 
