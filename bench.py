@@ -194,7 +194,7 @@ def reflection_2d(steps=2000, warmup=0):
 
 def time_to_l2():
     """Time to L2 error 1e-4 on 16 held-out seeded variable-material acoustic
-    instances (T = 0.5, judged against the FP64 1025^2 solve injected on each
+    instances (T = 0.5, measured against the FP64 1025^2 solve injected on each
     grid): plain FP64 single-problem path vs the FP32 batched ensemble vs the
     learned corrector on 257^2 with the trained weights shipped in the package."""
     C = pkg("corrector")

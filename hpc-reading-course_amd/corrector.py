@@ -262,7 +262,7 @@ def _l2(spec, v, ref):
 def time_to_accuracy(params, norm, band=BAND, n_eval=32, seed0=900000, t_end=0.5, plain_n=513, bound=1e-4,
                      log=print):
     """Wall time of the ways to solve n_eval held-out instances to t_end, each
-    judged by the L2 error against the FP64 1025^2 solution injected at its
+    measured by the L2 error against the FP64 1025^2 solution injected at its
     own grid nodes:
       plain         FP64 single-problem path, one instance after another, plain_n^2
       batched_fp32  FP32 ensemble, plain_n^2, one persistent kernel per stage
