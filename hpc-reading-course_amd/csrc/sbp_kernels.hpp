@@ -84,17 +84,11 @@ template <typename T, int ORDER>
 __device__ __forceinline__ T dline(const T *__restrict__ f, long stride, int n, int idx, T hi)
 {
   using G = Geom<ORDER>;
-  int row, start;
-  if (idx < G::CLS) {
-    row = idx;
-    start = 0;
-  } else if (idx >= n - G::CLS) {
-    row = G::CLS + (n - 1 - idx);
-    start = n - G::TAPS;
-  } else {
-    row = 2 * G::CLS;
-    start = idx - G::R;
-  }
+  // table row and first tap by selects, not branches: the loads of every
+  // line of an edge point then sit in one basic block and issue together
+  const bool lo = idx < G::CLS, hi_side = idx >= n - G::CLS;
+  const int row = lo ? idx : (hi_side ? G::CLS + (n - 1 - idx) : 2 * G::CLS);
+  const int start = lo ? 0 : (hi_side ? n - G::TAPS : idx - G::R);
   const T *c = c_edge<T, ORDER>.c[row];
   T acc = T(0);
 #pragma unroll
@@ -154,8 +148,39 @@ __device__ __forceinline__ T stage4_increment(const StageArgs<T> &a, T U, T Y2, 
   return ((Y2 - U) + 2 * (Y3 - U) + (Y4 - U)) * a.third + a.ca * K;
 }
 
+// Pointwise operands of the RK4 update at one edge point: U (stages 2-4), Y2,
+// Y3 and the compensation (stage 4).  edge_point loads them before anything
+// else and stores after everything else: the compiler cannot move a load past
+// a store through these pointers (they may alias as far as it knows), and
+// loads issued late, one load-store pair per component, put several
+// dependent memory round trips on the critical path of the frame launches.
+// Distinct components live in distinct planes, and each component's loads
+// precede its own stores, so the values are unchanged.
+template <typename T, int NC> struct RkIn {
+  T U[NC], Y2[NC], Y3[NC], C[NC];
+};
+
 template <typename T, int NC, int MODE, bool KAHAN>
-__device__ __forceinline__ void rk_point(const StageArgs<T> &a, long o, const T *K, const T *Ycen)
+__device__ __forceinline__ RkIn<T, NC> rk_load(const StageArgs<T> &a, long o)
+{
+  RkIn<T, NC> in;
+#pragma unroll
+  for (int c = 0; c < NC; c++) {
+    const long q = o + c * a.plane;
+    in.U[c] = in.Y2[c] = in.Y3[c] = in.C[c] = T(0);
+    if constexpr (MODE == MODE_S2 || MODE == MODE_S3 || MODE == MODE_S4) in.U[c] = a.uin[q];
+    if constexpr (MODE == MODE_S4) {
+      in.Y2[c] = a.y2[q];
+      in.Y3[c] = a.y3[q];
+      if constexpr (KAHAN) in.C[c] = a.comp[q];
+    }
+  }
+  return in;
+}
+
+template <typename T, int NC, int MODE, bool KAHAN>
+__device__ __forceinline__ void rk_store(const StageArgs<T> &a, long o, const T *K, const T *Ycen,
+                                         const RkIn<T, NC> &in)
 {
 #pragma unroll
   for (int c = 0; c < NC; c++) {
@@ -165,14 +190,14 @@ __device__ __forceinline__ void rk_point(const StageArgs<T> &a, long o, const T 
     } else if constexpr (MODE == MODE_S1) {
       a.yout[q] = Ycen[c] + a.cy * K[c];
     } else if constexpr (MODE == MODE_S2 || MODE == MODE_S3) {
-      a.yout[q] = a.uin[q] + a.cy * K[c];
+      a.yout[q] = in.U[c] + a.cy * K[c];
     } else {
-      const T U = a.uin[q];
-      const T inc = stage4_increment<T, KAHAN>(a, U, a.y2[q], a.y3[q], Ycen[c], K[c]);
+      const T U = in.U[c];
+      const T inc = stage4_increment<T, KAHAN>(a, U, in.Y2[c], in.Y3[c], Ycen[c], K[c]);
       if constexpr (!KAHAN) {
         a.uout[q] = U + inc;
       } else {
-        const T y = inc - a.comp[q];
+        const T y = inc - in.C[c];
         const T t = U + y;
         a.comp[q] = (t - U) - y;
         a.uout[q] = t;
@@ -261,22 +286,24 @@ __device__ __forceinline__ void edge_point(const StageArgs<T> &a, int e)
   const T qinv0 = T(Ops<ORDER>::hqinv[0]);
   const T hqx = a.hix * qinv0, hqy = a.hiy * qinv0;
   T K[NC], Uc[NC];
+  // every load of the point is issued before the first arithmetic on it
+  const RkIn<T, NC> rin = rk_load<T, NC, MODE, KAHAN>(a, o);
   if constexpr (AC) {
     const T *U_ = base, *V_ = base + a.plane, *P_ = base + 2 * a.plane;
-    const T dxp = dline<T, ORDER>(P_ + rowb, 1, nx, i, a.hix);
-    const T dyp = dline<T, ORDER>(P_ + colb, pitch, ny, j, a.hiy);
-    const T dxu = dline<T, ORDER>(U_ + rowb, 1, nx, i, a.hix);
-    const T dyv = dline<T, ORDER>(V_ + colb, pitch, ny, j, a.hiy);
     const T u = U_[o], v = V_[o], p = P_[o];
-    Uc[0] = u;
-    Uc[1] = v;
-    Uc[2] = p;
     T ri = T(1), Kb = T(1), fu = T(0), fv = T(0);
-    if constexpr (MAT == M_REF) ref_material(a, i, j, ri, fu, fv);
     if constexpr (MAT == M_ARR) {
       ri = a.rinv[o];
       Kb = a.bulk[o];
     }
+    const T dxp = dline<T, ORDER>(P_ + rowb, 1, nx, i, a.hix);
+    const T dyp = dline<T, ORDER>(P_ + colb, pitch, ny, j, a.hiy);
+    const T dxu = dline<T, ORDER>(U_ + rowb, 1, nx, i, a.hix);
+    const T dyv = dline<T, ORDER>(V_ + colb, pitch, ny, j, a.hiy);
+    Uc[0] = u;
+    Uc[1] = v;
+    Uc[2] = p;
+    if constexpr (MAT == M_REF) ref_material(a, i, j, ri, fu, fv);
     acoustic_F<T, MAT>(dxp, dyp, dxu, dyv, ri, Kb, fu, fv, K[0], K[1], K[2]);
     // Boundary terms in bc_serial order W, S, E, N (boundary_conditions.h:84-87).
     const int bc = a.bc == B_DEFAULT ? B_FSREF : a.bc;
@@ -297,9 +324,9 @@ __device__ __forceinline__ void edge_point(const StageArgs<T> &a, int e)
       if (j == ny - 1) K[2] += Kb * (hqy * v);
     }
   } else {  // 2D advection: advection_{ll..rr} + SAT_bc_{west..north}
+    const T u = base[o];
     const T dx = dline<T, ORDER>(base + rowb, 1, nx, i, a.hix);
     const T dy = dline<T, ORDER>(base + colb, pitch, ny, j, a.hiy);
-    const T u = base[o];
     Uc[0] = u;
     T k = -(a.ax * dx + a.ay * dy);
     const T ax = a.ax, ay = a.ay;
@@ -309,7 +336,7 @@ __device__ __forceinline__ void edge_point(const StageArgs<T> &a, int e)
     if (j == ny - 1) k += T(0.5) * (T(0.5) * (ay - fabs(ay))) * (hqy * u);
     K[0] = k;
   }
-  rk_point<T, NC, MODE, KAHAN>(a, o, K, Uc);
+  rk_store<T, NC, MODE, KAHAN>(a, o, K, Uc, rin);
 }
 
 // --------------------------------------------------------------------------
