@@ -103,12 +103,16 @@ def test_batch_unit_material_and_configs(capi):
             assert np.max(np.abs(g - ref)) <= 2e-6 * np.max(np.abs(ref))
 
 
-@pytest.mark.parametrize("prec", [P.FP64, P.FP32_KAHAN])
-def test_batch_two_stage_parity_and_switching(capi, orc, prec):
+@pytest.mark.parametrize("order", [2, 4, 6])
+@pytest.mark.parametrize("prec", [P.FP64, P.FP32, P.FP32_KAHAN])
+def test_batch_two_stage_parity_and_switching(capi, orc, prec, order):
     """Two-stage configurations: graph replays (10 steps), odd remainders and
     a switch back to a one-stage configuration mid-run keep every instance
-    on the single-problem path; FP64 within 1e-12 of the oracle."""
-    ens = ensemble(prec, sizes=(20, 77, 130, 41), order=6)
+    on the single-problem path; FP64 within 1e-12 of the oracle.  Ragged
+    instance sizes, so the frame work items of the frame-only launches end
+    mid-wave."""
+    print("batch two-stage", prec, order, flush=True)
+    ens = ensemble(prec, sizes=(20, 77, 130, 41), order=order)
     dts = [e[2] for e in ens]
     cfgs = two_stage_configs(capi)
     with capi.Batch([e[0] for e in ens]) as b:
