@@ -186,13 +186,23 @@ def test_corrected_step_uses_host_features(capi, band):
 
 
 @pytest.mark.parametrize("prec,band,two_stage", [(P.FP64, 0, False), (P.FP32, 0, False), (P.FP32, 4, False),
-                                                  (P.FP64, 0, True)])
+                                                  (P.FP64, 0, True), (P.FP32, 4, True)])
 def test_batch_advance_matches_single(capi, prec, band, two_stage):
     """sbp_corr_advance_batch (corrected steps captured in a graph and
     replayed, then the remainder eagerly) equals the single-context
     sbp_corr_advance on every instance, bit for bit, when no step is refused;
     two calls in a row (20 + 40 steps) with the state read in between; with
-    the plain steps on either schedule (one stage or two stages per launch)."""
+    the plain steps on either schedule (one stage or two stages per launch).
+    FP32 with two stages per launch: within 2e-6 relative (the two-stage
+    tiles' row loop is unrolled, and FP32 multiply-adds may be contracted
+    differently in the batch and single-problem kernels)."""
+    exact = prec == P.FP64 or not two_stage
+
+    def same(a, b):
+        if exact:
+            return np.array_equal(a, b)
+        return np.max(np.abs(a - b)) <= 2e-6 * np.max(np.abs(b))
+
     specs = [P.acoustic_box(49, 4, sigma=0.08, materials=True, seed=s, material_seed=s + 3, precision=prec)
              for s in (21, 22, 23)]
     dt = P.dt_from_cfl(specs[0][0], 0.25)
@@ -227,9 +237,9 @@ def test_batch_advance_matches_single(capi, prec, band, two_stage):
                 ctx.set_fused(two_stage)
                 ctx.set_state(v0)
                 assert c.advance(ctx, 0.0, dt, 20, 1.0) == 0
-                assert np.array_equal(ctx.get_state(), mid[k]), k
+                assert same(ctx.get_state(), mid[k]), k
                 assert c.advance(ctx, 20 * dt, dt, 40, 1.0) == 0
-                assert np.array_equal(ctx.get_state(), got[k]), k
+                assert same(ctx.get_state(), got[k]), k
         # the corrections did change the state
         with capi.Batch([sp for sp, _ in specs]) as b:
             for k, (_, v0) in enumerate(specs):
