@@ -747,8 +747,7 @@ __global__ __launch_bounds__(256) void batch_candidate_energy_kernel(const Batch
   const float *nk = net + (long)k * npts * C_OUT;
   const double hx = 1.0 / (double)d.hix, hy = 1.0 / (double)d.hiy;
   double acc = 0;
-  const long q = chunk * 256L + threadIdx.x;  // one point per thread: bpi = ceil(H W / 256)
-  if (q < npts) {
+  for (long q = chunk * 256L + threadIdx.x; q < npts; q += (long)bpi * 256) {
     const int j = (int)(q / W), i = (int)(q - (long)j * W);
     const long o = (long)j * d.pitch + d.xoff + i;
     const double w = hweight(order, W, i, hx) * hweight(order, H, j, hy);
@@ -761,7 +760,7 @@ __global__ __launch_bounds__(256) void batch_candidate_energy_kernel(const Batch
     }
     const double rho = d.rinv ? 1.0 / (double)d.rinv[o] : 1.0;
     const double kinv = d.bulk ? 1.0 / (double)d.bulk[o] : 1.0;
-    acc = w * (rho * (v[0] * v[0] + v[1] * v[1]) + kinv * v[2] * v[2]);
+    acc += w * (rho * (v[0] * v[0] + v[1] * v[1]) + kinv * v[2] * v[2]);
   }
   const double sm = block_sum_256(acc, sh);
   if (threadIdx.x == 0) partial[blockIdx.x] = sm;
@@ -836,7 +835,7 @@ struct sbp_corrector {
   double *lpart = nullptr, *scalars = nullptr;  // scalars: [0] loss, [1] E_old
   int *rejected = nullptr, *idx = nullptr;
   double *bpart = nullptr, *be_old = nullptr;  // batched energies: per-block partials, reference energies
-  double *cpart = nullptr;                      // candidate energies: one partial per 256 points
+  double *cpart = nullptr;                      // candidate energies: per-block partials (at most one per 256 points)
   int *brej = nullptr;
   float *ds_raw = nullptr, *ds_tgt = nullptr;
   int ds_n = 0;
@@ -1312,7 +1311,9 @@ int sbp_corr_advance_batch(sbp_corrector *c, sbp_batch *b, const double *dt, lon
   int rc = sbp::host::batch_upload_dt(b, dt);
   if (rc) return rc;
   const int count = b->count, bpi = 64;
-  const int cbpi = (int)(((long)c->H * c->W + 255) / 256);  // candidate-energy blocks per instance
+  // candidate-energy blocks per instance: few enough that batch_apply_kernel's
+  // blocks sum an instance's partials in one load per lane
+  const int cbpi = (int)std::min<long>(64, ((long)c->H * c->W + 255) / 256);
   const int pth_ = b->esz == 8 ? pth<double> : pth<float>;
   const int ptiles = ((c->W + PTW - 1) / PTW) * ((c->H + pth_ - 1) / pth_);
   const long npix = (long)count * c->H * c->W;
