@@ -120,7 +120,14 @@ def simulate_windows(order, nrows):
             assert gdx[0] == o and fmat[0] == o and wmid[r] == o and win[0] == o
 
 
-def check(nx, ny, order, rows=None, pf=3, esz=8):
+def pair_pf(esz, pf=2):
+    """Rows of loads in flight of a two-stage tile (sbp_kernels.hpp pair_pf):
+    one more for FP32."""
+    return pf + 1 if esz == 4 else pf
+
+
+def check(nx, ny, order, rows=None, pf=None, esz=8):
+    pf = pair_pf(esz) if pf is None else pf
     g = pair_geometry(nx, ny, order, rows, esz)
     NT = g["nt"]
     r, d, cls = R[order], g["d"], CLS[order]
@@ -165,7 +172,10 @@ def check(nx, ny, order, rows=None, pf=3, esz=8):
         j1 = min(j0 + g["rows"], ny - d)
         assert j1 > j0
         nk = (j1 - j0) + 4 * r
-        for k in range(nk + pf):
+        # the row loop runs whole groups of pf + 1 iterations, each loading
+        # row k + pf ahead: the last load is for k = groups * (pf + 1) - 1 + pf
+        kmax = -(-nk // (pf + 1)) * (pf + 1) + pf
+        for k in range(kmax):
             row = min(j0 - 2 * r + k, ny - 1)
             assert 0 <= row < ny and 0 <= max(row - r, 0) and 0 <= max(row - 2 * r, 0)
         # staged input columns of this tile (LDS position p <-> column x0 - 2R + p)
