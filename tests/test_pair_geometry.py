@@ -15,7 +15,9 @@ Checks, for many grid shapes and all orders, that
 * the LDS ring slots hold the rows each phase reads, and no slot a phase
   reads is overwritten before a barrier separates the two (pair_tile);
 * the register windows and FIFOs hold the rows each phase reads
-  (pair_tile_rw, the default).
+  (pair_tile_rw, the default);
+* the ensemble kernels' persistent loops take every work item of every XCD
+  chunk, and every one-wave part of a frame item, exactly once.
 """
 import numpy as np
 import pytest
@@ -262,3 +264,35 @@ def test_pair_geometry_large_grid_parameters():
     assert pair_geometry(1025, 1025, 4)["rows"] == 8
     check(4097, 300, 4)
     check(1025, 1025, 4)
+
+
+def batch_traversal(counts, grid, sub):
+    """Replays the persistent loops of the ensemble kernels (sbp_batch.hpp
+    batch_stage_kernel / batch_pair_kernel with sub = 1, batch_frame_kernel
+    with sub = NT / 64 and grid * sub blocks): block b works on XCD b % 8's
+    chunk, from item (b / 8) / sub on, striding by the XCD's block count.
+    Returns the (xcd, item, part) triples visited."""
+    assert grid % NXCD == 0
+    nb = grid * sub
+    seen = []
+    for b in range(nb):
+        xcd = b % NXCD
+        n = counts[xcd] * sub
+        v = b // NXCD
+        while v < n:
+            seen.append((xcd, v // sub, v % sub))
+            v += nb // NXCD
+    return seen
+
+
+@pytest.mark.parametrize("sub", [1, 4])
+@pytest.mark.parametrize("grid", [8, 256, 1024, 4096])
+def test_batch_work_traversal_visits_every_item_once(grid, sub):
+    """Every work item of every XCD chunk (and every wave-sized part of a
+    frame item) is taken by exactly one block, for chunks shorter and longer
+    than the XCD's share of the grid, and empty ones."""
+    rng = np.random.default_rng(grid + sub)
+    for counts in ([0] * 8, [1, 0, 3, 0, 0, 7, 0, 2], list(rng.integers(0, 2000, 8))):
+        seen = batch_traversal(counts, grid, sub)
+        want = [(x, i, p) for x in range(NXCD) for i in range(counts[x]) for p in range(sub)]
+        assert sorted(seen) == sorted(want)
