@@ -24,6 +24,7 @@ template <typename T> struct BatchInst {
   int bc, tiles_x, ty_rows, n_tiles, n_edge_blocks, n_edge;
   // two-stage configurations: the (CLS+R)-deep frame and the pair tiles
   int p_depth, p_tiles_x, p_ty_rows, p_n_tiles, p_n_edge, p_n_edge_blocks;
+  int p_nts;  // pair tiles' store policy (PairArgs::nts), one for the whole ensemble
 };
 
 template <typename T, int MODE>
@@ -100,6 +101,9 @@ __global__ __launch_bounds__(BX) void batch_stage_kernel(const BatchInst<T> *__r
   }
 }
 
+// flag or-ed into a two-stage launch's part number: cached pair-tile stores
+constexpr int kBatchCachedStores = 16;
+
 using BatchLaunchFn = hipError_t (*)(int mode, const void *inst, const int2 *work, int nwork, int grid,
                                      hipStream_t s);
 
@@ -141,6 +145,7 @@ __device__ __forceinline__ PairArgs<T> batch_pair_args(const BatchInst<T> &d)
   a.ty_rows = d.p_ty_rows;
   a.n_tiles = d.p_n_tiles;
   a.n_edge_blocks = d.p_n_edge_blocks;
+  a.nts = d.p_nts;
   const double dt = d.dt;
   if constexpr (PAIR == 1) {
     a.yin = d.U;
@@ -161,7 +166,7 @@ __device__ __forceinline__ PairArgs<T> batch_pair_args(const BatchInst<T> &d)
   return a;
 }
 
-template <typename T, int ORDER, int MAT, int PART, bool KAHAN, int NT, int PF>
+template <typename T, int ORDER, int MAT, int PART, bool KAHAN, int NT, int PF, bool NTS>
 __global__ __launch_bounds__(NT) void batch_pair_kernel(const BatchInst<T> *__restrict__ inst,
                                                         const int2 *__restrict__ work, int nwork)
 {
@@ -175,8 +180,8 @@ __global__ __launch_bounds__(NT) void batch_pair_kernel(const BatchInst<T> *__re
       edge_point<T, ORDER, PH_ACOUSTIC, MAT, MODE, KAHAN>(batch_frame_args<T, MODE>(d), it.y * NT + threadIdx.x);
     } else if constexpr (PART == 1 || PART == 3) {
       if constexpr (SBP_PAIR_RW)
-        pair_tile_rw<T, ORDER, MAT, PART == 1 ? 1 : 2, KAHAN, NT, PF>(batch_pair_args<T, PART == 1 ? 1 : 2>(d),
-                                                                      it.y - d.p_n_edge_blocks);
+        pair_tile_rw<T, ORDER, MAT, PART == 1 ? 1 : 2, KAHAN, NT, PF, NTS>(batch_pair_args<T, PART == 1 ? 1 : 2>(d),
+                                                                           it.y - d.p_n_edge_blocks);
       else
         pair_tile<T, ORDER, MAT, PART == 1 ? 1 : 2, KAHAN, NT, PF>(batch_pair_args<T, PART == 1 ? 1 : 2>(d),
                                                                    it.y - d.p_n_edge_blocks);
@@ -210,10 +215,10 @@ __global__ __launch_bounds__(kFrameBX) void batch_frame_kernel(const BatchInst<T
 // blocks (grid * NT / kFrameBX of them, still a multiple of kNumXcd)
 template <typename T, int ORDER, int MAT, bool KAHAN> struct BatchPairLauncher {
   static constexpr int NT = pair_nt(sizeof(T), ORDER);
-  template <int PART, int PF>
+  template <int PART, int PF, bool NTS>
   static hipError_t go(const void *inst, const int2 *work, int nwork, int grid, hipStream_t s)
   {
-    hipLaunchKernelGGL((batch_pair_kernel<T, ORDER, MAT, PART, KAHAN, NT, PF>), dim3(grid), dim3(NT), 0, s,
+    hipLaunchKernelGGL((batch_pair_kernel<T, ORDER, MAT, PART, KAHAN, NT, PF, NTS>), dim3(grid), dim3(NT), 0, s,
                        static_cast<const BatchInst<T> *>(inst), work, nwork);
     return hipGetLastError();
   }
@@ -224,12 +229,17 @@ template <typename T, int ORDER, int MAT, bool KAHAN> struct BatchPairLauncher {
                        dim3(kFrameBX), 0, s, static_cast<const BatchInst<T> *>(inst), work, nwork);
     return hipGetLastError();
   }
+  // part | kBatchCachedStores: the pair tiles store cached (pair_cached_stores)
   static hipError_t launch(int part, const void *inst, const int2 *work, int nwork, int grid, hipStream_t s)
   {
+    constexpr int PF1 = pair_pf(sizeof(T), SBP_PAIR_PF1), PF2 = pair_pf(sizeof(T), SBP_PAIR_PF2);
     switch (part) {
-      case 1: return go<1, pair_pf(sizeof(T), SBP_PAIR_PF1)>(inst, work, nwork, grid, s);
-      case 2: return frame<2>(inst, work, nwork, grid, s);
-      case 3: return go<3, pair_pf(sizeof(T), SBP_PAIR_PF2)>(inst, work, nwork, grid, s);
+      case 1: return go<1, PF1, true>(inst, work, nwork, grid, s);
+      case 1 | kBatchCachedStores: return go<1, PF1, false>(inst, work, nwork, grid, s);
+      case 3: return go<3, PF2, true>(inst, work, nwork, grid, s);
+      case 3 | kBatchCachedStores: return go<3, PF2, false>(inst, work, nwork, grid, s);
+      case 2:
+      case 2 | kBatchCachedStores: return frame<2>(inst, work, nwork, grid, s);
       default: return frame<4>(inst, work, nwork, grid, s);
     }
   }

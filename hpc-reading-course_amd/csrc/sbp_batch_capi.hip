@@ -81,11 +81,16 @@ template <typename T> bool fill_inst(const sbp_batch *b, std::vector<BatchInst<T
 {
   bool ok = true;
   out.resize(b->count);
+  // bytes a two-stage step streams over the ensemble (U, Y2, Y3, materials,
+  // compensation): decides the pair tiles' store policy
+  double streamed = 0;
+  for (int k = 0; k < b->count; k++)
+    streamed += (double)b->inst[k].plane * b->esz * (9 + (b->RI ? 2 : 0) + (b->KC ? 3 : 0));
+  const int nts = pair_cached_stores(streamed) ? 0 : 1;
   for (int k = 0; k < b->count; k++) {
     const InstHost &h = b->inst[k];
     BatchInst<T> &d = out[k];
-    const long reg = 3 * h.plane;
-    (void)reg;
+    d.p_nts = nts;
     d.U = static_cast<T *>(b->U) + 3 * h.off;
     d.A = static_cast<T *>(b->A) + 3 * h.off;
     d.Y0 = static_cast<T *>(b->Y0) + 3 * h.off;
@@ -217,6 +222,7 @@ int apply_config(sbp_batch *b, int cfg)
   b->nwork_frame = (int)frame.size();
   b->nwork = (int)(work.size() - frame.size());  // the frame list starts at work_dev + nwork
   b->pair = c.pair != 0;
+  b->pair_nts = b->esz == 8 ? dd[0].p_nts : df[0].p_nts;
   b->grid = std::max(kNumXcd, (b->num_cus * c.blocks_per_cu + kNumXcd - 1) / kNumXcd * kNumXcd);
   b->launch = fn;
   b->config = cfg;
@@ -252,7 +258,7 @@ hipError_t launch_step(sbp_batch *b)
     const void *inst = sbp::host::batch_inst(b);
     for (int p = 1; p <= 4 && e == hipSuccess; p++) {
       const bool frame_only = p == 2 || p == 4;
-      e = b->launch(p, inst, frame_only ? b->work_dev + b->nwork : b->work_dev,
+      e = b->launch(p | (b->pair_nts ? 0 : kBatchCachedStores), inst, frame_only ? b->work_dev + b->nwork : b->work_dev,
                     frame_only ? b->nwork_frame : b->nwork, b->grid, b->stream);
     }
     b->par ^= 1;

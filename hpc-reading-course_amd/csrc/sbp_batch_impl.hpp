@@ -31,6 +31,7 @@ struct sbp_batch {
   size_t desc_bytes = 0;     // bytes of one set
   int2 *work_dev = nullptr;
   int nwork = 0;             // items of the stage launches (two-stage: frame + pair tiles)
+  int pair_nts = 1;          // two-stage: nontemporal (1) or cached (0) pair-tile stores
   int nwork_frame = 0;       // two-stage: frame-only items stored after the nwork items
   bool pair = false;         // the configuration runs two RK4 stages per interior launch
   int par = 0;               // 1: a two-stage step left the state in the Y0 pool

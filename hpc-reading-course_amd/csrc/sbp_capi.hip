@@ -242,6 +242,9 @@ template <typename T> PairArgs<T> make_pair_args(const sbp_ctx *c, int pair, dou
   a.ty_rows = c->p_ty_rows;
   a.n_tiles = c->p_n_tiles;
   a.n_edge_blocks = c->p_n_edge_blocks;
+  // bytes a two-stage step streams: U, Y2, Y3, materials, compensation
+  const double streamed = (double)c->plane * c->esz * (9 + (c->rinv ? 2 : 0) + (c->KC ? 3 : 0));
+  a.nts = pair_cached_stores(streamed) ? 0 : 1;
   if (pair == 1) {  // stages 1-2: U -> Y2 (Y0), Y3 (Y1)
     a.yin = static_cast<const T *>(c->U);
     a.o1 = static_cast<T *>(c->Y0);

@@ -599,7 +599,18 @@ template <typename T> struct PairArgs {
   T hix, hiy;
   T c1, c2, third;  // pair 1: dt/2, dt/2;  pair 2: dt, dt/6
   int tiles_x, ty_rows, n_tiles, n_edge_blocks;
+  int nts;  // 1: nontemporal output stores; 0: cached (pair_cached_stores); selects the kernel
 };
+
+// Output stores of the register-window tiles: cached when the arrays a step
+// streams (U, Y2, Y3 and the materials) fit in the 256 MB Infinity Cache, so
+// the next launch reads them from there; nontemporal otherwise, where cached
+// stores only evict what the launch still reads.  Measured on gfx950 (ms or
+// us per step, nontemporal -> cached): 1025^2 FP64 0.0695 -> 0.0657, 16 x
+// 513^2 FP32 ensemble 122 -> 111 us; but 2049^2 FP64 0.176 -> 0.186, 4097^2
+// FP32 0.344 -> 0.363, 16 x 513^2 FP64 ensemble 211 -> 239 us.
+constexpr double kCachedStoreBytes = 256.0 * 1024 * 1024;
+inline bool pair_cached_stores(double streamed_bytes) { return streamed_bytes <= kCachedStoreBytes; }
 
 template <typename T> struct PairRow {
   T in[3];        // stencil operand (u, v, p) at input row r, mid column
@@ -802,7 +813,7 @@ __device__ __forceinline__ void pair_tile(const PairArgs<T> &a, int tile)
 // LDS rows are single-buffered: each row written in one phase is read in the
 // next, and the barrier between them also separates the read from the next
 // iteration's write.
-template <typename T, int ORDER, int MAT, int PAIR, bool KAHAN, int NT, int PF>
+template <typename T, int ORDER, int MAT, int PAIR, bool KAHAN, int NT, int PF, bool NTS>
 __device__ __forceinline__ void pair_tile_rw(const PairArgs<T> &a, int tile)
 {
   using O = Ops<ORDER>;
@@ -831,6 +842,9 @@ __device__ __forceinline__ void pair_tile_rw(const PairArgs<T> &a, int tile)
   const long col = a.xoff + mcc, hcol = a.xoff + hc;
   const int jmax = a.ny - 1;
   const T *__restrict__ yin = a.yin;
+  // (a template parameter: stores that differ only in the nontemporal hint
+  // on the two sides of a runtime branch are merged into one plain store)
+  auto put = [&](T *p, T v) { stg<NTS>(p, v); };
 
   auto fetch = [&](PairRow<T> &q, int k) {
     const int r = min(j0 - 2 * R + k, jmax);
@@ -967,8 +981,8 @@ __device__ __forceinline__ void pair_tile_rw(const PairArgs<T> &a, int tile)
         if constexpr (PAIR == 1) {
 #pragma unroll
           for (int c = 0; c < 3; c++) {
-            stg<true>(a.o1 + c * plane + qo, wmid[c][R]);
-            stg<true>(a.o2 + c * plane + qo, win[c][0] + a.c2 * F[c]);
+            put(a.o1 + c * plane + qo, wmid[c][R]);
+            put(a.o2 + c * plane + qo, win[c][0] + a.c2 * F[c]);
           }
         } else {
           const bool ring = mc - D < RW || a.nx - D - 1 - mc < RW || o - D < RW || a.ny - D - 1 - o < RW;
@@ -978,12 +992,12 @@ __device__ __forceinline__ void pair_tile_rw(const PairArgs<T> &a, int tile)
             const T Y4 = wmid[c][R];
             const T inc = ((cur.Y2[c] - U) + 2 * (win[c][0] - U) + (Y4 - U)) * a.third + a.c2 * F[c];
             if constexpr (!KAHAN) {
-              stg<true>(a.o1 + c * plane + qo, U + inc);
+              put(a.o1 + c * plane + qo, U + inc);
             } else {
               const T y = inc - cur.C[c];
               const T tt = U + y;
-              stg<true>(a.comp + c * plane + qo, (tt - U) - y);
-              stg<true>(a.o1 + c * plane + qo, tt);
+              put(a.comp + c * plane + qo, (tt - U) - y);
+              put(a.o1 + c * plane + qo, tt);
             }
             if (ring) a.o2[c * plane + qo] = Y4;
           }
@@ -1011,7 +1025,7 @@ constexpr int pair_pf(int esz, int pf) { return esz == 4 ? pf + 1 : pf; }
 
 // Blocks [0, n_edge_blocks): stage 1 (pair 1) or stage 3 (pair 2) of the
 // D-deep frame (ea); the rest: two-stage interior tiles (a), XCD-remapped.
-template <typename T, int ORDER, int MAT, int PAIR, bool KAHAN, int NT, int PF>
+template <typename T, int ORDER, int MAT, int PAIR, bool KAHAN, int NT, int PF, bool NTS>
 __global__ __launch_bounds__(NT) void pair2d_kernel(StageArgs<T> ea, PairArgs<T> a)
 {
   const int b = blockIdx.x;
@@ -1025,7 +1039,7 @@ __global__ __launch_bounds__(NT) void pair2d_kernel(StageArgs<T> ea, PairArgs<T>
   const int q = nt / kNumXcd, r = nt % kNumXcd;
   const int tile = xcd * q + min(xcd, r) + local;
   if constexpr (SBP_PAIR_RW)
-    pair_tile_rw<T, ORDER, MAT, PAIR, KAHAN, NT, PF>(a, tile);
+    pair_tile_rw<T, ORDER, MAT, PAIR, KAHAN, NT, PF, NTS>(a, tile);
   else
     pair_tile<T, ORDER, MAT, PAIR, KAHAN, NT, PF>(a, tile);
 }
@@ -1290,15 +1304,20 @@ template <typename T, int ORDER, int MAT, bool KAHAN, int NT, int PF1, int PF2> 
     const StageArgs<T> &e = *static_cast<const StageArgs<T> *>(ea);
     switch (part) {
       case 1:
-        hipLaunchKernelGGL((pair2d_kernel<T, ORDER, MAT, 1, KAHAN, NT, PF1>), dim3(grid), dim3(NT), 0, s, e,
-                           *static_cast<const PairArgs<T> *>(pa));
+      case 3: {
+        const PairArgs<T> &a = *static_cast<const PairArgs<T> *>(pa);
+        if (part == 1 && a.nts)
+          hipLaunchKernelGGL((pair2d_kernel<T, ORDER, MAT, 1, KAHAN, NT, PF1, true>), dim3(grid), dim3(NT), 0, s, e, a);
+        else if (part == 1)
+          hipLaunchKernelGGL((pair2d_kernel<T, ORDER, MAT, 1, KAHAN, NT, PF1, false>), dim3(grid), dim3(NT), 0, s, e, a);
+        else if (a.nts)
+          hipLaunchKernelGGL((pair2d_kernel<T, ORDER, MAT, 2, KAHAN, NT, PF2, true>), dim3(grid), dim3(NT), 0, s, e, a);
+        else
+          hipLaunchKernelGGL((pair2d_kernel<T, ORDER, MAT, 2, KAHAN, NT, PF2, false>), dim3(grid), dim3(NT), 0, s, e, a);
         break;
+      }
       case 2:
         hipLaunchKernelGGL((frame2d_kernel<T, ORDER, MAT, MODE_S2, KAHAN>), dim3(grid), dim3(kFrameBX), 0, s, e);
-        break;
-      case 3:
-        hipLaunchKernelGGL((pair2d_kernel<T, ORDER, MAT, 2, KAHAN, NT, PF2>), dim3(grid), dim3(NT), 0, s, e,
-                           *static_cast<const PairArgs<T> *>(pa));
         break;
       default:
         hipLaunchKernelGGL((frame2d_kernel<T, ORDER, MAT, MODE_S4, KAHAN>), dim3(grid), dim3(kFrameBX), 0, s, e);
