@@ -213,3 +213,18 @@ def test_fused_rejects_unsupported(capi):
     with capi.Context(spec) as ctx:
         with pytest.raises(capi.SbpError):
             ctx.set_fused(True)
+
+
+def test_fused_nontemporal_store_kernels(capi):
+    """2049^2 order 4 FP64 with rho/K arrays: the arrays a two-stage step
+    streams exceed the 256 MB Infinity Cache, so the library picks the pair
+    kernels with nontemporal stores (the GPU tests' other grids run the
+    cached-store ones).  5 two-stage steps are bit-identical to the
+    one-stage path."""
+    spec = material_spec(2049, 4, P.BC_WALL)
+    say("fused nontemporal 2049^2")
+    v = state(spec)
+    dt = P.dt_from_cfl(spec, 0.25)
+    fused = run(capi, spec, v, dt, 5, True)
+    plain = run(capi, spec, v, dt, 5, False)
+    assert np.array_equal(fused, plain), relerr(fused, plain)
