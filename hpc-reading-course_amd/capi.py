@@ -34,7 +34,10 @@ class SbpProblem(C.Structure):
                 ("xl", C.c_double), ("xr", C.c_double), ("yl", C.c_double), ("yr", C.c_double),
                 ("ax", C.c_double), ("ay", C.c_double),
                 ("material", C.c_int), ("bc", C.c_int), ("precision", C.c_int),
-                ("rho_inv", _DP), ("bulk", _DP)]
+                ("rho_inv", _DP), ("bulk", _DP), ("vel_x", _DP), ("vel_y", _DP)]
+
+
+_ARRAY_FIELDS = ("rho_inv", "bulk", "vel_x", "vel_y")  # copied by sbp_create / sbp_batch_create
 
 
 class SbpError(RuntimeError):
@@ -147,18 +150,7 @@ class Context:
         lib = load_library()
         self.spec = spec
         self._keep = []
-        s = SbpProblem()
-        for f, _ in SbpProblem._fields_:
-            if f in ("rho_inv", "bulk"):
-                arr = getattr(spec, f)
-                if arr is None:
-                    setattr(s, f, None)
-                else:
-                    a = np.ascontiguousarray(arr, dtype=np.float64)
-                    self._keep.append(a)
-                    setattr(s, f, _ptr(a))
-            else:
-                setattr(s, f, getattr(spec, f))
+        s = _problem_struct(spec, self._keep)
         h = C.c_void_p()
         _check(lib.sbp_create(C.byref(s), C.byref(h)))
         self._h = h
@@ -275,8 +267,8 @@ class Context:
 def _problem_struct(spec, keep):
     s = SbpProblem()
     for f, _ in SbpProblem._fields_:
-        if f in ("rho_inv", "bulk"):
-            arr = getattr(spec, f)
+        if f in _ARRAY_FIELDS:
+            arr = getattr(spec, f, None)
             if arr is None:
                 setattr(s, f, None)
             else:

@@ -161,6 +161,8 @@ StageArgs<T> make_args(const sbp_ctx *c, int mode, const void *yin, double dt, d
   a.tk = static_cast<T *>(c->TK);
   a.rinv = static_cast<const T *>(c->rinv);
   a.bulk = static_cast<const T *>(c->bulk);
+  a.velx = static_cast<const T *>(c->velx);
+  a.vely = static_cast<const T *>(c->vely);
   a.xs = c->xs;
   a.ys = c->ys;
   a.c3x = c->c3x;
@@ -499,6 +501,9 @@ int validate(const sbp_problem *d)
   } else if (d->bc != SBP_BC_DEFAULT) {
     return fail(SBP_EINVAL, "boundary kinds other than default are acoustic only");
   }
+  if ((d->vel_x || d->vel_y) && d->physics != SBP_ADVECTION_1D && d->physics != SBP_ADVECTION_2D)
+    return fail(SBP_EINVAL, "velocity fields are for advection problems only");
+  if (d->vel_y && d->physics == SBP_ADVECTION_1D) return fail(SBP_EINVAL, "1D advection has no y-velocity");
   if ((long)d->nx * d->ny > (1L << 30)) return fail(SBP_EINVAL, "grid too large");
   return SBP_OK;
 }
@@ -543,8 +548,8 @@ void release(sbp_ctx *c)
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   drop_graph(c);
-  void *ptrs[] = {c->U, c->A, c->Y0, c->Y1, c->KC, c->TK, c->rinv, c->bulk, c->xs, c->ys, c->c3x, c->s3x,
-                  c->c4y, c->s4y, c->aos, c->aos2, c->partials, c->scalar};
+  void *ptrs[] = {c->U,   c->A,   c->Y0,  c->Y1,  c->KC,  c->TK,  c->rinv,     c->bulk,     c->velx, c->vely,
+                  c->xs,  c->ys,  c->c3x, c->s3x, c->c4y, c->s4y, c->aos,      c->aos2,     c->partials, c->scalar};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   for (void *p : c->rkw)
@@ -563,11 +568,17 @@ int build_ctx(const sbp_problem *desc, sbp_ctx *c)
   c->d = *desc;
   c->d.rho_inv = nullptr;
   c->d.bulk = nullptr;
+  c->d.vel_x = nullptr;
+  c->d.vel_y = nullptr;
   c->phys = desc->physics;
   c->order = desc->order;
   c->cls = closure_of(c->order);
   c->C = components(c->phys);
   c->mat = c->phys == SBP_ACOUSTIC_2D ? desc->material : SBP_MAT_UNIT;
+  // 2D advection with velocity fields runs the kernels' per-point (M_ARR) form;
+  // 1D advection reads the field whenever its plane exists
+  const bool vel_fields = desc->vel_x || desc->vel_y;
+  if (c->phys == SBP_ADVECTION_2D && vel_fields) c->mat = SBP_MAT_ARRAYS;
   c->bc = desc->bc;
   c->prec = desc->precision;
   c->esz = c->prec == SBP_FP64 ? 8 : 4;
@@ -622,6 +633,24 @@ int build_ctx(const sbp_problem *desc, sbp_ctx *c)
     return rc;
   if (c->phys == SBP_ACOUSTIC_2D && c->mat == SBP_MAT_ARRAYS) {
     if ((rc = upload_plane(c, desc->rho_inv, &c->rinv)) || (rc = upload_plane(c, desc->bulk, &c->bulk))) return rc;
+  }
+  if (vel_fields) {
+    // a field not given is the constant ax / ay on every point
+    const size_t npts = (size_t)c->nx * c->ny;
+    std::vector<double> fx, fy;
+    const double *vx = desc->vel_x, *vy = desc->vel_y;
+    if (!vx) {
+      fx.assign(npts, desc->ax);
+      vx = fx.data();
+    }
+    if ((rc = upload_plane(c, vx, &c->velx))) return rc;
+    if (c->phys == SBP_ADVECTION_2D) {
+      if (!vy) {
+        fy.assign(npts, desc->ay);
+        vy = fy.data();
+      }
+      if ((rc = upload_plane(c, vy, &c->vely))) return rc;
+    }
   }
   if (c->phys == SBP_ACOUSTIC_2D && c->mat == SBP_MAT_REF_RHO) {
     // x = xl + i/hi as rho_inv and forcing_* compute it (wave_eq_rhs.h:43,52).

@@ -15,6 +15,7 @@ struct sbp_ctx {
   double hix = 0, hiy = 0;
   void *U = nullptr, *A = nullptr, *Y0 = nullptr, *Y1 = nullptr, *KC = nullptr, *TK = nullptr;
   void *rinv = nullptr, *bulk = nullptr;
+  void *velx = nullptr, *vely = nullptr;  // advection velocity fields (per-point a, b)
   double *xs = nullptr, *ys = nullptr, *c3x = nullptr, *s3x = nullptr, *c4y = nullptr, *s4y = nullptr;
   double *aos = nullptr, *aos2 = nullptr;  // AoS FP64 staging (aos2: second operand, on first use)
   double *partials = nullptr, *scalar = nullptr;

@@ -12,7 +12,10 @@ StageLaunchFn pick_launcher(int phys, int mat)
 {
   if (phys == PH_ADV1D) return &Launcher1D<T, ORDER, PH_ADV1D, KAHAN>::launch;
   if (phys == PH_REFL1D) return &Launcher1D<T, ORDER, PH_REFL1D, KAHAN>::launch;
-  if (phys == PH_ADV2D) return &Launcher2D<T, ORDER, PH_ADV2D, M_UNIT, KAHAN, BX>::launch;
+  if (phys == PH_ADV2D) {  // M_ARR: per-point velocity fields
+    if (mat == M_ARR) return &Launcher2D<T, ORDER, PH_ADV2D, M_ARR, KAHAN, BX>::launch;
+    return &Launcher2D<T, ORDER, PH_ADV2D, M_UNIT, KAHAN, BX>::launch;
+  }
   if (phys == PH_ACOUSTIC) {
     if (mat == M_UNIT) return &Launcher2D<T, ORDER, PH_ACOUSTIC, M_UNIT, KAHAN, BX>::launch;
     if (mat == M_ARR) return &Launcher2D<T, ORDER, PH_ACOUSTIC, M_ARR, KAHAN, BX>::launch;

@@ -63,6 +63,8 @@ template <typename T> struct StageArgs {
   T *tk;          // k2 register of the 1D stage sequence
   const T *rinv;  // rho^-1 plane (caller material arrays)
   const T *bulk;  // K plane
+  const T *velx;  // advection velocity a(i,j) plane (per-point velocity), or null
+  const T *vely;  // advection velocity b(i,j) plane (2D)
   const double *xs, *ys;     // x_i = xl + i/hix, y_j (reference rho and forcing)
   const double *c3x, *s3x;   // cos(3 pi x_i), sin(3 pi x_i)
   const double *c4y, *s4y;   // cos(4 pi y_j), sin(4 pi y_j)
@@ -324,12 +326,18 @@ __device__ __forceinline__ void edge_point(const StageArgs<T> &a, int e)
       if (j == ny - 1) K[2] += Kb * (hqy * v);
     }
   } else {  // 2D advection: advection_{ll..rr} + SAT_bc_{west..north}
+    // velocities at this point: per-point fields (MAT == M_ARR) or constants;
+    // every SAT takes tau from its own boundary point (advection_rhs.h:461-528)
+    T ax = a.ax, ay = a.ay;
+    if constexpr (MAT == M_ARR) {
+      ax = a.velx[o];
+      ay = a.vely[o];
+    }
     const T u = base[o];
     const T dx = dline<T, ORDER>(base + rowb, 1, nx, i, a.hix);
     const T dy = dline<T, ORDER>(base + colb, pitch, ny, j, a.hiy);
     Uc[0] = u;
-    T k = -(a.ax * dx + a.ay * dy);
-    const T ax = a.ax, ay = a.ay;
+    T k = -(ax * dx + ay * dy);
     if (i == 0) k += T(0.5) * (T(-0.5) * (ax + fabs(ax))) * (hqx * u);
     if (j == 0) k += T(0.5) * (T(-0.5) * (ay + fabs(ay))) * (hqy * u);
     if (i == nx - 1) k += T(0.5) * (T(0.5) * (ax - fabs(ax))) * (hqx * u);
@@ -391,7 +399,8 @@ __device__ __forceinline__ void interior_tile(const StageArgs<T> &a, int tile)
   constexpr bool LD_U = MODE >= MODE_S2;
   constexpr bool LD_Y = MODE == MODE_S4;
   constexpr bool LD_C = MODE == MODE_S4 && KAHAN;
-  constexpr bool LD_M = AC && MAT == M_ARR;
+  // materials (acoustics: rho^-1, K) or velocity fields (advection: a, b)
+  constexpr bool LD_M = MAT == M_ARR;
 
   __shared__ T lds[2][NF][LW];
 
@@ -444,8 +453,8 @@ __device__ __forceinline__ void interior_tile(const StageArgs<T> &a, int tile)
       if constexpr (LD_C) r.C[c] = ldg<POL::NTP>(a.comp + c * plane + ox + col);
     }
     if constexpr (LD_M) {
-      r.ri = ldg<POL::NTP>(a.rinv + ox + col);
-      r.K = ldg<POL::NTP>(a.bulk + ox + col);
+      r.ri = ldg<POL::NTP>((AC ? a.rinv : a.velx) + ox + col);
+      r.K = ldg<POL::NTP>((AC ? a.bulk : a.vely) + ox + col);
     }
   };
 #pragma unroll
@@ -503,7 +512,8 @@ __device__ __forceinline__ void interior_tile(const StageArgs<T> &a, int tile)
       Uc[1] = win[1][R];
       Uc[2] = win[0][R];
     } else {
-      K[0] = -(a.ax * d0x + a.ay * d0y);
+      const T ax = LD_M ? cur.ri : a.ax, ay = LD_M ? cur.K : a.ay;
+      K[0] = -(ax * d0x + ay * d0y);
       Uc[0] = win[0][R];
     }
 
@@ -1057,11 +1067,13 @@ __global__ __launch_bounds__(256) void stage1d_kernel(StageArgs<T> a)
   const T *f0 = a.yin + a.xoff;
   const T hq = a.hix * T(Ops<ORDER>::hqinv[0]);
   if constexpr (PHYS == PH_ADV1D) {
-    // advection_{l,i,r} (advection_rhs.h:13-51) + SAT_bc_{l,r} (:95-119)
+    // advection_{l,i,r} (advection_rhs.h:13-51) + SAT_bc_{l,r} (:95-119), with
+    // the velocity a(i) of this point (per-point field, or the constant)
+    const T av = a.velx ? a.velx[o] : a.ax;
     const T u = f0[i];
-    T k = -a.ax * dline<T, ORDER>(f0, 1, nx, i, a.hix);
-    if (i == 0) k += T(0.5) * (T(-0.5) * (a.ax + fabs(a.ax))) * (hq * u);
-    if (i == nx - 1) k += T(0.5) * (T(0.5) * (a.ax - fabs(a.ax))) * (hq * u);
+    T k = -av * dline<T, ORDER>(f0, 1, nx, i, a.hix);
+    if (i == 0) k += T(0.5) * (T(-0.5) * (av + fabs(av))) * (hq * u);
+    if (i == nx - 1) k += T(0.5) * (T(0.5) * (av - fabs(av))) * (hq * u);
     T K[1] = {k}, Uc[1] = {u};
     rk_point_exact<T, 1, MODE, KAHAN>(a, o, K, Uc);
   } else {

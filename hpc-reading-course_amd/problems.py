@@ -58,6 +58,8 @@ class ProblemSpec:
     precision: int = FP64
     rho_inv: Optional[np.ndarray] = None  # (ny, nx) float64
     bulk: Optional[np.ndarray] = None     # (ny, nx) float64
+    vel_x: Optional[np.ndarray] = None    # (ny, nx) advection a(i) / a(i,j) per point
+    vel_y: Optional[np.ndarray] = None    # (ny, nx) advection b(i,j) per point (2D)
 
     @property
     def components(self) -> int:
@@ -130,6 +132,43 @@ def adv2d_exact(spec: ProblemSpec, t: float, rstar: float = 0.1) -> np.ndarray:
     X, Y = np.meshgrid(spec.x(), spec.y())
     g = np.exp(-((X - spec.ax * t) ** 2 + (Y - spec.ay * t) ** 2) / (rstar * rstar))
     return g.reshape(-1).astype(np.float64)
+
+
+def adv1d_var(n: int, order: int = 4, width: float = 0.2, **kw):
+    """1D advection with the spatially varying velocity a(x) = x + 2 (the
+    AppCtx a(i) of advection_1D_sim.cpp:19,70 as a field) on [-1, 1]:
+    u_t + (x+2) u_x = 0, u0 = exp(-(x/width)^2).  a > 0 everywhere, so the
+    left end is the inflow boundary."""
+    spec = ProblemSpec(ADVECTION_1D, order, n, 1, -1.0, 1.0, ax=0.0, **kw)
+    spec.vel_x = (spec.x() + 2.0).reshape(1, n)
+    return spec, adv1d_var_exact(spec, 0.0, width)
+
+
+def adv1d_var_exact(spec: ProblemSpec, t: float, width: float = 0.2) -> np.ndarray:
+    """Characteristics x + 2 = (x0 + 2) e^t: u(x, t) = u0((x + 2) e^-t - 2).
+    At the inflow end x = -1 the exact solution is u0(e^-t - 2) <= exp(-25)
+    for width 0.2 and t >= 0, so the SAT's zero inflow data is consistent."""
+    x0 = (spec.x() + 2.0) * math.exp(-t) - 2.0
+    return np.exp(-(x0 / width) ** 2).astype(np.float64)
+
+
+def adv2d_var(n: int, order: int = 4, width: float = 0.2, **kw):
+    """2D advection with a(x, y) = x + 2 and b(x, y) = y - 2 per grid point
+    (the AppCtx a(i,j), b(i,j) of advection_2D_sim.cpp:21,74-75 as fields) on
+    [-1, 1]^2; inflow through the west and north boundaries."""
+    spec = ProblemSpec(ADVECTION_2D, order, n, n, -1.0, 1.0, -1.0, 1.0, ax=0.0, ay=0.0, **kw)
+    X, Y = np.meshgrid(spec.x(), spec.y())
+    spec.vel_x = X + 2.0
+    spec.vel_y = Y - 2.0
+    return spec, adv2d_var_exact(spec, 0.0, width)
+
+
+def adv2d_var_exact(spec: ProblemSpec, t: float, width: float = 0.2) -> np.ndarray:
+    """u(x, y, t) = u0((x + 2) e^-t - 2, 2 + (y - 2) e^-t)."""
+    X, Y = np.meshgrid(spec.x(), spec.y())
+    x0 = (X + 2.0) * math.exp(-t) - 2.0
+    y0 = 2.0 + (Y - 2.0) * math.exp(-t)
+    return np.exp(-(x0 * x0 + y0 * y0) / (width * width)).reshape(-1).astype(np.float64)
 
 
 def standing_mode(spec: ProblemSpec, t: float, n: int = 3, m: int = 4) -> np.ndarray:

@@ -68,6 +68,15 @@ typedef struct sbp_problem {
   int precision;        /* sbp_precision                                                  */
   const double *rho_inv;/* nx*ny values [j][i], SBP_MAT_ARRAYS only; copied at create    */
   const double *bulk;   /* nx*ny values [j][i] of K, SBP_MAT_ARRAYS only; copied         */
+  /* Advection velocity per grid point, nx*ny values [j][i] (1D: nx values), copied
+   * at create; NULL = the constant ax / ay above.  These are the AppCtx velocity
+   * functions a(i) (advection_1D_sim.cpp:19,70) and a(i,j), b(i,j)
+   * (advection_2D_sim.cpp:21,74-75) sampled on the grid: the RHS multiplies the
+   * derivatives by them point by point (advection_rhs.h:22,36,49,166-393) and
+   * each inflow SAT takes tau from the velocity at its own boundary point
+   * (advection_rhs.h:102,116,461-528).  Advection only. */
+  const double *vel_x;
+  const double *vel_y;  /* 2D only */
 } sbp_problem;
 
 /* Error codes. */

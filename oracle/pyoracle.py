@@ -2,7 +2,7 @@
 
 Problem descriptions are duck-typed: any object with the fields of
 ``sbp_problem`` (physics, order, nx, ny, xl, xr, yl, yr, ax, ay, material, bc,
-precision, rho_inv, bulk) is accepted.
+precision, rho_inv, bulk, vel_x, vel_y) is accepted; missing array fields are NULL.
 """
 from __future__ import annotations
 
@@ -20,7 +20,10 @@ class OrcProblem(C.Structure):
                 ("xl", C.c_double), ("xr", C.c_double), ("yl", C.c_double), ("yr", C.c_double),
                 ("ax", C.c_double), ("ay", C.c_double),
                 ("material", C.c_int), ("bc", C.c_int), ("precision", C.c_int),
-                ("rho_inv", C.POINTER(C.c_double)), ("bulk", C.POINTER(C.c_double))]
+                ("rho_inv", C.POINTER(C.c_double)), ("bulk", C.POINTER(C.c_double)),
+                ("vel_x", C.POINTER(C.c_double)), ("vel_y", C.POINTER(C.c_double))]
+
+_ARRAY_FIELDS = ("rho_inv", "bulk", "vel_x", "vel_y")
 
 
 class OrcOps(C.Structure):
@@ -75,7 +78,7 @@ class _Desc:
         self.keep = []
         s = OrcProblem()
         for f, _ in OrcProblem._fields_:
-            if f in ("rho_inv", "bulk"):
+            if f in _ARRAY_FIELDS:
                 arr = getattr(spec, f, None)
                 if arr is None:
                     setattr(s, f, None)

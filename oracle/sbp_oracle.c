@@ -111,6 +111,9 @@ int orc_validate(const sbp_problem *p)
   else if (p->ny < nmin) return -1;
   if (p->physics == SBP_ACOUSTIC_2D && p->material == SBP_MAT_ARRAYS &&
       (!p->rho_inv || !p->bulk)) return -1;
+  /* velocity fields: advection only, and no y-velocity in 1D */
+  if ((p->vel_x || p->vel_y) && p->physics != SBP_ADVECTION_1D && p->physics != SBP_ADVECTION_2D) return -1;
+  if (p->vel_y && p->physics == SBP_ADVECTION_1D) return -1;
   return 0;
 }
 
@@ -202,16 +205,28 @@ static inline double mat_bulk(const orc_setup *s, int i, int j)
   return 1.0;
 }
 
+/* Advection velocities at grid point (i, j): the AppCtx functions a(i) /
+ * a(i,j), b(i,j) (advection_1D_sim.cpp:19,70; advection_2D_sim.cpp:21,74-75)
+ * sampled on the grid, or the constants ax, ay. */
+static inline double vel_x(const orc_setup *s, int i, int j)
+{
+  return s->p->vel_x ? s->p->vel_x[(long)j * s->nx + i] : s->p->ax;
+}
+
+static inline double vel_y(const orc_setup *s, int i, int j)
+{
+  return s->p->vel_y ? s->p->vel_y[(long)j * s->nx + i] : s->p->ay;
+}
+
 /* ------------------------------------------------------------------------ */
 /* Region kernels over a block [i0,i1) x [j0,j1)                             */
 /* ------------------------------------------------------------------------ */
 
-/* 1D advection: advection_{l,i,r} (advection_rhs.h:13-51): dst = -a D1 u. */
+/* 1D advection: advection_{l,i,r} (advection_rhs.h:13-51): dst_i = -a(i) (D1 u)_i. */
 static void adv1d_block(const orc_setup *s, const double *src, double *dst, int i0, int i1)
 {
-  const double a = s->p->ax;
   for (int i = i0; i < i1; i++)
-    dst[i] = -a * orc_d1(&s->op, src, 1, s->nx, i, s->hix);
+    dst[i] = -vel_x(s, i, 0) * orc_d1(&s->op, src, 1, s->nx, i, s->hix);
 }
 
 /* 1D reflection: reflection_{l,i,r} (reflection_rhs.h:10-47):
@@ -225,12 +240,11 @@ static void refl_block(const orc_setup *s, const double *src, double *dst, int i
 }
 
 /* 2D advection: advection_{ll..rr} (advection_rhs.h:156-393):
- * dst = -(a_x D_x u + a_y D_y u). */
+ * dst = -(a_x(i,j) D_x u + a_y(i,j) D_y u). */
 static void adv2d_block(const orc_setup *s, const double *src, double *dst,
                         int i0, int i1, int j0, int j1)
 {
   const int nx = s->nx, ny = s->ny;
-  const double ax = s->p->ax, ay = s->p->ay;
   for (int j = j0; j < j1; j++) {
     const int ky = region(&s->op, ny, j);
     const double *row = src + (long)nx * j;
@@ -239,7 +253,7 @@ static void adv2d_block(const orc_setup *s, const double *src, double *dst,
       const int kx = region(&s->op, nx, i);
       double dx = d1_kind(&s->op, row, 1, nx, i, s->hix, kx);
       double dy = d1_kind(&s->op, col0 + i, nx, ny, j, s->hiy, ky);
-      dst[(long)nx * j + i] = -(ax * dx + ay * dy);
+      dst[(long)nx * j + i] = -(vel_x(s, i, j) * dx + vel_y(s, i, j) * dy);
     }
   }
 }
@@ -288,20 +302,21 @@ static void acoustic_block(const orc_setup *s, double t, const double *src, doub
 /* Boundary conditions on the part of the boundary inside a block            */
 /* ------------------------------------------------------------------------ */
 
-/* SAT_bc_l / SAT_bc_r (advection_rhs.h:95-119).  The right SAT is written
- * with the correct inverse weight closure_invquad[0]; the reference reads
- * closure_invquad[nx-1] there but multiplies it by tau = 0 for a >= 0. */
+/* SAT_bc_l / SAT_bc_r (advection_rhs.h:95-119), tau from the velocity at
+ * the boundary point itself.  The right SAT is written with the correct
+ * inverse weight closure_invquad[0]; the reference reads
+ * closure_invquad[nx-1] there but multiplies it by tau = 0 for a(nx-1) >= 0. */
 static void adv1d_bc(const orc_setup *s, const double *src, double *dst, int i0, int i1)
 {
   const int nx = s->nx;
   const double hq = s->hix * s->op.hqinv[0];
   if (i0 == 0) {
-    double a_l = s->p->ax;
+    double a_l = vel_x(s, 0, 0);
     double tau = -0.5 * (a_l + fabs(a_l));
     dst[0] += 0.5 * tau * (hq * src[0]);
   }
   if (i1 == nx) {
-    double a_r = s->p->ax;
+    double a_r = vel_x(s, nx - 1, 0);
     double tau = 0.5 * (a_r - fabs(a_r));
     dst[nx - 1] += 0.5 * tau * (hq * src[nx - 1]);
   }
@@ -315,34 +330,38 @@ static void refl_bc(const orc_setup *s, double *dst, int i0, int i1)
 }
 
 /* SAT_bc_{west,south,east,north} (advection_rhs.h:461-528), applied in the
- * bc_serial order W, S, E, N (boundary_conditions.h:84-87).  The south SAT
- * uses the y-spacing and closure_invquad[0]; the reference uses
- * hi[0]*closure_invquad[i] there, multiplied by tau_s = 0 for b <= 0. */
+ * bc_serial order W, S, E, N (boundary_conditions.h:84-87); each boundary
+ * point takes tau from the velocity at that point (a_x on W/E, a_y on S/N).
+ * The south SAT uses the y-spacing and closure_invquad[0]; the reference
+ * uses hi[0]*closure_invquad[i] there, multiplied by tau_s = 0 for b <= 0. */
 static void adv2d_bc(const orc_setup *s, const double *src, double *dst,
                      int i0, int i1, int j0, int j1)
 {
   const int nx = s->nx, ny = s->ny;
   const double hqx = s->hix * s->op.hqinv[0], hqy = s->hiy * s->op.hqinv[0];
-  const double ax = s->p->ax, ay = s->p->ay;
   if (i0 == 0)
     for (int j = j0; j < j1; j++) {
-      double tau = -0.5 * (ax + fabs(ax));
+      double a_w = vel_x(s, 0, j);
+      double tau = -0.5 * (a_w + fabs(a_w));
       dst[(long)nx * j] += 0.5 * tau * (hqx * src[(long)nx * j]);
     }
   if (j0 == 0)
     for (int i = i0; i < i1; i++) {
-      double tau = -0.5 * (ay + fabs(ay));
+      double a_s = vel_y(s, i, 0);
+      double tau = -0.5 * (a_s + fabs(a_s));
       dst[i] += 0.5 * tau * (hqy * src[i]);
     }
   if (i1 == nx)
     for (int j = j0; j < j1; j++) {
-      double tau = 0.5 * (ax - fabs(ax));
+      double a_e = vel_x(s, nx - 1, j);
+      double tau = 0.5 * (a_e - fabs(a_e));
       long k = (long)nx * j + nx - 1;
       dst[k] += 0.5 * tau * (hqx * src[k]);
     }
   if (j1 == ny)
     for (int i = i0; i < i1; i++) {
-      double tau = 0.5 * (ay - fabs(ay));
+      double a_n = vel_y(s, i, ny - 1);
+      double tau = 0.5 * (a_n - fabs(a_n));
       long k = (long)nx * (ny - 1) + i;
       dst[k] += 0.5 * tau * (hqy * src[k]);
     }
