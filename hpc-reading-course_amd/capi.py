@@ -6,8 +6,10 @@ Mirrors the reference's time-stepping interface for this path:
   the ``PetscErrorCode rhs(TS, PetscReal t, Vec v_src, Vec v_dst, void *ctx)``
   callbacks (e.g. demo/wave/wave_eq_sim.cpp:291-308);
 * ``ts_rk4(ctx, t_end, dt, v)`` advances ``v`` in place to ``t_end`` with
-  classical RK4, full steps of ``dt`` and a final clipped step, the
-  TS_EXACTFINALTIME_MATCHSTEP behaviour of ts_rk4 (src/time_stepping/ts_rk.cpp:15-39,54-65).
+  classical RK4 under PETSc's TS_EXACTFINALTIME_MATCHSTEP step schedule, as
+  ts_rk4 does (src/time_stepping/ts_rk.cpp:15-39,54-65);
+* ``rk4_custom(ctx, t_end, dt, v)`` is the reference's own RK4_custom loop
+  (include/time_stepping/time_stepping.h:98-152).
 
 Vectors are float64 numpy arrays in the reference AoS layout v[j][i][c].
 The library must be built (``build.py``); nothing here falls back to CPU code.
@@ -62,6 +64,11 @@ EXPORTS = {
     "sbp_step": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_long]),
     "sbp_rk45": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, C.c_long,
                            C.POINTER(C.c_long), C.POINTER(C.c_long), _DP]),
+    "sbp_ts_rk4": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_double, C.POINTER(C.c_long), _DP]),
+    "sbp_matchstep_schedule": (C.c_int, [C.c_double, C.c_double, C.c_double, C.c_int, _DP, C.POINTER(C.c_long),
+                                         C.POINTER(C.c_int), _DP]),
+    "sbp_rk4_custom": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.POINTER(C.c_long), _DP, _DP]),
+    "sbp_adapt_basic": (C.c_int, [C.c_double, C.c_double, C.c_int, C.c_int, C.POINTER(C.c_int), _DP]),
     "sbp_synchronize": (C.c_int, [C.c_void_p]),
     "sbp_time_steps": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_long, _DP]),
     "sbp_time_stages": (C.c_int, [C.c_void_p, C.c_double, C.c_long, _DP]),
@@ -201,6 +208,18 @@ class Context:
 
     def synchronize(self):
         _check(load_library().sbp_synchronize(self._h))
+
+    def ts_rk4(self, t0: float, t_end: float, dt: float):
+        """TSRK4 + MATCHSTEP of the resident state; returns (steps, final time)."""
+        n, tf = C.c_long(), C.c_double()
+        _check(load_library().sbp_ts_rk4(self._h, t0, t_end, dt, C.byref(n), C.byref(tf)))
+        return n.value, tf.value
+
+    def rk4_custom(self, t_end: float, dt: float):
+        """RK4_custom's loop on the resident state; returns (tlen, dt used, final t)."""
+        n, h, tf = C.c_long(), C.c_double(), C.c_double()
+        _check(load_library().sbp_rk4_custom(self._h, t_end, dt, C.byref(n), C.byref(h), C.byref(tf)))
+        return n.value, h.value, tf.value
 
     def rk45(self, t0: float, t_end: float, dt0: float, atol: float = 1e-4, rtol: float = 1e-4,
              max_steps: int = 1000000):
@@ -500,14 +519,47 @@ def rhs(ctx: Context, t: float, v_src: np.ndarray, v_dst: np.ndarray) -> int:
 
 
 def ts_rk4(ctx: Context, t_end: float, dt: float, v: np.ndarray, t0: float = 0.0) -> float:
-    """Advance v in place from t0 to t_end (MATCHSTEP); returns the final time."""
-    n, last = P.matchstep_schedule(t_end, dt, t0)
+    """Advance v in place from t0 to t_end with TSRK4 under
+    TS_EXACTFINALTIME_MATCHSTEP (PETSc's step schedule, sbp_ts_rk4); returns
+    the final time."""
     ctx.set_state(v)
-    ctx.step(t0, dt, n)
-    if last:
-        ctx.step(t0 + n * dt, last, 1)
+    _, tf = ctx.ts_rk4(t0, t_end, dt)
     v.reshape(-1)[:] = ctx.get_state()
-    return t_end if (n or last) else t0
+    return tf
+
+
+def rk4_custom(ctx: Context, t_end: float, dt: float, v: np.ndarray, verbose: bool = True):
+    """RK4_custom(da, Tend, dt, v, rhs, ctx) (time_stepping.h:98-152): v advanced
+    in place over round(Tend/dt) steps from t = 0, with the reference's prints;
+    returns (tlen, dt used, final t)."""
+    tlen = int(round(t_end / dt))
+    if verbose and abs(tlen * dt - t_end) > 1e-14:
+        print("Warning: Non-matching time step. dt*tlen = %.12f.\nChanging timestep from %e to %e."
+              % (dt * tlen, dt, t_end / tlen))
+    ctx.set_state(v)
+    n, h, tf = ctx.rk4_custom(t_end, dt)
+    if verbose:
+        print("tlen: %d, dt: %e" % (n, h))
+        print("Final t: %.8e" % tf)
+    v.reshape(-1)[:] = ctx.get_state()
+    return n, h, tf
+
+
+def matchstep_schedule(t_end: float, dt: float, t0: float = 0.0, max_runs: int = 64):
+    """ts_rk4's step sizes from t0 to t_end (host only): [(dt, count), ...]."""
+    rd = np.empty(max_runs)
+    rl = np.empty(max_runs, dtype=np.int64)
+    nr, tf = C.c_int(), C.c_double()
+    _check(load_library().sbp_matchstep_schedule(t0, t_end, dt, max_runs, _ptr(rd),
+                                                 rl.ctypes.data_as(C.POINTER(C.c_long)), C.byref(nr), C.byref(tf)))
+    return [(float(rd[k]), int(rl[k])) for k in range(nr.value)]
+
+
+def adapt_basic(enorm: float, h: float, order: int = 5, prev_rejected: bool = False):
+    """The library's TSAdaptChoose_Basic decision (host only): (accept, next_h)."""
+    a, nh = C.c_int(), C.c_double()
+    _check(load_library().sbp_adapt_basic(enorm, h, order, int(prev_rejected), C.byref(a), C.byref(nh)))
+    return bool(a.value), nh.value
 
 
 def ts_rk45(ctx: Context, t_end: float, dt: float, v: np.ndarray, t0: float = 0.0, atol: float = 1e-4,

@@ -261,7 +261,7 @@ hipError_t launch_step(sbp_batch *b)
       e = b->launch(p | (b->pair_nts ? 0 : kBatchCachedStores), inst, frame_only ? b->work_dev + b->nwork : b->work_dev,
                     frame_only ? b->nwork_frame : b->nwork, b->grid, b->stream);
     }
-    b->par ^= 1;
+    if (e == hipSuccess) b->par ^= 1;  // the new states sit in the other pool only if every part went in
     return e;
   }
   for (int m = MODE_S1; m <= MODE_S4 && e == hipSuccess; m++)
@@ -273,11 +273,13 @@ hipError_t launch_step(sbp_batch *b)
 int ensure_graph(sbp_batch *b)
 {
   if (b->gexec[b->par]) return SBP_OK;
+  const int par0 = b->par;  // captured steps flip the parity as if they had run
   hipGraph_t g = nullptr;
   HIP_OR_FAIL(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
   hipError_t le = hipSuccess;
   for (int s = 0; s < kBatchGraphSteps && le == hipSuccess; s++) le = launch_step(b);
   hipError_t ce = hipStreamEndCapture(b->stream, &g);
+  b->par = par0;
   if (le != hipSuccess || ce != hipSuccess) {
     if (g) (void)hipGraphDestroy(g);
     return fail(SBP_EHIP, std::string("batch graph capture: ") + hipGetErrorString(le != hipSuccess ? le : ce));

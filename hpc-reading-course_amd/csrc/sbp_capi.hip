@@ -18,6 +18,7 @@
 #include "sbp_ctx.hpp"
 #include "sbp_host.hpp"
 #include "sbp_kernels.hpp"
+#include "sbp_ts.hpp"
 
 using namespace sbp;
 
@@ -288,7 +289,8 @@ template <typename T> hipError_t launch_pair_part_t(sbp_ctx *c, int part, double
   }
   // frame-only launch: one wave per block
   const hipError_t err = c->pair_launch(part, &e, nullptr, (c->p_n_edge + kFrameBX - 1) / kFrameBX, c->stream);
-  if (part == 4) std::swap(c->U, c->Y0);
+  // the step is complete (or captured) only when its last launch went in
+  if (part == 4 && err == hipSuccess) std::swap(c->U, c->Y0);
   return err;
 }
 
@@ -341,7 +343,9 @@ int ensure_graph(sbp_ctx *c, double dt, hipGraphExec_t *out)
   c->g_next ^= 1;
   if (slot.exec) (void)hipGraphExecDestroy(slot.exec);
   slot = sbp_ctx::GraphSlot{};
-  void *u0 = c->U;
+  // Captured two-stage steps swap the U and Y0 roles as if they had run; a
+  // capture that fails part-way must leave the roles as they were.
+  void *u0 = c->U, *y0 = c->Y0;
   hipGraph_t g = nullptr;
   HIP_OR_FAIL(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   hipError_t le = hipSuccess;
@@ -349,6 +353,8 @@ int ensure_graph(sbp_ctx *c, double dt, hipGraphExec_t *out)
   hipError_t ce = hipStreamEndCapture(c->stream, &g);
   if (le != hipSuccess || ce != hipSuccess) {
     if (g) (void)hipGraphDestroy(g);
+    c->U = u0;
+    c->Y0 = y0;
     return fail(SBP_EHIP, std::string("graph capture: ") + hipGetErrorString(le != hipSuccess ? le : ce));
   }
   hipError_t ie = hipGraphInstantiate(&slot.exec, g, nullptr, nullptr, 0);
@@ -357,14 +363,21 @@ int ensure_graph(sbp_ctx *c, double dt, hipGraphExec_t *out)
     slot.exec = nullptr;
     return fail(SBP_EHIP, std::string("graph instantiate: ") + hipGetErrorString(ie));
   }
-  if (c->U != u0) return fail(SBP_EHIP, "graph capture left the state in another array");
+  if (c->U != u0) {
+    c->U = u0;
+    c->Y0 = y0;
+    return fail(SBP_EHIP, "graph capture left the state in another array");
+  }
   slot.dt = dt;
   slot.u = u0;
   *out = slot.exec;
   return SBP_OK;
 }
 
-int run_steps(sbp_ctx *c, double t0, double dt, long nsteps)
+// nsteps steps of dt from t0; *t_end (optional) receives the time the steps
+// accumulate to, t = t + dt per step as both integrators of the reference
+// advance it (RK4_custom time_stepping.h:141; PETSc's ptime += time_step).
+int run_steps(sbp_ctx *c, double t0, double dt, long nsteps, double *t_end = nullptr)
 {
   long n = 0;
   if (!time_dependent(c) && nsteps >= kGraphSteps) {
@@ -380,6 +393,7 @@ int run_steps(sbp_ctx *c, double t0, double dt, long nsteps)
     HIP_OR_FAIL(launch_step(c, t, dt));
     t = t + dt;
   }
+  if (t_end) *t_end = t;
   return SBP_OK;
 }
 
@@ -854,6 +868,67 @@ int sbp_step(sbp_ctx *ctx, double t0, double dt, long nsteps)
   if (!ctx || nsteps < 0) return fail(SBP_EINVAL, "bad argument");
   if (!ctx->have_state) return fail(SBP_ESTATE, "no state loaded");
   return run_steps(ctx, t0, dt, nsteps);
+}
+
+int sbp_matchstep_schedule(double t0, double t_end, double dt, int max_runs, double *run_dt, long *run_len,
+                           int *nruns, double *t_final)
+{
+  if (!nruns || max_runs < 0 || !(dt > 0) || !std::isfinite(t0) || !std::isfinite(t_end))
+    return fail(SBP_EINVAL, "bad time span or step");
+  const int n = ts::matchstep_runs(t0, t_end, dt, max_runs, run_dt, run_len, t_final);
+  if (n < 0) return fail(SBP_EINVAL, "the schedule needs more than max_runs runs");
+  *nruns = n;
+  return SBP_OK;
+}
+
+int sbp_ts_rk4(sbp_ctx *ctx, double t0, double t_end, double dt, long *nsteps, double *t_final)
+{
+  if (!ctx) return fail(SBP_EINVAL, "null context");
+  if (!ctx->have_state) return fail(SBP_ESTATE, "no state loaded");
+  if (!(dt > 0) || !std::isfinite(t0) || !std::isfinite(t_end)) return fail(SBP_EINVAL, "bad time span or step");
+  // A fixed-step MATCHSTEP run changes its step at most twice, at the end.
+  constexpr int kMaxRuns = 8;
+  double rdt[kMaxRuns];
+  long rlen[kMaxRuns];
+  const int nr = ts::matchstep_runs(t0, t_end, dt, kMaxRuns, rdt, rlen, nullptr);
+  if (nr < 0) return fail(SBP_EINVAL, "unexpected step schedule");
+  double t = t0;
+  long total = 0;
+  for (int k = 0; k < nr; k++) {
+    int rc = run_steps(ctx, t, rdt[k], rlen[k], &t);
+    if (rc) return rc;
+    total += rlen[k];
+  }
+  if (nsteps) *nsteps = total;
+  if (t_final) *t_final = t;
+  return SBP_OK;
+}
+
+int sbp_rk4_custom(sbp_ctx *ctx, double t_end, double dt, long *tlen, double *dt_used, double *t_final)
+{
+  if (!ctx) return fail(SBP_EINVAL, "null context");
+  if (!ctx->have_state) return fail(SBP_ESTATE, "no state loaded");
+  if (!(dt > 0) || !(t_end > 0) || !std::isfinite(t_end)) return fail(SBP_EINVAL, "bad time span or step");
+  double h = dt;
+  const long n = ts::rk4_custom_steps(t_end, dt, &h, nullptr);
+  if (n < 1) return fail(SBP_EINVAL, "round(Tend/dt) is zero");
+  double t = 0.0;
+  int rc = run_steps(ctx, 0.0, h, n, &t);
+  if (rc) return rc;
+  if (tlen) *tlen = n;
+  if (dt_used) *dt_used = h;
+  if (t_final) *t_final = t;
+  return SBP_OK;
+}
+
+int sbp_adapt_basic(double enorm, double h, int order, int prev_rejected, int *accept, double *next_h)
+{
+  if (!accept || !next_h || order < 1 || !(h > 0)) return fail(SBP_EINVAL, "bad argument");
+  if (!std::isfinite(enorm)) return fail(SBP_EDIVERGED, "non-finite error estimate");
+  const ts::Choice ch = ts::adapt_basic(enorm, h, order, prev_rejected != 0);
+  *accept = ch.accept ? 1 : 0;
+  *next_h = ch.next_h;
+  return SBP_OK;
 }
 
 int sbp_synchronize(sbp_ctx *ctx)

@@ -1363,11 +1363,13 @@ int sbp_corr_advance_batch(sbp_corrector *c, sbp_batch *b, const double *dt, lon
   if (nsteps >= 2 * kGraphSteps) {
     hipGraph_t g = nullptr;
     hipGraphExec_t ge = nullptr;
+    const int par0 = b->par;  // captured two-stage steps flip the batch's parity as if they had run
     HIP_OR_FAIL(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     hipError_t le = hipSuccess;
     for (long k = 0; k < kGraphSteps && le == hipSuccess; k++) le = enqueue_step();
     hipError_t ce = hipStreamEndCapture(s, &g);
     if (le != hipSuccess || ce != hipSuccess) {
+      b->par = par0;
       if (g) (void)hipGraphDestroy(g);
       return fail(SBP_EHIP, std::string("corrector graph capture: ") + hipGetErrorString(le != hipSuccess ? le : ce));
     }

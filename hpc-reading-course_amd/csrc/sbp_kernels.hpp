@@ -584,12 +584,8 @@ __global__ __launch_bounds__(BX) void stage2d_kernel(StageArgs<T> a)
 //
 // A block of NT threads owns NT mid columns (x0-R .. x0-R+NT-1) and the
 // NT-2R output columns inside them, and marches down the rows: iteration k
-// takes input row r = j0-2R+k, computes mid row r-R, then output row r-2R.
-// Two forms: pair_tile keeps the rows in LDS rings (ring depths 2R+2 for the
-// input and 2R+1 for the mid stage keep every slot a row reads alive until
-// the barrier that precedes its overwrite); pair_tile_rw (the default,
-// SBP_PAIR_RW) keeps each thread's column in register windows and passes
-// only the rows being x-differentiated through LDS.  Two barriers per row.
+// takes input row r = j0-2R+k, computes mid row r-R, then output row r-2R
+// (pair_tile_rw below).  Two barriers per row.
 //
 // Threads (mid columns) per two-stage tile; the host sizes the grid with it.
 // (With the LDS-ring tile, FP64 order 6 needed 128 to fit three blocks per
@@ -630,192 +626,12 @@ template <typename T> struct PairRow {
   T Y2[3], C[3];  // pair 2: Y2 and compensation at output row r-2R
 };
 
-template <typename T, int ORDER, int MAT, int PAIR, bool KAHAN, int NT, int PF>
-__device__ __forceinline__ void pair_tile(const PairArgs<T> &a, int tile)
-{
-  using O = Ops<ORDER>;
-  constexpr int R = O::R, D = O::CLS + O::R;
-  constexpr int DI = 2 * R + 2, DM = 2 * R + 1;
-  constexpr int LW = NT + 2 * R;
-  constexpr int BXO = NT - 2 * R;
-  constexpr bool LD_M = MAT == M_ARR;
-  // Y4 band next to the frame: the frame's table rows read TAPS consecutive
-  // points from i-R (zero padded past i+R), so up to TAPS-R-1 into the tiles.
-  constexpr int RW = Geom<ORDER>::TAPS - R - 1;
-
-  __shared__ T s_in[3][DI][LW];
-  __shared__ T s_mid[3][DM][NT];
-
-  const int t = threadIdx.x;
-  // row-major tiles (measured 4% faster than column-major, which shares the
-  // vertical halos in L2 but splits rows across XCDs)
-  const int tx = tile % a.tiles_x, tyb = tile / a.tiles_x;
-  const int x0 = D + tx * BXO;               // first output column of the tile
-  const int mc = x0 - R + t;                 // this thread's mid column
-  const int mcc = min(mc, a.nx - 1);
-  const bool hl = t < R, hr = t >= NT - R, halo = hl || hr;
-  const int hc = hl ? mc - R : min(mc + R, a.nx - 1);
-  const int hslot = hl ? t : t + 2 * R;
-  const bool out_col = t >= R && t < NT - R && mc < a.nx - D;
-  const int j0 = D + tyb * a.ty_rows;
-  const int j1 = min(j0 + a.ty_rows, a.ny - D);
-  const int nk = (j1 - j0) + 4 * R;
-  const long plane = a.plane, pitch = a.pitch;
-  const long col = a.xoff + mcc, hcol = a.xoff + hc;
-  const int jmax = a.ny - 1;
-  const T *__restrict__ yin = a.yin;
-
-  auto fetch = [&](PairRow<T> &q, int k) {
-    const int r = min(j0 - 2 * R + k, jmax);  // j0 - 2R >= CLS - R >= 0
-    const long orow = (long)r * pitch;
-#pragma unroll
-    for (int c = 0; c < 3; c++) q.in[c] = yin[c * plane + orow + col];
-    if (halo) {
-      q.h[0] = yin[orow + hcol];
-      q.h[1] = yin[2 * plane + orow + hcol];
-    }
-    const long om = (long)max(r - R, 0) * pitch + col;
-    if constexpr (PAIR == 2) {
-#pragma unroll
-      for (int c = 0; c < 3; c++) q.U[c] = a.u[c * plane + om];
-    }
-    if constexpr (LD_M) {
-      q.ri = a.rinv[om];
-      q.K = a.bulk[om];
-    }
-    if constexpr (PAIR == 2) {
-      const long oo = (long)max(r - 2 * R, 0) * pitch + col;
-#pragma unroll
-      for (int c = 0; c < 3; c++) {
-        q.Y2[c] = ldg<true>(a.y2 + c * plane + oo);
-        if constexpr (KAHAN) q.C[c] = ldg<true>(a.comp + c * plane + oo);
-      }
-    }
-  };
-
-  // Prefetch queue of PF+1 slots: iteration k consumes slot k % (PF+1) and
-  // loads row k+PF into slot (k+PF) % (PF+1), which iteration k-1 emptied, so
-  // no slot is overwritten while its old value is live and every slot keeps
-  // one register set (no copies of in-flight loads at the loop latch, which
-  // would have to wait for them).
-  constexpr int S = PF + 1;
-  PairRow<T> q[S];
-#pragma unroll
-  for (int k = 0; k < S; k++) {
-    q[k] = PairRow<T>{};
-    if (k < PF) fetch(q[k], k);
-  }
-  // pointwise values of the last R+1 mid rows, oldest first: U (pair 2), rho^-1, K
-  T fU[R + 1][3], fri[R + 1], fK[R + 1];
-#pragma unroll
-  for (int p = 0; p <= R; p++) {
-    fri[p] = fK[p] = T(1);
-#pragma unroll
-    for (int c = 0; c < 3; c++) fU[p][c] = T(0);
-  }
-
-  int si = 0;  // input-ring slot of row r (k % DI)
-  int sm = 0;  // mid-ring slot of mid row r-R ((k - 2R) % DM)
-  // The row loop is unrolled by S so that the queue is indexed statically.
-  // It runs whole groups of S iterations (no exit inside a group, which
-  // would also make the compiler drain every load at the loop latch);
-  // iterations k >= nk stage clamped rows and store nothing.
-  for (int kb = 0; kb < nk; kb += S) {
-#pragma unroll
-    for (int pq = 0; pq < S; pq++) {
-      const int k = kb + pq;
-      const PairRow<T> &cur = q[pq];
-      fetch(q[(pq + PF) % S], k + PF);
-
-#pragma unroll
-      for (int c = 0; c < 3; c++) s_in[c][si][t + R] = cur.in[c];
-      if (halo) {
-        s_in[0][si][hslot] = cur.h[0];
-        s_in[2][si][hslot] = cur.h[1];
-      }
-      __syncthreads();
-
-      if (k >= 2 * R) {  // mid row m = r - R: outer stage on the grown tile
-        int sy[2 * R + 1];  // input slots of rows m-R .. m+R
-#pragma unroll
-        for (int s = 0; s <= 2 * R; s++) sy[s] = (si + DI - 2 * R + s) % DI;
-        const int sr = sy[R];
-        const T dxp = dint<T, ORDER>([&](int s) { return s_in[2][sr][t + s]; }, a.hix);
-        const T dyp = dint<T, ORDER>([&](int s) { return s_in[2][sy[s]][t + R]; }, a.hiy);
-        const T dxu = dint<T, ORDER>([&](int s) { return s_in[0][sr][t + s]; }, a.hix);
-        const T dyv = dint<T, ORDER>([&](int s) { return s_in[1][sy[s]][t + R]; }, a.hiy);
-        T Um[3];
-#pragma unroll
-        for (int c = 0; c < 3; c++) Um[c] = PAIR == 1 ? s_in[c][sr][t + R] : cur.U[c];
-        const T ri = LD_M ? cur.ri : T(1), Kb = LD_M ? cur.K : T(1);
-        T F[3];
-        acoustic_F<T, MAT>(dxp, dyp, dxu, dyv, ri, Kb, T(0), T(0), F[0], F[1], F[2]);
-#pragma unroll
-        for (int c = 0; c < 3; c++) s_mid[c][sm][t] = Um[c] + a.c1 * F[c];
-#pragma unroll
-        for (int p = 0; p < R; p++) {
-          fri[p] = fri[p + 1];
-          fK[p] = fK[p + 1];
-#pragma unroll
-          for (int c = 0; c < 3; c++) fU[p][c] = fU[p + 1][c];
-        }
-        fri[R] = ri;
-        fK[R] = Kb;
-#pragma unroll
-        for (int c = 0; c < 3; c++) fU[R][c] = Um[c];
-      }
-      __syncthreads();
-
-      if (k >= 4 * R && k < nk && out_col) {  // output row o = r - 2R: inner stage on the tile
-        const int o = j0 - 4 * R + k;
-        int sz[2 * R + 1];  // mid slots of rows o-R .. o+R
-#pragma unroll
-        for (int s = 0; s <= 2 * R; s++) sz[s] = (sm + DM - 2 * R + s) % DM;
-        const int so = sz[R];
-        const T dxp = dint<T, ORDER>([&](int s) { return s_mid[2][so][t - R + s]; }, a.hix);
-        const T dyp = dint<T, ORDER>([&](int s) { return s_mid[2][sz[s]][t]; }, a.hiy);
-        const T dxu = dint<T, ORDER>([&](int s) { return s_mid[0][so][t - R + s]; }, a.hix);
-        const T dyv = dint<T, ORDER>([&](int s) { return s_mid[1][sz[s]][t]; }, a.hiy);
-        T F[3];
-        acoustic_F<T, MAT>(dxp, dyp, dxu, dyv, fri[0], fK[0], T(0), T(0), F[0], F[1], F[2]);
-        const int sio = (si + DI - 2 * R) % DI;  // input slot of row o
-        const long qo = (long)o * pitch + a.xoff + mc;
-        if constexpr (PAIR == 1) {
-#pragma unroll
-          for (int c = 0; c < 3; c++) {
-            stg<true>(a.o1 + c * plane + qo, s_mid[c][so][t]);
-            stg<true>(a.o2 + c * plane + qo, s_in[c][sio][t + R] + a.c2 * F[c]);
-          }
-        } else {
-          const bool ring = mc - D < RW || a.nx - D - 1 - mc < RW || o - D < RW || a.ny - D - 1 - o < RW;
-#pragma unroll
-          for (int c = 0; c < 3; c++) {
-            const T U = fU[0][c];
-            const T Y4 = s_mid[c][so][t];
-            const T inc = ((cur.Y2[c] - U) + 2 * (s_in[c][sio][t + R] - U) + (Y4 - U)) * a.third + a.c2 * F[c];
-            if constexpr (!KAHAN) {
-              stg<true>(a.o1 + c * plane + qo, U + inc);
-            } else {
-              const T y = inc - cur.C[c];
-              const T tt = U + y;
-              stg<true>(a.comp + c * plane + qo, (tt - U) - y);
-              stg<true>(a.o1 + c * plane + qo, tt);
-            }
-            if (ring) a.o2[c * plane + qo] = Y4;
-          }
-        }
-      }
-      si = si + 1 == DI ? 0 : si + 1;
-      if (k >= 2 * R) sm = sm + 1 == DM ? 0 : sm + 1;
-    }
-  }
-}
-
-// Register-window form of pair_tile: each thread keeps its own column's last
-// 2R+1 rows of the operand and of the mid stage in registers (the
-// y-derivatives and pointwise values come from there), and only the rows
-// being x-differentiated pass through LDS, once each: 8 KB of LDS per FP64
-// block instead of 68 KB.  Same arithmetic per point, term for term.
+// Two-stage tile: each thread keeps its own column's last 2R+1 rows of the
+// operand and of the mid stage in registers (the y-derivatives and pointwise
+// values come from there), and only the rows being x-differentiated pass
+// through LDS, once each: 8 KB of LDS per FP64 block (an earlier form kept
+// whole rows of all three fields in LDS rings, 68 KB, and ran up to 7%
+// slower).
 //   phase 0: stage operand row r (u, p with the x halo) in LDS
 //   phase 1: dx of row r into a register FIFO; mid row m = r-R from the
 //            windows and the FIFO's oldest entry; its u, p into LDS
@@ -1017,9 +833,6 @@ __device__ __forceinline__ void pair_tile_rw(const PairArgs<T> &a, int tile)
   }
 }
 
-#ifndef SBP_PAIR_RW
-#define SBP_PAIR_RW 1
-#endif
 // Rows of loads in flight in the pair-1 and pair-2 tiles (sweep on gfx950):
 // FP64 tiles run SBP_PAIR_PF1/PF2 rows ahead; FP32 rows take half the
 // registers, and their tiles run one row further ahead (4% faster on
@@ -1048,10 +861,7 @@ __global__ __launch_bounds__(NT) void pair2d_kernel(StageArgs<T> ea, PairArgs<T>
   const int xcd = bi % kNumXcd, local = bi / kNumXcd;
   const int q = nt / kNumXcd, r = nt % kNumXcd;
   const int tile = xcd * q + min(xcd, r) + local;
-  if constexpr (SBP_PAIR_RW)
-    pair_tile_rw<T, ORDER, MAT, PAIR, KAHAN, NT, PF, NTS>(a, tile);
-  else
-    pair_tile<T, ORDER, MAT, PAIR, KAHAN, NT, PF>(a, tile);
+  pair_tile_rw<T, ORDER, MAT, PAIR, KAHAN, NT, PF, NTS>(a, tile);
 }
 
 // --------------------------------------------------------------------------

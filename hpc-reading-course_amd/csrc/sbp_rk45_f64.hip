@@ -3,10 +3,13 @@
 // PETSc -- TSRK5F (the fifth-order solution is propagated, the embedded
 // fourth-order one estimates the error), TSADAPTBASIC, and
 // TS_EXACTFINALTIME_MATCHSTEP.  PETSc's source is not part of the reference;
-// the controller follows its documented behaviour and defaults:
+// the step loop restates TSSolve / TSStep_RK / TSAdaptChoose_Basic of PETSc
+// 3.12 with their defaults (sbp_ts.hpp):
 //   enorm = sqrt(mean((y - y_hat)^2 / tol^2)), tol = atol + rtol*max(|y|, |y_hat|)
-//   accept iff enorm <= 1;  dt_next = h * clip(0.9 * enorm^(-1/5) [* 0.5 if rejected], 0.1, 10)
-// where h is the attempted step (clipped to land on t_end).
+//   accept iff enorm <= 1;  next = h * clip(0.9 [* 0.5 after a rejected attempt
+//   of the same step] * enorm^(-1/5), 0.1, 10); after an accepted step the
+//   MATCHSTEP rule stretches or halves the next step near t_end; more than 10
+//   rejections of one step, or a non-finite enorm, fail with SBP_EDIVERGED.
 //
 // Every attempt is six right-hand-side launches (the RHS mode of the stage
 // kernels) plus combination kernels and one weighted-norm reduction, whose
@@ -23,6 +26,7 @@
 #include "../../include/sbp_hip.h"
 #include "sbp_ctx.hpp"
 #include "sbp_host.hpp"
+#include "sbp_ts.hpp"
 
 using sbp::host::fail;
 #define HIP_OR_FAIL(expr) SBP_HIP_OR_FAIL(expr)
@@ -158,53 +162,67 @@ int sbp_rk45(sbp_ctx *ctx, double t0, double t_end, double dt0, double atol, dou
   const int pgrid = (int)((npts + 255) / 256);
   const int fgrid = std::min(pgrid, ctx->npartials);
   hipStream_t s = ctx->stream;
-  double t = t0, dt = dt0;
+  namespace ts = sbp::ts;
+  // TSSolve: the first step does not overshoot t_end
+  double t = t0, h = ts::first_step(t0, dt0, t_end);
   long acc = 0, rej = 0;
-  while (t < t_end) {
-    if (acc + rej >= max_steps) return fail(SBP_EINVAL, "step budget exhausted before t_end");
-    bool last = false;
-    double h = dt;
-    if (t + h >= t_end) {  // MATCHSTEP: land on t_end
-      h = t_end - t;
-      last = true;
-    }
-    double *U = static_cast<double *>(ctx->U);
-    HIP_OR_FAIL(sbp::host::launch_rhs(ctx, t, U, K[0]));
-    for (int st = 1; st < 6; st++) {
-      Row a{};
-      for (int q = 0; q < st; q++) a.c[q] = kA[st][q];
-      hipLaunchKernelGGL(rkf_stage_kernel, dim3(pgrid), dim3(256), 0, s, r, st, a, h, U, Y);
+  auto report = [&]() {
+    if (accepted) *accepted = acc;
+    if (rejected) *rejected = rej;
+    if (dt_next) *dt_next = h;
+  };
+  while (t < t_end) {  // TS_CONVERGED_TIME once ptime >= max_time
+    bool prev_rejected = false;  // TSStep_RK's accept flag, reset every step
+    int rejections = 0;
+    for (;;) {
+      if (acc + rej >= max_steps) {
+        report();
+        return fail(SBP_EINVAL, "step budget exhausted before t_end");
+      }
+      double *U = static_cast<double *>(ctx->U);
+      HIP_OR_FAIL(sbp::host::launch_rhs(ctx, t, U, K[0]));
+      for (int st = 1; st < 6; st++) {
+        Row a{};
+        for (int q = 0; q < st; q++) a.c[q] = kA[st][q];
+        hipLaunchKernelGGL(rkf_stage_kernel, dim3(pgrid), dim3(256), 0, s, r, st, a, h, U, Y);
+        HIP_OR_FAIL(hipGetLastError());
+        HIP_OR_FAIL(sbp::host::launch_rhs(ctx, t + kC[st] * h, Y, K[st]));
+      }
+      hipLaunchKernelGGL(rkf_finish_kernel, dim3(fgrid), dim3(256), 0, s, r, b, e, h, atol, rtol, U, U5,
+                         ctx->partials);
+      hipLaunchKernelGGL(rkf_sum_kernel, dim3(1), dim3(256), 0, s, ctx->partials, fgrid, ctx->scalar);
       HIP_OR_FAIL(hipGetLastError());
-      HIP_OR_FAIL(sbp::host::launch_rhs(ctx, t + kC[st] * h, Y, K[st]));
-    }
-    hipLaunchKernelGGL(rkf_finish_kernel, dim3(fgrid), dim3(256), 0, s, r, b, e, h, atol, rtol, U, U5,
-                       ctx->partials);
-    hipLaunchKernelGGL(rkf_sum_kernel, dim3(1), dim3(256), 0, s, ctx->partials, fgrid, ctx->scalar);
-    HIP_OR_FAIL(hipGetLastError());
-    double ss = 0;
-    HIP_OR_FAIL(hipMemcpyAsync(&ss, ctx->scalar, sizeof(double), hipMemcpyDeviceToHost, s));
-    HIP_OR_FAIL(hipStreamSynchronize(s));
-    const double enorm = std::sqrt(ss / (double)(npts * ctx->C));
-    const bool accept = enorm <= 1.0;
-    double hfac = enorm > 0 ? 0.9 * std::pow(enorm, -1.0 / 5.0) : 10.0;
-    if (!accept) hfac *= 0.5;
-    hfac = std::fmin(std::fmax(hfac, 0.1), 10.0);
-    if (accept) {
-      // the candidate becomes the state: swap the registers (the captured
-      // RK4 step graph refers to the old one)
-      std::swap(ctx->U, ctx->rkw[7]);
-      U5 = static_cast<double *>(ctx->rkw[7]);
-      sbp::host::drop_step_graph(ctx);
-      t = last ? t_end : t + h;
-      acc++;
-    } else {
+      double ss = 0;
+      HIP_OR_FAIL(hipMemcpyAsync(&ss, ctx->scalar, sizeof(double), hipMemcpyDeviceToHost, s));
+      HIP_OR_FAIL(hipStreamSynchronize(s));
+      const double enorm = std::sqrt(ss / (double)(npts * ctx->C));
+      if (!std::isfinite(enorm)) {  // TSErrorWeightedNorm: PETSC_ERR_FP
+        report();
+        return fail(SBP_EDIVERGED, "non-finite error estimate (the state is not finite)");
+      }
+      const ts::Choice ch = ts::adapt_basic(enorm, h, 5, prev_rejected);
+      if (ch.accept) {
+        // the candidate becomes the state: swap the registers (the captured
+        // RK4 step graph refers to the old one)
+        std::swap(ctx->U, ctx->rkw[7]);
+        U5 = static_cast<double *>(ctx->rkw[7]);
+        sbp::host::drop_step_graph(ctx);
+        const double next = ts::matchstep(t + h, ch.next_h, t_end);
+        t = t + h;
+        h = next;
+        acc++;
+        break;
+      }
       rej++;
+      h = ch.next_h;
+      prev_rejected = true;
+      if (++rejections > ts::kMaxReject) {  // TS_DIVERGED_STEP_REJECTED
+        report();
+        return fail(SBP_EDIVERGED, "step rejected more than 10 times (TS_DIVERGED_STEP_REJECTED)");
+      }
     }
-    dt = h * hfac;
   }
-  if (accepted) *accepted = acc;
-  if (rejected) *rejected = rej;
-  if (dt_next) *dt_next = dt;
+  report();
   return SBP_OK;
 }
 

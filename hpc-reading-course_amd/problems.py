@@ -233,18 +233,6 @@ def dt_from_cfl(spec: ProblemSpec, cfl: float) -> float:
     return cfl / min(spec.hix, spec.hiy)
 
 
-def matchstep_schedule(t_end: float, dt: float, t0: float = 0.0):
-    """Steps of ts_rk4's TS_EXACTFINALTIME_MATCHSTEP run (ts_rk.cpp:28): full
-    steps of dt while they fit, then one clipped step landing on t_end.
-    Returns (n_full, dt_last) with dt_last = 0 when none is needed."""
-    span = t_end - t0
-    n = int(math.floor(span / dt + 1e-9))
-    rest = span - n * dt
-    if rest <= 1e-9 * dt:
-        return n, 0.0
-    return n, rest
-
-
 # --------------------------------------------------------------------------
 # Benchmark workloads
 # --------------------------------------------------------------------------

@@ -8,8 +8,9 @@
 //   ts_rk4(da, t_end, dt, v, rhs, ctx)          (code/src/time_stepping/ts_rk.cpp:54-65).
 // Here sbp::AppCtx owns the device context, sbp::rhs has the callback's
 // argument order and meaning (vectors are the AoS arrays a Vec holds on one
-// rank), and sbp::ts_rk4 runs TSRK4 with TS_EXACTFINALTIME_MATCHSTEP
-// (ts_rk.cpp:28) with the state kept on the GPU for the whole solve: the
+// rank), sbp::ts_rk4 runs TSRK4 with TS_EXACTFINALTIME_MATCHSTEP
+// (ts_rk.cpp:28) and sbp::RK4_custom the reference's own loop
+// (time_stepping.h:98-152), with the state kept on the GPU for the whole solve: the
 // right-hand side is fused into the RK4 stage kernels, so the rhs callback
 // argument of the reference's ts_rk4 has no counterpart.
 //
@@ -18,6 +19,7 @@
 #pragma once
 
 #include <cmath>
+#include <cstdio>
 #include <stdexcept>
 #include <string>
 
@@ -62,27 +64,33 @@ inline int rhs(void * /*ts*/, double t, const double *v_src, double *v_dst, void
   return sbp_rhs(static_cast<AppCtx *>(ctx)->get(), t, v_src, v_dst);
 }
 
-// Steps of a MATCHSTEP run from t0 to t_end: n full steps of dt, then one
-// clipped step of *dt_last (0 when the full steps land on t_end).
-inline long matchstep_schedule(double t0, double t_end, double dt, double *dt_last)
-{
-  const double span = t_end - t0;
-  long n = (long)std::floor(span / dt + 1e-9);
-  double rest = span - n * dt;
-  *dt_last = rest <= 1e-9 * dt ? 0.0 : rest;
-  return n;
-}
-
-// ts_rk4: advance v (AoS, in/out) from t0 to t_end with classical RK4.
+// ts_rk4: advance v (AoS, in/out) from t0 to t_end with classical RK4 and
+// PETSc's TS_EXACTFINALTIME_MATCHSTEP step schedule (sbp_ts_rk4).
 inline int ts_rk4(double t_end, double dt, double *v, AppCtx &ctx, double t0 = 0.0)
 {
-  double last = 0;
-  const long n = matchstep_schedule(t0, t_end, dt, &last);
   int rc = sbp_set_state(ctx.get(), v);
-  if (!rc && n > 0) rc = sbp_step(ctx.get(), t0, dt, n);
-  if (!rc && last > 0) rc = sbp_step(ctx.get(), t0 + n * dt, last, 1);
+  if (!rc) rc = sbp_ts_rk4(ctx.get(), t0, t_end, dt, nullptr, nullptr);
   if (!rc) rc = sbp_get_state(ctx.get(), v);
   return rc;
+}
+
+// RK4_custom(da, Tend, dt, v, rhs, ctx) (time_stepping.h:98-152): round(Tend/dt)
+// steps from t = 0, the step rescaled to land on Tend, with the reference's
+// prints (PetscPrintf on rank 0 there, stdout here).
+inline int RK4_custom(double t_end, double dt, double *v, AppCtx &ctx)
+{
+  const long tlen = std::lround(t_end / dt);
+  if (std::fabs(tlen * dt - t_end) > 1e-14)
+    std::printf("Warning: Non-matching time step. dt*tlen = %.12f.\nChanging timestep from %e to %e.\n",
+                dt * tlen, dt, t_end / tlen);
+  long n = 0;
+  double h = dt, t = 0;
+  int rc = sbp_set_state(ctx.get(), v);
+  if (!rc) rc = sbp_rk4_custom(ctx.get(), t_end, dt, &n, &h, &t);
+  if (rc) return rc;
+  std::printf("tlen: %ld, dt: %e\n", n, h);
+  std::printf("Final t: %.8e\n", t);
+  return sbp_get_state(ctx.get(), v);
 }
 
 // ts_rk45 (ts_rk.cpp:41-52): adaptive Runge-Kutta-Fehlberg 4(5) of v from t0

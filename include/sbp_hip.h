@@ -86,7 +86,10 @@ enum {
   SBP_EHIP        = -2, /* HIP runtime error                           */
   SBP_ENOMEM      = -3, /* device or host allocation failed            */
   SBP_ENODEV      = -4, /* no usable gfx950 device                     */
-  SBP_ESTATE      = -5  /* call out of sequence (e.g. no state loaded) */
+  SBP_ESTATE      = -5, /* call out of sequence (e.g. no state loaded) */
+  SBP_EDIVERGED   = -6  /* adaptive integrator failed: a step rejected more than 10
+                           times (PETSc's TS_DIVERGED_STEP_REJECTED) or a non-finite
+                           error estimate (PETSc's TSErrorWeightedNorm, PETSC_ERR_FP) */
 };
 
 typedef struct sbp_ctx sbp_ctx; /* opaque: device buffers, stream, graphs */
@@ -132,19 +135,53 @@ int  sbp_rhs(sbp_ctx *ctx, double t, const double *src, double *dst);
  * (code/include/time_stepping/time_stepping.h:127-142,267-269). */
 int  sbp_rk4_advance(sbp_ctx *ctx, double *v, double t0, double dt, long nsteps);
 
+/* ts_rk4 on the resident state: TSRK4 with TSADAPTNONE under
+ * TS_EXACTFINALTIME_MATCHSTEP from t0 to t_end, first step dt
+ * (ts_rk.cpp:15-39,54-65).  The step schedule is PETSc 3.12's: TSSolve clips a
+ * first step that would overshoot; after each step ending at t with
+ * hmax = t_end - t, a next step that would overshoot becomes hmax, one with
+ * 2 dt > hmax becomes hmax/2, one with 1.01 dt > hmax becomes hmax (so
+ * t_end = 10.5 dt runs 9 steps of dt and 2 of 0.75 dt; 10.005 dt runs 9 of dt
+ * and one of 1.005 dt).  Step times accumulate as t = t + h.  nsteps and
+ * t_final (optional) receive the steps taken and the final time. */
+int  sbp_ts_rk4(sbp_ctx *ctx, double t0, double t_end, double dt, long *nsteps, double *t_final);
+
+/* That schedule without a context (host only, no GPU): runs of equal steps,
+ * run_dt[k] repeated run_len[k] times, k < *nruns <= max_runs (SBP_EINVAL if
+ * more would be needed; the arrays may be null to count). */
+int  sbp_matchstep_schedule(double t0, double t_end, double dt, int max_runs, double *run_dt,
+                            long *run_len, int *nruns, double *t_final);
+
+/* RK4_custom(da, Tend, dt, v, rhs, ctx) on the resident state
+ * (code/include/time_stepping/time_stepping.h:98-152): tlen = round(Tend/dt)
+ * steps from t = 0; when tlen*dt misses Tend by more than 1e-14 the step
+ * becomes Tend/tlen.  tlen, dt_used and t_final (the values the reference
+ * prints as "tlen: %d, dt: %e" and "Final t: %.8e") are optional. */
+int  sbp_rk4_custom(sbp_ctx *ctx, double t_end, double dt, long *tlen, double *dt_used, double *t_final);
+
 /* Adaptive Runge-Kutta-Fehlberg 4(5) on the resident state (FP64 contexts)
  * from t0 to t_end, first attempted step dt0.  Replaces ts_rk45
  * (ts_rk.cpp:41-52: TSRK5F, the fifth-order solution propagated and the
  * embedded fourth-order one estimating the error; TSADAPTBASIC;
- * TS_EXACTFINALTIME_MATCHSTEP).  A step is accepted when
- *   sqrt(mean(((y - y_hat) / (atol + rtol*max(|y|, |y_hat|)))^2)) <= 1
- * and the next step is h * clip(0.9 * enorm^(-1/5) (halved after a
- * rejection), 0.1, 10) -- PETSc's documented basic adaptor with its default
- * safety factors (its defaults for the tolerances are atol = rtol = 1e-4).
- * Fails with SBP_EINVAL after max_steps attempts.  accepted, rejected and
- * dt_next (the step the controller would take next) are optional. */
+ * TS_EXACTFINALTIME_MATCHSTEP).  The step loop is PETSc 3.12's TSStep_RK with
+ * TSAdaptChoose_Basic at its defaults: an attempt is accepted when
+ *   enorm = sqrt(mean(((y - y_hat) / (atol + rtol*max(|y|, |y_hat|)))^2)) <= 1,
+ * the next step is h * clip(s * enorm^(-1/5), 0.1, 10) with s = 0.9, or 0.45
+ * when the previous attempt of the same step was rejected too; after an
+ * accepted step the MATCHSTEP rule of sbp_ts_rk4 adjusts it near t_end (its
+ * defaults for the tolerances are atol = rtol = 1e-4).  Fails with
+ * SBP_EDIVERGED when one step is rejected more than 10 times or enorm is not
+ * finite, and with SBP_EINVAL after max_steps attempts (a safety bound with
+ * no PETSc counterpart).  accepted, rejected and dt_next (the step the
+ * controller would take next) are optional and also set on failure. */
 int  sbp_rk45(sbp_ctx *ctx, double t0, double t_end, double dt0, double atol, double rtol,
               long max_steps, long *accepted, long *rejected, double *dt_next);
+
+/* The basic adaptor's decision for one attempt (host only, no GPU):
+ * TSAdaptChoose_Basic with the defaults above for an embedded pair of the
+ * given order; prev_rejected: the previous attempt of the same step was
+ * rejected.  SBP_EDIVERGED for a non-finite enorm. */
+int  sbp_adapt_basic(double enorm, double h, int order, int prev_rejected, int *accept, double *next_h);
 
 /* ---- device-resident state (the time loop stays on the GPU) -------------- */
 int  sbp_set_state(sbp_ctx *ctx, const double *v);  /* host AoS -> device state */

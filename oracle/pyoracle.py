@@ -63,6 +63,10 @@ def _load(name):
         lib.orc_rk45.argtypes = [P, _DP, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, C.c_long,
                                  _LP, _LP, _DP]
         lib.orc_rkf45_tableau.argtypes = [_DP, _DP, _DP, _DP]
+        lib.orc_ts_rk4.argtypes = [P, _DP, C.c_double, C.c_double, C.c_double, _LP, _DP, C.c_int]
+        lib.orc_matchstep_steps.argtypes = [C.c_double, C.c_double, C.c_double, C.c_long, _DP, _DP]
+        lib.orc_adapt_basic.argtypes = [C.c_double, C.c_double, C.c_int, C.c_int, C.POINTER(C.c_int), _DP]
+        lib.orc_adapt_basic.restype = None
         _libs[name] = lib
     return _libs[name]
 
@@ -135,17 +139,50 @@ def rk4(spec, v: np.ndarray, t0: float, dt: float, nsteps: int, threads: int = 1
     return out
 
 
+def ts_rk4(spec, v: np.ndarray, t_end: float, dt: float, t0: float = 0.0, threads: int = 1):
+    """ts_rk4 (ts_rk.cpp:54-65): RK4 under TS_EXACTFINALTIME_MATCHSTEP from t0
+    to t_end, first step dt; returns (v, steps taken, final time)."""
+    d = _Desc(spec)
+    out = np.array(v, dtype=np.float64, copy=True)
+    n, tf = C.c_long(), C.c_double()
+    if lib().orc_ts_rk4(d.ref, _ptr(out), t0, t_end, dt, C.byref(n), C.byref(tf), threads):
+        raise ValueError("orc_ts_rk4 rejected the problem")
+    return out, n.value, tf.value
+
+
+def matchstep_steps(t0: float, t_end: float, dt: float, max_steps: int = 1 << 20):
+    """Step sizes of a fixed-step MATCHSTEP run; returns (steps, final time)."""
+    buf = np.empty(max_steps)
+    tf = C.c_double()
+    n = lib().orc_matchstep_steps(t0, t_end, dt, max_steps, _ptr(buf), C.byref(tf))
+    if n < 0:
+        raise ValueError("more than max_steps steps")
+    return buf[:n].copy(), tf.value
+
+
+def adapt_basic(enorm: float, h: float, order: int = 5, prev_rejected: bool = False):
+    """TSAdaptChoose_Basic's (accept, next_h) for one attempt."""
+    acc, nh = C.c_int(), C.c_double()
+    lib().orc_adapt_basic(enorm, h, order, int(prev_rejected), C.byref(acc), C.byref(nh))
+    return bool(acc.value), nh.value
+
+
 def rk45(spec, v: np.ndarray, t0: float, t_end: float, dt0: float, atol: float = 1e-4,
-         rtol: float = 1e-4, max_steps: int = 1000000):
-    """Adaptive RKF45 (orc_rk45); returns (v, accepted, rejected, dt_next)."""
+         rtol: float = 1e-4, max_steps: int = 1000000, check: bool = True):
+    """Adaptive RKF45 (orc_rk45); returns (v, accepted, rejected, dt_next), and
+    the return code as a fifth value when check is False (no exception then:
+    -2 = max_steps attempts spent, -3 = more than 10 rejections of one step
+    or a non-finite error estimate)."""
     d = _Desc(spec)
     out = np.array(v, dtype=np.float64, copy=True)
     acc, rej, dtn = C.c_long(), C.c_long(), C.c_double()
     rc = lib().orc_rk45(d.ref, _ptr(out), t0, t_end, dt0, atol, rtol, max_steps, C.byref(acc), C.byref(rej),
                         C.byref(dtn))
-    if rc:
-        raise ValueError("orc_rk45 failed (%d)" % rc)
-    return out, acc.value, rej.value, dtn.value
+    if check:
+        if rc:
+            raise ValueError("orc_rk45 failed (%d)" % rc)
+        return out, acc.value, rej.value, dtn.value
+    return out, acc.value, rej.value, dtn.value, rc
 
 
 def rkf45_tableau():

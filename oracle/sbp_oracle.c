@@ -16,6 +16,7 @@
 #define _POSIX_C_SOURCE 200809L
 #include "sbp_oracle.h"
 
+#include <float.h>
 #include <math.h>
 #include <pthread.h>
 #include <stdlib.h>
@@ -644,12 +645,97 @@ void orc_error_norms(const sbp_problem *p, const double *v, const double *w,
 }
 
 /* ------------------------------------------------------------------------ */
+/* PETSc 3.12 TS step-size rules behind ts_rk4 / ts_rk45 (ts_rk.cpp:15-65):   */
+/* TS_EXACTFINALTIME_MATCHSTEP with TSADAPTNONE (TSRK4) or TSADAPTBASIC       */
+/* (TSRK5F).  PETSc's source is not part of the reference; these restate      */
+/* TSSolve's first-step clip, TSAdaptChoose's MATCHSTEP adjustment            */
+/* (matchstepfac {0.01, 2}), TSAdaptChoose_Basic (safety 0.9, reject safety   */
+/* 0.5 after a rejected attempt of the same step, clip [0.1, 10], dt in       */
+/* [1e-20, 1e50]) and TSStep_RK's max_reject = 10.                            */
+/* ------------------------------------------------------------------------ */
+static double orc_clip(double x, double lo, double hi) { return x < lo ? lo : (x > hi ? hi : x); }
+
+static double ts_first_step(double ptime, double dt, double t_max)
+{
+  const double maxdt = t_max - ptime;
+  if (dt >= maxdt) return maxdt;
+  const double d = fabs(dt - maxdt);  /* PetscIsCloseAtTol(dt, maxdt, 10 eps, 0) */
+  if (d <= 10 * DBL_EPSILON * fmax(fabs(dt), fabs(maxdt))) return maxdt;
+  return dt;
+}
+
+/* After an accepted step ending at t, the proposal h becomes: */
+static double ts_matchstep(double t, double h, double t_max)
+{
+  const double tend = t + h, hmax = t_max - t;
+  double next = h;
+  if (t < t_max && tend > t_max) next = hmax;
+  if (t < t_max && tend < t_max && h * 2.0 > hmax) next = hmax / 2;
+  if (t < t_max && tend < t_max && h * (1.0 + 0.01) > hmax) next = hmax;
+  return next;
+}
+
+void orc_adapt_basic(double enorm, double h, int order, int prev_rejected, int *accept, double *next_h)
+{
+  double safety = 0.9;
+  int acc = 1;
+  if (enorm > 1) {
+    if (prev_rejected) safety *= 0.5;
+    acc = h < (1 + sqrt(DBL_EPSILON)) * 1e-20;
+  }
+  const double hfac = enorm > 0 ? safety * pow(enorm, -1.0 / order) : INFINITY;
+  *accept = acc;
+  *next_h = orc_clip(h * orc_clip(hfac, 0.1, 10.0), 1e-20, 1e50);
+}
+
+int orc_matchstep_steps(double t0, double t_end, double dt, long max_steps, double *steps, double *t_final)
+{
+  double t = t0, h = ts_first_step(t0, dt, t_end);
+  long n = 0;
+  while (t < t_end) {
+    if (n >= max_steps) return -1;
+    if (steps) steps[n] = h;
+    n++;
+    const double next = ts_matchstep(t + h, h, t_end);
+    t = t + h;
+    h = next;
+  }
+  if (t_final) *t_final = t;
+  return (int)n;
+}
+
+/* ts_rk4: TSRK4 (the RK4_custom combination) over the MATCHSTEP schedule. */
+int orc_ts_rk4(const sbp_problem *p, double *v, double t0, double t_end, double dt, long *nsteps,
+               double *t_final, int nthreads)
+{
+  if (orc_validate(p) || !(dt > 0)) return -1;
+  double t = t0, h = ts_first_step(t0, dt, t_end);
+  long n = 0;
+  while (t < t_end) {
+    /* a run of equal steps goes to the (threaded) fixed-step integrator */
+    long run = 0;
+    double tt = t, hh = h;
+    while (tt < t_end && hh == h) {
+      const double next = ts_matchstep(tt + hh, hh, t_end);
+      tt = tt + hh;
+      hh = next;
+      run++;
+    }
+    if (orc_rk4_par(p, v, t, h, run, nthreads)) return -1;
+    for (long k = 0; k < run; k++) t = t + h;
+    n += run;
+    h = hh;
+  }
+  if (nsteps) *nsteps = n;
+  if (t_final) *t_final = t;
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
 /* Adaptive Runge-Kutta-Fehlberg 4(5): what ts_rk45 (ts_rk.cpp:41-52) asks   */
 /* PETSc for -- TSRK5F (fifth-order solution propagated, fourth-order        */
-/* embedded estimate), TSADAPTBASIC, TS_EXACTFINALTIME_MATCHSTEP.  PETSc's   */
-/* source is not part of the reference; the controller restates its          */
-/* documented behaviour and defaults (safety 0.9, reject safety 0.5, clip    */
-/* [0.1, 10], weighted 2-norm with tol = atol + rtol*max(|y|, |y_hat|)).     */
+/* embedded estimate) with the rules above; weighted 2-norm with             */
+/* tol = atol + rtol*max(|y|, |y_hat|).                                        */
 /* ------------------------------------------------------------------------ */
 /* Fehlberg's coefficients (E. Fehlberg, NASA TR R-315, 1969). */
 static const double RKF_C[6] = {0.0, 1.0 / 4.0, 3.0 / 8.0, 12.0 / 13.0, 1.0, 1.0 / 2.0};
@@ -684,56 +770,64 @@ int orc_rk45(const sbp_problem *p, double *v, double t0, double t_end, double dt
   if (!ok) return -1;
   double e[6];
   for (int s = 0; s < 6; s++) e[s] = RKF_B5[s] - RKF_B4[s];
-  double t = t0, dt = dt0;
+  double t = t0, h = ts_first_step(t0, dt0, t_end);
   long acc = 0, rej = 0;
   int rc = 0;
-  while (t < t_end) {
-    if (acc + rej >= max_steps) {
-      rc = -2;
-      break;
-    }
-    int last = 0;
-    double h = dt;
-    if (t + h >= t_end) {  /* MATCHSTEP: land on t_end */
-      h = t_end - t;
-      last = 1;
-    }
-    orc_rhs(p, t, v, K[0]);
-    for (int s = 1; s < 6; s++) {
+  while (t < t_end && !rc) {
+    int prev_rejected = 0, rejections = 0;
+    for (;;) {
+      if (acc + rej >= max_steps) {
+        rc = -2;
+        break;
+      }
+      orc_rhs(p, t, v, K[0]);
+      for (int s = 1; s < 6; s++) {
+        for (long k = 0; k < n; k++) {
+          double sum = RKF_A[s][0] * K[0][k];
+          for (int j = 1; j < s; j++) sum = sum + RKF_A[s][j] * K[j][k];
+          Y[k] = v[k] + h * sum;
+        }
+        orc_rhs(p, t + RKF_C[s] * h, Y, K[s]);
+      }
+      double ss = 0;
       for (long k = 0; k < n; k++) {
-        double sum = RKF_A[s][0] * K[0][k];
-        for (int j = 1; j < s; j++) sum = sum + RKF_A[s][j] * K[j][k];
-        Y[k] = v[k] + h * sum;
+        double sb = RKF_B5[0] * K[0][k], se = e[0] * K[0][k];
+        for (int j = 1; j < 6; j++) {
+          sb = sb + RKF_B5[j] * K[j][k];
+          se = se + e[j] * K[j][k];
+        }
+        const double u5 = v[k] + h * sb, err = h * se, y4 = u5 - err;
+        const double tol = atol + rtol * fmax(fabs(u5), fabs(y4));
+        const double w = err / tol;
+        ss += w * w;
+        U5[k] = u5;
       }
-      orc_rhs(p, t + RKF_C[s] * h, Y, K[s]);
-    }
-    double ss = 0;
-    for (long k = 0; k < n; k++) {
-      double sb = RKF_B5[0] * K[0][k], se = e[0] * K[0][k];
-      for (int j = 1; j < 6; j++) {
-        sb = sb + RKF_B5[j] * K[j][k];
-        se = se + e[j] * K[j][k];
+      const double enorm = sqrt(ss / (double)n);
+      if (!isfinite(enorm)) { /* TSErrorWeightedNorm: PETSC_ERR_FP */
+        rc = -3;
+        break;
       }
-      const double u5 = v[k] + h * sb, err = h * se, y4 = u5 - err;
-      const double tol = atol + rtol * fmax(fabs(u5), fabs(y4));
-      const double w = err / tol;
-      ss += w * w;
-      U5[k] = u5;
-    }
-    const double enorm = sqrt(ss / (double)n);
-    const int accept = enorm <= 1.0;
-    double hfac = enorm > 0 ? 0.9 * pow(enorm, -1.0 / 5.0) : 10.0;
-    if (!accept) hfac *= 0.5;
-    hfac = fmin(fmax(hfac, 0.1), 10.0);
-    if (accept) {
-      memcpy(v, U5, sizeof(double) * n);
-      t = last ? t_end : t + h;
-      acc++;
-    } else {
+      int accept;
+      double next;
+      orc_adapt_basic(enorm, h, 5, prev_rejected, &accept, &next);
+      if (accept) {
+        memcpy(v, U5, sizeof(double) * n);
+        next = ts_matchstep(t + h, next, t_end);
+        t = t + h;
+        h = next;
+        acc++;
+        break;
+      }
       rej++;
+      h = next;
+      prev_rejected = 1;
+      if (++rejections > 10) { /* TS_DIVERGED_STEP_REJECTED */
+        rc = -3;
+        break;
+      }
     }
-    dt = h * hfac;
   }
+  double dt = h;
   for (int s = 0; s < 6; s++) free(K[s]);
   free(Y);
   free(U5);

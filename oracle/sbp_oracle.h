@@ -44,9 +44,20 @@ int    orc_rk4(const sbp_problem *p, double *v, double t0, double dt, long nstep
 int    orc_rk4_par(const sbp_problem *p, double *v, double t0, double dt, long nsteps,
                    int nthreads);
 
+/* ts_rk4 (ts_rk.cpp:54-65): TSRK4 with TSADAPTNONE under
+ * TS_EXACTFINALTIME_MATCHSTEP from t0 to t_end, first step dt. */
+int    orc_ts_rk4(const sbp_problem *p, double *v, double t0, double t_end, double dt, long *nsteps,
+                  double *t_final, int nthreads);
+/* The step sizes of that run (at most max_steps; returns the count or -1). */
+int    orc_matchstep_steps(double t0, double t_end, double dt, long max_steps, double *steps,
+                           double *t_final);
+/* TSAdaptChoose_Basic's decision for one attempt. */
+void   orc_adapt_basic(double enorm, double h, int order, int prev_rejected, int *accept,
+                       double *next_h);
 /* Adaptive Runge-Kutta-Fehlberg 4(5) from t0 to t_end (ts_rk45, ts_rk.cpp:41-52:
  * TSRK5F + TSADAPTBASIC + MATCHSTEP).  Returns 0, -1 on bad input, -2 when
- * max_steps attempts did not reach t_end. */
+ * max_steps attempts did not reach t_end, -3 when a step was rejected more
+ * than 10 times or the error estimate was not finite. */
 int    orc_rk45(const sbp_problem *p, double *v, double t0, double t_end, double dt0, double atol,
                 double rtol, long max_steps, long *accepted, long *rejected, double *dt_next);
 void   orc_rkf45_tableau(double c[6], double a[6][5], double b5[6], double b4[6]);

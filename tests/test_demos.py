@@ -72,10 +72,7 @@ def oracle_errors(orc, demo, order, n, tend, cfl):
              "reflection": P.reflection}[demo]
     spec, v0 = build(n, order)
     dt = P.dt_from_cfl(spec, cfl)
-    k, last = P.matchstep_schedule(tend, dt)
-    v = orc.rk4(spec, v0, 0.0, dt, k)
-    if last:
-        v = orc.rk4(spec, v, k * dt, last, 1)
+    v, _, _ = orc.ts_rk4(spec, v0, tend, dt)
     exact = {"adv_1D": lambda: P.adv1d_exact(spec, tend), "adv_2D": lambda: P.adv2d_exact(spec, tend),
              "wave": lambda: P.standing_mode(spec, tend), "wave_hom": lambda: P.standing_mode(spec, tend),
              "reflection": lambda: P.reflection_exact(spec, tend)}[demo]()

@@ -145,15 +145,19 @@ def test_rk4_1d(capi, orc, kind, order):
 def test_config_1d_advection_full(capi, orc):
     """N=201 order 4, a=1, Gaussian exp(-(x/0.2)^2), CFL 0.2 to t=2: 1000 steps."""
     spec, v0, dt, T = P.baseline_1d_advection()
-    n, last = P.matchstep_schedule(T, dt)
-    assert n == 1000 and last == 0.0
+    steps, _ = orc.matchstep_steps(0.0, T, dt)
+    assert len(steps) == 1000 and np.allclose(steps, dt, rtol=1e-12, atol=0)
+    n = 1000
     say("config 1: 1000 steps")
     ref = orc.rk4(spec, v0, 0.0, dt, n)
     with capi.Context(spec) as ctx:
         got = ctx.rk4_advance(v0, 0.0, dt, n)
         e_gpu = ctx.energy(got)
+        v = v0.copy()
+        capi.ts_rk4(ctx, T, dt, v)  # the run as ts_rk4 schedules it
     assert np.array_equal(got, ref), relerr(got, ref)
     assert e_gpu == pytest.approx(orc.energy(spec, ref), rel=1e-13)
+    assert np.array_equal(v, orc.ts_rk4(spec, v0, T, dt)[0])
 
 
 def test_config_acoustic_1025_full(capi, orc):
@@ -224,10 +228,8 @@ def test_ts_rk4_matchstep(capi, orc):
     spec, v0 = P.wave(41, 4)
     dt = P.dt_from_cfl(spec, 0.3)
     T = 0.1234
-    n, last = P.matchstep_schedule(T, dt)
-    assert last > 0
-    ref = orc.rk4(spec, v0, 0.0, dt, n)
-    ref = orc.rk4(spec, ref, n * dt, last, 1)
+    ref, n, _ = orc.ts_rk4(spec, v0, T, dt, threads=THREADS)
+    assert n > 1
     v = v0.copy()
     with capi.Context(spec) as ctx:
         capi.ts_rk4(ctx, T, dt, v)

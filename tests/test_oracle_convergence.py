@@ -24,10 +24,7 @@ CASES = [
 def solve_l2(orc, maker, exact, n, order, T, cfl=0.1):
     spec, v0 = getattr(P, maker)(n, order)
     dt = P.dt_from_cfl(spec, cfl)
-    m, last = P.matchstep_schedule(T, dt)
-    v = orc.rk4(spec, v0, 0.0, dt, m)
-    if last:
-        v = orc.rk4(spec, v, m * dt, last, 1)
+    v, _, _ = orc.ts_rk4(spec, v0, T, dt)
     return orc.error_norms(spec, v, exact(spec, T))[0]
 
 

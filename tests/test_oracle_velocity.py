@@ -128,10 +128,7 @@ def solve_l2(orc, maker, exact, n, order, T, cfl=0.1):
     vmax = max(float(np.max(np.abs(spec.vel_x))),
                float(np.max(np.abs(spec.vel_y))) if spec.vel_y is not None else 0.0)
     dt = P.dt_from_cfl(spec, cfl) / vmax
-    m, last = P.matchstep_schedule(T, dt)
-    v = orc.rk4(spec, v0, 0.0, dt, m)
-    if last:
-        v = orc.rk4(spec, v, m * dt, last, 1)
+    v, _, _ = orc.ts_rk4(spec, v0, T, dt)
     return orc.error_norms(spec, v, exact(spec, T))[0]
 
 

@@ -1,6 +1,6 @@
 """CPU model of the two-stage interior launches (sbp_capi.hip
-compute_pair_geometry / default_pair_rows, sbp_kernels.hpp pair_tile /
-pair_tile_rw / pair2d_kernel, edge_point with a D = CLS + R deep frame).
+compute_pair_geometry / default_pair_rows, sbp_kernels.hpp pair_tile_rw /
+pair2d_kernel, edge_point with a D = CLS + R deep frame).
 
 Checks, for many grid shapes and all orders, that
 * every global load of a pair tile (operand rows with prefetch, halo
@@ -12,10 +12,8 @@ Checks, for many grid shapes and all orders, that
   interior point of the operator (so the interior stencil is the right row);
 * every interior point the frame's stage 4 reads (zero-padded taps
   included) is one of the points pair 2 stores Y4 at;
-* the LDS ring slots hold the rows each phase reads, and no slot a phase
-  reads is overwritten before a barrier separates the two (pair_tile);
 * the register windows and FIFOs hold the rows each phase reads
-  (pair_tile_rw, the default);
+  (pair_tile_rw);
 * the ensemble kernels' persistent loops take every work item of every XCD
   chunk, and every one-wave part of a frame item, exactly once.
 """
@@ -54,43 +52,6 @@ def pair_geometry(nx, ny, order, rows=None, esz=8):
     eb = (n_edge + NT - 1) // NT
     neb = (eb + NXCD - 1) // NXCD * NXCD
     return dict(d=d, rows=rows, tiles_x=tiles_x, n_tiles=n_tiles, n_edge=n_edge, n_edge_blocks=neb, nt=NT)
-
-
-def simulate_rings(order, nrows):
-    """Replays the slot arithmetic of pair_tile for a tile of nrows output
-    rows (rows numbered from j0 = 0) and checks contents and hazards."""
-    r = R[order]
-    DI, DM = 2 * r + 2, 2 * r + 1
-    s_in = [None] * DI
-    s_mid = [None] * DM
-    # (slot, iteration) of the last read of each ring slot in the output phase
-    last_out_read_in = {}
-    si = sm = 0
-    nk = nrows + 4 * r
-    for k in range(nk):
-        row = -2 * r + k
-        # phase 0: store input row; must not overwrite a slot read by out(k-1)
-        assert last_out_read_in.get(si) != k - 1, ("input slot race", order, k)
-        s_in[si] = row
-        # barrier
-        if k >= 2 * r:
-            m = row - r
-            sy = [(si + DI - 2 * r + s) % DI for s in range(2 * r + 1)]
-            assert [s_in[x] for x in sy] == list(range(m - r, m + r + 1))
-            s_mid[sm] = m
-        # barrier
-        if k >= 4 * r:
-            o = -4 * r + k
-            sz = [(sm + DM - 2 * r + s) % DM for s in range(2 * r + 1)]
-            assert [s_mid[x] for x in sz] == list(range(o - r, o + r + 1)), (order, k)
-            sio = (si + DI - 2 * r) % DI
-            assert s_in[sio] == o
-            last_out_read_in[sio] = k
-            # the mid slots read here are overwritten at mid(k+1) at the
-            # earliest, which follows the first barrier of iteration k+1
-        si = 0 if si + 1 == DI else si + 1
-        if k >= 2 * r:
-            sm = 0 if sm + 1 == DM else sm + 1
 
 
 def simulate_windows(order, nrows):
@@ -230,31 +191,8 @@ def test_pair_geometry_tile_heights(rows):
 
 @pytest.mark.parametrize("order", [2, 4, 6])
 @pytest.mark.parametrize("nrows", [1, 2, 5, 8, 64])
-def test_pair_ring_slots(order, nrows):
-    simulate_rings(order, nrows)
-
-
-@pytest.mark.parametrize("order", [2, 4, 6])
-@pytest.mark.parametrize("nrows", [1, 2, 5, 8, 64])
 def test_pair_register_windows(order, nrows):
     simulate_windows(order, nrows)
-
-
-def test_pair_ring_race_is_detected():
-    """The hazard check fails for an input ring one row shorter (2R+1 deep),
-    where the store of row r would overwrite the row the previous output
-    phase read without a barrier in between."""
-    r = 2
-    DI = 2 * r + 1
-    last = {}
-    si, clash = 0, False
-    for k in range(20):
-        if last.get(si) == k - 1:
-            clash = True
-        if k >= 4 * r:
-            last[(si + DI - 2 * r) % DI] = k
-        si = (si + 1) % DI
-    assert clash
 
 
 def test_pair_geometry_large_grid_parameters():
