@@ -9,14 +9,14 @@ state resident in HBM.
 
 Prints one JSON line:
   value       DOF-updates/s (nx*ny*3 values advanced one RK4 step per update)
-  roofline    HBM roofline of one RK4 step's four launches: algorithmic
-              bytes per DOF-update of a 3-register RK4 with one launch per
-              stage (16 words: 9 reads + 7 writes, 128 B in FP64) over the
-              summed mean launch durations (HIP events on the library's
-              stream).  At this size the library runs two RK4 stages per
-              interior launch, which needs 7 words (kernel_bytes_per_dof,
-              kernel_frac), so the 16-word rate can exceed the peak; the
-              measured HBM traffic is `traffic`
+  roofline    HBM roofline of the dominant kernel: the step's longest launch
+              (at 4097^2 the pair2d_kernel launch of RK4 stages 3-4), its
+              algorithmic bytes per launch (each value it must read or write
+              once: launch_bytes) over its mean duration (HIP events on the
+              library's stream), against the 8 TB/s peak; `traffic` is the
+              same launch's HBM bytes measured by rocprofv3 counters.  The
+              whole step's rate on its own bytes, and on the 16-word basis of
+              a 3-register RK4 with one launch per stage, are reported beside it
   cpu_baseline the oracle compiled with the reference's optimisation flags,
               pthreads over a near-square block decomposition, timed on a
               bounded number of steps of the same workload on this host
@@ -34,14 +34,13 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
-# HBM bytes per DOF-update of one FP64 RK4 step of the default workload
-# (4097^2, order 4, rho = K = 1), summed over the step's four launches, from
-# rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (factors from the 1 GiB copy
-# kernel in the same runs), per schedule:
-# profiles/stage2d_acoustic_4097sq_order4_fp64_hbm_counters.md (one stage per
-# launch) and profiles/pair2d_acoustic_4097sq_order4_fp64_hbm_counters.md
-# (two-stage launches).
-MEASURED_TRAFFIC_PER_DOF = {(4097, 4, 0, "one stage per launch"): 107.41, (4097, 4, 0, "two-stage"): 68.49}
+# HBM bytes per launch of the dominant kernel of the default workload (4097^2,
+# order 4, rho = K = 1, FP64, two-stage schedule: the pair2d_kernel launch of
+# RK4 stages 3-4), from rocprofv3 FETCH_SIZE / 2.0 + WRITE_SIZE / 1.0 (the
+# factors a 1 GiB plain copy reads in the same runs), with the commit it was
+# measured at: profiles/pair2d_acoustic_4097sq_order4_fp64_hbm_counters.md.
+MEASURED_TRAFFIC = {(4097, 4, 0, "two-stage", 3): None}
+MEASURED_AT = None
 
 
 def pkg(sub):
@@ -65,21 +64,41 @@ def bytes_per_dof(precision, materials=False, C=3):
     return (words + extra) * w
 
 
-def kernel_bytes_per_dof(precision, materials=False, C=3, fused=False):
-    """Bytes per DOF-update the kernels as built need.  One launch per stage:
-    the 13-word scheme (U, Y2, Y3, Y4: 9 reads + 4 writes; 15 words with
-    Kahan), materials read by 4 launches.  Two-stage launches: 7 words
-    (stages 1-2 read U and write Y2, Y3; stages 3-4 read Y3, U, Y2 and write
-    U'; 9 with Kahan), materials read by 2 launches.  The frame's own
-    one-stage launches (a (CLS+R)-deep band, 0.6% of 4097^2) are not counted."""
-    w = 8 if precision == 0 else 4
-    if fused:
-        words = 7 if precision != 2 else 9
-        extra = (2 * 2 / C) if materials else 0.0
-    else:
-        words = 13 if precision != 2 else 15
-        extra = (4 * 2 / C) if materials else 0.0
-    return (words + extra) * w
+def launch_bytes(spec, fused):
+    """Algorithmic HBM bytes of each of a step's four launches: every value a
+    launch must read or write, once.  Per DOF (point and component), words:
+      one stage per launch: U -> Y2 (2), Y2, U -> Y3 (3), Y3, U -> Y4 (3),
+                            Y4, U, Y2, Y3 -> U (5; Kahan + compensation R/W 2)
+      two-stage:  launch 1  interior U -> Y2, Y3 (3); frame U -> Y2 (2)
+                  launch 2  frame Y2, U -> Y3 (3)
+                  launch 3  interior Y3, U, Y2 -> U' (4; Kahan +2) and Y4 on
+                            the band next to the frame (+1); frame Y3, U -> Y4 (3)
+                  launch 4  frame Y4, U, Y2, Y3 -> U' (5; Kahan +2)
+    Stored rho^-1 and K add 2 words per point to every launch that differentiates
+    with them (two-stage: per interior or frame point a launch covers)."""
+    w = 8 if spec.precision == 0 else 4
+    kahan = spec.precision == 2
+    mat = spec.material == 2
+    C = spec.components
+    order = spec.order
+    npts = spec.nx * spec.ny
+    if not fused:
+        words = [2, 3, 3, 5 + (2 if kahan else 0)]
+        return [float(npts * (C * k + (2 if mat else 0)) * w) for k in words]
+    cls, r = {2: 1, 4: 4, 6: 6}[order], order // 2
+    d = cls + r
+    taps = {2: 3, 4: 6, 6: 9}[order]
+    rw = taps - r - 1  # Y4 band width next to the frame
+    ni = (spec.nx - 2 * d) * (spec.ny - 2 * d)
+    nf = npts - ni
+    inner = (spec.nx - 2 * d - 2 * rw) * (spec.ny - 2 * d - 2 * rw)
+    band = ni - max(inner, 0)
+    m = 2 if mat else 0
+    b1 = ni * (3 * C + m) + nf * (2 * C + m)
+    b2 = nf * (3 * C + m)
+    b3 = ni * ((4 + (2 if kahan else 0)) * C + m) + band * C + nf * (3 * C + m)
+    b4 = nf * ((5 + (2 if kahan else 0)) * C + m)
+    return [float(b * w) for b in (b1, b2, b3, b4)]
 
 
 def gpu_run(spec, v0, dt, steps, warmup, stage_steps=None, fused=None):
@@ -108,15 +127,18 @@ def measure(spec, v0, dt, steps, warmup, fused=None):
     dofs = spec.nx * spec.ny * spec.components
     mat = spec.material == 2
     bpd = bytes_per_dof(spec.precision, mat, spec.components)
-    kbpd = kernel_bytes_per_dof(spec.precision, mat, spec.components, fused)
     step_kernel_s = float(np.sum(stage_ms)) * 1e-3
     achieved = dofs * bpd / step_kernel_s / 1e9
-    k_achieved = dofs * kbpd / step_kernel_s / 1e9
+    lb = launch_bytes(spec, fused)
+    k_achieved = sum(lb) / step_kernel_s / 1e9
+    dom = int(np.argmax(stage_ms))
+    dom_gbs = lb[dom] / (float(stage_ms[dom]) * 1e-3) / 1e9
     return dict(
         wall_s=wall, event_ms=ev_ms, ms_per_step=wall / steps * 1e3,
         dof_updates_per_s=dofs * steps / wall, stage_ms=[float(x) for x in stage_ms],
         bytes_per_dof=bpd, achieved_gbs=achieved, frac=achieved / PEAK_HBM_GBS, energy=energy,
-        kernel_bytes_per_dof=kbpd, kernel_achieved_gbs=k_achieved, kernel_frac=k_achieved / PEAK_HBM_GBS,
+        kernel_bytes_per_dof=sum(lb) / dofs, kernel_achieved_gbs=k_achieved, kernel_frac=k_achieved / PEAK_HBM_GBS,
+        launch_bytes=lb, dominant_launch=dom + 1, dominant_gbs=dom_gbs, dominant_frac=dom_gbs / PEAK_HBM_GBS,
         schedule="two-stage" if fused else "one stage per launch")
 
 
@@ -144,7 +166,7 @@ def batched(count=256, steps=500, warmup=10, prec=1):
     dofs = sum(sp.nx * sp.ny * 3 for sp in specs)
     bpd = bytes_per_dof(prec, materials=True)
     achieved = dofs * bpd * steps / (ev_ms * 1e-3) / 1e9
-    kbpd = kernel_bytes_per_dof(prec, materials=True, fused=cfg.get("two_stage", False))
+    kbpd = sum(sum(launch_bytes(sp, cfg.get("two_stage", False))) for sp in specs) / dofs
     return {"instances": count, "dofs": dofs, "steps": steps, "precision": ["fp64", "fp32", "fp32_kahan"][prec],
             "dof_updates_per_s": dofs * steps / wall, "ms_per_step": wall / steps * 1e3,
             "event_ms_per_step": ev_ms / steps, "bytes_per_dof": bpd, "achieved_gbs": achieved,
@@ -174,7 +196,7 @@ def reflection_2d(steps=2000, warmup=0):
             fused = ctx.fused
         dofs = spec.nx * spec.ny * 3
         bpd = bytes_per_dof(prec, True)
-        kbpd = kernel_bytes_per_dof(prec, True, 3, fused)
+        kbpd = sum(launch_bytes(spec, fused)) / dofs
         step_s = float(np.sum(stage_ms)) * 1e-3
         out[name] = {"dof_updates_per_s": dofs * steps / wall, "ms_per_step": wall / steps * 1e3,
                      "bytes_per_dof": bpd, "frac": dofs * bpd / step_s / 1e9 / PEAK_HBM_GBS,
@@ -279,7 +301,8 @@ def main():
         secondary["acoustic2d_wall_1025sq_order4_fp64"] = {
             "dof_updates_per_s": m2["dof_updates_per_s"], "ms_per_step": m2["ms_per_step"],
             "achieved_gbs": m2["achieved_gbs"], "frac": m2["frac"], "kernel_frac": m2["kernel_frac"],
-            "stage_ms": m2["stage_ms"], "schedule": m2["schedule"]}
+            "stage_ms": m2["stage_ms"], "schedule": m2["schedule"], "dominant_launch": m2["dominant_launch"],
+            "dominant_frac": m2["dominant_frac"]}
         log("1025^2:", secondary["acoustic2d_wall_1025sq_order4_fp64"])
         for prec, name in ((1, "fp32"), (2, "fp32_kahan")):
             s3, v3, dt3 = P.baseline_acoustic(args.n, args.order, precision=prec)
@@ -288,13 +311,15 @@ def main():
                 "dof_updates_per_s": m3["dof_updates_per_s"], "ms_per_step": m3["ms_per_step"],
                 "bytes_per_dof": m3["bytes_per_dof"], "achieved_gbs": m3["achieved_gbs"],
                 "frac": m3["frac"], "kernel_bytes_per_dof": m3["kernel_bytes_per_dof"],
-                "kernel_frac": m3["kernel_frac"], "stage_ms": m3["stage_ms"], "schedule": m3["schedule"]}
+                "kernel_frac": m3["kernel_frac"], "stage_ms": m3["stage_ms"], "schedule": m3["schedule"],
+                "dominant_launch": m3["dominant_launch"], "dominant_frac": m3["dominant_frac"]}
             log(name, secondary["acoustic2d_wall_%dsq_order%d_%s" % (args.n, args.order, name)])
         s4, v4, dt4 = P.baseline_acoustic(args.n, args.order)
         m4 = measure(s4, v4, dt4, args.steps, args.warmup, fused=False)
         secondary["acoustic2d_wall_%dsq_order%d_fp64_one_stage_per_launch" % (args.n, args.order)] = {
             k: m4[k] for k in ("dof_updates_per_s", "ms_per_step", "achieved_gbs", "frac", "kernel_bytes_per_dof",
-                               "kernel_achieved_gbs", "kernel_frac", "stage_ms", "schedule")}
+                               "kernel_achieved_gbs", "kernel_frac", "stage_ms", "schedule", "dominant_launch",
+                               "dominant_frac")}
         log("one stage per launch:", m4)
         del v4
         bm = batched()
@@ -313,14 +338,14 @@ def main():
         cpu = cpu_baseline(spec, v0, dt)
         log("cpu:", cpu)
 
-    traffic_dof = MEASURED_TRAFFIC_PER_DOF.get((spec.nx, spec.order, spec.precision, main_m["schedule"]))
     dofs = spec.nx * spec.ny * spec.components
-    traffic = traffic_dof * dofs / 4 if traffic_dof else None  # mean HBM bytes per launch of a step
+    dom = main_m["dominant_launch"]
+    traffic = MEASURED_TRAFFIC.get((spec.nx, spec.order, spec.precision, main_m["schedule"], dom))
     if main_m["schedule"] == "two-stage":
-        kernel = ("pair2d_kernel x2 (interior stages 1-2 and 3-4, each launch with the frame's stage 1 or 3) "
-                  "+ frame2d_kernel x2 (frame stages 2 and 4): one RK4 step, mean over its four launches")
+        names = {1: "pair2d_kernel, RK4 stages 1-2 (+ the frame's stage 1)", 2: "frame2d_kernel, frame stage 2",
+                 3: "pair2d_kernel, RK4 stages 3-4 (+ the frame's stage 3)", 4: "frame2d_kernel, frame stage 4"}
     else:
-        kernel = "stage2d_kernel x4 (one RK4 step), mean over the four stages"
+        names = {k: "stage2d_kernel, RK4 stage %d" % k for k in range(1, 5)}
     out = {
         "metric": "DOF-updates/s & % HBM roofline; time to L2 err 1e-4: chosen vs plain FP64",
         "value": main_m["dof_updates_per_s"],
@@ -338,20 +363,24 @@ def main():
                    "nx": spec.nx, "ny": spec.ny, "components": 3, "order": spec.order,
                    "cfl": 0.25, "bc": "wall SAT", "material": "rho=K=1",
                    "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "achieved": main_m["achieved_gbs"], "peak": PEAK_HBM_GBS,
-                     "unit": "GB/s", "frac": main_m["frac"], "traffic": traffic,
-                     "kernel": kernel, "schedule": main_m["schedule"],
-                     "bytes_basis": "16 words per DOF-update (3-register RK4, one launch per stage); "
-                                    "a two-stage schedule needs fewer (kernel_bytes_per_dof), so this "
-                                    "rate can exceed the HBM peak",
-                     "bytes_per_dof": main_m["bytes_per_dof"],
-                     "algorithmic_bytes_per_launch": main_m["bytes_per_dof"] * dofs / 4,
-                     "traffic_unit": "mean HBM bytes per launch of one RK4 step "
-                                     "(rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE)",
-                     "kernel_bytes_per_dof": main_m["kernel_bytes_per_dof"],
-                     "kernel_achieved": main_m["kernel_achieved_gbs"],
-                     "kernel_frac": main_m["kernel_frac"],
-                     "stage_ms": main_m["stage_ms"],
+        "roofline": {"bound": "hbm", "achieved": main_m["dominant_gbs"], "peak": PEAK_HBM_GBS,
+                     "unit": "GB/s", "frac": main_m["dominant_frac"], "traffic": traffic,
+                     "kernel": names[dom], "schedule": main_m["schedule"],
+                     "algorithmic_bytes_per_launch": main_m["launch_bytes"][dom - 1],
+                     "mean_launch_ms": main_m["stage_ms"][dom - 1],
+                     "bytes_basis": "each value the launch must read or write, once (bench.launch_bytes)",
+                     "traffic_unit": "HBM bytes per launch of that kernel (rocprofv3 FETCH_SIZE/2.0 + "
+                                     "WRITE_SIZE/1.0, copy-calibrated)",
+                     "traffic_measured_at": MEASURED_AT,
+                     "launch_bytes": main_m["launch_bytes"], "stage_ms": main_m["stage_ms"],
+                     "step": {"bytes": sum(main_m["launch_bytes"]), "bytes_per_dof": main_m["kernel_bytes_per_dof"],
+                              "kernel_ms": float(np.sum(main_m["stage_ms"])),
+                              "achieved": main_m["kernel_achieved_gbs"], "frac": main_m["kernel_frac"]},
+                     "rate_16word_basis": {"bytes_per_dof": main_m["bytes_per_dof"],
+                                           "achieved": main_m["achieved_gbs"], "frac_of_peak": main_m["frac"],
+                                           "note": "3-register RK4 with one launch per stage (16 words); "
+                                                   "the two-stage schedule needs fewer bytes, so this rate "
+                                                   "is not bounded by the peak"},
                      "copy_ceiling_gbs": copy_gbs},
         "cpu_baseline": cpu,
         "time_to_l2_1e-4": tta,

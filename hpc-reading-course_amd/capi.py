@@ -57,6 +57,8 @@ EXPORTS = {
     "sbp_set_tile_rows": (C.c_int, [C.c_void_p, C.c_int]),
     "sbp_set_fused": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "sbp_get_fused": (C.c_int, [C.c_void_p]),
+    "sbp_set_store_policy": (C.c_int, [C.c_void_p, C.c_int]),
+    "sbp_batch_set_store_policy": (C.c_int, [C.c_void_p, C.c_int]),
     "sbp_rhs": (C.c_int, [C.c_void_p, C.c_double, _DP, _DP]),
     "sbp_rk4_advance": (C.c_int, [C.c_void_p, _DP, C.c_double, C.c_double, C.c_long]),
     "sbp_set_state": (C.c_int, [C.c_void_p, _DP]),
@@ -260,6 +262,10 @@ class Context:
     def fused(self) -> bool:
         return bool(load_library().sbp_get_fused(self._h))
 
+    def set_store_policy(self, policy: int):
+        """Two-stage tile stores: 0 by streamed size (default), 1 cached, 2 nontemporal."""
+        _check(load_library().sbp_set_store_policy(self._h, int(policy)))
+
     # -- norms -------------------------------------------------------------
     def energy(self, v: np.ndarray) -> float:
         e = C.c_double()
@@ -390,6 +396,10 @@ class Batch:
         _check(load_library().sbp_batch_get_config(self._h, *[C.byref(x) for x in v]))
         return dict(config=v[0].value, tile_width=v[1].value, tile_rows=v[2].value,
                     blocks_per_cu=v[3].value, two_stage=bool(v[4].value))
+
+    def set_store_policy(self, policy: int):
+        """Two-stage tile stores: 0 by streamed size (default), 1 cached, 2 nontemporal."""
+        _check(load_library().sbp_batch_set_store_policy(self._h, int(policy)))
 
     def autotune(self, dt) -> int:
         d = self._dt(dt)

@@ -222,7 +222,8 @@ int apply_config(sbp_batch *b, int cfg)
   b->nwork_frame = (int)frame.size();
   b->nwork = (int)(work.size() - frame.size());  // the frame list starts at work_dev + nwork
   b->pair = c.pair != 0;
-  b->pair_nts = b->esz == 8 ? dd[0].p_nts : df[0].p_nts;
+  b->auto_nts = b->esz == 8 ? dd[0].p_nts : df[0].p_nts;
+  b->pair_nts = b->store_policy == 0 ? b->auto_nts : (b->store_policy == 2 ? 1 : 0);
   b->grid = std::max(kNumXcd, (b->num_cus * c.blocks_per_cu + kNumXcd - 1) / kNumXcd * kNumXcd);
   b->launch = fn;
   b->config = cfg;
@@ -612,6 +613,16 @@ int sbp_batch_set_config(sbp_batch *b, int config)
   if (!b) return fail(SBP_EINVAL, "null batch");
   if (config < 0 || config >= kNumConfigs) return fail(SBP_EINVAL, "configuration index out of range");
   return apply_config(b, config);
+}
+
+int sbp_batch_set_store_policy(sbp_batch *b, int policy)
+{
+  if (!b || policy < 0 || policy > 2) return fail(SBP_EINVAL, "store policy must be 0, 1 or 2");
+  HIP_OR_FAIL(hipStreamSynchronize(b->stream));
+  drop_graph(b);
+  b->store_policy = policy;
+  b->pair_nts = policy == 0 ? b->auto_nts : (policy == 2 ? 1 : 0);
+  return SBP_OK;
 }
 
 int sbp_batch_get_config(const sbp_batch *b, int *config, int *tile_width, int *tile_rows, int *blocks_per_cu,

@@ -247,7 +247,7 @@ template <typename T> PairArgs<T> make_pair_args(const sbp_ctx *c, int pair, dou
   a.n_edge_blocks = c->p_n_edge_blocks;
   // bytes a two-stage step streams: U, Y2, Y3, materials, compensation
   const double streamed = (double)c->plane * c->esz * (9 + (c->rinv ? 2 : 0) + (c->KC ? 3 : 0));
-  a.nts = pair_cached_stores(streamed) ? 0 : 1;
+  a.nts = c->store_policy == 0 ? (pair_cached_stores(streamed) ? 0 : 1) : (c->store_policy == 2 ? 1 : 0);
   if (pair == 1) {  // stages 1-2: U -> Y2 (Y0), Y3 (Y1)
     a.yin = static_cast<const T *>(c->U);
     a.o1 = static_cast<T *>(c->Y0);
@@ -510,8 +510,6 @@ int validate(const sbp_problem *d)
     if (d->material < SBP_MAT_UNIT || d->material > SBP_MAT_ARRAYS) return fail(SBP_EINVAL, "bad material");
     if (d->material == SBP_MAT_ARRAYS && (!d->rho_inv || !d->bulk))
       return fail(SBP_EINVAL, "SBP_MAT_ARRAYS needs rho_inv and bulk");
-    if (d->material == SBP_MAT_REF_RHO && d->precision != SBP_FP64)
-      return fail(SBP_EINVAL, "the reference rho = 2+xy problem is FP64 only");
   } else if (d->bc != SBP_BC_DEFAULT) {
     return fail(SBP_EINVAL, "boundary kinds other than default are acoustic only");
   }
@@ -812,6 +810,15 @@ int sbp_set_fused(sbp_ctx *ctx, int on, int rows)
 }
 
 int sbp_get_fused(const sbp_ctx *ctx) { return ctx && ctx->fused ? 1 : 0; }
+
+int sbp_set_store_policy(sbp_ctx *ctx, int policy)
+{
+  if (!ctx || policy < 0 || policy > 2) return fail(SBP_EINVAL, "store policy must be 0, 1 or 2");
+  HIP_OR_FAIL(hipStreamSynchronize(ctx->stream));
+  drop_graph(ctx);
+  ctx->store_policy = policy;
+  return SBP_OK;
+}
 
 int sbp_rhs(sbp_ctx *ctx, double t, const double *src, double *dst)
 {

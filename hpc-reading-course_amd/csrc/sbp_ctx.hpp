@@ -29,6 +29,7 @@ struct sbp_ctx {
   // launcher, D = CLS + R deep frame, tile geometry of the pair tiles.
   sbp::PairLaunchFn pair_launch = nullptr;
   bool fused = false;
+  int store_policy = 0;  // two-stage tile stores: 0 by streamed size, 1 cached, 2 nontemporal
   int p_depth = 0, p_ty_rows = 0, p_tiles_x = 0, p_n_tiles = 0, p_n_edge = 0, p_n_edge_blocks = 0;
   // Captured multi-step graphs, keyed by dt and by the array holding U when
   // the capture started (the two-stage path swaps U and Y2 every step).

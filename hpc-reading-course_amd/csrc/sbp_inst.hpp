@@ -19,8 +19,9 @@ StageLaunchFn pick_launcher(int phys, int mat)
   if (phys == PH_ACOUSTIC) {
     if (mat == M_UNIT) return &Launcher2D<T, ORDER, PH_ACOUSTIC, M_UNIT, KAHAN, BX>::launch;
     if (mat == M_ARR) return &Launcher2D<T, ORDER, PH_ACOUSTIC, M_ARR, KAHAN, BX>::launch;
-    if constexpr (sizeof(T) == 8 && !KAHAN)
-      if (mat == M_REF) return &Launcher2D<T, ORDER, PH_ACOUSTIC, M_REF, KAHAN, BX>::launch;
+    // rho = 2 + xy and the manufactured forcing: evaluated in FP64 from FP64
+    // tables in every precision, then rounded to T
+    if (mat == M_REF) return &Launcher2D<T, ORDER, PH_ACOUSTIC, M_REF, KAHAN, BX>::launch;
   }
   return nullptr;
 }

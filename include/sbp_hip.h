@@ -118,6 +118,13 @@ int  sbp_set_tile_rows(sbp_ctx *ctx, int rows);
  * device (ts_rk.cpp:54-65), not what they compute. */
 int  sbp_set_fused(sbp_ctx *ctx, int on, int rows);
 int  sbp_get_fused(const sbp_ctx *ctx);
+/* Store hint of the two-stage tiles' outputs: 0 (default) chooses from the
+ * bytes a step streams (cached stores while U, Y2, Y3 and the materials fit
+ * the 256 MB Infinity Cache, nontemporal ones beyond), 1 forces cached and 2
+ * nontemporal stores.  A performance knob with no reference counterpart;
+ * results do not depend on it (it selects between two builds of the same
+ * kernel, which is how tests run both on small grids). */
+int  sbp_set_store_policy(sbp_ctx *ctx, int policy);
 
 /* ---- right-hand side ----------------------------------------------------
  * dst = F(t, src) on host AoS arrays.  Replaces the rhs_serial callbacks
@@ -244,6 +251,8 @@ int  sbp_batch_get_config(const sbp_batch *b, int *config, int *tile_width, int 
 /* Time every candidate on this ensemble (resident states are saved and
  * restored) and keep the fastest; *choice receives its index. */
 int  sbp_batch_autotune(sbp_batch *b, const double *dt, int *choice);
+/* sbp_set_store_policy for the ensemble's two-stage configurations. */
+int  sbp_batch_set_store_policy(sbp_batch *b, int policy);
 
 /* ---- learned coarse-grid corrector ----------------------------------------
  * A 3-layer 3x3 convolutional network (8 -> 32 -> 32 -> 3 channels, ReLU)

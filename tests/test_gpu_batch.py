@@ -200,3 +200,34 @@ def test_batch_errors(capi):
         assert e.value.code == -5
         with pytest.raises(capi.SbpError):
             b.set_state(5, ens[0][1])
+
+
+@pytest.mark.parametrize("policy", [1, 2], ids=["cached", "nontemporal"])
+@pytest.mark.parametrize("prec", [P.FP32, P.FP32_KAHAN, P.FP64])
+def test_batch_store_policies(capi, orc, prec, policy):
+    """Both store builds of the ensemble's two-stage kernels (parts 1 and 3;
+    the 256-instance workload runs the nontemporal ones): every instance
+    within 2e-6 (FP32; FP64: bit-identical) of the two-stage single-problem
+    path, FP64 within 1e-12 of the oracle."""
+    print("batch store policy", prec, policy, flush=True)
+    ens = ensemble(prec, sizes=(20, 77, 130, 41, 257))
+    dts = [e[2] for e in ens]
+    cfgs = two_stage_configs(capi)
+    with capi.Batch([e[0] for e in ens]) as b:
+        b.set_config(cfgs[0])
+        b.set_store_policy(policy)
+        for k, (_, v0, _) in enumerate(ens):
+            b.set_state(k, v0)
+        b.step(dts, 13)
+        got = [b.get_state(k) for k in range(len(ens))]
+    for (spec, v0, dt), g in zip(ens, got):
+        with capi.Context(spec) as ctx:
+            ctx.set_fused(True)
+            ctx.set_store_policy(policy)
+            ref = ctx.rk4_advance(v0, 0.0, dt, 13)
+        if prec == P.FP64:
+            assert np.array_equal(g, ref)
+            o = orc.rk4(spec, v0, 0.0, dt, 13)
+            assert np.max(np.abs(g - o)) <= 1e-12 * np.max(np.abs(o))
+        else:
+            assert np.max(np.abs(g - ref)) <= 2e-6 * np.max(np.abs(ref))

@@ -228,3 +228,45 @@ def test_fused_nontemporal_store_kernels(capi):
     fused = run(capi, spec, v, dt, 5, True)
     plain = run(capi, spec, v, dt, 5, False)
     assert np.array_equal(fused, plain), relerr(fused, plain)
+
+
+STORE_CASES = [("unit", 4, P.FP64), ("unit", 4, P.FP32), ("unit", 4, P.FP32_KAHAN),
+               ("arrays", 6, P.FP64), ("arrays", 6, P.FP32), ("arrays", 6, P.FP32_KAHAN),
+               ("arrays", 4, P.FP64), ("unit", 2, P.FP32)]
+
+
+@pytest.mark.parametrize("policy", [1, 2], ids=["cached", "nontemporal"])
+@pytest.mark.parametrize("kind,order,prec", STORE_CASES, ids=lambda c: str(c))
+def test_store_policies(capi, orc, kind, order, prec, policy):
+    """Both builds of every two-stage tile (cached and nontemporal stores;
+    the library picks the nontemporal one above 256 MB of streamed arrays,
+    e.g. for the 4097^2 workloads) on a small ragged grid: FP64 bit-identical
+    to the one-stage path and within 1e-12 of the oracle; FP32 / Kahan within
+    2e-6 of the one-stage path and 2e-5 of the FP64 oracle."""
+    spec = make(kind, order, 300, 281, P.BC_WALL, precision=prec)
+    say("store policy", kind, order, prec, policy)
+    v = state(spec)
+    dt = P.dt_from_cfl(spec, 0.25)
+    plain = run(capi, spec, v, dt, 23, False)
+    with capi.Context(spec) as ctx:
+        ctx.set_fused(True)
+        ctx.set_store_policy(policy)
+        ctx.set_state(v)
+        ctx.step(0.0, dt, 23)   # a 16-step graph + 7 direct steps
+        got = ctx.get_state()
+    ref = orc.rk4(spec.with_(precision=P.FP64), v, 0.0, dt, 23, threads=THREADS)
+    if prec == P.FP64:
+        assert np.array_equal(got, plain), relerr(got, plain)
+        assert relerr(got, ref) <= 1e-12
+    else:
+        assert relerr(got, plain) <= 2e-6
+        assert relerr(got, ref) <= 2e-5
+
+
+def test_store_policy_rejects_bad_values(capi):
+    spec, _ = P.wave_hom(40, 4)
+    with capi.Context(spec) as ctx:
+        for bad in (-1, 3):
+            with pytest.raises(capi.SbpError) as e:
+                ctx.set_store_policy(bad)
+            assert e.value.code == -1

@@ -130,3 +130,21 @@ def test_fp32_energy_non_increasing(capi):
     # FP32 rounding allows relative wiggles of order 1e-6 around a flat energy
     assert all(E[k + 1] <= E[k] * (1 + 1e-6) for k in range(len(E) - 1)), E
     assert E[-1] >= 0.95 * E[0]
+
+
+@pytest.mark.parametrize("order", [2, 4, 6])
+@pytest.mark.parametrize("prec", [P.FP32, P.FP32_KAHAN])
+def test_fp32_reference_rho_problem(capi, orc, order, prec):
+    """demo/wave (rho = 2 + xy with the manufactured forcing, wave_eq_rhs.h:42-64)
+    in FP32: the right-hand side at a stage time where the forcing is on is
+    within 1e-5 of the FP64 oracle, and 200 RK4 steps within 2e-5."""
+    spec, v0 = P.wave(65, order, precision=prec)
+    X = np.linspace(0, 1, spec.length)
+    v = np.sin(7 * X) + 0.3 * np.cos(31 * X)
+    f64 = spec.with_(precision=P.FP64)
+    with capi.Context(spec) as ctx:
+        got = ctx.rhs(0.03, v)
+        dt = P.dt_from_cfl(spec, 0.25)
+        run = ctx.rk4_advance(v0, 0.0, dt, 200)
+    assert relerr(got, orc.rhs(f64, 0.03, v)) <= 1e-5
+    assert relerr(run, orc.rk4(f64, v0, 0.0, dt, 200, threads=8)) <= 2e-5
