@@ -78,6 +78,8 @@ EXPORTS = {
     "sbp_state_energy": (C.c_int, [C.c_void_p, _DP]),
     "sbp_hnorm": (C.c_int, [C.c_void_p, _DP, _DP]),
     "sbp_error_norms": (C.c_int, [C.c_void_p, _DP, _DP, _DP, _DP]),
+    "sbp_error_norms_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_long, C.c_double, C.c_double,
+                                          C.c_void_p, C.c_void_p]),
     "sbp_batch_create": (C.c_int, [C.POINTER(SbpProblem), C.c_int, C.POINTER(C.c_void_p)]),
     "sbp_batch_destroy": (C.c_int, [C.c_void_p]),
     "sbp_batch_count": (C.c_int, [C.c_void_p]),
@@ -580,6 +582,14 @@ def ts_rk45(ctx: Context, t_end: float, dt: float, v: np.ndarray, t0: float = 0.
     a, r, _ = ctx.rk45(t0, t_end, dt, atol, rtol)
     v.reshape(-1)[:] = ctx.get_state()
     return a, r
+
+
+def error_norms_noalloc(v_ptr: int, w_ptr: int, n: int, hx: float, hy: float, out_ptr: int,
+                        stream: int = 0):
+    """sbp_error_norms_noalloc on device pointers (integers): out[0] = l2,
+    out[1] = max, written asynchronously on `stream` without any allocation."""
+    _check(load_library().sbp_error_norms_noalloc(C.c_void_p(v_ptr), C.c_void_p(w_ptr), int(n), float(hx),
+                                                  float(hy), C.c_void_p(out_ptr), C.c_void_p(stream or None)))
 
 
 def copy_bandwidth(nbytes: int = 1 << 30, reps: int = 5) -> float:

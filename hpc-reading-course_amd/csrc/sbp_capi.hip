@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -742,6 +743,68 @@ static __global__ __launch_bounds__(256) void diff_norms_final_kernel(const doub
   }
 }
 
+// ---- no-allocation error norms (sbp_error_norms_noalloc) -----------------
+// The per-block partials of diff_norms_kernel live in this file-scope device
+// array instead of memory the library allocates: one (sum of squares, max)
+// pair per block, at most kNoAllocBlocks blocks (8 KB), so any vector length
+// reduces into it.  Each GPU holds its own copy until the process ends; every
+// call on a GPU shares it (calls that overlap on one GPU race).  Every block
+// of a launch writes its pair before the final kernel reads it, so nothing
+// has to start at zero.
+constexpr int kNoAllocBlocks = 512;
+static __device__ __align__(256) double scratch[2 * kNoAllocBlocks];
+
+static __global__ __launch_bounds__(256) void error_norms_out_kernel(const double *__restrict__ partial, int nb,
+                                                                      double hprod, double *__restrict__ out)
+{
+  __shared__ double ss[4], mm[4];
+  double s2 = 0, mx = 0;
+  for (int k = threadIdx.x; k < nb; k += 256) {
+    s2 += partial[2 * k];
+    mx = fmax(mx, partial[2 * k + 1]);
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    s2 += __shfl_down(s2, off, 64);
+    mx = fmax(mx, __shfl_down(mx, off, 64));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    ss[threadIdx.x >> 6] = s2;
+    mm[threadIdx.x >> 6] = mx;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    out[0] = sqrt(hprod) * sqrt((ss[0] + ss[1]) + (ss[2] + ss[3]));
+    out[1] = fmax(fmax(mm[0], mm[1]), fmax(mm[2], mm[3]));
+  }
+}
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch[kMaxDevices] = {};
+
+// The device address of `scratch` on the current GPU, looked up once per GPU
+// with that GPU current (host code must never pass the symbol itself: on the
+// host it names a stand-in, not device memory).
+int scratch_on_current_device(double **out)
+{
+  int dev = 0;
+  HIP_OR_FAIL(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(SBP_ENODEV, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(SBP_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch[dev] = static_cast<double *>(p);
+  }
+  *out = g_scratch[dev];
+  return SBP_OK;
+}
+}  // namespace
+
 hipError_t sbp::host::launch_step(sbp_ctx *c, double t, double dt) { return ::launch_step(c, t, dt); }
 hipError_t sbp::host::launch_rhs(sbp_ctx *c, double t, const void *src, void *dst)
 {
@@ -1049,6 +1112,28 @@ int sbp_error_norms(sbp_ctx *ctx, const double *v, const double *w, double *l2, 
   if (!is1d(ctx->phys)) hprod *= 1.0 / ctx->hiy;
   if (l2) *l2 = std::sqrt(hprod) * std::sqrt(r[0]);
   if (maxerr) *maxerr = r[1];
+  return SBP_OK;
+}
+
+int sbp_error_norms_noalloc(const double *v, const double *w, long n, double hx, double hy, double *out,
+                            void *stream)
+{
+  if (!v || !w || !out) return fail(SBP_EINVAL, "null argument");
+  if (n < 1 || n > SBP_NOALLOC_MAX_VALUES)
+    return fail(SBP_EINVAL, "vector length must be 1 .. SBP_NOALLOC_MAX_VALUES (2^40)");
+  if (!(hx > 0) || !(hy > 0) || !std::isfinite(hx) || !std::isfinite(hy))
+    return fail(SBP_EINVAL, "grid spacings must be positive");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(SBP_ENODEV, "no HIP device");
+  double *part = nullptr;
+  int rc = scratch_on_current_device(&part);
+  if (rc) return rc;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  // the block count of sbp_error_norms, so both reduce in the same order
+  const int grid = (int)std::max<long>(1, std::min<long>((n + 255) / 256, kNoAllocBlocks));
+  hipLaunchKernelGGL(diff_norms_kernel, dim3(grid), dim3(256), 0, s, v, w, n, part);
+  hipLaunchKernelGGL(error_norms_out_kernel, dim3(1), dim3(256), 0, s, part, grid, hx * hy, out);
+  HIP_OR_FAIL(hipGetLastError());
   return SBP_OK;
 }
 

@@ -220,6 +220,20 @@ int  sbp_hnorm(sbp_ctx *ctx, const double *v, double *norm);
 int  sbp_error_norms(sbp_ctx *ctx, const double *v, const double *w,
                      double *l2, double *maxerr);  /* on the device, FP64 */
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): sbp_error_norms on n FP64 values v, w already in device
+ * memory, with no context, writing out[0] = sqrt(hx*hy)*||v-w||_2 (hy = 1 for a
+ * 1D grid) and out[1] = ||v-w||_inf to device memory `out` on `stream` (a
+ * hipStream_t; null = the null stream), asynchronously, with the same block
+ * partition and summation order as sbp_error_norms.  It allocates nothing:
+ * its partial sums live in a fixed 8 KB device array per GPU, held until the
+ * process ends and shared by every call on that GPU, so calls that overlap on
+ * one GPU race.  n is 1 .. SBP_NOALLOC_MAX_VALUES (SBP_EINVAL beyond). */
+#define SBP_NOALLOC_MAX_VALUES (1L << 40)
+int  sbp_error_norms_noalloc(const double *v, const double *w, long n, double hx, double hy,
+                             double *out, void *stream);
+
 /* ---- batched ensemble ---------------------------------------------------
  * Many independent 2D acoustic problems (uneven sizes, own materials, own dt)
  * advanced together: one persistent kernel per RK4 stage walks a flattened
