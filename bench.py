@@ -69,10 +69,11 @@ def launch_bytes(spec, fused):
     launch must read or write, once.  Per DOF (point and component), words:
       one stage per launch: U -> Y2 (2), Y2, U -> Y3 (3), Y3, U -> Y4 (3),
                             Y4, U, Y2, Y3 -> U (5; Kahan + compensation R/W 2)
-      two-stage:  launch 1  interior U -> Y2, Y3 (3); frame U -> Y2 (2)
+      two-stage:  launch 1  interior U -> Y3 (2) and Y2 on the band next to
+                            the frame (+1); frame U -> Y2 (2)
                   launch 2  frame Y2, U -> Y3 (3)
-                  launch 3  interior Y3, U, Y2 -> U' (4; Kahan +2) and Y4 on
-                            the band next to the frame (+1); frame Y3, U -> Y4 (3)
+                  launch 3  interior Y3, U -> U' (3; Kahan +2; Y2 is recomputed)
+                            and Y4 on the band (+1); frame Y3, U -> Y4 (3)
                   launch 4  frame Y4, U, Y2, Y3 -> U' (5; Kahan +2)
     Stored rho^-1 and K add 2 words per point to every launch that differentiates
     with them (two-stage: per interior or frame point a launch covers)."""
@@ -94,9 +95,9 @@ def launch_bytes(spec, fused):
     inner = (spec.nx - 2 * d - 2 * rw) * (spec.ny - 2 * d - 2 * rw)
     band = ni - max(inner, 0)
     m = 2 if mat else 0
-    b1 = ni * (3 * C + m) + nf * (2 * C + m)
+    b1 = ni * (2 * C + m) + band * C + nf * (2 * C + m)
     b2 = nf * (3 * C + m)
-    b3 = ni * ((4 + (2 if kahan else 0)) * C + m) + band * C + nf * (3 * C + m)
+    b3 = ni * ((3 + (2 if kahan else 0)) * C + m) + band * C + nf * (3 * C + m)
     b4 = nf * ((5 + (2 if kahan else 0)) * C + m)
     return [float(b * w) for b in (b1, b2, b3, b4)]
 
@@ -109,6 +110,7 @@ def gpu_run(spec, v0, dt, steps, warmup, stage_steps=None, fused=None):
             ctx.set_fused(fused)
         fused = ctx.fused
         ctx.set_state(v0)
+        rows = ctx.tune_fused(dt)  # tile height measured on this grid (state kept), before the timed region
         if warmup:
             ctx.step(0.0, dt, warmup)
         ctx.synchronize()
@@ -119,11 +121,11 @@ def gpu_run(spec, v0, dt, steps, warmup, stage_steps=None, fused=None):
         energy = ctx.state_energy()
     finally:
         ctx.close()
-    return wall, ev_ms, stage_ms, energy, fused
+    return wall, ev_ms, stage_ms, energy, fused, rows
 
 
 def measure(spec, v0, dt, steps, warmup, fused=None):
-    wall, ev_ms, stage_ms, energy, fused = gpu_run(spec, v0, dt, steps, warmup, fused=fused)
+    wall, ev_ms, stage_ms, energy, fused, rows = gpu_run(spec, v0, dt, steps, warmup, fused=fused)
     dofs = spec.nx * spec.ny * spec.components
     mat = spec.material == 2
     bpd = bytes_per_dof(spec.precision, mat, spec.components)
@@ -139,7 +141,7 @@ def measure(spec, v0, dt, steps, warmup, fused=None):
         bytes_per_dof=bpd, achieved_gbs=achieved, frac=achieved / PEAK_HBM_GBS, energy=energy,
         kernel_bytes_per_dof=sum(lb) / dofs, kernel_achieved_gbs=k_achieved, kernel_frac=k_achieved / PEAK_HBM_GBS,
         launch_bytes=lb, dominant_launch=dom + 1, dominant_gbs=dom_gbs, dominant_frac=dom_gbs / PEAK_HBM_GBS,
-        schedule="two-stage" if fused else "one stage per launch")
+        schedule="two-stage" if fused else "one stage per launch", tile_rows=rows)
 
 
 def batched(count=256, steps=500, warmup=10, prec=1):
@@ -187,6 +189,7 @@ def reflection_2d(steps=2000, warmup=0):
         spec, v0, dt = P.baseline_reflection(precision=prec)
         with capi.Context(spec) as ctx:
             ctx.set_state(v0)
+            ctx.tune_fused(dt)
             ctx.synchronize()
             t0 = time.perf_counter()
             ctx.time_steps(0.0, dt, steps)

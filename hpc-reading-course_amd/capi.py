@@ -58,6 +58,7 @@ EXPORTS = {
     "sbp_set_fused": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "sbp_get_fused": (C.c_int, [C.c_void_p]),
     "sbp_set_store_policy": (C.c_int, [C.c_void_p, C.c_int]),
+    "sbp_tune_fused": (C.c_int, [C.c_void_p, C.c_double, C.POINTER(C.c_int)]),
     "sbp_batch_set_store_policy": (C.c_int, [C.c_void_p, C.c_int]),
     "sbp_rhs": (C.c_int, [C.c_void_p, C.c_double, _DP, _DP]),
     "sbp_rk4_advance": (C.c_int, [C.c_void_p, _DP, C.c_double, C.c_double, C.c_long]),
@@ -267,6 +268,13 @@ class Context:
     def set_store_policy(self, policy: int):
         """Two-stage tile stores: 0 by streamed size (default), 1 cached, 2 nontemporal."""
         _check(load_library().sbp_set_store_policy(self._h, int(policy)))
+
+    def tune_fused(self, dt: float) -> int:
+        """Time the two-stage tile heights on this grid and keep the fastest
+        (state untouched); returns the rows chosen (0: one launch per stage)."""
+        r = C.c_int()
+        _check(load_library().sbp_tune_fused(self._h, float(dt), C.byref(r)))
+        return r.value
 
     # -- norms -------------------------------------------------------------
     def energy(self, v: np.ndarray) -> float:

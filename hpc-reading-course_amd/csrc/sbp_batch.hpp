@@ -162,6 +162,7 @@ __device__ __forceinline__ PairArgs<T> batch_pair_args(const BatchInst<T> &d)
     a.comp = d.comp;
     a.c1 = T(dt);
     a.c2 = T(dt / 6);
+    a.c0 = T(0.5 * dt);
   }
   return a;
 }

@@ -44,7 +44,8 @@ constexpr BatchConfig kConfigs[] = {
     {256, 64, 2, 1, 0}, {128, 32, 8, 1, 0}, {128, 64, 8, 1, 0}, {128, 16, 8, 1, 0}, {128, 32, 4, 1, 0},
     {128, 64, 4, 1, 0}, {128, 32, 16, 1, 0}, {256, 8, 4, 1, 0},  {256, 4, 4, 1, 0},  {128, 8, 8, 1, 0},
     {128, 4, 8, 1, 0},  {256, 8, 4, 4, 0},  {256, 4, 4, 4, 0},  {128, 8, 8, 4, 0},  {128, 4, 8, 4, 0},
-    {256, 16, 4, 0, 1}, {256, 8, 4, 0, 1},  {256, 32, 4, 0, 1}, {256, 16, 2, 0, 1},
+    {256, 16, 4, 0, 1}, {256, 8, 4, 0, 1},  {256, 32, 4, 0, 1}, {256, 16, 2, 0, 1}, {256, 16, 3, 0, 1},
+    {256, 24, 3, 0, 1}, {256, 12, 3, 0, 1}, {256, 24, 2, 0, 1}, {256, 32, 2, 0, 1},
 };
 constexpr int kNumConfigs = sizeof(kConfigs) / sizeof(kConfigs[0]);
 constexpr int kAlignB = 64;

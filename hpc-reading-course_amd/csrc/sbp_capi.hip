@@ -104,10 +104,13 @@ void compute_pair_geometry(sbp_ctx *c)
   c->p_n_edge_blocks = (eb + kNumXcd - 1) / kNumXcd * kNumXcd;
 }
 
-// Default pair-tile height (measured on gfx950 at 2049^2 and 4097^2): 16
-// rows for orders 2 and 4, 48 for order 6 (its 4R = 12 halo rows per tile
-// favour taller tiles), halved down to 8 while the grid has fewer than 768
-// tiles (two FP64 blocks per CU on 256 CUs, 1.5 rounds).
+// Default pair-tile height, from sweeps on gfx950 (ms per two-stage step):
+// FP64 orders 2 and 4 take about ny/100 rows, at most 32 (4097^2: 32 rows
+// 0.48 against 16 rows 0.50; 1025^2: 10 rows 0.063 against 8 rows 0.071);
+// FP64 order 6 68 rows (4097^2 with rho/K arrays: 0.77 against 48 rows
+// 0.83), its 4R = 12 halo rows per tile favouring tall tiles; FP32 16 rows
+// (order 6: 48).  Halved while the grid would have fewer than 400 tiles.
+// sbp_tune_fused measures the candidates on the real grid instead.
 int default_pair_rows(const sbp_ctx *c)
 {
   const int R = c->order / 2, D = c->cls + R;
@@ -115,8 +118,12 @@ int default_pair_rows(const sbp_ctx *c)
   if (wi <= 0 || hi <= 0) return 16;
   const int nt = pair_nt((int)c->esz, c->order);
   const int tx = (wi + nt - 2 * R - 1) / (nt - 2 * R);
-  int t = c->order == 6 ? 48 : 16;
-  while (t > 8 && (long)tx * ((hi + t - 1) / t) < 768) t /= 2;
+  int t;
+  if (c->esz == 8)
+    t = c->order == 6 ? 68 : std::min(32, std::max(8, (hi + 50) / 100 / 2 * 2));
+  else
+    t = c->order == 6 ? 48 : 16;
+  while (t > 8 && (long)tx * ((hi + t - 1) / t) < 400) t /= 2;
   return t;
 }
 
@@ -246,16 +253,16 @@ template <typename T> PairArgs<T> make_pair_args(const sbp_ctx *c, int pair, dou
   a.ty_rows = c->p_ty_rows;
   a.n_tiles = c->p_n_tiles;
   a.n_edge_blocks = c->p_n_edge_blocks;
-  // bytes a two-stage step streams: U, Y2, Y3, materials, compensation
+  // bytes a two-stage step streams: U, Y3, U' (in the Y2 array), materials, compensation
   const double streamed = (double)c->plane * c->esz * (9 + (c->rinv ? 2 : 0) + (c->KC ? 3 : 0));
   a.nts = c->store_policy == 0 ? (pair_cached_stores(streamed) ? 0 : 1) : (c->store_policy == 2 ? 1 : 0);
-  if (pair == 1) {  // stages 1-2: U -> Y2 (Y0), Y3 (Y1)
+  if (pair == 1) {  // stages 1-2: U -> Y3 (Y1), Y2 (Y0) next to the frame
     a.yin = static_cast<const T *>(c->U);
     a.o1 = static_cast<T *>(c->Y0);
     a.o2 = static_cast<T *>(c->Y1);
     a.c1 = T(0.5 * dt);
     a.c2 = T(0.5 * dt);
-  } else {  // stages 3-4: Y3, U, Y2 -> U' (into Y0), Y4 (A) next to the frame
+  } else {  // stages 3-4: Y3, U -> U' (into Y0), Y4 (A) next to the frame
     a.yin = static_cast<const T *>(c->Y1);
     a.u = static_cast<const T *>(c->U);
     a.y2 = static_cast<const T *>(c->Y0);
@@ -264,6 +271,7 @@ template <typename T> PairArgs<T> make_pair_args(const sbp_ctx *c, int pair, dou
     a.comp = static_cast<T *>(c->KC);
     a.c1 = T(dt);
     a.c2 = T(dt / 6);
+    a.c0 = T(0.5 * dt);
   }
   return a;
 }
@@ -873,6 +881,69 @@ int sbp_set_fused(sbp_ctx *ctx, int on, int rows)
 }
 
 int sbp_get_fused(const sbp_ctx *ctx) { return ctx && ctx->fused ? 1 : 0; }
+
+int sbp_tune_fused(sbp_ctx *ctx, double dt, int *rows)
+{
+  if (!ctx || !(dt > 0)) return fail(SBP_EINVAL, "bad argument");
+  if (!ctx->have_state) return fail(SBP_ESTATE, "no state loaded");
+  if (!ctx->fused) {
+    if (rows) *rows = 0;
+    return SBP_OK;
+  }
+  const int R = ctx->order / 2, D = ctx->cls + R;
+  const int hi = ctx->ny - 2 * D;
+  static const int cand[] = {8, 10, 12, 16, 20, 24, 32, 48, 68, 96, 136};
+  // the trial steps advance the state: save it (and its compensation) first
+  const size_t reg = ctx->esz * (size_t)ctx->C * ctx->plane;
+  void *save_u = nullptr, *save_c = nullptr;
+  int rc = dalloc(&save_u, reg);
+  if (!rc && ctx->KC) rc = dalloc(&save_c, reg);
+  auto cleanup = [&]() {
+    if (save_u) (void)hipFree(save_u);
+    if (save_c) (void)hipFree(save_c);
+  };
+  if (rc) {
+    cleanup();
+    return rc;
+  }
+  hipError_t e = hipMemcpyAsync(save_u, ctx->U, reg, hipMemcpyDeviceToDevice, ctx->stream);
+  if (e == hipSuccess && ctx->KC) e = hipMemcpyAsync(save_c, ctx->KC, reg, hipMemcpyDeviceToDevice, ctx->stream);
+  const int start_rows = ctx->p_ty_rows;
+  int best = start_rows;
+  float best_ms = 1e30f;
+  for (int t : cand) {
+    if (e != hipSuccess) break;
+    if (t > hi && t != cand[0]) continue;
+    drop_graph(ctx);
+    ctx->p_ty_rows = t;
+    compute_pair_geometry(ctx);
+    e = launch_step(ctx, 0.0, dt);  // warm-up
+    float total = 0;
+    for (int rep = 0; rep < 3 && e == hipSuccess; rep++) {
+      e = hipEventRecord(ctx->ev[0], ctx->stream);
+      if (e == hipSuccess) e = launch_step(ctx, 0.0, dt);
+      if (e == hipSuccess) e = hipEventRecord(ctx->ev[1], ctx->stream);
+      if (e == hipSuccess) e = hipEventSynchronize(ctx->ev[1]);
+      float ms = 0;
+      if (e == hipSuccess) e = hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]);
+      total += ms;
+    }
+    if (e == hipSuccess && total < best_ms) {
+      best_ms = total;
+      best = t;
+    }
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(ctx->U, save_u, reg, hipMemcpyDeviceToDevice, ctx->stream);
+  if (e == hipSuccess && ctx->KC) e = hipMemcpyAsync(ctx->KC, save_c, reg, hipMemcpyDeviceToDevice, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  cleanup();
+  drop_graph(ctx);
+  ctx->p_ty_rows = e == hipSuccess ? best : start_rows;
+  compute_pair_geometry(ctx);
+  if (e != hipSuccess) return fail(SBP_EHIP, std::string("tile tuning: ") + hipGetErrorString(e));
+  if (rows) *rows = best;
+  return SBP_OK;
+}
 
 int sbp_set_store_policy(sbp_ctx *ctx, int policy)
 {

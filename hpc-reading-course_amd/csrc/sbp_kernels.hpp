@@ -566,21 +566,26 @@ __global__ __launch_bounds__(BX) void stage2d_kernel(StageArgs<T> a)
 // --------------------------------------------------------------------------
 // Two RK4 stages per launch on the interior (temporal blocking, acoustics).
 // --------------------------------------------------------------------------
-//   pair 1: K1 = F(U),  Y2 = U + dt/2 K1 on the tile grown by R (LDS only),
-//           K2 = F(Y2), Y3 = U + dt/2 K2 on the tile.        (reads U; writes Y2, Y3)
-//   pair 2: K3 = F(Y3), Y4 = U + dt K3 on the tile grown by R (LDS only),
+//   pair 1: K1 = F(U),  Y2 = U + dt/2 K1 on the tile grown by R (on chip),
+//           K2 = F(Y2), Y3 = U + dt/2 K2 on the tile.        (reads U; writes Y3)
+//   pair 2: K3 = F(Y3), Y4 = U + dt K3 on the tile grown by R (on chip),
+//           K1 = F(U),  Y2 = U + dt/2 K1 on the tile (recomputed, not read),
 //           K4 = F(Y4), U' = U + ((Y2-U) + 2(Y3-U) + (Y4-U))/3 + dt/6 K4
-//                                              (reads Y3, U, Y2; writes U')
-// 3 + 4 = 7 words per value and step instead of the 13 of one launch per
-// stage.  Tiles cover [D, nx-D) x [D, ny-D) with D = CLS + R, so every point
-// of a grown tile is an interior point of the operator.  The D-deep frame runs
-// the one-stage edge path: stage 1 (pair-1 launch), stage 2 (own launch),
-// stage 3 (pair-2 launch), stage 4 (own launch).  Pair 2 therefore also stores
-// Y4 on the interior points next to the frame, which the frame's stage 4
-// differentiates, and writes U' into the Y2 array (Y2 is read only at the
-// point itself there, while U is read R points around it by neighbouring
-// tiles); the host swaps the roles of the U and Y2 arrays after each step.
-// Per point the arithmetic is that of the one-stage kernels, term for term.
+//                                              (reads Y3, U; writes U')
+// 2 + 3 = 5 words per value and step instead of the 13 of one launch per
+// stage.  Recomputing K1 from U, which pair 2 reads R rows around every
+// output point anyway, costs about 40 FLOP per point and saves a whole
+// array pass; K1 is evaluated by the same code on the same values as in
+// pair 1, so Y2 is the same to the bit.  Tiles cover [D, nx-D) x [D, ny-D)
+// with D = CLS + R, so every point of a grown tile is an interior point of
+// the operator.  The D-deep frame runs the one-stage edge path: stage 1
+// (pair-1 launch), stage 2 (own launch), stage 3 (pair-2 launch), stage 4
+// (own launch).  The frame's stages 2 and 4 differentiate Y2 and Y4 at the
+// interior points next to the frame, so pair 1 stores Y2 and pair 2 stores
+// Y4 on that band; pair 2 writes U' into the Y2 array (U is read R points
+// around each point by neighbouring tiles); the host swaps the roles of the
+// U and Y2 arrays after each step.  Per point the arithmetic is that of the
+// one-stage kernels, term for term.
 //
 // A block of NT threads owns NT mid columns (x0-R .. x0-R+NT-1) and the
 // NT-2R output columns inside them, and marches down the rows: iteration k
@@ -595,15 +600,16 @@ constexpr int pair_nt(int esz, int order) { return (void)esz, (void)order, 256; 
 template <typename T> struct PairArgs {
   const T *yin;  // stencil operand of the outer stage: U (pair 1), Y3 (pair 2)
   const T *u;    // U (pair 2)
-  const T *y2;   // Y2 (pair 2)
-  T *o1;         // pair 1: Y2;  pair 2: U' (the y2 array, read first at each point)
-  T *o2;         // pair 1: Y3;  pair 2: Y4 on the points next to the frame
+  const T *y2;   // unused (pair 2 recomputes Y2)
+  T *o1;         // pair 1: Y2 on the band next to the frame;  pair 2: U' (the Y2 array)
+  T *o2;         // pair 1: Y3;  pair 2: Y4 on the band next to the frame
   T *comp;       // Kahan compensation of U (pair 2)
   const T *rinv, *bulk;
   long plane;
   int pitch, xoff, nx, ny;
   T hix, hiy;
   T c1, c2, third;  // pair 1: dt/2, dt/2;  pair 2: dt, dt/6
+  T c0;             // pair 2: dt/2 (Y2 = U + dt/2 K1)
   int tiles_x, ty_rows, n_tiles, n_edge_blocks;
   int nts;  // 1: nontemporal output stores; 0: cached (pair_cached_stores); selects the kernel
 };
@@ -623,7 +629,7 @@ template <typename T> struct PairRow {
   T h[2];         // u, p at input row r, halo column
   T U[3];         // pair 2: U at mid row r-R
   T ri, K;        // materials at mid row r-R
-  T Y2[3], C[3];  // pair 2: Y2 and compensation at output row r-2R
+  T C[3];         // pair 2 (Kahan): compensation at output row r-2R
 };
 
 // Two-stage tile: each thread keeps its own column's last 2R+1 rows of the
@@ -634,8 +640,11 @@ template <typename T> struct PairRow {
 // slower).
 //   phase 0: stage operand row r (u, p with the x halo) in LDS
 //   phase 1: dx of row r into a register FIFO; mid row m = r-R from the
-//            windows and the FIFO's oldest entry; its u, p into LDS
-//   phase 2: dx of mid row m into a second FIFO; output row o = r-2R
+//            windows and the FIFO's oldest entry; its u, p into LDS (pair 2:
+//            also the u, p of U at row m)
+//   phase 2: dx of mid row m (pair 2: and of U at row m) into second FIFOs;
+//            output row o = r-2R (pair 2: K1 at row o from the window of U
+//            rows o-R..o+R and the FIFO of its x-derivatives)
 // LDS rows are single-buffered: each row written in one phase is read in the
 // next, and the barrier between them also separates the read from the next
 // iteration's write.
@@ -651,6 +660,7 @@ __device__ __forceinline__ void pair_tile_rw(const PairArgs<T> &a, int tile)
 
   __shared__ T s_u[LW], s_p[LW];      // operand row r (x halo included)
   __shared__ T s_mu[NT], s_mp[NT];    // mid row m
+  __shared__ T s_Uu[PAIR == 2 ? NT : 1], s_Up[PAIR == 2 ? NT : 1];  // pair 2: U at row m
 
   const int t = threadIdx.x;
   const int tx = tile % a.tiles_x, tyb = tile / a.tiles_x;
@@ -690,13 +700,10 @@ __device__ __forceinline__ void pair_tile_rw(const PairArgs<T> &a, int tile)
       q.ri = a.rinv[om];
       q.K = a.bulk[om];
     }
-    if constexpr (PAIR == 2) {
+    if constexpr (PAIR == 2 && KAHAN) {
       const long oo = (long)max(r - 2 * R, 0) * pitch + col;
 #pragma unroll
-      for (int c = 0; c < 3; c++) {
-        q.Y2[c] = ldg<true>(a.y2 + c * plane + oo);
-        if constexpr (KAHAN) q.C[c] = ldg<true>(a.comp + c * plane + oo);
-      }
+      for (int c = 0; c < 3; c++) q.C[c] = ldg<true>(a.comp + c * plane + oo);
     }
   };
 
@@ -710,19 +717,23 @@ __device__ __forceinline__ void pair_tile_rw(const PairArgs<T> &a, int tile)
   // windows of the last 2R+1 rows (oldest first): operand rows r-2R..r, mid rows m-2R..m
   T win[3][WN], wmid[3][WN];
   // FIFOs of the last R+1 rows: dx of the operand (u, p), dx of the mid stage,
-  // materials and (pair 2) U of the mid rows
-  T fdx[2][R + 1], gdx[2][R + 1], fri[R + 1], fK[R + 1], fU[R + 1][3];
+  // (pair 2) dx of U (u, p), materials; (pair 2) window of U over the last
+  // 2R+1 mid rows (oldest first; fU[R] is U at the output row)
+  constexpr int NU = PAIR == 2 ? WN : R + 1;
+  T fdx[2][R + 1], gdx[2][R + 1], hdx[2][R + 1], fri[R + 1], fK[R + 1], fU[NU][3];
 #pragma unroll
   for (int s = 0; s < WN; s++)
 #pragma unroll
     for (int c = 0; c < 3; c++) win[c][s] = wmid[c][s] = T(0);
 #pragma unroll
   for (int p = 0; p <= R; p++) {
-    fdx[0][p] = fdx[1][p] = gdx[0][p] = gdx[1][p] = T(0);
+    fdx[0][p] = fdx[1][p] = gdx[0][p] = gdx[1][p] = hdx[0][p] = hdx[1][p] = T(0);
     fri[p] = fK[p] = T(1);
+  }
+#pragma unroll
+  for (int p = 0; p < NU; p++)
 #pragma unroll
     for (int c = 0; c < 3; c++) fU[p][c] = T(0);
-  }
 
   for (int kb = 0; kb < nk; kb += S) {
 #pragma unroll
@@ -771,17 +782,23 @@ __device__ __forceinline__ void pair_tile_rw(const PairArgs<T> &a, int tile)
         }
         s_mu[t] = wmid[0][WN - 1];
         s_mp[t] = wmid[2][WN - 1];
+        if constexpr (PAIR == 2) {
+          s_Uu[t] = Um[0];
+          s_Up[t] = Um[2];
+        }
 #pragma unroll
         for (int p = 0; p < R; p++) {
           fri[p] = fri[p + 1];
           fK[p] = fK[p + 1];
-#pragma unroll
-          for (int c = 0; c < 3; c++) fU[p][c] = fU[p + 1][c];
         }
         fri[R] = ri;
         fK[R] = Kb;
 #pragma unroll
-        for (int c = 0; c < 3; c++) fU[R][c] = Um[c];
+        for (int p = 0; p < NU - 1; p++)
+#pragma unroll
+          for (int c = 0; c < 3; c++) fU[p][c] = fU[p + 1][c];
+#pragma unroll
+        for (int c = 0; c < 3; c++) fU[NU - 1][c] = Um[c];
       }
       __syncthreads();
 
@@ -792,9 +809,20 @@ __device__ __forceinline__ void pair_tile_rw(const PairArgs<T> &a, int tile)
           gdx[0][p] = gdx[0][p + 1];
           gdx[1][p] = gdx[1][p + 1];
         }
+        if constexpr (PAIR == 2) {
+#pragma unroll
+          for (int p = 0; p < R; p++) {
+            hdx[0][p] = hdx[0][p + 1];
+            hdx[1][p] = hdx[1][p + 1];
+          }
+        }
         if (out_col) {
           gdx[0][R] = dint<T, ORDER>([&](int s) { return s_mu[t - R + s]; }, a.hix);
           gdx[1][R] = dint<T, ORDER>([&](int s) { return s_mp[t - R + s]; }, a.hix);
+          if constexpr (PAIR == 2) {
+            hdx[0][R] = dint<T, ORDER>([&](int s) { return s_Uu[t - R + s]; }, a.hix);
+            hdx[1][R] = dint<T, ORDER>([&](int s) { return s_Up[t - R + s]; }, a.hix);
+          }
         }
       }
       if (k >= 4 * R && k < nk && out_col) {
@@ -804,19 +832,26 @@ __device__ __forceinline__ void pair_tile_rw(const PairArgs<T> &a, int tile)
         T F[3];
         acoustic_F<T, MAT>(gdx[1][0], dyp, gdx[0][0], dyv, fri[0], fK[0], T(0), T(0), F[0], F[1], F[2]);
         const long qo = (long)o * pitch + a.xoff + mc;
+        // the band next to the frame, whose stage-2 and stage-4 rows read Y2 and Y4 here
+        const bool ring = mc - D < RW || a.nx - D - 1 - mc < RW || o - D < RW || a.ny - D - 1 - o < RW;
         if constexpr (PAIR == 1) {
 #pragma unroll
           for (int c = 0; c < 3; c++) {
-            put(a.o1 + c * plane + qo, wmid[c][R]);
+            if (ring) a.o1[c * plane + qo] = wmid[c][R];
             put(a.o2 + c * plane + qo, win[c][0] + a.c2 * F[c]);
           }
         } else {
-          const bool ring = mc - D < RW || a.nx - D - 1 - mc < RW || o - D < RW || a.ny - D - 1 - o < RW;
+          // K1 = F(U) at the output row and Y2 = U + dt/2 K1, as pair 1 formed them
+          const T dyp1 = dint<T, ORDER>([&](int s) { return fU[s][2]; }, a.hiy);
+          const T dyv1 = dint<T, ORDER>([&](int s) { return fU[s][1]; }, a.hiy);
+          T F1[3];
+          acoustic_F<T, MAT>(hdx[1][0], dyp1, hdx[0][0], dyv1, fri[0], fK[0], T(0), T(0), F1[0], F1[1], F1[2]);
 #pragma unroll
           for (int c = 0; c < 3; c++) {
-            const T U = fU[0][c];
+            const T U = fU[R][c];
+            const T Y2 = U + a.c0 * F1[c];
             const T Y4 = wmid[c][R];
-            const T inc = ((cur.Y2[c] - U) + 2 * (win[c][0] - U) + (Y4 - U)) * a.third + a.c2 * F[c];
+            const T inc = ((Y2 - U) + 2 * (win[c][0] - U) + (Y4 - U)) * a.third + a.c2 * F[c];
             if constexpr (!KAHAN) {
               put(a.o1 + c * plane + qo, U + inc);
             } else {

@@ -118,6 +118,12 @@ int  sbp_set_tile_rows(sbp_ctx *ctx, int rows);
  * device (ts_rk.cpp:54-65), not what they compute. */
 int  sbp_set_fused(sbp_ctx *ctx, int on, int rows);
 int  sbp_get_fused(const sbp_ctx *ctx);
+/* Time the two-stage schedule's candidate tile heights (8 .. 136 rows) on
+ * this grid for step dt and keep the fastest; *rows (optional) receives it
+ * (0 when the context runs one launch per stage).  The resident state is
+ * saved and restored, so tuning leaves it untouched; a tuning knob with no
+ * reference counterpart, results do not depend on it. */
+int  sbp_tune_fused(sbp_ctx *ctx, double dt, int *rows);
 /* Store hint of the two-stage tiles' outputs: 0 (default) chooses from the
  * bytes a step streams (cached stores while U, Y2, Y3 and the materials fit
  * the 256 MB Infinity Cache, nontemporal ones beyond), 1 forces cached and 2

@@ -270,3 +270,30 @@ def test_store_policy_rejects_bad_values(capi):
             with pytest.raises(capi.SbpError) as e:
                 ctx.set_store_policy(bad)
             assert e.value.code == -1
+
+
+@pytest.mark.parametrize("prec", [P.FP64, P.FP32_KAHAN])
+def test_tune_fused_keeps_state_and_results(capi, prec):
+    """sbp_tune_fused times the tile heights with real steps but restores the
+    state (and its compensation) afterwards; a tuned context then steps
+    exactly like an untuned one (FP64: bit-identical; every tile height gives
+    the same FP64 result)."""
+    spec, v0, dt = P.baseline_acoustic(1025, 4, precision=prec)
+    say("tune fused", prec)
+    with capi.Context(spec) as ctx:
+        ctx.set_state(v0)
+        before = ctx.get_state()
+        rows = ctx.tune_fused(dt)
+        assert rows in (8, 10, 12, 16, 20, 24, 32, 48, 68, 96, 136)
+        assert np.array_equal(ctx.get_state(), before)
+        ctx.step(0.0, dt, 21)
+        tuned = ctx.get_state()
+    untuned = run(capi, spec, v0, dt, 21, True)
+    if prec == P.FP64:
+        assert np.array_equal(tuned, untuned)
+    else:
+        assert relerr(tuned, untuned) <= 2e-6
+    small, v1 = P.wave_hom(60, 4)
+    with capi.Context(small) as ctx:   # one launch per stage: nothing to tune
+        ctx.set_state(v1)
+        assert ctx.tune_fused(0.001) == 0

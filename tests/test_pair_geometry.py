@@ -10,8 +10,10 @@ Checks, for many grid shapes and all orders, that
 * every mid point an output point differentiates is computed by the same
   tile, at an unclamped column, from input columns the tile staged, and is an
   interior point of the operator (so the interior stencil is the right row);
-* every interior point the frame's stage 4 reads (zero-padded taps
-  included) is one of the points pair 2 stores Y4 at;
+* every interior point the frame's stages 2 and 4 read (zero-padded taps
+  included) is one of the points pair 1 stores Y2 and pair 2 stores Y4 at
+  (the band next to the frame: pair 1 stores Y2 nowhere else, pair 2
+  recomputes it);
 * the register windows and FIFOs hold the rows each phase reads
   (pair_tile_rw);
 * the ensemble kernels' persistent loops take every work item of every XCD
@@ -34,8 +36,11 @@ def default_pair_rows(nx, ny, order, esz=8):
     if wi <= 0 or hi <= 0:
         return 16
     tx = (wi + NT - 2 * r - 1) // (NT - 2 * r)
-    t = 48 if order == 6 else 16
-    while t > 8 and tx * ((hi + t - 1) // t) < 768:
+    if esz == 8:
+        t = 68 if order == 6 else min(32, max(8, (hi + 50) // 100 // 2 * 2))
+    else:
+        t = 48 if order == 6 else 16
+    while t > 8 and tx * ((hi + t - 1) // t) < 400:
         t //= 2
     return t
 
@@ -64,7 +69,9 @@ def simulate_windows(order, nrows):
     wmid = [None] * WN         # mid rows
     fdx = [None] * (r + 1)     # dx of operand rows
     gdx = [None] * (r + 1)     # dx of mid rows
-    fmat = [None] * (r + 1)    # materials / U of mid rows
+    hdx = [None] * (r + 1)     # pair 2: dx of U at the mid rows
+    fmat = [None] * (r + 1)    # materials of mid rows
+    fU = [None] * WN           # pair 2: U of the mid rows, oldest first
     nk = nrows + 4 * r
     for k in range(nk):
         row = -2 * r + k
@@ -76,11 +83,15 @@ def simulate_windows(order, nrows):
             assert win[r] == m                     # U(m) of pair 1
             wmid = wmid[1:] + [m]
             fmat = fmat[1:] + [m]
+            fU = fU[1:] + [m]
             gdx = gdx[1:] + [m]                    # phase 2: dx of mid row m
+            hdx = hdx[1:] + [m]                    #          and of U at row m
         if k >= 4 * r:
             o = -4 * r + k
             assert wmid == list(range(o - r, o + r + 1)), (order, k, wmid)
             assert gdx[0] == o and fmat[0] == o and wmid[r] == o and win[0] == o
+            # pair 2 recomputes K1 at row o: U over rows o-R..o+R, dx of U at row o
+            assert fU == list(range(o - r, o + r + 1)) and fU[r] == o and hdx[0] == o
 
 
 def pair_pf(esz, pf=2):
@@ -197,9 +208,12 @@ def test_pair_register_windows(order, nrows):
 
 def test_pair_geometry_large_grid_parameters():
     g = pair_geometry(4097, 4097, 4)
-    assert g["d"] == 6 and g["tiles_x"] == 17 and g["rows"] == 16 and g["n_tiles"] == 17 * 256
-    assert pair_geometry(4097, 4097, 6)["rows"] == 48 and pair_geometry(4097, 4097, 6)["nt"] == 256
-    assert pair_geometry(1025, 1025, 4)["rows"] == 8
+    assert g["d"] == 6 and g["tiles_x"] == 17 and g["rows"] == 32 and g["n_tiles"] == 17 * 128
+    assert pair_geometry(4097, 4097, 6)["rows"] == 68 and pair_geometry(4097, 4097, 6)["nt"] == 256
+    assert pair_geometry(4097, 4097, 6, esz=4)["rows"] == 48
+    assert pair_geometry(4097, 4097, 4, esz=4)["rows"] == 16
+    assert pair_geometry(1025, 1025, 4)["rows"] == 10
+    assert pair_geometry(1025, 1025, 4, esz=4)["rows"] == 8
     check(4097, 300, 4)
     check(1025, 1025, 4)
 
