@@ -102,6 +102,9 @@ def launch_bytes(spec, fused):
     return [float(b * w) for b in (b1, b2, b3, b4)]
 
 
+TILE_ROWS = 0  # --tile-rows: fixed two-stage tile height instead of sbp_tune_fused (profiling runs)
+
+
 def gpu_run(spec, v0, dt, steps, warmup, stage_steps=None, fused=None):
     capi = pkg("capi")
     ctx = capi.Context(spec)
@@ -110,7 +113,11 @@ def gpu_run(spec, v0, dt, steps, warmup, stage_steps=None, fused=None):
             ctx.set_fused(fused)
         fused = ctx.fused
         ctx.set_state(v0)
-        rows = ctx.tune_fused(dt)  # tile height measured on this grid (state kept), before the timed region
+        if fused and TILE_ROWS:
+            ctx.set_fused(True, TILE_ROWS)
+            rows = TILE_ROWS
+        else:
+            rows = ctx.tune_fused(dt)  # tile height measured on this grid (state kept), before the timed region
         if warmup:
             ctx.step(0.0, dt, warmup)
         ctx.synchronize()
@@ -271,7 +278,13 @@ def main():
                     help="only run the 1 GiB plain copy kernel (HBM counter calibration)")
     ap.add_argument("--tta-only", action="store_true",
                     help="only run the time-to-L2-1e-4 comparison (plain FP64, batched FP32, learned)")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="only run the 256-instance FP32 ensemble (--steps timed steps)")
+    ap.add_argument("--tile-rows", type=int, default=0,
+                    help="two-stage tile height of the main workload (default: tuned on the grid)")
     args = ap.parse_args()
+    global TILE_ROWS
+    TILE_ROWS = args.tile_rows
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the SBP-SAT path here is built and timed on a single "
               "MI355X (multi-GPU sharding is a design note, not a measured path); run with --gpus 1.",
@@ -289,6 +302,9 @@ def main():
         return
     if args.tta_only:
         print(json.dumps(time_to_l2()), flush=True)
+        return
+    if args.batched_only:
+        print(json.dumps(batched(steps=args.steps, warmup=args.warmup)), flush=True)
         return
 
     spec, v0, dt = P.baseline_acoustic(args.n, args.order)

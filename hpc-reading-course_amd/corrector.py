@@ -259,15 +259,29 @@ def _l2(spec, v, ref):
     return float(np.sqrt(spec.hx * spec.hy) * np.linalg.norm(v - ref))
 
 
+def interp_time_at_error(points, target):
+    """Time at error `target` on a curve of (time, error) points, piecewise
+    linear in (log error, log time) between the two points that bracket it;
+    None when no pair of points brackets it."""
+    pts = sorted(points, key=lambda p: p[1])
+    for (t_a, e_a), (t_b, e_b) in zip(pts, pts[1:]):
+        if e_a <= target <= e_b and e_b > e_a:
+            w = (np.log(target) - np.log(e_a)) / (np.log(e_b) - np.log(e_a))
+            return float(np.exp(np.log(t_a) + w * (np.log(t_b) - np.log(t_a))))
+    return None
+
+
 def time_to_accuracy(params, norm, band=BAND, n_eval=32, seed0=900000, t_end=0.5, plain_n=513, bound=1e-4,
-                     log=print):
+                     log=print, equal_error_grids=(321, 385, 449)):
     """Wall time of the ways to solve n_eval held-out instances to t_end, each
     measured by the L2 error against the FP64 1025^2 solution injected at its
     own grid nodes:
       plain         FP64 single-problem path, one instance after another, plain_n^2
       batched_fp32  FP32 ensemble, plain_n^2, one persistent kernel per stage
       learned       FP32 ensemble on 257^2 + the corrector after every step
-    and, for comparison, the FP64 ensemble at plain_n^2 (batched_fp64), the
+    the plain path's time-versus-error curve over 257^2 .. plain_n^2 (grids in
+    between measured against their own 4x-refined FP64 references) and the
+    time it needs at the learned form's mean and worst error; and, for comparison, the FP64 ensemble at plain_n^2 (batched_fp64), the
     learned form with FP64 stages (learned_fp64), the plain 257^2 solve
     (the method without the network), and the plain FP64 path and the FP32
     ensemble with the order-6 operators on 257^2."""
@@ -359,6 +373,41 @@ def time_to_accuracy(params, norm, band=BAND, n_eval=32, seed0=900000, t_end=0.5
     res["speedup_batched_fp32"] = tp / tb
     res["speedup_learned"] = tp / tl
     res["gain_learned_vs_plain_coarse_l2"] = float(np.mean(ec) / res["learned"]["mean_l2"])
+
+    # Equal error: the plain FP64 path on grids between 257^2 and plain_n^2,
+    # each against its own 4x-refined FP64 reference (they do not nest in the
+    # 1025^2 one), gives its time-versus-error curve; the time it needs at the
+    # learned form's error (mean and worst instance) is interpolated on it
+    # (piecewise linear in log time against log error).
+    curve = [(N_COARSE, tc, float(np.mean(ec)), max(ec)), (plain_n, tp, float(np.mean(ep)), max(ep))]
+    for n in equal_error_grids:
+        nfine = RATIO * (n - 1) + 1
+        fine_n = _instances(seeds, nfine, P.FP64)
+        dtf = P.dt_from_cfl(fine_n[0][0], CFL)
+        with capi.Batch([f[0] for f in fine_n]) as b:
+            for k, (_, v0) in enumerate(fine_n):
+                b.set_state(k, v0)
+            b.step(dtf, int(round(t_end / dtf)))
+            b.synchronize()
+            refs_n = [b.get_state_strided(k, RATIO) for k in range(n_eval)]
+        del fine_n
+        tn, en = run_plain(n, P.FP64, refs_n)
+        curve.append((n, tn, float(np.mean(en)), max(en)))
+        log("plain FP64 %d^2 (vs its %d^2 reference): %.3f s, mean L2 %.2e, max %.2e"
+            % (n, nfine, tn, float(np.mean(en)), max(en)))
+    curve.sort()
+    res["plain_fp64_curve"] = [{"grid": n, "wall_s": t_, "mean_l2": em, "max_l2": ex} for n, t_, em, ex in curve]
+    for which, col in (("mean", 2), ("max", 3)):
+        target = res["learned"][which + "_l2"]
+        t_eq = interp_time_at_error([(c[1], c[col]) for c in curve], target)
+        res["plain_time_at_learned_%s_l2" % which] = t_eq
+        res["speedup_learned_equal_%s_l2" % which] = t_eq / tl if t_eq else None
+        tb_err = res["batched_fp32"][which + "_l2"]
+        res["speedup_batched_fp32_equal_%s_l2" % which] = (
+            interp_time_at_error([(c[1], c[col]) for c in curve], tb_err) or 0) / tb
+    log("equal error: plain FP64 needs %.3f s (mean) / %.3f s (max) at the learned form's error: "
+        "%.2fx / %.2fx" % (res["plain_time_at_learned_mean_l2"] or -1, res["plain_time_at_learned_max_l2"] or -1,
+                           res["speedup_learned_equal_mean_l2"] or -1, res["speedup_learned_equal_max_l2"] or -1))
 
     # the same comparison with the order-6 operators on the coarse grid: the
     # plain FP64 path one instance after another and the FP32 ensemble

@@ -130,8 +130,7 @@ int default_pair_rows(const sbp_ctx *c)
 bool pair_supported(const sbp_ctx *c)
 {
   const int D = c->cls + c->order / 2;
-  return c->pair_launch && c->phys == SBP_ACOUSTIC_2D && (c->mat == SBP_MAT_UNIT || c->mat == SBP_MAT_ARRAYS) &&
-         c->nx > 2 * D && c->ny > 2 * D;
+  return c->pair_launch && c->phys == SBP_ACOUSTIC_2D && c->nx > 2 * D && c->ny > 2 * D;
 }
 
 // Two-stage stepping is the default where it was measured faster: grids of
@@ -236,9 +235,22 @@ hipError_t launch_stage(sbp_ctx *c, int mode, double dt, double t, const void *y
   return c->launch(mode, &a, c->geom, c->stream);
 }
 
-template <typename T> PairArgs<T> make_pair_args(const sbp_ctx *c, int pair, double dt)
+template <typename T> PairArgs<T> make_pair_args(const sbp_ctx *c, int pair, double t, double dt)
 {
   PairArgs<T> a{};
+  // rho = 2 + xy and its forcing (SBP_MAT_REF_RHO): tables and cos(5 pi t)
+  // at the stage times t, t + dt/2 (stages 2, 3) and t + dt, as make_args forms them
+  a.xs = c->xs;
+  a.ys = c->ys;
+  a.c3x = c->c3x;
+  a.s3x = c->s3x;
+  a.c4y = c->c4y;
+  a.s4y = c->s4y;
+  const double c5_1 = std::cos(5 * kPi * t), c5_23 = std::cos(5 * kPi * (t + 0.5 * dt)),
+               c5_4 = std::cos(5 * kPi * (t + dt));
+  a.c5_mid = pair == 1 ? c5_1 : c5_23;
+  a.c5_out = pair == 1 ? c5_23 : c5_4;
+  a.c5_k1 = c5_1;
   a.rinv = static_cast<const T *>(c->rinv);
   a.bulk = static_cast<const T *>(c->bulk);
   a.plane = c->plane;
@@ -293,7 +305,7 @@ template <typename T> hipError_t launch_pair_part_t(sbp_ctx *c, int part, double
   e.n_tiles = 0;
   if (part == 4) e.uout = static_cast<T *>(c->Y0);  // U' next to the interior's; U read from uin
   if (part == 1 || part == 3) {
-    const PairArgs<T> p = make_pair_args<T>(c, part == 1 ? 1 : 2, dt);
+    const PairArgs<T> p = make_pair_args<T>(c, part == 1 ? 1 : 2, t, dt);
     return c->pair_launch(part, &e, &p, c->p_n_edge_blocks + c->p_n_tiles, c->stream);
   }
   // frame-only launch: one wave per block

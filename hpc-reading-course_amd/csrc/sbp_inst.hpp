@@ -35,6 +35,7 @@ PairLaunchFn pick_pair(int mat)
   constexpr int NT = pair_nt(sizeof(T), ORDER);
   if (mat == M_UNIT) return &PairLauncher<T, ORDER, M_UNIT, KAHAN, NT, PF1, PF2>::launch;
   if (mat == M_ARR) return &PairLauncher<T, ORDER, M_ARR, KAHAN, NT, PF1, PF2>::launch;
+  if (mat == M_REF) return &PairLauncher<T, ORDER, M_REF, KAHAN, NT, PF1, PF2>::launch;
   return nullptr;
 }
 

@@ -142,6 +142,20 @@ __device__ __forceinline__ void ref_material(const StageArgs<T> &a, int i, int j
   fv = T(-(4 * kPi * a.c5t * a.c4y[j] * a.s3x[i] * (xy + 1)) / (xy + 2));
 }
 
+// rho^-1 and the manufactured forcing at (x_i, y_j) and time factor c5t =
+// cos(5 pi t), from per-column values (x, cos 3 pi x, sin 3 pi x) and the row
+// tables: the expressions of ref_material, so every path rounds them alike.
+template <typename T>
+__device__ __forceinline__ void ref_material_at(double x, double c3x, double s3x, const double *ys, const double *c4y,
+                                                const double *s4y, int j, double c5t, T &ri, T &fu, T &fv)
+{
+  const double y = ys[j];
+  const double xy = x * y;
+  ri = T(1. / (2 + xy));
+  fu = T(-(3 * kPi * c5t * c3x * s4y[j] * (xy + 1)) / (xy + 2));
+  fv = T(-(4 * kPi * c5t * c4y[j] * s3x * (xy + 1)) / (xy + 2));
+}
+
 // RK4 stage arithmetic at one point (edge path of the 2D kernels).  Ycen are
 // the centre values of the stage operand: U in stage 1, Y4 in stage 4.
 template <typename T, bool KAHAN>
@@ -610,6 +624,11 @@ template <typename T> struct PairArgs {
   T hix, hiy;
   T c1, c2, third;  // pair 1: dt/2, dt/2;  pair 2: dt, dt/6
   T c0;             // pair 2: dt/2 (Y2 = U + dt/2 K1)
+  // the reference's rho = 2 + xy with its forcing (M_REF): per-column and
+  // per-row tables, and cos(5 pi t) at the times of the outer stage, the
+  // inner stage and (pair 2) the recomputed stage 1
+  const double *xs, *ys, *c3x, *s3x, *c4y, *s4y;
+  double c5_mid, c5_out, c5_k1;
   int tiles_x, ty_rows, n_tiles, n_edge_blocks;
   int nts;  // 1: nontemporal output stores; 0: cached (pair_cached_stores); selects the kernel
 };
@@ -707,6 +726,13 @@ __device__ __forceinline__ void pair_tile_rw(const PairArgs<T> &a, int tile)
     }
   };
 
+  double xc = 0, c3c = 0, s3c = 0;  // M_REF: x, cos 3 pi x, sin 3 pi x of this thread's column
+  if constexpr (MAT == M_REF) {
+    xc = a.xs[mcc];
+    c3c = a.c3x[mcc];
+    s3c = a.s3x[mcc];
+  }
+
   constexpr int S = PF + 1;
   PairRow<T> q[S];
 #pragma unroll
@@ -771,9 +797,11 @@ __device__ __forceinline__ void pair_tile_rw(const PairArgs<T> &a, int tile)
         T Um[3];
 #pragma unroll
         for (int c = 0; c < 3; c++) Um[c] = PAIR == 1 ? win[c][R] : cur.U[c];
-        const T ri = LD_M ? cur.ri : T(1), Kb = LD_M ? cur.K : T(1);
+        T ri = LD_M ? cur.ri : T(1), Kb = LD_M ? cur.K : T(1), fu = T(0), fv = T(0);
+        if constexpr (MAT == M_REF)
+          ref_material_at<T>(xc, c3c, s3c, a.ys, a.c4y, a.s4y, min(j0 - 3 * R + k, a.ny - 1), a.c5_mid, ri, fu, fv);
         T F[3];
-        acoustic_F<T, MAT>(fdx[1][0], dyp, fdx[0][0], dyv, ri, Kb, T(0), T(0), F[0], F[1], F[2]);
+        acoustic_F<T, MAT>(fdx[1][0], dyp, fdx[0][0], dyv, ri, Kb, fu, fv, F[0], F[1], F[2]);
 #pragma unroll
         for (int c = 0; c < 3; c++) {
 #pragma unroll
@@ -829,8 +857,14 @@ __device__ __forceinline__ void pair_tile_rw(const PairArgs<T> &a, int tile)
         const int o = j0 - 4 * R + k;
         const T dyp = dint<T, ORDER>([&](int s) { return wmid[2][s]; }, a.hiy);
         const T dyv = dint<T, ORDER>([&](int s) { return wmid[1][s]; }, a.hiy);
+        T fu = T(0), fv = T(0), fu1 = T(0), fv1 = T(0);
+        if constexpr (MAT == M_REF) {
+          T ri_o;
+          ref_material_at<T>(xc, c3c, s3c, a.ys, a.c4y, a.s4y, o, a.c5_out, ri_o, fu, fv);
+          if constexpr (PAIR == 2) ref_material_at<T>(xc, c3c, s3c, a.ys, a.c4y, a.s4y, o, a.c5_k1, ri_o, fu1, fv1);
+        }
         T F[3];
-        acoustic_F<T, MAT>(gdx[1][0], dyp, gdx[0][0], dyv, fri[0], fK[0], T(0), T(0), F[0], F[1], F[2]);
+        acoustic_F<T, MAT>(gdx[1][0], dyp, gdx[0][0], dyv, fri[0], fK[0], fu, fv, F[0], F[1], F[2]);
         const long qo = (long)o * pitch + a.xoff + mc;
         // the band next to the frame, whose stage-2 and stage-4 rows read Y2 and Y4 here
         const bool ring = mc - D < RW || a.nx - D - 1 - mc < RW || o - D < RW || a.ny - D - 1 - o < RW;
@@ -845,7 +879,7 @@ __device__ __forceinline__ void pair_tile_rw(const PairArgs<T> &a, int tile)
           const T dyp1 = dint<T, ORDER>([&](int s) { return fU[s][2]; }, a.hiy);
           const T dyv1 = dint<T, ORDER>([&](int s) { return fU[s][1]; }, a.hiy);
           T F1[3];
-          acoustic_F<T, MAT>(hdx[1][0], dyp1, hdx[0][0], dyv1, fri[0], fK[0], T(0), T(0), F1[0], F1[1], F1[2]);
+          acoustic_F<T, MAT>(hdx[1][0], dyp1, hdx[0][0], dyv1, fri[0], fK[0], fu1, fv1, F1[0], F1[1], F1[2]);
 #pragma unroll
           for (int c = 0; c < 3; c++) {
             const T U = fU[R][c];

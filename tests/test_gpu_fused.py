@@ -73,12 +73,16 @@ CASES = [
     ("unit", 4, 300, 280, P.BC_FREE_SURFACE), ("arrays", 4, 300, 280, P.BC_WALL),
     ("arrays", 6, 129, 129, P.BC_FREE_SURFACE), ("arrays", 2, 40, 33, P.BC_WALL),
     ("arrays", 6, 531, 270, P.BC_WALL), ("arrays", 4, 19, 19, P.BC_WALL), ("unit", 6, 19, 25, P.BC_WALL),
+    ("ref", 4, 77, 70, P.BC_DEFAULT), ("ref", 6, 300, 281, P.BC_DEFAULT), ("ref", 2, 45, 60, P.BC_FREE_SURFACE),
 ]
 
 
 def make(kind, order, nx, ny, bc, precision=P.FP64):
     if kind == "unit":
         spec, _ = P.wave_hom(nx, order, bc=bc)
+        return spec.with_(ny=ny, precision=precision)
+    if kind == "ref":  # demo/wave: rho = 2 + xy with the manufactured forcing
+        spec, _ = P.wave(nx, order, bc=bc)
         return spec.with_(ny=ny, precision=precision)
     return material_spec(nx, order, bc, ny=ny, precision=precision)
 
@@ -107,7 +111,7 @@ def test_fused_tile_rows_do_not_change_results(capi, rows):
 
 
 @pytest.mark.parametrize("prec,tol", [(P.FP32, 2e-6), (P.FP32_KAHAN, 2e-6)])
-@pytest.mark.parametrize("kind,order", [("unit", 4), ("arrays", 6), ("arrays", 2)])
+@pytest.mark.parametrize("kind,order", [("unit", 4), ("arrays", 6), ("arrays", 2), ("ref", 4)])
 def test_fused_fp32(capi, orc, prec, tol, kind, order):
     spec = make(kind, order, 200, 190, P.BC_WALL, precision=prec)
     say("fused fp32", prec, kind, order)
@@ -203,7 +207,7 @@ def test_fused_default_by_grid_size(capi):
 
 
 def test_fused_rejects_unsupported(capi):
-    for spec in (P.wave(41, 4)[0], P.adv2d(40, 4)[0], P.adv1d(50, 4)[0]):
+    for spec in (P.adv2d(40, 4)[0], P.adv1d(50, 4)[0]):
         with capi.Context(spec) as ctx:
             with pytest.raises(capi.SbpError) as e:
                 ctx.set_fused(True)
