@@ -23,6 +23,7 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/sbp_hip.h"
@@ -349,31 +350,77 @@ __global__ __launch_bounds__(256) void pack_raw_kernel(const float *__restrict__
 
 // --------------------------------------------------------------------------
 // Fused inference: the three convolutions of one image window in one kernel,
-// activations kept in LDS.  A block owns a window of WW = 64 columns
-// (image columns c0 .. c0+63, c0 = x0 - 3) and a chunk of output rows; it
-// marches down the rows keeping the last four rows of the input, of layer 1
-// and of layer 2 in LDS rings.  In row step t every layer reads only rows
-// written in steps t-3 .. t-1 and writes its new row into slot t % 4, so one
-// barrier per step orders everything: layer 1 produces row y0 - 5 + t, layer
-// 2 row y0 - 7 + t and layer 3 (the output) row y0 - 9 + t.  Every layer is evaluated on all 64 window
-// columns; a layer's value at window column w is exact when its inputs at
-// w-1..w+1 are, so the output is exact on w in [3, 60]: 58 output columns
-// per window.  Activations at image positions outside the image are stored
-// as zero (the zero padding of the layer-by-layer kernels).  The GEMMs run on
-// v_mfma_f32_16x16x32_bf16 (K = 32 per instruction: one tap of 32 channels,
-// or four taps of the 8 input channels), so the output equals forward()'s up
-// to the FP32 accumulation order inside an instruction.
+// activations kept in LDS.  A block owns a window of FW = 64 columns (image
+// columns c0 .. c0+63, c0 = strip * 58 - 3) and a chunk of at most
+// FUSED_MAX_ROWS output rows y0 .. y1-1, and marches down the rows.  Row step t:
+//   * stores input row y0-3+t into slot t % 2 of the input ring;
+//   * layer 1 reads input row q = y0-4+t (slot (t+1) % 2, stored in step
+//     t-1) ONCE and adds its three row taps to the partial sums of layer-1
+//     rows q+1 (tap dy = 0), q (dy = 1) and q-1 (dy = 2); row q-1 is then
+//     complete and goes (bias, ReLU, bf16) into slot t % 2 of ring 1;
+//   * layer 2 does the same with layer-1 row y0-6+t (completing row y0-7+t
+//     into ring 2), layer 3 with layer-2 row y0-8+t (completing output row
+//     y0-9+t into an LDS staging area that is written out after the march).
+// Each layer reads only what the previous step wrote, so two slots per ring
+// and one barrier per step order everything, and every activation row is
+// read from LDS once per layer instead of once per row tap.  The three
+// partial rows of layers 1 and 2 are three accumulators whose roles rotate
+// each step; the step loop is unrolled by six, so ring slots and roles are
+// compile-time constants.  Layer 3 has 3 output channels: its MFMA rows are
+// (channel c, row tap dy) = 4c + dy, so one MFMA per column tap covers all
+// three row taps, and the running sum moves on by one row tap per step inside
+// each lane's four accumulator registers.  The input rows are fetched two
+// steps ahead, and no global store is issued during the march, so a step
+// never waits for more than the load issued two steps earlier.
+//
+// The GEMMs run on v_mfma_f32_16x16x32_bf16 with the weights as the A operand
+// (M = 16 output channels) and the activations as B (N = 16 pixels of a
+// segment, K = 32: one column tap of 32 channels, or the three column taps of
+// the 8 input channels plus 8 zeros); a lane then holds 4 consecutive
+// channels of one pixel, stored to LDS as one 8-byte write.  Every layer is
+// evaluated on all 64 window columns; a value at window column w is exact
+// when its inputs at w-1..w+1 are, so the output is exact on w in [3, 60]: 58
+// output columns per window.  Activations at positions outside the image are
+// zero (the zero padding of the layer-by-layer kernels), so the output equals
+// forward()'s up to the FP32 accumulation order.
 // --------------------------------------------------------------------------
 using bf16x8 = __attribute__((ext_vector_type(8))) short;
 using vbf8 = __attribute__((ext_vector_type(8))) __bf16;
-constexpr int FW = 64, FS = FW / 16, FOUT = FW - 6;  // window width, segments, output columns
-constexpr int FPX = FW + 2;                            // pixels per LDS row (one zero pad each side)
-constexpr int F1S = 40;                                // LDS channel stride of the 32-channel rows (80 B)
+using vbf2 = __attribute__((ext_vector_type(2))) __bf16;
+using f32x2 = __attribute__((ext_vector_type(2))) float;
+using vi16x2 = __attribute__((ext_vector_type(2))) short;
+constexpr int FW = 64, FOUT = FW - 6;  // window width, output columns
+constexpr int FPX = FW + 2;            // pixels per LDS row (one zero pad each side)
+constexpr int F1S = 40;                // LDS channel stride of the 32-channel rows (80 B)
+constexpr int FUSED_HALO = 9;          // row steps beyond the chunk's rows
+constexpr int FUSED_MAX_ROWS = 32;     // output rows per block (LDS staging)
+constexpr int FIN_PARTS = FPX * C_INP / 4;  // 8-byte parts of an input row
 
 __device__ __forceinline__ f32x4 mfma32(bf16x8 a, bf16x8 b, f32x4 c)
 {
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(vbf8, a), __builtin_bit_cast(vbf8, b), c, 0, 0, 0);
 }
+
+// 4 channels of one pixel (bias already in the sum): bf16 (round to nearest
+// even), then ReLU as a signed 16-bit max with 0 (a bf16 is negative exactly
+// when its sign bit is set, and -0 becomes +0 as the float ReLU makes it), or
+// zero outside the image
+__device__ __forceinline__ uint2 relu_pack4(f32x4 acc, bool in)
+{
+  const vi16x2 z = {0, 0};
+  const vi16x2 lo = __builtin_elementwise_max(
+      __builtin_bit_cast(vi16x2, __builtin_convertvector(f32x2{acc[0], acc[1]}, vbf2)), z);
+  const vi16x2 hi = __builtin_elementwise_max(
+      __builtin_bit_cast(vi16x2, __builtin_convertvector(f32x2{acc[2], acc[3]}, vbf2)), z);
+  return uint2{in ? __builtin_bit_cast(unsigned, lo) : 0u, in ? __builtin_bit_cast(unsigned, hi) : 0u};
+}
+
+// The partial rows of layers 1 and 2 that one role holds (two segments each).
+struct FusedRows {
+  f32x4 l1[2], l2[2];
+};
+
+template <int V> using IC = std::integral_constant<int, V>;
 
 __global__ __launch_bounds__(256) void fused_net_kernel(const bf16 *__restrict__ X, int nimg, int H, int W, int rows,
                                                         const bf16 *__restrict__ w1, const float *__restrict__ b1,
@@ -381,10 +428,10 @@ __global__ __launch_bounds__(256) void fused_net_kernel(const bf16 *__restrict__
                                                         const bf16 *__restrict__ w3, const float *__restrict__ b3,
                                                         float *__restrict__ out)
 {
-  constexpr int NR = 4;  // ring depth
-  __shared__ __attribute__((aligned(16))) bf16 s_in[NR][FPX * C_INP];
-  __shared__ __attribute__((aligned(16))) bf16 s_a1[NR][FPX * F1S];
-  __shared__ __attribute__((aligned(16))) bf16 s_a2[NR][FPX * F1S];
+  __shared__ __attribute__((aligned(16))) bf16 s_in[2][FPX * C_INP];
+  __shared__ __attribute__((aligned(16))) bf16 s_a1[2][FPX * F1S];
+  __shared__ __attribute__((aligned(16))) bf16 s_a2[2][FPX * F1S];
+  __shared__ float s_out[FUSED_MAX_ROWS * FOUT * C_OUT];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int m = lane & 15, kg = lane >> 4;
   const int strips = (W + FOUT - 1) / FOUT, chunks = (H + rows - 1) / rows;
@@ -395,129 +442,147 @@ __global__ __launch_bounds__(256) void fused_net_kernel(const bf16 *__restrict__
   const int y0 = chunk * rows, y1 = min(y0 + rows, H);
   const long ibase = (long)img * H * W;
 
-  // zero every LDS row once: pads stay zero, rows outside the image too
-  for (int q = threadIdx.x; q < NR * FPX * C_INP; q += 256) (&s_in[0][0])[q] = 0;
-  for (int q = threadIdx.x; q < NR * FPX * F1S; q += 256) {
+  // zero the activation rings once: the pad pixels stay zero
+  for (int q = threadIdx.x; q < 2 * FPX * F1S; q += 256) {
     (&s_a1[0][0])[q] = 0;
     (&s_a2[0][0])[q] = 0;
   }
 
-  // this wave's weights: output-channel tile tc (layers 1, 2), all of layer 3.
-  // Fragment k range of lane (m, kg) in K-step ks: 32 * ks + 8 * kg .. + 7.
-  const int tc = wv & 1, sA = wv >> 1;  // layers 1-2: segments sA and sA+2
-  constexpr int NK1 = 3, NK2 = KP2 / 32, NK3 = KP3 / 32;
-  bf16x8 bw1[NK1], bw2[NK2], bw3[NK3];
+  // Weights (A operand: lane (kg, m) holds row m, k = 8 kg .. 8 kg + 7).
+  // Layers 1, 2: this wave's output-channel tile tc, segments sA and sA + 2.
+  //   layer 1, row tap dy: k = 8 dx + ci (dx < 3), zero for kg = 3;
+  //   layer 2, tap 3 dy + dx: k = ci;
+  //   layer 3, column tap dx: row m = 4c + dy (c, dy < 3), k = ci.
+  const int tc = wv & 1, sA = wv >> 1;
+  bf16x8 bw1[3], bw2[9], bw3[3];
 #pragma unroll
-  for (int ks = 0; ks < NK1; ks++) {
-    const int k0 = ks * 32 + kg * 8;  // w1 rows hold KP1 = 80 entries (k >= 72 are zero)
-    bw1[ks] = k0 < KP1 ? *reinterpret_cast<const bf16x8 *>(w1 + (long)(tc * 16 + m) * KP1 + k0) : bf16x8{};
+  for (int dy = 0; dy < 3; dy++)
+    bw1[dy] = kg < 3 ? *reinterpret_cast<const bf16x8 *>(w1 + (long)(tc * 16 + m) * KP1 + dy * 24 + kg * 8) : bf16x8{};
+#pragma unroll
+  for (int tp = 0; tp < 9; tp++)
+    bw2[tp] = *reinterpret_cast<const bf16x8 *>(w2 + (long)(tc * 16 + m) * KP2 + tp * 32 + kg * 8);
+  const int c3 = m >> 2, dy3 = m & 3;
+#pragma unroll
+  for (int dx = 0; dx < 3; dx++)
+    bw3[dx] = (c3 < C_OUT && dy3 < 3) ? *reinterpret_cast<const bf16x8 *>(w3 + (long)c3 * KP3 + (dy3 * 3 + dx) * 32 + kg * 8)
+                                      : bf16x8{};
+  f32x4 bv1, bv2;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    bv1[i] = b1[tc * 16 + 4 * kg + i];
+    bv2[i] = b2[tc * 16 + 4 * kg + i];
   }
-#pragma unroll
-  for (int ks = 0; ks < NK2; ks++)
-    bw2[ks] = *reinterpret_cast<const bf16x8 *>(w2 + (long)(tc * 16 + m) * KP2 + ks * 32 + kg * 8);
-#pragma unroll
-  for (int ks = 0; ks < NK3; ks++)
-    bw3[ks] = *reinterpret_cast<const bf16x8 *>(w3 + (long)m * KP3 + ks * 32 + kg * 8);
-  const float bv1 = b1[tc * 16 + m], bv2 = b2[tc * 16 + m], bv3 = b3[m];
+  const float bv3 = kg < C_OUT ? b3[kg] : 0.f;
+
+  const int nsteps = (y1 - y0) + FUSED_HALO;
+  // 8-byte part threadIdx.x of input row ri: pixel threadIdx.x / 2 (window
+  // column threadIdx.x / 2 - 1), channels 4 (threadIdx.x & 1) .. + 3.  Every
+  // lane loads from a clamped address, and what lies outside the image is
+  // zeroed when the row is stored to LDS, so each step issues exactly one load
+  // per wave and waits only for the load of two steps earlier.
+  const int ipx = min((int)threadIdx.x, FIN_PARTS - 1) >> 1, icol = c0 - 1 + ipx;
+  const bool iload = icol >= 0 && icol < W;
+  const long ioff = ibase * C_INP + (long)min(max(icol, 0), W - 1) * C_INP + 4 * (threadIdx.x & 1);
+  auto load_in = [&](int ri) {
+    return *reinterpret_cast<const uint2 *>(X + ioff + (long)min(max(ri, 0), H - 1) * W * C_INP);
+  };
+  const f32x4 zero = {0, 0, 0, 0};
+  uint2 in_q[2] = {load_in(y0 - 3), load_in(y0 - 2)};  // rows of steps t, t+1 (slot t % 2)
+  f32x4 q3 = zero;  // layer 3 running sums: register dy of lane (c, pixel)
   __syncthreads();
 
-  const int nsteps = (y1 - y0) + 9;
-  // input pixel (row ri, window column threadIdx.x - 1) of thread threadIdx.x < FPX
-  auto load_in = [&](int ri) {
-    const int col = c0 - 1 + (int)threadIdx.x;
-    uint4 v = {0, 0, 0, 0};
-    if (threadIdx.x < FPX && ri >= 0 && ri < H && col >= 0 && col < W)
-      v = *reinterpret_cast<const uint4 *>(X + (ibase + (long)ri * W + col) * C_INP);
-    return v;
-  };
-  uint4 in_next = load_in(y0 - 3);
-  for (int t = 0; t < nsteps; t++) {
-    // input row y0 - 3 + t -> ring slot t % 4 (pixels w = -1 .. FW), first
-    // read in step t + 1; the next row's load is issued now and lands while
-    // this step computes
-    if (threadIdx.x < FPX) *reinterpret_cast<uint4 *>(s_in[t % NR] + threadIdx.x * C_INP) = in_next;
-    if (t + 1 < nsteps) in_next = load_in(y0 - 2 + t);
-    // layer 1, row r1 = y0 - 5 + t, from the input rows of steps t-3 .. t-1
-    // (slot (t + 1 + dy) % 4); K-step ks covers taps 4 ks + kg (all 8
-    // channels of one input pixel)
-    if (t >= 3) {
-      const int r1 = y0 - 5 + t;
-      const bool rin = r1 >= 0 && r1 < H;
+  // Step t with ring slot P = t % 2: X completes its rows, Y is one row
+  // behind, Z starts.
+  auto step = [&](int t, auto P, FusedRows &Xr, FusedRows &Yr, FusedRows &Zr) {
+    constexpr int ws = decltype(P)::value, rs = ws ^ 1;
+    // input row y0 - 3 + t -> slot ws, and the load of the row of step t + 2
+    if (threadIdx.x < FIN_PARTS) {
+      const int ri = y0 - 3 + t;
+      const bool in = iload && ri >= 0 && ri < H;
+      *reinterpret_cast<uint2 *>(s_in[ws] + 4 * threadIdx.x) = uint2{in ? in_q[ws].x : 0u, in ? in_q[ws].y : 0u};
+    }
+    in_q[ws] = load_in(y0 - 1 + t);
+    // layer 1: input rows y0-3 .. y1+2 (steps 1 .. rows+6); completes row y0-5+t
+    if (t >= 1 && t <= nsteps - 3) {
+      const int r = y0 - 5 + t;
+      const bool rin = r >= 0 && r < H;
 #pragma unroll
       for (int u = 0; u < 2; u++) {
         const int sg = sA + 2 * u;
-        f32x4 acc = {0, 0, 0, 0};
-#pragma unroll
-        for (int ks = 0; ks < NK1; ks++) {
-          const int tap = ks * 4 + kg;
-          bf16x8 a = {};
-          if (tap < 9) {
-            const bf16 *row = s_in[(t + 1 + tap / 3) % NR];  // step t - 3 + tap/3
-            const int px = 16 * sg + m + tap % 3;       // window column m + dx plus the pad column
-            a = *reinterpret_cast<const bf16x8 *>(row + px * C_INP);
-          }
-          acc = mfma32(a, bw1[ks], acc);
-        }
-        bf16 *dst = s_a1[t % NR];
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-          const int w = 16 * sg + 4 * kg + i, col = c0 + w;
-          const float v = acc[i] + bv1;
-          const bool in = rin && col >= 0 && col < W;
-          dst[(w + 1) * F1S + tc * 16 + m] = in ? f2bf(v > 0.f ? v : 0.f) : (bf16)0;
-        }
+        // lanes kg = 3 meet zero weights: they re-read column tap 2
+        const bf16x8 a = *reinterpret_cast<const bf16x8 *>(s_in[rs] + (16 * sg + m + min(kg, 2)) * C_INP);
+        Zr.l1[u] = mfma32(bw1[0], a, bv1);  // a new row's sum starts at its bias
+        Yr.l1[u] = mfma32(bw1[1], a, Yr.l1[u]);
+        Xr.l1[u] = mfma32(bw1[2], a, Xr.l1[u]);
+        const int col = c0 + 16 * sg + m;
+        *reinterpret_cast<uint2 *>(s_a1[ws] + (16 * sg + m + 1) * F1S + tc * 16 + 4 * kg) =
+            relu_pack4(Xr.l1[u], rin && col >= 0 && col < W);
       }
     }
-    // layer 2, row r2 = y0 - 7 + t, from the layer-1 rows of steps t-3 .. t-1;
-    // K-step ks is tap ks, channels 8 kg .. 8 kg + 7
-    if (t >= 6) {
-      const int r2 = y0 - 7 + t;
-      const bool rin = r2 >= 0 && r2 < H;
+    // layer 2: layer-1 rows y0-2 .. y1+1 (steps 4 .. rows+7); completes row y0-7+t
+    if (t >= 4 && t <= nsteps - 2) {
+      const int r = y0 - 7 + t;
+      const bool rin = r >= 0 && r < H;
 #pragma unroll
       for (int u = 0; u < 2; u++) {
         const int sg = sA + 2 * u;
-        f32x4 acc = {0, 0, 0, 0};
+        bf16x8 a[3];
 #pragma unroll
-        for (int ks = 0; ks < NK2; ks++) {
-          const bf16 *row = s_a1[(t + 1 + ks / 3) % NR];
-          const int px = 16 * sg + m + ks % 3;
-          const bf16x8 a = *reinterpret_cast<const bf16x8 *>(row + px * F1S + kg * 8);
-          acc = mfma32(a, bw2[ks], acc);
-        }
-        bf16 *dst = s_a2[t % NR];
+        for (int dx = 0; dx < 3; dx++)
+          a[dx] = *reinterpret_cast<const bf16x8 *>(s_a1[rs] + (16 * sg + m + dx) * F1S + kg * 8);
+        Zr.l2[u] = mfma32(bw2[0], a[0], bv2);
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-          const int w = 16 * sg + 4 * kg + i, col = c0 + w;
-          const float v = acc[i] + bv2;
-          const bool in = rin && col >= 0 && col < W;
-          dst[(w + 1) * F1S + tc * 16 + m] = in ? f2bf(v > 0.f ? v : 0.f) : (bf16)0;
-        }
+        for (int dx = 1; dx < 3; dx++) Zr.l2[u] = mfma32(bw2[dx], a[dx], Zr.l2[u]);
+#pragma unroll
+        for (int dx = 0; dx < 3; dx++) Yr.l2[u] = mfma32(bw2[3 + dx], a[dx], Yr.l2[u]);
+#pragma unroll
+        for (int dx = 0; dx < 3; dx++) Xr.l2[u] = mfma32(bw2[6 + dx], a[dx], Xr.l2[u]);
+        const int col = c0 + 16 * sg + m;
+        *reinterpret_cast<uint2 *>(s_a2[ws] + (16 * sg + m + 1) * F1S + tc * 16 + 4 * kg) =
+            relu_pack4(Xr.l2[u], rin && col >= 0 && col < W);
       }
     }
-    // layer 3, row r3 = y0 - 9 + t, from the layer-2 rows of steps t-3 .. t-1
-    if (t >= 9) {
-      const int r3 = y0 - 9 + t;
+    // layer 3: layer-2 rows y0-1 .. y1 (steps 7 .. rows+8); completes output
+    // row y0-9+t, staged from step 9 on
+    if (t >= 7 && t < nsteps) {
       const int sg = wv;
-      f32x4 acc = {0, 0, 0, 0};
+      bf16x8 a[3];
 #pragma unroll
-      for (int ks = 0; ks < NK3; ks++) {
-        const bf16 *row = s_a2[(t + 1 + ks / 3) % NR];
-        const int px = 16 * sg + m + ks % 3;
-        const bf16x8 a = *reinterpret_cast<const bf16x8 *>(row + px * F1S + kg * 8);
-        acc = mfma32(a, bw3[ks], acc);
-      }
-      if (m < C_OUT) {
+      for (int dx = 0; dx < 3; dx++)
+        a[dx] = *reinterpret_cast<const bf16x8 *>(s_a2[rs] + (16 * sg + m + dx) * F1S + kg * 8);
+      f32x4 acc = {0.f, q3[0], q3[1], q3[2]};  // each partial moves on one row tap
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-          const int w = 16 * sg + 4 * kg + i, col = c0 + w;
-          if (w >= 3 && w < FW - 3 && col < W)
-            out[(ibase + (long)r3 * W + col) * C_OUT + m] = acc[i] + bv3;
-        }
-      }
+      for (int dx = 0; dx < 3; dx++) acc = mfma32(bw3[dx], a[dx], acc);
+      q3 = acc;
+      const int w = 16 * sg + m;
+      if (t >= FUSED_HALO && kg < C_OUT && w >= 3 && w < FW - 3)
+        s_out[((t - FUSED_HALO) * FOUT + w - 3) * C_OUT + kg] = acc[2] + bv3;
     }
-    // this step's rows become readable, and the slots read in this step
-    // ((t + 1) % 4 of each ring) may be overwritten in the next
+    // this step's rows become readable, and the slots read in this step may
+    // be overwritten in the next
     __syncthreads();
+  };
+
+  FusedRows ra, rb, rc;
+#pragma unroll
+  for (int u = 0; u < 2; u++) ra.l1[u] = rb.l1[u] = rc.l1[u] = ra.l2[u] = rb.l2[u] = rc.l2[u] = zero;
+  // whole groups of six steps (the ones past nsteps only load and wait), so
+  // every step issues its load and the wait for it is the same on every path
+  for (int t = 0; t < nsteps; t += 6) {
+    step(t, IC<0>{}, ra, rb, rc);
+    step(t + 1, IC<1>{}, rb, rc, ra);
+    step(t + 2, IC<0>{}, rc, ra, rb);
+    step(t + 3, IC<1>{}, ra, rb, rc);
+    step(t + 4, IC<0>{}, rb, rc, ra);
+    step(t + 5, IC<1>{}, rc, ra, rb);
+  }
+
+  // the staged output rows y0 .. y1-1, window columns 3 .. 60
+  const int nval = (y1 - y0) * FOUT * C_OUT;
+  for (int q = threadIdx.x; q < nval; q += 256) {
+    const int r = q / (FOUT * C_OUT), e = q - r * (FOUT * C_OUT);
+    const int col = c0 + 3 + e / C_OUT;
+    if (col < W) out[(ibase + (long)(y0 + r) * W + c0 + 3) * C_OUT + e] = s_out[q];
   }
 }
 
@@ -899,8 +964,8 @@ int forward(sbp_corrector *c, int nb)
 // Network output Yo of the nb packed images X0 (inference: the fused kernel).
 void launch_infer(sbp_corrector *c, int nb, hipStream_t stream)
 {
-  // at most 32 rows per chunk, the chunks of an image as even as possible
-  const int nchunks = (c->H + 31) / 32, rows = (c->H + nchunks - 1) / nchunks;
+  // at most FUSED_MAX_ROWS rows per chunk, the chunks of an image as even as possible
+  const int nchunks = (c->H + FUSED_MAX_ROWS - 1) / FUSED_MAX_ROWS, rows = (c->H + nchunks - 1) / nchunks;
   const long blocks = (long)nb * ((c->W + FOUT - 1) / FOUT) * ((c->H + rows - 1) / rows);
   hipLaunchKernelGGL(fused_net_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, c->X0, nb, c->H, c->W, rows,
                      c->w1, c->b1, c->w2, c->b2, c->w3, c->b3, c->Yo);
