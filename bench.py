@@ -36,11 +36,12 @@ sys.path.insert(0, ROOT)
 PEAK_HBM_GBS = 8000.0
 # HBM bytes per launch of the dominant kernel of the default workload (4097^2,
 # order 4, rho = K = 1, FP64, two-stage schedule: the pair2d_kernel launch of
-# RK4 stages 3-4), from rocprofv3 FETCH_SIZE / 2.0 + WRITE_SIZE / 1.0 (the
-# factors a 1 GiB plain copy reads in the same runs), with the commit it was
-# measured at: profiles/pair2d_acoustic_4097sq_order4_fp64_hbm_counters.md.
-MEASURED_TRAFFIC = {(4097, 4, 0, "two-stage", 3): None}
-MEASURED_AT = None
+# RK4 stages 3-4), from rocprofv3 as FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 KiB
+# (the factors that turn the counters of a 1 GiB plain copy in the same runs
+# into the bytes it reads and writes), with the commit it was measured at:
+# profiles/pair2d_acoustic_4097sq_order4_fp64_hbm_counters.md.
+MEASURED_TRAFFIC = {(4097, 4, 0, "two-stage", 3): (535295 * 2.0 + 406522 * 1.0) * 1024}
+MEASURED_AT = "885b11b"
 
 
 def pkg(sub):
@@ -388,8 +389,8 @@ def main():
                      "algorithmic_bytes_per_launch": main_m["launch_bytes"][dom - 1],
                      "mean_launch_ms": main_m["stage_ms"][dom - 1],
                      "bytes_basis": "each value the launch must read or write, once (bench.launch_bytes)",
-                     "traffic_unit": "HBM bytes per launch of that kernel (rocprofv3 FETCH_SIZE/2.0 + "
-                                     "WRITE_SIZE/1.0, copy-calibrated)",
+                     "traffic_unit": "HBM bytes per launch of that kernel (rocprofv3 FETCH_SIZE x 2.0 + "
+                                     "WRITE_SIZE x 1.0 KiB, copy-calibrated)",
                      "traffic_measured_at": MEASURED_AT,
                      "launch_bytes": main_m["launch_bytes"], "stage_ms": main_m["stage_ms"],
                      "step": {"bytes": sum(main_m["launch_bytes"]), "bytes_per_dof": main_m["kernel_bytes_per_dof"],
