@@ -180,8 +180,8 @@ __global__ __launch_bounds__(NT) void batch_pair_kernel(const BatchInst<T> *__re
     if (it.y < d.p_n_edge_blocks) {
       edge_point<T, ORDER, PH_ACOUSTIC, MAT, MODE, KAHAN>(batch_frame_args<T, MODE>(d), it.y * NT + threadIdx.x);
     } else if constexpr (PART == 1 || PART == 3) {
-      pair_tile_rw<T, ORDER, MAT, PART == 1 ? 1 : 2, KAHAN, NT, PF, NTS>(batch_pair_args<T, PART == 1 ? 1 : 2>(d),
-                                                                         it.y - d.p_n_edge_blocks);
+      pair_tile_rw<T, ORDER, PH_ACOUSTIC, MAT, PART == 1 ? 1 : 2, KAHAN, NT, PF, NTS>(
+          batch_pair_args<T, PART == 1 ? 1 : 2>(d), it.y - d.p_n_edge_blocks);
     }
     __syncthreads();  // the next item reuses the LDS rows
   }

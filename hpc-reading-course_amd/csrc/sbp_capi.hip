@@ -130,17 +130,24 @@ int default_pair_rows(const sbp_ctx *c)
 bool pair_supported(const sbp_ctx *c)
 {
   const int D = c->cls + c->order / 2;
-  return c->pair_launch && c->phys == SBP_ACOUSTIC_2D && c->nx > 2 * D && c->ny > 2 * D;
+  return c->pair_launch && (c->phys == SBP_ACOUSTIC_2D || c->phys == SBP_ADVECTION_2D) && c->nx > 2 * D &&
+         c->ny > 2 * D;
 }
 
-// Two-stage stepping is the default where it was measured faster: grids of
-// at least 1000^2 interior points (order 4 FP64, ms per step against one
-// launch per stage: 1025^2 0.072 / 0.079, 1537^2 0.120 / 0.157, 2049^2
-// 0.176 / 0.238, 4097^2 0.667 / 0.980).
+// Two-stage stepping is the default where it was measured faster.  2D
+// acoustics: grids of at least 1000^2 interior points (order 4 FP64, ms per
+// step against one launch per stage: 1025^2 0.072 / 0.079, 1537^2 0.120 /
+// 0.157, 2049^2 0.176 / 0.238, 4097^2 0.667 / 0.980).  2D advection: at
+// least 1500^2 (order 4, one-stage / two-stage ms, constant velocity and
+// per-point fields: 1025^2 FP64 0.033 / 0.031 and 0.044 / 0.032, FP32 0.018
+// / 0.022 and 0.026 / 0.023; 2049^2 FP64 0.096 / 0.060 and 0.131 / 0.067,
+// FP32 0.064 / 0.038 and 0.078 / 0.042; 4097^2 FP64 0.316 / 0.202 and 0.450
+// / 0.263, FP32 0.220 / 0.090 and 0.264 / 0.125).
 bool pair_default(const sbp_ctx *c)
 {
   const int D = c->cls + c->order / 2;
-  return pair_supported(c) && (long)(c->nx - 2 * D) * (c->ny - 2 * D) >= 1000L * 1000L;
+  const long side = c->phys == SBP_ADVECTION_2D ? 1500L : 1000L;
+  return pair_supported(c) && (long)(c->nx - 2 * D) * (c->ny - 2 * D) >= side * side;
 }
 
 int default_ty_rows(int nx, int ny, int cls)
@@ -253,6 +260,10 @@ template <typename T> PairArgs<T> make_pair_args(const sbp_ctx *c, int pair, dou
   a.c5_k1 = c5_1;
   a.rinv = static_cast<const T *>(c->rinv);
   a.bulk = static_cast<const T *>(c->bulk);
+  a.velx = static_cast<const T *>(c->velx);
+  a.vely = static_cast<const T *>(c->vely);
+  a.ax = T(c->d.ax);
+  a.ay = T(c->d.ay);
   a.plane = c->plane;
   a.pitch = c->pitch;
   a.xoff = c->xoff;
@@ -265,8 +276,10 @@ template <typename T> PairArgs<T> make_pair_args(const sbp_ctx *c, int pair, dou
   a.ty_rows = c->p_ty_rows;
   a.n_tiles = c->p_n_tiles;
   a.n_edge_blocks = c->p_n_edge_blocks;
-  // bytes a two-stage step streams: U, Y3, U' (in the Y2 array), materials, compensation
-  const double streamed = (double)c->plane * c->esz * (9 + (c->rinv ? 2 : 0) + (c->KC ? 3 : 0));
+  // bytes a two-stage step streams: U, Y3, U' (in the Y2 array), materials or
+  // velocity fields, compensation
+  const bool fields = c->rinv || c->velx;
+  const double streamed = (double)c->plane * c->esz * (3 * c->C + (fields ? 2 : 0) + (c->KC ? c->C : 0));
   a.nts = c->store_policy == 0 ? (pair_cached_stores(streamed) ? 0 : 1) : (c->store_policy == 2 ? 1 : 0);
   if (pair == 1) {  // stages 1-2: U -> Y3 (Y1), Y2 (Y0) next to the frame
     a.yin = static_cast<const T *>(c->U);
@@ -637,14 +650,15 @@ int build_ctx(const sbp_problem *desc, sbp_ctx *c)
     c->launch = reg(c->phys, c->mat, c->prec == SBP_FP32_KAHAN, kBX);
   }
   if (!c->launch) return fail(SBP_EINVAL, "no kernel for this physics/material/precision");
-  if (c->phys == SBP_ACOUSTIC_2D) {
+  if (c->phys == SBP_ACOUSTIC_2D || c->phys == SBP_ADVECTION_2D) {
     if (c->prec == SBP_FP64)
       c->pair_launch = (c->order == 2 ? pair_launcher_o2_f64
-                                      : (c->order == 4 ? pair_launcher_o4_f64 : pair_launcher_o6_f64))(c->mat);
+                                      : (c->order == 4 ? pair_launcher_o4_f64 : pair_launcher_o6_f64))(c->phys,
+                                                                                                        c->mat);
     else
       c->pair_launch = (c->order == 2 ? pair_launcher_o2_f32
                                       : (c->order == 4 ? pair_launcher_o4_f32 : pair_launcher_o6_f32))(
-          c->mat, c->prec == SBP_FP32_KAHAN);
+          c->phys, c->mat, c->prec == SBP_FP32_KAHAN);
   }
   c->fused = pair_default(c);
 
@@ -882,8 +896,8 @@ int sbp_set_fused(sbp_ctx *ctx, int on, int rows)
 {
   if (!ctx || rows < 0 || rows > 4096) return fail(SBP_EINVAL, "bad argument");
   if (on && !pair_supported(ctx))
-    return fail(SBP_EINVAL, "two-stage stepping needs a 2D acoustic context with unit or array materials "
-                            "and more than 2(CLS+R) points per direction");
+    return fail(SBP_EINVAL, "two-stage stepping needs a 2D acoustic or 2D advection context with more than "
+                            "2(CLS+R) points per direction");
   HIP_OR_FAIL(hipStreamSynchronize(ctx->stream));
   drop_graph(ctx);
   ctx->fused = on != 0;

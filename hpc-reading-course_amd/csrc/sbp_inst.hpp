@@ -27,15 +27,23 @@ StageLaunchFn pick_launcher(int phys, int mat)
 }
 
 // Two-stage interior tiles: pair_nt threads; prefetch depths (rows of loads in
-// flight per wave) from a sweep on gfx950 (pair_pf).
+// flight per wave) from a sweep on gfx950 (pair_pf).  2D acoustics with unit,
+// array or reference materials; 2D advection with constant or per-point
+// velocity (M_ARR).
 template <typename T, int ORDER, bool KAHAN>
-PairLaunchFn pick_pair(int mat)
+PairLaunchFn pick_pair(int phys, int mat)
 {
   constexpr int PF1 = pair_pf(sizeof(T), SBP_PAIR_PF1), PF2 = pair_pf(sizeof(T), SBP_PAIR_PF2);
   constexpr int NT = pair_nt(sizeof(T), ORDER);
-  if (mat == M_UNIT) return &PairLauncher<T, ORDER, M_UNIT, KAHAN, NT, PF1, PF2>::launch;
-  if (mat == M_ARR) return &PairLauncher<T, ORDER, M_ARR, KAHAN, NT, PF1, PF2>::launch;
-  if (mat == M_REF) return &PairLauncher<T, ORDER, M_REF, KAHAN, NT, PF1, PF2>::launch;
+  if (phys == PH_ACOUSTIC) {
+    if (mat == M_UNIT) return &PairLauncher<T, ORDER, PH_ACOUSTIC, M_UNIT, KAHAN, NT, PF1, PF2>::launch;
+    if (mat == M_ARR) return &PairLauncher<T, ORDER, PH_ACOUSTIC, M_ARR, KAHAN, NT, PF1, PF2>::launch;
+    if (mat == M_REF) return &PairLauncher<T, ORDER, PH_ACOUSTIC, M_REF, KAHAN, NT, PF1, PF2>::launch;
+  }
+  if (phys == PH_ADV2D) {
+    if (mat == M_ARR) return &PairLauncher<T, ORDER, PH_ADV2D, M_ARR, KAHAN, NT, PF1, PF2>::launch;
+    return &PairLauncher<T, ORDER, PH_ADV2D, M_UNIT, KAHAN, NT, PF1, PF2>::launch;
+  }
   return nullptr;
 }
 
@@ -48,9 +56,9 @@ PairLaunchFn pick_pair(int mat)
     if (bx == 256) return pick_launcher<double, ORDER, false, 256>(phys, mat);  \
     return nullptr;                                                             \
   }                                                                             \
-  PairLaunchFn pair_launcher_o##ORDER##_f64(int mat)                            \
+  PairLaunchFn pair_launcher_o##ORDER##_f64(int phys, int mat)                  \
   {                                                                             \
-    return pick_pair<double, ORDER, false>(mat);                                \
+    return pick_pair<double, ORDER, false>(phys, mat);                          \
   }                                                                             \
   }
 
@@ -62,9 +70,9 @@ PairLaunchFn pick_pair(int mat)
     return kahan ? pick_launcher<float, ORDER, true, 256>(phys, mat)            \
                  : pick_launcher<float, ORDER, false, 256>(phys, mat);          \
   }                                                                             \
-  PairLaunchFn pair_launcher_o##ORDER##_f32(int mat, bool kahan)                \
+  PairLaunchFn pair_launcher_o##ORDER##_f32(int phys, int mat, bool kahan)      \
   {                                                                             \
-    return kahan ? pick_pair<float, ORDER, true>(mat)                           \
-                 : pick_pair<float, ORDER, false>(mat);                         \
+    return kahan ? pick_pair<float, ORDER, true>(phys, mat)                     \
+                 : pick_pair<float, ORDER, false>(phys, mat);                   \
   }                                                                             \
   }

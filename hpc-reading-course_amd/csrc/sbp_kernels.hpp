@@ -618,10 +618,12 @@ template <typename T> struct PairArgs {
   T *o1;         // pair 1: Y2 on the band next to the frame;  pair 2: U' (the Y2 array)
   T *o2;         // pair 1: Y3;  pair 2: Y4 on the band next to the frame
   T *comp;       // Kahan compensation of U (pair 2)
-  const T *rinv, *bulk;
+  const T *rinv, *bulk;  // acoustics: rho^-1, K planes (M_ARR)
+  const T *velx, *vely;  // 2D advection: velocity planes a(i,j), b(i,j) (M_ARR)
   long plane;
   int pitch, xoff, nx, ny;
   T hix, hiy;
+  T ax, ay;         // 2D advection: constant velocity (M_UNIT)
   T c1, c2, third;  // pair 1: dt/2, dt/2;  pair 2: dt, dt/6
   T c0;             // pair 2: dt/2 (Y2 = U + dt/2 K1)
   // the reference's rho = 2 + xy with its forcing (M_REF): per-column and
@@ -667,19 +669,25 @@ template <typename T> struct PairRow {
 // LDS rows are single-buffered: each row written in one phase is read in the
 // next, and the barrier between them also separates the read from the next
 // iteration's write.
-template <typename T, int ORDER, int MAT, int PAIR, bool KAHAN, int NT, int PF, bool NTS>
+template <typename T, int ORDER, int PHYS, int MAT, int PAIR, bool KAHAN, int NT, int PF, bool NTS>
 __device__ __forceinline__ void pair_tile_rw(const PairArgs<T> &a, int tile)
 {
   using O = Ops<ORDER>;
   constexpr int R = O::R, D = O::CLS + O::R, WN = 2 * R + 1;
+  // acoustics: (u, v, p), u and p differentiated in x, v and p in y;
+  // 2D advection: u, differentiated in both
+  constexpr bool AC = PHYS == PH_ACOUSTIC;
+  constexpr int NC = AC ? 3 : 1;
+  static_assert(AC || PHYS == PH_ADV2D, "two-stage tiles: 2D acoustics or 2D advection");
+  static_assert(AC || MAT != M_REF, "rho = 2 + xy is an acoustic material");
   constexpr int LW = NT + 2 * R;
   constexpr int BXO = NT - 2 * R;
   constexpr bool LD_M = MAT == M_ARR;
   constexpr int RW = Geom<ORDER>::TAPS - R - 1;
 
-  __shared__ T s_u[LW], s_p[LW];      // operand row r (x halo included)
-  __shared__ T s_mu[NT], s_mp[NT];    // mid row m
-  __shared__ T s_Uu[PAIR == 2 ? NT : 1], s_Up[PAIR == 2 ? NT : 1];  // pair 2: U at row m
+  __shared__ T s_u[LW], s_p[AC ? LW : 1];      // operand row r (x halo included)
+  __shared__ T s_mu[NT], s_mp[AC ? NT : 1];    // mid row m
+  __shared__ T s_Uu[PAIR == 2 ? NT : 1], s_Up[PAIR == 2 && AC ? NT : 1];  // pair 2: U at row m
 
   const int t = threadIdx.x;
   const int tx = tile % a.tiles_x, tyb = tile / a.tiles_x;
@@ -705,24 +713,24 @@ __device__ __forceinline__ void pair_tile_rw(const PairArgs<T> &a, int tile)
     const int r = min(j0 - 2 * R + k, jmax);
     const long orow = (long)r * pitch;
 #pragma unroll
-    for (int c = 0; c < 3; c++) q.in[c] = yin[c * plane + orow + col];
+    for (int c = 0; c < NC; c++) q.in[c] = yin[c * plane + orow + col];
     if (halo) {
       q.h[0] = yin[orow + hcol];
-      q.h[1] = yin[2 * plane + orow + hcol];
+      if constexpr (AC) q.h[1] = yin[2 * plane + orow + hcol];
     }
     const long om = (long)max(r - R, 0) * pitch + col;
     if constexpr (PAIR == 2) {
 #pragma unroll
-      for (int c = 0; c < 3; c++) q.U[c] = a.u[c * plane + om];
+      for (int c = 0; c < NC; c++) q.U[c] = a.u[c * plane + om];
     }
     if constexpr (LD_M) {
-      q.ri = a.rinv[om];
-      q.K = a.bulk[om];
+      q.ri = (AC ? a.rinv : a.velx)[om];
+      q.K = (AC ? a.bulk : a.vely)[om];
     }
     if constexpr (PAIR == 2 && KAHAN) {
       const long oo = (long)max(r - 2 * R, 0) * pitch + col;
 #pragma unroll
-      for (int c = 0; c < 3; c++) q.C[c] = ldg<true>(a.comp + c * plane + oo);
+      for (int c = 0; c < NC; c++) q.C[c] = ldg<true>(a.comp + c * plane + oo);
     }
   };
 
@@ -770,16 +778,16 @@ __device__ __forceinline__ void pair_tile_rw(const PairArgs<T> &a, int tile)
 
       // phase 0: operand row r into the windows and (u, p) into LDS
 #pragma unroll
-      for (int c = 0; c < 3; c++) {
+      for (int c = 0; c < NC; c++) {
 #pragma unroll
         for (int s = 0; s < WN - 1; s++) win[c][s] = win[c][s + 1];
         win[c][WN - 1] = cur.in[c];
       }
       s_u[t + R] = cur.in[0];
-      s_p[t + R] = cur.in[2];
+      if constexpr (AC) s_p[t + R] = cur.in[2];
       if (halo) {
         s_u[hslot] = cur.h[0];
-        s_p[hslot] = cur.h[1];
+        if constexpr (AC) s_p[hslot] = cur.h[1];
       }
       __syncthreads();
 
@@ -790,29 +798,37 @@ __device__ __forceinline__ void pair_tile_rw(const PairArgs<T> &a, int tile)
         fdx[1][p] = fdx[1][p + 1];
       }
       fdx[0][R] = dint<T, ORDER>([&](int s) { return s_u[t + s]; }, a.hix);
-      fdx[1][R] = dint<T, ORDER>([&](int s) { return s_p[t + s]; }, a.hix);
+      if constexpr (AC) fdx[1][R] = dint<T, ORDER>([&](int s) { return s_p[t + s]; }, a.hix);
       if (k >= 2 * R) {
-        const T dyp = dint<T, ORDER>([&](int s) { return win[2][s]; }, a.hiy);
-        const T dyv = dint<T, ORDER>([&](int s) { return win[1][s]; }, a.hiy);
-        T Um[3];
+        T Um[NC];
 #pragma unroll
-        for (int c = 0; c < 3; c++) Um[c] = PAIR == 1 ? win[c][R] : cur.U[c];
-        T ri = LD_M ? cur.ri : T(1), Kb = LD_M ? cur.K : T(1), fu = T(0), fv = T(0);
-        if constexpr (MAT == M_REF)
-          ref_material_at<T>(xc, c3c, s3c, a.ys, a.c4y, a.s4y, min(j0 - 3 * R + k, a.ny - 1), a.c5_mid, ri, fu, fv);
-        T F[3];
-        acoustic_F<T, MAT>(fdx[1][0], dyp, fdx[0][0], dyv, ri, Kb, fu, fv, F[0], F[1], F[2]);
+        for (int c = 0; c < NC; c++) Um[c] = PAIR == 1 ? win[c][R] : cur.U[c];
+        T ri = LD_M ? cur.ri : T(1), Kb = LD_M ? cur.K : T(1);
+        T F[NC];
+        if constexpr (AC) {
+          const T dyp = dint<T, ORDER>([&](int s) { return win[2][s]; }, a.hiy);
+          const T dyv = dint<T, ORDER>([&](int s) { return win[1][s]; }, a.hiy);
+          T fu = T(0), fv = T(0);
+          if constexpr (MAT == M_REF)
+            ref_material_at<T>(xc, c3c, s3c, a.ys, a.c4y, a.s4y, min(j0 - 3 * R + k, a.ny - 1), a.c5_mid, ri, fu,
+                               fv);
+          acoustic_F<T, MAT>(fdx[1][0], dyp, fdx[0][0], dyv, ri, Kb, fu, fv, F[0], F[1], F[2]);
+        } else {
+          const T dy = dint<T, ORDER>([&](int s) { return win[0][s]; }, a.hiy);
+          const T ax = LD_M ? ri : a.ax, ay = LD_M ? Kb : a.ay;
+          F[0] = -(ax * fdx[0][0] + ay * dy);
+        }
 #pragma unroll
-        for (int c = 0; c < 3; c++) {
+        for (int c = 0; c < NC; c++) {
 #pragma unroll
           for (int s = 0; s < WN - 1; s++) wmid[c][s] = wmid[c][s + 1];
           wmid[c][WN - 1] = Um[c] + a.c1 * F[c];
         }
         s_mu[t] = wmid[0][WN - 1];
-        s_mp[t] = wmid[2][WN - 1];
+        if constexpr (AC) s_mp[t] = wmid[2][WN - 1];
         if constexpr (PAIR == 2) {
           s_Uu[t] = Um[0];
-          s_Up[t] = Um[2];
+          if constexpr (AC) s_Up[t] = Um[2];
         }
 #pragma unroll
         for (int p = 0; p < R; p++) {
@@ -824,9 +840,9 @@ __device__ __forceinline__ void pair_tile_rw(const PairArgs<T> &a, int tile)
 #pragma unroll
         for (int p = 0; p < NU - 1; p++)
 #pragma unroll
-          for (int c = 0; c < 3; c++) fU[p][c] = fU[p + 1][c];
+          for (int c = 0; c < NC; c++) fU[p][c] = fU[p + 1][c];
 #pragma unroll
-        for (int c = 0; c < 3; c++) fU[NU - 1][c] = Um[c];
+        for (int c = 0; c < NC; c++) fU[NU - 1][c] = Um[c];
       }
       __syncthreads();
 
@@ -846,42 +862,54 @@ __device__ __forceinline__ void pair_tile_rw(const PairArgs<T> &a, int tile)
         }
         if (out_col) {
           gdx[0][R] = dint<T, ORDER>([&](int s) { return s_mu[t - R + s]; }, a.hix);
-          gdx[1][R] = dint<T, ORDER>([&](int s) { return s_mp[t - R + s]; }, a.hix);
+          if constexpr (AC) gdx[1][R] = dint<T, ORDER>([&](int s) { return s_mp[t - R + s]; }, a.hix);
           if constexpr (PAIR == 2) {
             hdx[0][R] = dint<T, ORDER>([&](int s) { return s_Uu[t - R + s]; }, a.hix);
-            hdx[1][R] = dint<T, ORDER>([&](int s) { return s_Up[t - R + s]; }, a.hix);
+            if constexpr (AC) hdx[1][R] = dint<T, ORDER>([&](int s) { return s_Up[t - R + s]; }, a.hix);
           }
         }
       }
       if (k >= 4 * R && k < nk && out_col) {
         const int o = j0 - 4 * R + k;
-        const T dyp = dint<T, ORDER>([&](int s) { return wmid[2][s]; }, a.hiy);
-        const T dyv = dint<T, ORDER>([&](int s) { return wmid[1][s]; }, a.hiy);
         T fu = T(0), fv = T(0), fu1 = T(0), fv1 = T(0);
         if constexpr (MAT == M_REF) {
           T ri_o;
           ref_material_at<T>(xc, c3c, s3c, a.ys, a.c4y, a.s4y, o, a.c5_out, ri_o, fu, fv);
           if constexpr (PAIR == 2) ref_material_at<T>(xc, c3c, s3c, a.ys, a.c4y, a.s4y, o, a.c5_k1, ri_o, fu1, fv1);
         }
-        T F[3];
-        acoustic_F<T, MAT>(gdx[1][0], dyp, gdx[0][0], dyv, fri[0], fK[0], fu, fv, F[0], F[1], F[2]);
+        // velocity at the output row (2D advection)
+        const T axo = LD_M ? fri[0] : a.ax, ayo = LD_M ? fK[0] : a.ay;
+        T F[NC];
+        if constexpr (AC) {
+          const T dyp = dint<T, ORDER>([&](int s) { return wmid[2][s]; }, a.hiy);
+          const T dyv = dint<T, ORDER>([&](int s) { return wmid[1][s]; }, a.hiy);
+          acoustic_F<T, MAT>(gdx[1][0], dyp, gdx[0][0], dyv, fri[0], fK[0], fu, fv, F[0], F[1], F[2]);
+        } else {
+          const T dy = dint<T, ORDER>([&](int s) { return wmid[0][s]; }, a.hiy);
+          F[0] = -(axo * gdx[0][0] + ayo * dy);
+        }
         const long qo = (long)o * pitch + a.xoff + mc;
         // the band next to the frame, whose stage-2 and stage-4 rows read Y2 and Y4 here
         const bool ring = mc - D < RW || a.nx - D - 1 - mc < RW || o - D < RW || a.ny - D - 1 - o < RW;
         if constexpr (PAIR == 1) {
 #pragma unroll
-          for (int c = 0; c < 3; c++) {
+          for (int c = 0; c < NC; c++) {
             if (ring) a.o1[c * plane + qo] = wmid[c][R];
             put(a.o2 + c * plane + qo, win[c][0] + a.c2 * F[c]);
           }
         } else {
           // K1 = F(U) at the output row and Y2 = U + dt/2 K1, as pair 1 formed them
-          const T dyp1 = dint<T, ORDER>([&](int s) { return fU[s][2]; }, a.hiy);
-          const T dyv1 = dint<T, ORDER>([&](int s) { return fU[s][1]; }, a.hiy);
-          T F1[3];
-          acoustic_F<T, MAT>(hdx[1][0], dyp1, hdx[0][0], dyv1, fri[0], fK[0], fu1, fv1, F1[0], F1[1], F1[2]);
+          T F1[NC];
+          if constexpr (AC) {
+            const T dyp1 = dint<T, ORDER>([&](int s) { return fU[s][2]; }, a.hiy);
+            const T dyv1 = dint<T, ORDER>([&](int s) { return fU[s][1]; }, a.hiy);
+            acoustic_F<T, MAT>(hdx[1][0], dyp1, hdx[0][0], dyv1, fri[0], fK[0], fu1, fv1, F1[0], F1[1], F1[2]);
+          } else {
+            const T dy1 = dint<T, ORDER>([&](int s) { return fU[s][0]; }, a.hiy);
+            F1[0] = -(axo * hdx[0][0] + ayo * dy1);
+          }
 #pragma unroll
-          for (int c = 0; c < 3; c++) {
+          for (int c = 0; c < NC; c++) {
             const T U = fU[R][c];
             const T Y2 = U + a.c0 * F1[c];
             const T Y4 = wmid[c][R];
@@ -917,12 +945,12 @@ constexpr int pair_pf(int esz, int pf) { return esz == 4 ? pf + 1 : pf; }
 
 // Blocks [0, n_edge_blocks): stage 1 (pair 1) or stage 3 (pair 2) of the
 // D-deep frame (ea); the rest: two-stage interior tiles (a), XCD-remapped.
-template <typename T, int ORDER, int MAT, int PAIR, bool KAHAN, int NT, int PF, bool NTS>
+template <typename T, int ORDER, int PHYS, int MAT, int PAIR, bool KAHAN, int NT, int PF, bool NTS>
 __global__ __launch_bounds__(NT) void pair2d_kernel(StageArgs<T> ea, PairArgs<T> a)
 {
   const int b = blockIdx.x;
   if (b < a.n_edge_blocks) {
-    edge_point<T, ORDER, PH_ACOUSTIC, MAT, PAIR == 1 ? MODE_S1 : MODE_S3, KAHAN>(ea, b * NT + threadIdx.x);
+    edge_point<T, ORDER, PHYS, MAT, PAIR == 1 ? MODE_S1 : MODE_S3, KAHAN>(ea, b * NT + threadIdx.x);
     return;
   }
   const int bi = b - a.n_edge_blocks;
@@ -930,7 +958,7 @@ __global__ __launch_bounds__(NT) void pair2d_kernel(StageArgs<T> ea, PairArgs<T>
   const int xcd = bi % kNumXcd, local = bi / kNumXcd;
   const int q = nt / kNumXcd, r = nt % kNumXcd;
   const int tile = xcd * q + min(xcd, r) + local;
-  pair_tile_rw<T, ORDER, MAT, PAIR, KAHAN, NT, PF, NTS>(a, tile);
+  pair_tile_rw<T, ORDER, PHYS, MAT, PAIR, KAHAN, NT, PF, NTS>(a, tile);
 }
 
 // --------------------------------------------------------------------------
@@ -1175,10 +1203,10 @@ struct Launcher1D {
 // Frame-only stages 2 and 4 of the two-stage schedule: one wave per block,
 // so the few thousand frame points spread over every CU.
 constexpr int kFrameBX = 64;
-template <typename T, int ORDER, int MAT, int MODE, bool KAHAN>
+template <typename T, int ORDER, int PHYS, int MAT, int MODE, bool KAHAN>
 __global__ __launch_bounds__(kFrameBX) void frame2d_kernel(StageArgs<T> a)
 {
-  edge_point<T, ORDER, PH_ACOUSTIC, MAT, MODE, KAHAN>(a, blockIdx.x * kFrameBX + threadIdx.x);
+  edge_point<T, ORDER, PHYS, MAT, MODE, KAHAN>(a, blockIdx.x * kFrameBX + threadIdx.x);
 }
 
 // part 1: pair2d_kernel pair 1 (+ frame stage 1); part 2: frame stage 2;
@@ -1189,7 +1217,7 @@ using PairLaunchFn = hipError_t (*)(int part, const void *edge_args, const void 
 
 // PF1 / PF2: rows of loads in flight ahead of the row being computed in the
 // pair-1 (one stencil operand per row) and pair-2 (operand + U + Y2) launches.
-template <typename T, int ORDER, int MAT, bool KAHAN, int NT, int PF1, int PF2> struct PairLauncher {
+template <typename T, int ORDER, int PHYS, int MAT, bool KAHAN, int NT, int PF1, int PF2> struct PairLauncher {
   static hipError_t launch(int part, const void *ea, const void *pa, int grid, hipStream_t s)
   {
     const StageArgs<T> &e = *static_cast<const StageArgs<T> *>(ea);
@@ -1198,20 +1226,20 @@ template <typename T, int ORDER, int MAT, bool KAHAN, int NT, int PF1, int PF2> 
       case 3: {
         const PairArgs<T> &a = *static_cast<const PairArgs<T> *>(pa);
         if (part == 1 && a.nts)
-          hipLaunchKernelGGL((pair2d_kernel<T, ORDER, MAT, 1, KAHAN, NT, PF1, true>), dim3(grid), dim3(NT), 0, s, e, a);
+          hipLaunchKernelGGL((pair2d_kernel<T, ORDER, PHYS, MAT, 1, KAHAN, NT, PF1, true>), dim3(grid), dim3(NT), 0, s, e, a);
         else if (part == 1)
-          hipLaunchKernelGGL((pair2d_kernel<T, ORDER, MAT, 1, KAHAN, NT, PF1, false>), dim3(grid), dim3(NT), 0, s, e, a);
+          hipLaunchKernelGGL((pair2d_kernel<T, ORDER, PHYS, MAT, 1, KAHAN, NT, PF1, false>), dim3(grid), dim3(NT), 0, s, e, a);
         else if (a.nts)
-          hipLaunchKernelGGL((pair2d_kernel<T, ORDER, MAT, 2, KAHAN, NT, PF2, true>), dim3(grid), dim3(NT), 0, s, e, a);
+          hipLaunchKernelGGL((pair2d_kernel<T, ORDER, PHYS, MAT, 2, KAHAN, NT, PF2, true>), dim3(grid), dim3(NT), 0, s, e, a);
         else
-          hipLaunchKernelGGL((pair2d_kernel<T, ORDER, MAT, 2, KAHAN, NT, PF2, false>), dim3(grid), dim3(NT), 0, s, e, a);
+          hipLaunchKernelGGL((pair2d_kernel<T, ORDER, PHYS, MAT, 2, KAHAN, NT, PF2, false>), dim3(grid), dim3(NT), 0, s, e, a);
         break;
       }
       case 2:
-        hipLaunchKernelGGL((frame2d_kernel<T, ORDER, MAT, MODE_S2, KAHAN>), dim3(grid), dim3(kFrameBX), 0, s, e);
+        hipLaunchKernelGGL((frame2d_kernel<T, ORDER, PHYS, MAT, MODE_S2, KAHAN>), dim3(grid), dim3(kFrameBX), 0, s, e);
         break;
       default:
-        hipLaunchKernelGGL((frame2d_kernel<T, ORDER, MAT, MODE_S4, KAHAN>), dim3(grid), dim3(kFrameBX), 0, s, e);
+        hipLaunchKernelGGL((frame2d_kernel<T, ORDER, PHYS, MAT, MODE_S4, KAHAN>), dim3(grid), dim3(kFrameBX), 0, s, e);
     }
     return hipGetLastError();
   }
@@ -1227,13 +1255,13 @@ StageLaunchFn launcher_o6_f64(int phys, int mat, int bx);
 StageLaunchFn launcher_o2_f32(int phys, int mat, bool kahan, int bx);
 StageLaunchFn launcher_o4_f32(int phys, int mat, bool kahan, int bx);
 StageLaunchFn launcher_o6_f32(int phys, int mat, bool kahan, int bx);
-// Two-stage interior launchers of the acoustic system (material M_UNIT or
-// M_ARR); nullptr where none is built.
-PairLaunchFn pair_launcher_o2_f64(int mat);
-PairLaunchFn pair_launcher_o4_f64(int mat);
-PairLaunchFn pair_launcher_o6_f64(int mat);
-PairLaunchFn pair_launcher_o2_f32(int mat, bool kahan);
-PairLaunchFn pair_launcher_o4_f32(int mat, bool kahan);
-PairLaunchFn pair_launcher_o6_f32(int mat, bool kahan);
+// Two-stage launchers of 2D acoustics (unit, array or reference materials)
+// and 2D advection (constant or per-point velocity); nullptr where none is built.
+PairLaunchFn pair_launcher_o2_f64(int phys, int mat);
+PairLaunchFn pair_launcher_o4_f64(int phys, int mat);
+PairLaunchFn pair_launcher_o6_f64(int phys, int mat);
+PairLaunchFn pair_launcher_o2_f32(int phys, int mat, bool kahan);
+PairLaunchFn pair_launcher_o4_f32(int phys, int mat, bool kahan);
+PairLaunchFn pair_launcher_o6_f32(int phys, int mat, bool kahan);
 
 }  // namespace sbp

@@ -110,9 +110,11 @@ int  sbp_set_tile_rows(sbp_ctx *ctx, int rows);
 /* Two-stage stepping (on = 1): each RK4 step runs stages 1-2 and stages 3-4
  * of the interior in one launch each, the inner stage kept on chip, and the
  * (CLS+R)-deep boundary frame in the one-stage kernels; 5 array passes per
- * step instead of 13, same arithmetic per point.  2D acoustic contexts only
- * (every material model; SBP_EINVAL otherwise); on by default for those with
- * at least 1000^2 interior points.  rows: tile height of the two-stage tiles
+ * step instead of 13, same arithmetic per point.  2D acoustic contexts (every
+ * material model) and 2D advection contexts (constant or per-point velocity);
+ * SBP_EINVAL otherwise.  On by default for 2D acoustics with at least 1000^2
+ * interior points and 2D advection with at least 1500^2 (where each was
+ * measured faster).  rows: tile height of the two-stage tiles
  * (0 = chosen from the grid).  A performance knob with no reference
  * counterpart: it changes how ts_rk4's steps are scheduled on the device
  * (ts_rk.cpp:54-65), not what they compute. */

@@ -194,8 +194,8 @@ def test_fused_then_rk45_and_time_stages(capi, orc):
 
 
 def test_fused_default_by_grid_size(capi):
-    """On by default for 2D acoustics with >= 1000^2 interior points (unit or
-    array materials), off below and for every other problem."""
+    """On by default for 2D acoustics with >= 1000^2 interior points and 2D
+    advection with >= 1500^2, off below and for the 1D problems."""
     big, _, _ = P.baseline_acoustic(1025, 4)
     with capi.Context(big) as ctx:
         assert ctx.fused
@@ -204,19 +204,120 @@ def test_fused_default_by_grid_size(capi):
     small, _, _ = P.baseline_acoustic(1001, 4)   # 989^2 interior points
     with capi.Context(small) as ctx:
         assert not ctx.fused
+    with capi.Context(adv_spec("adv", 4, 1513, 1513)) as ctx:   # 1501^2 interior points
+        assert ctx.fused
+    with capi.Context(adv_spec("adv", 4, 1511, 1511)) as ctx:
+        assert not ctx.fused
+    with capi.Context(P.adv1d(5000, 4)[0]) as ctx:
+        assert not ctx.fused
 
 
 def test_fused_rejects_unsupported(capi):
-    for spec in (P.adv2d(40, 4)[0], P.adv1d(50, 4)[0]):
+    for spec in (P.adv1d(50, 4)[0], P.reflection(60, 4)[0]):
         with capi.Context(spec) as ctx:
             with pytest.raises(capi.SbpError) as e:
                 ctx.set_fused(True)
             assert e.value.code == -1
             assert not ctx.fused
-    spec, _ = P.wave_hom(18, 6)   # no interior beyond the 9-deep frame
+    for spec in (P.wave_hom(18, 6)[0], adv_spec("adv", 6, 18, 40)):   # no interior beyond the 9-deep frame
+        with capi.Context(spec) as ctx:
+            with pytest.raises(capi.SbpError):
+                ctx.set_fused(True)
+
+
+# ---------------------------------------------------------------------------
+# 2D advection (advection_2D_sim.cpp): u_t + a u_x + b u_y = 0 with the upwind
+# SATs, constant velocity or per-point fields
+# ---------------------------------------------------------------------------
+
+def adv_spec(kind, order, nx, ny, precision=P.FP64):
+    """adv: the demo's a = 1.5, b = -1 on [-1, 1]^2; var: a = x + 2,
+    b = y - 2 (inflow west and north); swirl: a = 1.2 sin(pi y), b = cos(pi x),
+    whose sign changes along every boundary (the upwind SAT switches on and off
+    from point to point)."""
+    if kind == "adv":
+        return P.ProblemSpec(P.ADVECTION_2D, order, nx, ny, -1.0, 1.0, -1.0, 1.0, ax=1.5, ay=-1.0,
+                             precision=precision)
+    spec = P.ProblemSpec(P.ADVECTION_2D, order, nx, ny, -1.0, 1.0, -1.0, 1.0, ax=0.0, ay=0.0, precision=precision)
+    X, Y = np.meshgrid(spec.x(), spec.y())
+    if kind == "var":
+        spec.vel_x, spec.vel_y = X + 2.0, Y - 2.0
+    else:
+        spec.vel_x, spec.vel_y = 1.2 * np.sin(np.pi * Y), np.cos(np.pi * X)
+    return spec
+
+
+def adv_state(spec, seed=5):
+    rng = np.random.default_rng(seed)
+    X, Y = np.meshgrid(spec.x(), spec.y())
+    g = np.exp(-((X - 0.1) ** 2 + (Y + 0.2) ** 2) * 8)
+    return g.reshape(-1) + 1e-3 * rng.standard_normal(spec.length)
+
+
+ADV_CASES = [("adv", 4, 77, 70), ("adv", 2, 50, 41), ("adv", 6, 60, 45), ("adv", 4, 19, 19),
+             ("var", 4, 300, 281), ("var", 6, 129, 129), ("var", 2, 40, 33), ("swirl", 4, 131, 97),
+             ("swirl", 6, 45, 60)]
+
+
+@pytest.mark.parametrize("case", ADV_CASES, ids=lambda c: "%s-o%d-%dx%d" % c)
+@pytest.mark.parametrize("nsteps", [1, 2, 37])
+def test_fused_adv2d_fp64_bit_identical_and_oracle(capi, orc, case, nsteps):
+    """FP64 two-stage 2D advection is bit-identical to the one-stage path and
+    within 1e-12 of the oracle."""
+    spec = adv_spec(*case)
+    say("fused adv2d fp64", case, nsteps)
+    v = adv_state(spec)
+    dt = P.dt_from_cfl(spec, 0.2)
+    plain = run(capi, spec, v, dt, nsteps, False)
+    fused = run(capi, spec, v, dt, nsteps, True)
+    assert np.array_equal(fused, plain), relerr(fused, plain)
+    ref = orc.rk4(spec, v, 0.0, dt, nsteps, threads=THREADS)
+    assert relerr(fused, ref) <= 1e-12
+
+
+@pytest.mark.parametrize("policy", [1, 2], ids=["cached", "nontemporal"])
+@pytest.mark.parametrize("prec", [P.FP64, P.FP32, P.FP32_KAHAN])
+@pytest.mark.parametrize("kind,order", [("adv", 4), ("swirl", 6), ("var", 2)])
+def test_fused_adv2d_precisions_and_stores(capi, orc, kind, order, prec, policy):
+    """Both store builds of the advection tiles in every precision, 23 steps
+    (a 16-step graph + 7 direct steps): FP64 bit-identical to the one-stage
+    path and within 1e-12 of the oracle; FP32 / Kahan within 2e-6 of the
+    one-stage path of the same precision and 2e-5 of the FP64 oracle."""
+    spec = adv_spec(kind, order, 300, 281, precision=prec)
+    say("fused adv2d", kind, order, prec, policy)
+    v = adv_state(spec)
+    dt = P.dt_from_cfl(spec, 0.2)
+    plain = run(capi, spec, v, dt, 23, False)
     with capi.Context(spec) as ctx:
-        with pytest.raises(capi.SbpError):
-            ctx.set_fused(True)
+        ctx.set_fused(True)
+        ctx.set_store_policy(policy)
+        ctx.set_state(v)
+        ctx.step(0.0, dt, 23)
+        got = ctx.get_state()
+    ref = orc.rk4(spec.with_(precision=P.FP64), v, 0.0, dt, 23, threads=THREADS)
+    if prec == P.FP64:
+        assert np.array_equal(got, plain), relerr(got, plain)
+        assert relerr(got, ref) <= 1e-12
+    else:
+        assert relerr(got, plain) <= 2e-6
+        assert relerr(got, ref) <= 2e-5
+
+
+def test_fused_adv2d_demo_error(capi, orc):
+    """The demo's Gaussian (advection_2D_sim.cpp, rstar 0.1) advected to
+    t = 0.2 on 201^2 at order 4 with the two-stage path: the same error
+    against the analytic solution as the oracle's run."""
+    spec, v0 = P.adv2d(201, 4)
+    dt = P.dt_from_cfl(spec, 0.2)
+    n = int(round(0.2 / dt))
+    say("fused adv2d demo")
+    got = run(capi, spec, v0, dt, n, True)
+    ref = orc.rk4(spec, v0, 0.0, dt, n, threads=THREADS)
+    exact = P.adv2d_exact(spec, n * dt)
+    e_gpu = orc.error_norms(spec, got, exact)
+    e_orc = orc.error_norms(spec, ref, exact)
+    assert e_gpu[0] == pytest.approx(e_orc[0], rel=1e-9)
+    assert e_gpu[0] < 1e-2
 
 
 def test_fused_nontemporal_store_kernels(capi):
