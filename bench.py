@@ -283,6 +283,8 @@ def main():
                     help="only run the 256-instance FP32 ensemble (--steps timed steps)")
     ap.add_argument("--tile-rows", type=int, default=0,
                     help="two-stage tile height of the main workload (default: tuned on the grid)")
+    ap.add_argument("--one-stage", action="store_true",
+                    help="run the main workload with one launch per RK4 stage (the stage2d kernels)")
     args = ap.parse_args()
     global TILE_ROWS
     TILE_ROWS = args.tile_rows
@@ -311,7 +313,7 @@ def main():
     spec, v0, dt = P.baseline_acoustic(args.n, args.order)
     log("workload %dx%d order %d FP64: %d warmup + %d timed steps" % (spec.nx, spec.ny, spec.order,
                                                                       args.warmup, args.steps))
-    main_m = measure(spec, v0, dt, args.steps, args.warmup)
+    main_m = measure(spec, v0, dt, args.steps, args.warmup, fused=False if args.one_stage else None)
     log("main:", json.dumps({k: v for k, v in main_m.items() if k != "stage_ms"}))
 
     secondary = {}

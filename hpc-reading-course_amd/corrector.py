@@ -259,15 +259,20 @@ def _l2(spec, v, ref):
     return float(np.sqrt(spec.hx * spec.hy) * np.linalg.norm(v - ref))
 
 
-def interp_time_at_error(points, target):
+def interp_time_at_error(points, target, rtol=0.01):
     """Time at error `target` on a curve of (time, error) points, piecewise
     linear in (log error, log time) between the two points that bracket it;
-    None when no pair of points brackets it."""
+    a target within rtol (relative) of the curve's smallest or largest error
+    takes that end point's time (the same grid solved another way lands a
+    hair off its error); None when nothing brackets it."""
     pts = sorted(points, key=lambda p: p[1])
     for (t_a, e_a), (t_b, e_b) in zip(pts, pts[1:]):
         if e_a <= target <= e_b and e_b > e_a:
             w = (np.log(target) - np.log(e_a)) / (np.log(e_b) - np.log(e_a))
             return float(np.exp(np.log(t_a) + w * (np.log(t_b) - np.log(t_a))))
+    for t_e, e_e in (pts[0], pts[-1]):
+        if abs(target - e_e) <= rtol * e_e:
+            return float(t_e)
     return None
 
 
@@ -403,8 +408,8 @@ def time_to_accuracy(params, norm, band=BAND, n_eval=32, seed0=900000, t_end=0.5
         res["plain_time_at_learned_%s_l2" % which] = t_eq
         res["speedup_learned_equal_%s_l2" % which] = t_eq / tl if t_eq else None
         tb_err = res["batched_fp32"][which + "_l2"]
-        res["speedup_batched_fp32_equal_%s_l2" % which] = (
-            interp_time_at_error([(c[1], c[col]) for c in curve], tb_err) or 0) / tb
+        tb_eq = interp_time_at_error([(c[1], c[col]) for c in curve], tb_err)
+        res["speedup_batched_fp32_equal_%s_l2" % which] = tb_eq / tb if tb_eq else None
     log("equal error: plain FP64 needs %.3f s (mean) / %.3f s (max) at the learned form's error: "
         "%.2fx / %.2fx" % (res["plain_time_at_learned_mean_l2"] or -1, res["plain_time_at_learned_max_l2"] or -1,
                            res["speedup_learned_equal_mean_l2"] or -1, res["speedup_learned_equal_max_l2"] or -1))

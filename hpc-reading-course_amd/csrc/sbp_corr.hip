@@ -619,14 +619,25 @@ __global__ __launch_bounds__(256) void gather_kernel(const float *__restrict__ r
 // The host restatement is corrector.raw_features.
 //
 // A block packs a PTW x pth<T> tile of one image (taller for FP32 states,
-// whose staging takes half the LDS).  The three state planes of the
-// tile plus a 3-point halo are staged in LDS (in the state's precision) as
+// whose staging takes half the LDS).  The three state planes of the tile
+// plus a 3-point halo are staged in LDS (in the state's precision) as
 // f[clamp(j)][clamp(i)], so a neighbour read with its index clamped is a
 // plain LDS read; the centres of the differenced fourth differences are
-// clamped into the image first.
-constexpr int PTW = 64, PTX = PTW + 6;
+// clamped into the image first.  Lane L of a wave owns image column
+// x0 - 1 + L and pth/4 consecutive rows: the y-differences of its rows come
+// from one window down its column (each fourth difference computed once, not
+// once per point that uses it), and the x-differences of the columns beside
+// it from the neighbouring lanes by shuffles.  Lanes 0 and 63 are the halo
+// columns; lanes 1..62 own the PTW = 62 output columns.
+constexpr int PTW = 62, PTX = PTW + 6;
 template <typename T> constexpr int pth = sizeof(T) == 4 ? 32 : 16;
 template <typename T> constexpr int pty = pth<T> + 6;
+
+// the fourth difference a - 4b + 6c - 4d + e, left to right in double
+__device__ __forceinline__ double d4(double a, double b, double c, double d, double e)
+{
+  return a - 4 * b + 6 * c - 4 * d + e;
+}
 
 template <typename T>
 __device__ __forceinline__ void pack_tile(const T *__restrict__ U, const T *__restrict__ rinv, const T *__restrict__ bulk,
@@ -635,7 +646,9 @@ __device__ __forceinline__ void pack_tile(const T *__restrict__ U, const T *__re
                                           T (*s)[pty<T>][PTX])
 {
   constexpr int PTH = pth<T>, PTY = pty<T>;
+  constexpr int RPT = PTH / 4;  // rows per lane
   constexpr int NSTAGE = 3 * PTY * PTX, NLOAD = (NSTAGE + 255) / 256;
+  static_assert(PTX == 64 + 4, "one wave of columns plus the stencil's reach");
   const int x0 = tx * PTW, y0 = ty * PTH;
   // every load of the staging is issued before the first LDS store, so the
   // block waits for one memory latency instead of NLOAD of them
@@ -650,17 +663,17 @@ __device__ __forceinline__ void pack_tile(const T *__restrict__ U, const T *__re
       v[k] = U[c * plane + (long)gj * pitch + xoff + gi];
     }
   }
-  // the material values of this thread's points, loaded with the staging
-  static_assert(PTW * PTH % 256 == 0, "whole passes over the tile");
-  constexpr int NPT = PTW * PTH / 256;
-  T mri[NPT], mK[NPT];
+  const int lane = threadIdx.x & 63;
+  const int cx = x0 - 1 + lane;                 // this lane's image column
+  const int jb = y0 + (threadIdx.x >> 6) * RPT;  // its first row
+  // the material values of this lane's points, loaded with the staging
+  T mri[RPT], mK[RPT];
 #pragma unroll
-  for (int k = 0; k < NPT; k++) {
-    const int q = threadIdx.x + 256 * k;
-    const int i = min(x0 + (q % PTW), nx - 1), j = min(y0 + (q / PTW), ny - 1);
+  for (int r = 0; r < RPT; r++) {
+    const int i = min(max(cx, 0), nx - 1), j = min(jb + r, ny - 1);
     const long o = (long)j * pitch + xoff + i;
-    mri[k] = rinv ? rinv[o] : T(1);
-    mK[k] = bulk ? bulk[o] : T(1);
+    mri[r] = rinv ? rinv[o] : T(1);
+    mK[r] = bulk ? bulk[o] : T(1);
   }
 #pragma unroll
   for (int k = 0; k < NLOAD; k++) {
@@ -668,30 +681,66 @@ __device__ __forceinline__ void pack_tile(const T *__restrict__ U, const T *__re
     if (q < NSTAGE) (&s[0][0][0])[q] = v[k];
   }
   __syncthreads();
-  // image (i, j) -> LDS slot; valid for |i - x0 - PTW/2| <= PTW/2 + 3 etc.
-  auto at = [&](int c, int i, int j) { return (double)s[c][j - y0 + 3][i - x0 + 3]; };
-  auto d4x = [&](int c, int i, int j) {
-    i = min(max(i, 0), nx - 1);
-    return at(c, i - 2, j) - 4 * at(c, i - 1, j) + 6 * at(c, i, j) - 4 * at(c, i + 1, j) + at(c, i + 2, j);
-  };
-  auto d4y = [&](int c, int i, int j) {
-    j = min(max(j, 0), ny - 1);
-    return at(c, i, j - 2) - 4 * at(c, i, j - 1) + 6 * at(c, i, j) - 4 * at(c, i, j + 1) + at(c, i, j + 2);
-  };
+  // LDS slot of image column i is i - x0 + 3, of image row j is j - y0 + 3
+  auto at = [&](int c, int yy, int xx) { return (double)s[c][yy][xx]; };
+  const int xs = lane + 2;                           // this lane's column
+  const int xc = min(max(cx, 0), nx - 1) - x0 + 3;  // its centre for x-differences, clamped into the image
+  const int yb = jb - y0 + 3;                        // row jb
+  // y-differences down the column: p and v at centres jb-1 .. jb+RPT, u at jb .. jb+RPT-1
+  double dyp[RPT + 2], dyv[RPT + 2], dyu[RPT];
+  {
+    double cp[RPT + 6], cv[RPT + 6];
 #pragma unroll
-  for (int k = 0; k < NPT; k++) {
-    const int q = threadIdx.x + 256 * k;
-    const int i = x0 + (q % PTW), j = y0 + (q / PTW);
-    if (i >= nx || j >= ny) continue;
-    const double ri = (double)mri[k];
-    const double K = (double)mK[k];
+    for (int m = 0; m < RPT + 6; m++) {
+      cp[m] = at(2, yb - 3 + m, xs);
+      cv[m] = at(1, yb - 3 + m, xs);
+    }
+#pragma unroll
+    for (int m = 0; m < RPT + 2; m++) {
+      dyp[m] = d4(cp[m], cp[m + 1], cp[m + 2], cp[m + 3], cp[m + 4]);
+      dyv[m] = d4(cv[m], cv[m + 1], cv[m + 2], cv[m + 3], cv[m + 4]);
+    }
+  }
+  {
+    double cu[RPT + 4];
+#pragma unroll
+    for (int m = 0; m < RPT + 4; m++) cu[m] = at(0, yb - 2 + m, xs);
+#pragma unroll
+    for (int m = 0; m < RPT; m++) dyu[m] = d4(cu[m], cu[m + 1], cu[m + 2], cu[m + 3], cu[m + 4]);
+  }
+  // a centre outside the image takes the nearest image row's value
+  if (jb == 0) {
+    dyp[0] = dyp[1];
+    dyv[0] = dyv[1];
+  }
+#pragma unroll
+  for (int m = 1; m < RPT + 2; m++)
+    if (jb - 1 + m > ny - 1) {
+      dyp[m] = dyp[m - 1];
+      dyv[m] = dyv[m - 1];
+    }
+  const bool out_lane = lane >= 1 && lane <= PTW && cx < nx;
+#pragma unroll
+  for (int r = 0; r < RPT; r++) {
+    const int yy = yb + r;
+    const double dxu = d4(at(0, yy, xc - 2), at(0, yy, xc - 1), at(0, yy, xc), at(0, yy, xc + 1), at(0, yy, xc + 2));
+    const double dxv = d4(at(1, yy, xc - 2), at(1, yy, xc - 1), at(1, yy, xc), at(1, yy, xc + 1), at(1, yy, xc + 2));
+    const double dxp = d4(at(2, yy, xc - 2), at(2, yy, xc - 1), at(2, yy, xc), at(2, yy, xc + 1), at(2, yy, xc + 2));
+    // the x-differences at columns i - 1 and i + 1 (every lane takes part)
+    const double dxu_l = __shfl_up(dxu, 1), dxu_r = __shfl_down(dxu, 1);
+    const double dxp_l = __shfl_up(dxp, 1), dxp_r = __shfl_down(dxp, 1);
+    const int j = jb + r;
+    if (!out_lane || j >= ny) continue;
+    const int i = cx;
+    const double ri = (double)mri[r];
+    const double K = (double)mK[r];
     float raw[C_IN];
-    raw[0] = (float)(ri * (0.5 * (d4x(2, i + 1, j) - d4x(2, i - 1, j))));
-    raw[1] = (float)(ri * (0.5 * (d4y(2, i, j + 1) - d4y(2, i, j - 1))));
-    raw[2] = (float)(K * (0.5 * (d4x(0, i + 1, j) - d4x(0, i - 1, j)) + 0.5 * (d4y(1, i, j + 1) - d4y(1, i, j - 1))));
-    raw[3] = (float)(d4x(0, i, j) + d4y(0, i, j));
-    raw[4] = (float)(d4x(1, i, j) + d4y(1, i, j));
-    raw[5] = (float)(d4x(2, i, j) + d4y(2, i, j));
+    raw[0] = (float)(ri * (0.5 * (dxp_r - dxp_l)));
+    raw[1] = (float)(ri * (0.5 * (dyp[r + 2] - dyp[r])));
+    raw[2] = (float)(K * (0.5 * (dxu_r - dxu_l) + 0.5 * (dyv[r + 2] - dyv[r])));
+    raw[3] = (float)(dxu + dyu[r]);
+    raw[4] = (float)(dxv + dyv[r + 1]);
+    raw[5] = (float)(dxp + dyp[r + 1]);
     raw[6] = (float)(1.0 / ri);
     raw[7] = (float)K;
     __attribute__((aligned(16))) bf16 px[C_INP];

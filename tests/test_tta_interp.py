@@ -1,6 +1,7 @@
 """The equal-error interpolation of the time-to-accuracy comparison
 (corrector.interp_time_at_error): piecewise linear in log time against log
-error, exact on power laws, None outside the measured range."""
+error, exact on power laws, the end point just beyond either end, None
+further outside the measured range."""
 import importlib
 
 import numpy as np
@@ -25,3 +26,14 @@ def test_brackets_and_outside():
     assert C.interp_time_at_error(pts, 1e-6) is None
     assert C.interp_time_at_error(pts, 1e-2) is None
     assert C.interp_time_at_error(pts, 4.9e-4) == pytest.approx(0.172)
+
+
+def test_end_points_within_tolerance():
+    """A target a hair beyond the curve's end (the same grid solved by the
+    FP32 ensemble: 3.1965e-5 against the plain path's 3.1984e-5) takes the end
+    point's time; 2% beyond it is outside the curve."""
+    pts = [(0.179, 4.90e-4), (0.265, 1.03e-4), (0.569, 3.1984e-5)]
+    assert C.interp_time_at_error(pts, 3.1965e-5) == pytest.approx(0.569)
+    assert C.interp_time_at_error(pts, 4.93e-4) == pytest.approx(0.179)
+    assert C.interp_time_at_error(pts, 3.1984e-5 * 0.98) is None
+    assert C.interp_time_at_error(pts, 3.1965e-5, rtol=0.0) is None
