@@ -66,16 +66,15 @@ def bytes_per_dof(precision, materials=False, C=3):
 
 
 def launch_bytes(spec, fused):
-    """Algorithmic HBM bytes of each of a step's four launches: every value a
-    launch must read or write, once.  Per DOF (point and component), words:
+    """Algorithmic HBM bytes of each of a step's four stage slots: every value
+    a launch must read or write, once.  Per DOF (point and component), words:
       one stage per launch: U -> Y2 (2), Y2, U -> Y3 (3), Y3, U -> Y4 (3),
                             Y4, U, Y2, Y3 -> U (5; Kahan + compensation R/W 2)
-      two-stage:  launch 1  interior U -> Y3 (2) and Y2 on the band next to
-                            the frame (+1); frame U -> Y2 (2)
-                  launch 2  frame Y2, U -> Y3 (3)
-                  launch 3  interior Y3, U -> U' (3; Kahan +2; Y2 is recomputed)
-                            and Y4 on the band (+1); frame Y3, U -> Y4 (3)
-                  launch 4  frame Y4, U, Y2, Y3 -> U' (5; Kahan +2)
+      two-stage (two launches; slots 2 and 4 are empty):
+                  launch 1  interior U -> Y3 (2); frame U -> Y2, Y3 (3: Y2 is
+                            kept for the frame's stage 4)
+                  launch 3  interior Y3, U -> U' (3; Kahan +2; Y2 is
+                            recomputed); frame Y3, U, Y2 -> U' (4; Kahan +2)
     Stored rho^-1 and K add 2 words per point to every launch that differentiates
     with them (two-stage: per interior or frame point a launch covers)."""
     w = 8 if spec.precision == 0 else 4
@@ -89,18 +88,13 @@ def launch_bytes(spec, fused):
         return [float(npts * (C * k + (2 if mat else 0)) * w) for k in words]
     cls, r = {2: 1, 4: 4, 6: 6}[order], order // 2
     d = cls + r
-    taps = {2: 3, 4: 6, 6: 9}[order]
-    rw = taps - r - 1  # Y4 band width next to the frame
     ni = (spec.nx - 2 * d) * (spec.ny - 2 * d)
     nf = npts - ni
-    inner = (spec.nx - 2 * d - 2 * rw) * (spec.ny - 2 * d - 2 * rw)
-    band = ni - max(inner, 0)
     m = 2 if mat else 0
-    b1 = ni * (2 * C + m) + band * C + nf * (2 * C + m)
-    b2 = nf * (3 * C + m)
-    b3 = ni * ((3 + (2 if kahan else 0)) * C + m) + band * C + nf * (3 * C + m)
-    b4 = nf * ((5 + (2 if kahan else 0)) * C + m)
-    return [float(b * w) for b in (b1, b2, b3, b4)]
+    k2 = 2 if kahan else 0
+    b1 = ni * (2 * C + m) + nf * (3 * C + m)
+    b3 = ni * ((3 + k2) * C + m) + nf * ((4 + k2) * C + m)
+    return [float(b1 * w), 0.0, float(b3 * w), 0.0]
 
 
 TILE_ROWS = 0  # --tile-rows: fixed two-stage tile height instead of sbp_tune_fused (profiling runs)
@@ -364,8 +358,8 @@ def main():
     dom = main_m["dominant_launch"]
     traffic = MEASURED_TRAFFIC.get((spec.nx, spec.order, spec.precision, main_m["schedule"], dom))
     if main_m["schedule"] == "two-stage":
-        names = {1: "pair2d_kernel, RK4 stages 1-2 (+ the frame's stage 1)", 2: "frame2d_kernel, frame stage 2",
-                 3: "pair2d_kernel, RK4 stages 3-4 (+ the frame's stage 3)", 4: "frame2d_kernel, frame stage 4"}
+        names = {1: "pair2d_kernel, RK4 stages 1-2 (interior tiles and frame patches)", 2: "(no launch)",
+                 3: "pair2d_kernel, RK4 stages 3-4 (interior tiles and frame patches)", 4: "(no launch)"}
     else:
         names = {k: "stage2d_kernel, RK4 stage %d" % k for k in range(1, 5)}
     out = {

@@ -109,9 +109,10 @@ using BatchLaunchFn = hipError_t (*)(int mode, const void *inst, const int2 *wor
 
 // --------------------------------------------------------------------------
 // Two-stage configurations (the schedule of sbp_kernels.hpp pair2d_kernel):
-// four persistent launches per step over the same kind of work list,
-//   part 1: frame stage 1 blocks + pair-1 tiles      part 2: frame stage 2
-//   part 3: frame stage 3 blocks + pair-2 tiles      part 4: frame stage 4
+// two persistent launches per step over a work list of frame patches and
+// pair tiles,
+//   part 1: stages 1-2 (frame_patch_pair and pair-1 tiles)
+//   part 3: stages 3-4 (frame_patch_pair and pair-2 tiles)
 // A step leaves U' in each instance's Y0 array; the host then passes the
 // second descriptor set, in which U and Y0 trade places.
 // --------------------------------------------------------------------------
@@ -171,45 +172,24 @@ template <typename T, int ORDER, int MAT, int PART, bool KAHAN, int NT, int PF, 
 __global__ __launch_bounds__(NT) void batch_pair_kernel(const BatchInst<T> *__restrict__ inst,
                                                         const int2 *__restrict__ work, int nwork)
 {
-  constexpr int MODE = PART == 1 ? MODE_S1 : (PART == 2 ? MODE_S2 : (PART == 3 ? MODE_S3 : MODE_S4));
+  constexpr int PAIR = PART == 1 ? 1 : 2;
+  constexpr int MA = PAIR == 1 ? MODE_S1 : MODE_S3, MB = PAIR == 1 ? MODE_S2 : MODE_S4;
   const int2 chunk = work[blockIdx.x % kNumXcd];  // as in batch_stage_kernel
   const int2 *items = work + kNumXcd + chunk.x;
   for (int w = blockIdx.x / kNumXcd; w < chunk.y; w += gridDim.x / kNumXcd) {
     const int2 it = items[w];
     const BatchInst<T> &d = inst[it.x];
-    if (it.y < d.p_n_edge_blocks) {
-      edge_point<T, ORDER, PH_ACOUSTIC, MAT, MODE, KAHAN>(batch_frame_args<T, MODE>(d), it.y * NT + threadIdx.x);
-    } else if constexpr (PART == 1 || PART == 3) {
-      pair_tile_rw<T, ORDER, PH_ACOUSTIC, MAT, PART == 1 ? 1 : 2, KAHAN, NT, PF, NTS>(
-          batch_pair_args<T, PART == 1 ? 1 : 2>(d), it.y - d.p_n_edge_blocks);
-    }
+    if (it.y < d.p_n_edge_blocks)
+      frame_patch_pair<T, ORDER, PH_ACOUSTIC, MAT, PAIR, KAHAN, NT>(batch_frame_args<T, MA>(d),
+                                                                   batch_frame_args<T, MB>(d), it.y);
+    else
+      pair_tile_rw<T, ORDER, PH_ACOUSTIC, MAT, PAIR, KAHAN, NT, PF, NTS>(batch_pair_args<T, PAIR>(d),
+                                                                        it.y - d.p_n_edge_blocks);
     __syncthreads();  // the next item reuses the LDS rows
   }
 }
 
-// Frame-only parts 2 and 4 (stages 2 and 4 on the D-deep frames): each
-// frame work item holds NT points, taken by NT / kFrameBX one-wave blocks.
-template <typename T, int ORDER, int MAT, int PART, bool KAHAN, int NT>
-__global__ __launch_bounds__(kFrameBX) void batch_frame_kernel(const BatchInst<T> *__restrict__ inst,
-                                                               const int2 *__restrict__ work, int nwork)
-{
-  constexpr int MODE = PART == 2 ? MODE_S2 : MODE_S4;
-  constexpr int SUB = NT / kFrameBX;
-  static_assert(NT % kFrameBX == 0, "frame items split into whole waves");
-  const int2 chunk = work[blockIdx.x % kNumXcd];
-  const int2 *items = work + kNumXcd + chunk.x;
-  const int n = chunk.y * SUB;
-  for (int v = blockIdx.x / kNumXcd; v < n; v += gridDim.x / kNumXcd) {
-    const int2 it = items[v / SUB];
-    const BatchInst<T> &d = inst[it.x];
-    edge_point<T, ORDER, PH_ACOUSTIC, MAT, MODE, KAHAN>(batch_frame_args<T, MODE>(d),
-                                                         it.y * NT + (v % SUB) * kFrameBX + threadIdx.x);
-  }
-}
-
-// part: 1..4; the pair launches take the prefetch depths of sbp_kernels.hpp;
-// the frame-only parts run as many threads as a pair launch, in one-wave
-// blocks (grid * NT / kFrameBX of them, still a multiple of kNumXcd)
+// part: 1 or 3; the pair launches take the prefetch depths of sbp_kernels.hpp
 template <typename T, int ORDER, int MAT, bool KAHAN> struct BatchPairLauncher {
   static constexpr int NT = pair_nt(sizeof(T), ORDER);
   template <int PART, int PF, bool NTS>
@@ -217,13 +197,6 @@ template <typename T, int ORDER, int MAT, bool KAHAN> struct BatchPairLauncher {
   {
     hipLaunchKernelGGL((batch_pair_kernel<T, ORDER, MAT, PART, KAHAN, NT, PF, NTS>), dim3(grid), dim3(NT), 0, s,
                        static_cast<const BatchInst<T> *>(inst), work, nwork);
-    return hipGetLastError();
-  }
-  template <int PART>
-  static hipError_t frame(const void *inst, const int2 *work, int nwork, int grid, hipStream_t s)
-  {
-    hipLaunchKernelGGL((batch_frame_kernel<T, ORDER, MAT, PART, KAHAN, NT>), dim3(grid * (NT / kFrameBX)),
-                       dim3(kFrameBX), 0, s, static_cast<const BatchInst<T> *>(inst), work, nwork);
     return hipGetLastError();
   }
   // part | kBatchCachedStores: the pair tiles store cached (pair_cached_stores)
@@ -234,10 +207,7 @@ template <typename T, int ORDER, int MAT, bool KAHAN> struct BatchPairLauncher {
       case 1: return go<1, PF1, true>(inst, work, nwork, grid, s);
       case 1 | kBatchCachedStores: return go<1, PF1, false>(inst, work, nwork, grid, s);
       case 3: return go<3, PF2, true>(inst, work, nwork, grid, s);
-      case 3 | kBatchCachedStores: return go<3, PF2, false>(inst, work, nwork, grid, s);
-      case 2:
-      case 2 | kBatchCachedStores: return frame<2>(inst, work, nwork, grid, s);
-      default: return frame<4>(inst, work, nwork, grid, s);
+      default: return go<3, PF2, false>(inst, work, nwork, grid, s);
     }
   }
 };

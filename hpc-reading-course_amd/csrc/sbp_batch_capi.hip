@@ -123,7 +123,7 @@ template <typename T> bool fill_inst(const sbp_batch *b, std::vector<BatchInst<T
     d.p_ty_rows = ty;
     d.p_n_tiles = (pw > 0 && ph > 0) ? d.p_tiles_x * ((ph + ty - 1) / ty) : 0;
     d.p_n_edge = 2 * D * h.nx + 2 * D * (h.ny - 2 * D);
-    d.p_n_edge_blocks = (d.p_n_edge + nt - 1) / nt;
+    d.p_n_edge_blocks = pw > 0 && ph > 0 ? frame_patch_count(h.nx, h.ny, D) : 0;  // frame patches
   }
   return ok;
 }
@@ -144,16 +144,17 @@ int apply_config(sbp_batch *b, int cfg)
   // Work lists: kNumXcd {offset, count} headers, then the items of each
   // XCD's chunk.  Whole instances are bin-packed onto the XCDs by work
   // (heaviest first onto the least loaded XCD); a chunk holds the tiles of
-  // its instances (largest first), then their frame blocks, so long items
-  // start early and short ones fill the tail on every XCD alike.
-  std::vector<int2> work, frame;
+  // its instances (largest first), then their frame blocks (two-stage: frame
+  // patches), so long items start early and short ones fill the tail on every
+  // XCD alike.
+  std::vector<int2> work;
   auto build = [&](auto &desc) {
     const int R = b->order / 2;
     auto tiles = [&](int k) { return c.pair ? desc[k].p_n_tiles : desc[k].n_tiles; };
     auto eblocks = [&](int k) { return c.pair ? desc[k].p_n_edge_blocks : desc[k].n_edge_blocks; };
-    std::vector<double> weight(b->count);  // in 256-point rows
+    std::vector<double> weight(b->count);  // in 256-point rows (a frame patch: about 4, two stages on its grown points)
     for (int k = 0; k < b->count; k++)
-      weight[k] = c.pair ? (double)tiles(k) * (c.ty + 4 * R) + eblocks(k)
+      weight[k] = c.pair ? (double)tiles(k) * (c.ty + 4 * R) + 4.0 * eblocks(k)
                          : (double)tiles(k) * (c.ty + 2 * R) * c.bx / 256.0 + eblocks(k);
     std::vector<int> order(b->count);
     for (int k = 0; k < b->count; k++) order[k] = k;
@@ -179,8 +180,7 @@ int apply_config(sbp_batch *b, int cfg)
       }
       list.insert(list.end(), items.begin(), items.end());
     };
-    emit(work, true);                 // every stage (one-stage), parts 1 and 3 (two-stage)
-    if (c.pair) emit(frame, false);   // parts 2 and 4: the frame's blocks only
+    emit(work, true);  // every stage (one-stage), parts 1 and 3 (two-stage)
   };
   // the second descriptor set trades U and Y0 (state of an odd two-stage step)
   auto swapped = [](auto v) {
@@ -205,7 +205,6 @@ int apply_config(sbp_batch *b, int cfg)
     df.insert(df.end(), sw.begin(), sw.end());
   }
   if (!fits) return fail(SBP_EINVAL, "two-stage configurations need every instance larger than 2(CLS+R) points");
-  work.insert(work.end(), frame.begin(), frame.end());
   int rc = normalize_parity(b);
   if (rc) return rc;
   HIP_OR_FAIL(hipStreamSynchronize(b->stream));
@@ -220,8 +219,7 @@ int apply_config(sbp_batch *b, int cfg)
   HIP_OR_FAIL(hipMemcpy(b->inst_dev, b->esz == 8 ? (const void *)dd.data() : (const void *)df.data(), 2 * desc_bytes,
                         hipMemcpyHostToDevice));
   b->desc_bytes = desc_bytes;
-  b->nwork_frame = (int)frame.size();
-  b->nwork = (int)(work.size() - frame.size());  // the frame list starts at work_dev + nwork
+  b->nwork = (int)work.size();
   b->pair = c.pair != 0;
   b->auto_nts = b->esz == 8 ? dd[0].p_nts : df[0].p_nts;
   b->pair_nts = b->store_policy == 0 ? b->auto_nts : (b->store_policy == 2 ? 1 : 0);
@@ -256,14 +254,11 @@ int upload_dt(sbp_batch *b, const double *dt)
 hipError_t launch_step(sbp_batch *b)
 {
   hipError_t e = hipSuccess;
-  if (b->pair) {  // parts 1..4 on the descriptors of the current parity
+  if (b->pair) {  // parts 1 and 3 on the descriptors of the current parity
     const void *inst = sbp::host::batch_inst(b);
-    for (int p = 1; p <= 4 && e == hipSuccess; p++) {
-      const bool frame_only = p == 2 || p == 4;
-      e = b->launch(p | (b->pair_nts ? 0 : kBatchCachedStores), inst, frame_only ? b->work_dev + b->nwork : b->work_dev,
-                    frame_only ? b->nwork_frame : b->nwork, b->grid, b->stream);
-    }
-    if (e == hipSuccess) b->par ^= 1;  // the new states sit in the other pool only if every part went in
+    for (int p = 1; p <= 3 && e == hipSuccess; p += 2)
+      e = b->launch(p | (b->pair_nts ? 0 : kBatchCachedStores), inst, b->work_dev, b->nwork, b->grid, b->stream);
+    if (e == hipSuccess) b->par ^= 1;  // the new states sit in the other pool only if both parts went in
     return e;
   }
   for (int m = MODE_S1; m <= MODE_S4 && e == hipSuccess; m++)

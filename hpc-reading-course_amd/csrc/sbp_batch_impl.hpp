@@ -34,7 +34,6 @@ struct sbp_batch {
   int pair_nts = 1;          // two-stage: nontemporal (1) or cached (0) pair-tile stores
   int auto_nts = 1;          // the same chosen from the streamed bytes
   int store_policy = 0;      // 0: auto_nts, 1: cached, 2: nontemporal
-  int nwork_frame = 0;       // two-stage: frame-only items stored after the nwork items
   bool pair = false;         // the configuration runs two RK4 stages per interior launch
   int par = 0;               // 1: a two-stage step left the state in the Y0 pool
   double *aos = nullptr, *partials = nullptr, *scalar = nullptr;

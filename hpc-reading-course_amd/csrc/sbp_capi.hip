@@ -100,7 +100,9 @@ void compute_pair_geometry(sbp_ctx *c)
   c->p_tiles_x = wi > 0 ? (wi + bxo - 1) / bxo : 0;
   c->p_n_tiles = (wi > 0 && hi > 0) ? c->p_tiles_x * ((hi + c->p_ty_rows - 1) / c->p_ty_rows) : 0;
   c->p_n_edge = 2 * D * c->nx + 2 * D * (c->ny - 2 * D);
-  const int eb = (c->p_n_edge + nt - 1) / nt;
+  // one block per frame patch, rounded up to whole XCD rounds so the tiles'
+  // block index stays XCD-aligned
+  const int eb = frame_patch_count(c->nx, c->ny, D);
   c->p_n_edge_blocks = (eb + kNumXcd - 1) / kNumXcd * kNumXcd;
 }
 
@@ -301,30 +303,28 @@ template <typename T> PairArgs<T> make_pair_args(const sbp_ctx *c, int pair, dou
   return a;
 }
 
-// Launch `part` (1..4) of a two-stage step: 1 = frame stage 1 + interior
-// stages 1-2, 2 = frame stage 2, 3 = frame stage 3 + interior stages 3-4,
-// 4 = frame stage 4, after which U' sits in Y0 and the two arrays swap roles.
+// Launch `part` of a two-stage step: 1 = stages 1-2 (frame patches and
+// interior tiles), 3 = stages 3-4, after which U' sits in Y0 and the two
+// arrays swap roles.
 template <typename T> hipError_t launch_pair_part_t(sbp_ctx *c, int part, double t, double dt)
 {
   static const int mode[4] = {MODE_S1, MODE_S2, MODE_S3, MODE_S4};
   const double ts[4] = {t, t + 0.5 * dt, t + 0.5 * dt, t + dt};
   const void *yin[4] = {c->U, c->Y0, c->Y1, c->A};
   void *yout[4] = {c->Y0, c->Y1, c->A, nullptr};
-  StageArgs<T> e = make_args<T>(c, mode[part - 1], yin[part - 1], dt, ts[part - 1]);
-  e.yout = static_cast<T *>(yout[part - 1]);
-  e.depth = c->p_depth;
-  e.n_edge = c->p_n_edge;
-  e.n_edge_blocks = c->p_n_edge_blocks;
-  e.n_tiles = 0;
-  if (part == 4) e.uout = static_cast<T *>(c->Y0);  // U' next to the interior's; U read from uin
-  if (part == 1 || part == 3) {
-    const PairArgs<T> p = make_pair_args<T>(c, part == 1 ? 1 : 2, t, dt);
-    return c->pair_launch(part, &e, &p, c->p_n_edge_blocks + c->p_n_tiles, c->stream);
+  StageArgs<T> e[2];
+  for (int k = 0; k < 2; k++) {
+    const int s = part - 1 + k;  // stage index 0..3
+    e[k] = make_args<T>(c, mode[s], yin[s], dt, ts[s]);
+    e[k].yout = static_cast<T *>(yout[s]);
+    e[k].depth = c->p_depth;
+    e[k].n_tiles = 0;
   }
-  // frame-only launch: one wave per block
-  const hipError_t err = c->pair_launch(part, &e, nullptr, (c->p_n_edge + kFrameBX - 1) / kFrameBX, c->stream);
+  if (part == 3) e[1].uout = static_cast<T *>(c->Y0);  // U' into the Y2 array; U read from uin
+  const PairArgs<T> p = make_pair_args<T>(c, part == 1 ? 1 : 2, t, dt);
+  const hipError_t err = c->pair_launch(part, &e[0], &e[1], &p, c->p_n_edge_blocks + c->p_n_tiles, c->stream);
   // the step is complete (or captured) only when its last launch went in
-  if (part == 4 && err == hipSuccess) std::swap(c->U, c->Y0);
+  if (part == 3 && err == hipSuccess) std::swap(c->U, c->Y0);
   return err;
 }
 
@@ -336,8 +336,8 @@ hipError_t launch_pair_part(sbp_ctx *c, int part, double t, double dt)
 hipError_t launch_step(sbp_ctx *c, double t, double dt)
 {
   if (c->fused) {
-    hipError_t e = hipSuccess;
-    for (int p = 1; p <= 4 && e == hipSuccess; p++) e = launch_pair_part(c, p, t, dt);
+    hipError_t e = launch_pair_part(c, 1, t, dt);
+    if (e == hipSuccess) e = launch_pair_part(c, 3, t, dt);
     return e;
   }
   const double h = 0.5 * dt;
@@ -1138,11 +1138,14 @@ int sbp_time_stages(sbp_ctx *ctx, double dt, long nsteps, double ms_stage[4])
   if (!ctx->have_state) return fail(SBP_ESTATE, "no state loaded");
   double acc[4] = {0, 0, 0, 0};
   const double ts[4] = {0, 0.5 * dt, 0.5 * dt, dt};
+  // one launch per stage: four timed launches; two-stage: the launches of
+  // stages 1-2 and 3-4 are timed into slots 0 and 2, slots 1 and 3 stay 0
+  const int nl = ctx->fused ? 2 : 4;
   for (long n = 0; n < nsteps; n++) {
     HIP_OR_FAIL(hipEventRecord(ctx->ev[0], ctx->stream));
-    for (int s = 0; s < 4; s++) {
-      if (ctx->fused) {  // the four launches of a two-stage step
-        HIP_OR_FAIL(launch_pair_part(ctx, s + 1, n * dt, dt));
+    for (int s = 0; s < nl; s++) {
+      if (ctx->fused) {
+        HIP_OR_FAIL(launch_pair_part(ctx, 2 * s + 1, n * dt, dt));
       } else {
         const void *yin;
         void *yout;
@@ -1151,11 +1154,11 @@ int sbp_time_stages(sbp_ctx *ctx, double dt, long nsteps, double ms_stage[4])
       }
       HIP_OR_FAIL(hipEventRecord(ctx->ev[s + 1], ctx->stream));
     }
-    HIP_OR_FAIL(hipEventSynchronize(ctx->ev[4]));
-    for (int s = 0; s < 4; s++) {
+    HIP_OR_FAIL(hipEventSynchronize(ctx->ev[nl]));
+    for (int s = 0; s < nl; s++) {
       float f = 0;
       HIP_OR_FAIL(hipEventElapsedTime(&f, ctx->ev[s], ctx->ev[s + 1]));
-      acc[s] += f;
+      acc[ctx->fused ? 2 * s : s] += f;
     }
   }
   for (int s = 0; s < 4; s++) ms_stage[s] = nsteps > 0 ? acc[s] / nsteps : 0.0;

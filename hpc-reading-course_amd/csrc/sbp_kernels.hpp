@@ -266,56 +266,35 @@ __device__ __forceinline__ void rk_point_exact(const StageArgs<T> &a, long o, co
 // deep: the two-stage interior tiles below leave a frame of CLS + R).  Points
 // of the frame beyond the closure rows take the table's interior row.
 // --------------------------------------------------------------------------
-template <typename T, int ORDER, int PHYS, int MAT, int MODE, bool KAHAN>
-__device__ __forceinline__ void edge_point(const StageArgs<T> &a, int e)
+// The right-hand side K at point (i, j) of a 2D problem (interior, closure
+// and SAT terms), with the stencil operand read through a view: component c
+// at (i', j') is f[c * fplane + j' * fpitch + i'] (the state in HBM, or a
+// patch of it in LDS that holds every point the stencils of (i, j) reach).
+// Materials, velocities and the SAT data come from a, at HBM offset o.  Uc
+// receives the operand at the point.
+template <typename T, int ORDER, int PHYS, int MAT>
+__device__ __forceinline__ void edge_rhs(const StageArgs<T> &a, const T *f, long fplane, long fpitch, int i, int j,
+                                         long o, T *K, T *Uc)
 {
-  constexpr int CLS = Ops<ORDER>::CLS;
   constexpr bool AC = PHYS == PH_ACOUSTIC;
-  constexpr int NC = AC ? 3 : 1;
-  if (e >= a.n_edge) return;
   const int nx = a.nx, ny = a.ny;
-  const int dep = a.depth > 0 ? a.depth : CLS;
-  const int nstrip = dep * nx;
-  const int nside = dep * (ny - 2 * dep);
-  int i, j;
-  if (e < nstrip) {  // bottom strip, rows 0..dep-1
-    j = e / nx;
-    i = e - j * nx;
-  } else if (e < 2 * nstrip) {  // top strip
-    e -= nstrip;
-    j = ny - dep + e / nx;
-    i = e - (e / nx) * nx;
-  } else if (e < 2 * nstrip + nside) {  // left strip between them
-    e -= 2 * nstrip;
-    j = dep + e / dep;
-    i = e - (e / dep) * dep;
-  } else {  // right strip
-    e -= 2 * nstrip + nside;
-    j = dep + e / dep;
-    i = nx - dep + (e - (e / dep) * dep);
-  }
-  const long pitch = a.pitch;
-  const long o = j * pitch + a.xoff + i;
-  const T *base = a.yin;
-  const long rowb = j * pitch + a.xoff;  // start of row j
-  const long colb = a.xoff + i;          // column i at row 0
+  const long rowb = j * fpitch;  // start of row j
+  const long colb = i;           // column i at row 0
+  const long q = rowb + i;
   const T qinv0 = T(Ops<ORDER>::hqinv[0]);
   const T hqx = a.hix * qinv0, hqy = a.hiy * qinv0;
-  T K[NC], Uc[NC];
-  // every load of the point is issued before the first arithmetic on it
-  const RkIn<T, NC> rin = rk_load<T, NC, MODE, KAHAN>(a, o);
   if constexpr (AC) {
-    const T *U_ = base, *V_ = base + a.plane, *P_ = base + 2 * a.plane;
-    const T u = U_[o], v = V_[o], p = P_[o];
+    const T *U_ = f, *V_ = f + fplane, *P_ = f + 2 * fplane;
+    const T u = U_[q], v = V_[q], p = P_[q];
     T ri = T(1), Kb = T(1), fu = T(0), fv = T(0);
     if constexpr (MAT == M_ARR) {
       ri = a.rinv[o];
       Kb = a.bulk[o];
     }
     const T dxp = dline<T, ORDER>(P_ + rowb, 1, nx, i, a.hix);
-    const T dyp = dline<T, ORDER>(P_ + colb, pitch, ny, j, a.hiy);
+    const T dyp = dline<T, ORDER>(P_ + colb, fpitch, ny, j, a.hiy);
     const T dxu = dline<T, ORDER>(U_ + rowb, 1, nx, i, a.hix);
-    const T dyv = dline<T, ORDER>(V_ + colb, pitch, ny, j, a.hiy);
+    const T dyv = dline<T, ORDER>(V_ + colb, fpitch, ny, j, a.hiy);
     Uc[0] = u;
     Uc[1] = v;
     Uc[2] = p;
@@ -347,9 +326,9 @@ __device__ __forceinline__ void edge_point(const StageArgs<T> &a, int e)
       ax = a.velx[o];
       ay = a.vely[o];
     }
-    const T u = base[o];
-    const T dx = dline<T, ORDER>(base + rowb, 1, nx, i, a.hix);
-    const T dy = dline<T, ORDER>(base + colb, pitch, ny, j, a.hiy);
+    const T u = f[q];
+    const T dx = dline<T, ORDER>(f + rowb, 1, nx, i, a.hix);
+    const T dy = dline<T, ORDER>(f + colb, fpitch, ny, j, a.hiy);
     Uc[0] = u;
     T k = -(ax * dx + ay * dy);
     if (i == 0) k += T(0.5) * (T(-0.5) * (ax + fabs(ax))) * (hqx * u);
@@ -358,6 +337,40 @@ __device__ __forceinline__ void edge_point(const StageArgs<T> &a, int e)
     if (j == ny - 1) k += T(0.5) * (T(0.5) * (ay - fabs(ay))) * (hqy * u);
     K[0] = k;
   }
+}
+
+template <typename T, int ORDER, int PHYS, int MAT, int MODE, bool KAHAN>
+__device__ __forceinline__ void edge_point(const StageArgs<T> &a, int e)
+{
+  constexpr int CLS = Ops<ORDER>::CLS;
+  constexpr int NC = PHYS == PH_ACOUSTIC ? 3 : 1;
+  if (e >= a.n_edge) return;
+  const int nx = a.nx, ny = a.ny;
+  const int dep = a.depth > 0 ? a.depth : CLS;
+  const int nstrip = dep * nx;
+  const int nside = dep * (ny - 2 * dep);
+  int i, j;
+  if (e < nstrip) {  // bottom strip, rows 0..dep-1
+    j = e / nx;
+    i = e - j * nx;
+  } else if (e < 2 * nstrip) {  // top strip
+    e -= nstrip;
+    j = ny - dep + e / nx;
+    i = e - (e / nx) * nx;
+  } else if (e < 2 * nstrip + nside) {  // left strip between them
+    e -= 2 * nstrip;
+    j = dep + e / dep;
+    i = e - (e / dep) * dep;
+  } else {  // right strip
+    e -= 2 * nstrip + nside;
+    j = dep + e / dep;
+    i = nx - dep + (e - (e / dep) * dep);
+  }
+  const long o = j * (long)a.pitch + a.xoff + i;
+  T K[NC], Uc[NC];
+  // every load of the point is issued before the first arithmetic on it
+  const RkIn<T, NC> rin = rk_load<T, NC, MODE, KAHAN>(a, o);
+  edge_rhs<T, ORDER, PHYS, MAT>(a, a.yin + a.xoff, a.plane, a.pitch, i, j, o, K, Uc);
   rk_store<T, NC, MODE, KAHAN>(a, o, K, Uc, rin);
 }
 
@@ -578,7 +591,7 @@ __global__ __launch_bounds__(BX) void stage2d_kernel(StageArgs<T> a)
 }
 
 // --------------------------------------------------------------------------
-// Two RK4 stages per launch on the interior (temporal blocking, acoustics).
+// Two RK4 stages per launch (temporal blocking; 2D acoustics and advection).
 // --------------------------------------------------------------------------
 //   pair 1: K1 = F(U),  Y2 = U + dt/2 K1 on the tile grown by R (on chip),
 //           K2 = F(Y2), Y3 = U + dt/2 K2 on the tile.        (reads U; writes Y3)
@@ -592,13 +605,12 @@ __global__ __launch_bounds__(BX) void stage2d_kernel(StageArgs<T> a)
 // array pass; K1 is evaluated by the same code on the same values as in
 // pair 1, so Y2 is the same to the bit.  Tiles cover [D, nx-D) x [D, ny-D)
 // with D = CLS + R, so every point of a grown tile is an interior point of
-// the operator.  The D-deep frame runs the one-stage edge path: stage 1
-// (pair-1 launch), stage 2 (own launch), stage 3 (pair-2 launch), stage 4
-// (own launch).  The frame's stages 2 and 4 differentiate Y2 and Y4 at the
-// interior points next to the frame, so pair 1 stores Y2 and pair 2 stores
-// Y4 on that band; pair 2 writes U' into the Y2 array (U is read R points
-// around each point by neighbouring tiles); the host swaps the roles of the
-// U and Y2 arrays after each step.  Per point the arithmetic is that of the
+// the operator.  The D-deep frame is cut into patches that run both stages
+// of the launch themselves (frame_patch_pair: the first stage on the patch
+// grown into the interior, in LDS, then the second), so a step is two
+// launches; pair 2 writes U' into the Y2 array (U is read R points around
+// each point by neighbouring tiles); the host swaps the roles of the U and
+// Y2 arrays after each step.  Per point the arithmetic is that of the
 // one-stage kernels, term for term.
 //
 // A block of NT threads owns NT mid columns (x0-R .. x0-R+NT-1) and the
@@ -615,8 +627,8 @@ template <typename T> struct PairArgs {
   const T *yin;  // stencil operand of the outer stage: U (pair 1), Y3 (pair 2)
   const T *u;    // U (pair 2)
   const T *y2;   // unused (pair 2 recomputes Y2)
-  T *o1;         // pair 1: Y2 on the band next to the frame;  pair 2: U' (the Y2 array)
-  T *o2;         // pair 1: Y3;  pair 2: Y4 on the band next to the frame
+  T *o1;         // pair 2: U' (the Y2 array)
+  T *o2;         // pair 1: Y3
   T *comp;       // Kahan compensation of U (pair 2)
   const T *rinv, *bulk;  // acoustics: rho^-1, K planes (M_ARR)
   const T *velx, *vely;  // 2D advection: velocity planes a(i,j), b(i,j) (M_ARR)
@@ -683,7 +695,6 @@ __device__ __forceinline__ void pair_tile_rw(const PairArgs<T> &a, int tile)
   constexpr int LW = NT + 2 * R;
   constexpr int BXO = NT - 2 * R;
   constexpr bool LD_M = MAT == M_ARR;
-  constexpr int RW = Geom<ORDER>::TAPS - R - 1;
 
   __shared__ T s_u[LW], s_p[AC ? LW : 1];      // operand row r (x halo included)
   __shared__ T s_mu[NT], s_mp[AC ? NT : 1];    // mid row m
@@ -889,14 +900,9 @@ __device__ __forceinline__ void pair_tile_rw(const PairArgs<T> &a, int tile)
           F[0] = -(axo * gdx[0][0] + ayo * dy);
         }
         const long qo = (long)o * pitch + a.xoff + mc;
-        // the band next to the frame, whose stage-2 and stage-4 rows read Y2 and Y4 here
-        const bool ring = mc - D < RW || a.nx - D - 1 - mc < RW || o - D < RW || a.ny - D - 1 - o < RW;
         if constexpr (PAIR == 1) {
 #pragma unroll
-          for (int c = 0; c < NC; c++) {
-            if (ring) a.o1[c * plane + qo] = wmid[c][R];
-            put(a.o2 + c * plane + qo, win[c][0] + a.c2 * F[c]);
-          }
+          for (int c = 0; c < NC; c++) put(a.o2 + c * plane + qo, win[c][0] + a.c2 * F[c]);
         } else {
           // K1 = F(U) at the output row and Y2 = U + dt/2 K1, as pair 1 formed them
           T F1[NC];
@@ -922,11 +928,121 @@ __device__ __forceinline__ void pair_tile_rw(const PairArgs<T> &a, int tile)
               put(a.comp + c * plane + qo, (tt - U) - y);
               put(a.o1 + c * plane + qo, tt);
             }
-            if (ring) a.o2[c * plane + qo] = Y4;
           }
         }
       }
     }
+  }
+}
+
+// --------------------------------------------------------------------------
+// Frame patches of the two-stage step.  The D-deep frame (D = CLS + R) is cut
+// into patches of kPatchW points along each strip: the bottom and top strips
+// (D rows, every column) in runs of kPatchW columns, the left and right strips
+// (D columns, rows D .. ny-D-1) in runs of kPatchW rows.  A patch block
+// evaluates the pair's first stage (1 or 3) on the patch grown by
+// RW = TAPS - R - 1 points into the interior and sideways -- every point any
+// stencil of the second stage reaches from the patch (closure rows read the
+// first TAPS points of their line, the padded interior row reads
+// i-R .. i-R+TAPS-1) -- keeps it in LDS, and then evaluates the second stage
+// (2 or 4) on the patch from there, so the frame needs no launch of its own.
+// Grown points beyond the frame are interior points (the table's interior
+// row): the same values the interior tiles form there.  Each point's
+// arithmetic is edge_point's, term for term.
+// --------------------------------------------------------------------------
+constexpr int kPatchW = 64;
+
+// Number of frame patches of an nx x ny grid with a D-deep frame.
+__host__ __device__ inline int frame_patch_count(int nx, int ny, int D)
+{
+  return 2 * ((nx + kPatchW - 1) / kPatchW) + 2 * ((ny - 2 * D + kPatchW - 1) / kPatchW);
+}
+
+struct FramePatch {
+  int i0, i1, j0, j1;      // the patch: columns [i0, i1), rows [j0, j1)
+  int ei0, ei1, ej0, ej1;  // grown by RW into the interior and sideways, clipped to the grid
+};
+
+template <int ORDER> __device__ inline FramePatch frame_patch(int p, int nx, int ny)
+{
+  using G = Geom<ORDER>;
+  constexpr int D = G::CLS + G::R, RW = G::TAPS - G::R - 1;
+  const int nb = (nx + kPatchW - 1) / kPatchW;
+  FramePatch f;
+  if (p < 2 * nb) {  // bottom (p < nb) or top strip
+    const bool top = p >= nb;
+    const int k = top ? p - nb : p;
+    f.i0 = k * kPatchW;
+    f.i1 = min(f.i0 + kPatchW, nx);
+    f.j0 = top ? ny - D : 0;
+    f.j1 = top ? ny : D;
+    // sideways: RW, and at least the TAPS-point lines of the closure rows
+    // (a narrow last patch of right-closure points reads from nx - TAPS)
+    f.ei0 = max(min(f.i0 - RW, nx - G::TAPS), 0);
+    f.ei1 = min(max(f.i1 + RW, G::TAPS), nx);
+    f.ej0 = top ? max(ny - D - RW, 0) : 0;
+    f.ej1 = top ? ny : min(D + RW, ny);
+  } else {  // left or right strip, rows D .. ny-D-1
+    const int nl = (ny - 2 * D + kPatchW - 1) / kPatchW;
+    const bool right = p - 2 * nb >= nl;
+    const int k = p - 2 * nb - (right ? nl : 0);
+    f.j0 = D + k * kPatchW;
+    f.j1 = min(f.j0 + kPatchW, ny - D);
+    f.i0 = right ? nx - D : 0;
+    f.i1 = right ? nx : D;
+    f.ej0 = max(f.j0 - RW, 0);
+    f.ej1 = min(f.j1 + RW, ny);
+    f.ei0 = right ? max(nx - D - RW, 0) : 0;
+    f.ei1 = right ? nx : min(D + RW, nx);
+  }
+  return f;
+}
+
+// Stages 1-2 (PAIR 1) or 3-4 (PAIR 2) of frame patch p; eA / eB are the
+// arguments of the first / second stage (stage 1: U -> Y2, also stored on the
+// patch for stage 4; stage 2: -> Y3; stage 3: -> Y4, kept in LDS only;
+// stage 4: U, Y2, Y3, Y4 -> U').
+template <typename T, int ORDER, int PHYS, int MAT, int PAIR, bool KAHAN, int NT>
+__device__ __forceinline__ void frame_patch_pair(const StageArgs<T> &eA, const StageArgs<T> &eB, int p)
+{
+  using G = Geom<ORDER>;
+  constexpr int R = G::R, D = G::CLS + G::R, RW = G::TAPS - G::R - 1;
+  constexpr int NC = PHYS == PH_ACOUSTIC ? 3 : 1;
+  constexpr int MA = PAIR == 1 ? MODE_S1 : MODE_S3, MB = PAIR == 1 ? MODE_S2 : MODE_S4;
+  constexpr int FP_N = (D + RW) * (kPatchW + 2 * RW);  // points of the largest grown patch
+  static_assert(RW >= R, "the grown patch covers the interior stencil");
+  __shared__ T s_f[NC][FP_N];
+  const int nx = eA.nx, ny = eA.ny;
+  const FramePatch f = frame_patch<ORDER>(p, nx, ny);
+  const int ew = f.ei1 - f.ei0, ne = ew * (f.ej1 - f.ej0);
+  const long pitch = eA.pitch, plane = eA.plane;
+  // first stage on the grown patch
+  for (int q = threadIdx.x; q < ne; q += NT) {
+    const int jj = f.ej0 + q / ew, ii = f.ei0 + q % ew;
+    const long o = jj * pitch + eA.xoff + ii;
+    T K[NC], Uc[NC];
+    const RkIn<T, NC> rin = rk_load<T, NC, MA, KAHAN>(eA, o);
+    edge_rhs<T, ORDER, PHYS, MAT>(eA, eA.yin + eA.xoff, plane, pitch, ii, jj, o, K, Uc);
+    const bool on = ii >= f.i0 && ii < f.i1 && jj >= f.j0 && jj < f.j1;
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+      const T y = (MA == MODE_S1 ? Uc[c] : rin.U[c]) + eA.cy * K[c];
+      s_f[c][q] = y;
+      if (PAIR == 1 && on) eA.yout[o + c * plane] = y;  // Y2 on the frame (stage 4 reads it)
+    }
+  }
+  __syncthreads();
+  // second stage on the patch, its operand from LDS: element (c, i, j) of the
+  // grown patch is s_f[c][(j - ej0) * ew + (i - ei0)]
+  const T *view = &s_f[0][0] - ((long)f.ej0 * ew + f.ei0);
+  const int pw = f.i1 - f.i0, np = pw * (f.j1 - f.j0);
+  for (int q = threadIdx.x; q < np; q += NT) {
+    const int jj = f.j0 + q / pw, ii = f.i0 + q % pw;
+    const long o = jj * pitch + eB.xoff + ii;
+    T K[NC], Uc[NC];
+    const RkIn<T, NC> rin = rk_load<T, NC, MB, KAHAN>(eB, o);
+    edge_rhs<T, ORDER, PHYS, MAT>(eB, view, FP_N, ew, ii, jj, o, K, Uc);
+    rk_store<T, NC, MB, KAHAN>(eB, o, K, Uc, rin);
   }
 }
 
@@ -943,14 +1059,16 @@ __device__ __forceinline__ void pair_tile_rw(const PairArgs<T> &a, int tile)
 #endif
 constexpr int pair_pf(int esz, int pf) { return esz == 4 ? pf + 1 : pf; }
 
-// Blocks [0, n_edge_blocks): stage 1 (pair 1) or stage 3 (pair 2) of the
-// D-deep frame (ea); the rest: two-stage interior tiles (a), XCD-remapped.
+// Blocks [0, n_edge_blocks): frame patches, stages 1-2 (pair 1) or 3-4
+// (pair 2) with eA / eB the two stages' arguments; the rest: two-stage
+// interior tiles (a), XCD-remapped.
 template <typename T, int ORDER, int PHYS, int MAT, int PAIR, bool KAHAN, int NT, int PF, bool NTS>
-__global__ __launch_bounds__(NT) void pair2d_kernel(StageArgs<T> ea, PairArgs<T> a)
+__global__ __launch_bounds__(NT) void pair2d_kernel(StageArgs<T> eA, StageArgs<T> eB, PairArgs<T> a)
 {
   const int b = blockIdx.x;
   if (b < a.n_edge_blocks) {
-    edge_point<T, ORDER, PHYS, MAT, PAIR == 1 ? MODE_S1 : MODE_S3, KAHAN>(ea, b * NT + threadIdx.x);
+    if (b < frame_patch_count(eA.nx, eA.ny, Ops<ORDER>::CLS + Ops<ORDER>::R))
+      frame_patch_pair<T, ORDER, PHYS, MAT, PAIR, KAHAN, NT>(eA, eB, b);
     return;
   }
   const int bi = b - a.n_edge_blocks;
@@ -1200,47 +1318,32 @@ struct Launcher1D {
   }
 };
 
-// Frame-only stages 2 and 4 of the two-stage schedule: one wave per block,
-// so the few thousand frame points spread over every CU.
-constexpr int kFrameBX = 64;
-template <typename T, int ORDER, int PHYS, int MAT, int MODE, bool KAHAN>
-__global__ __launch_bounds__(kFrameBX) void frame2d_kernel(StageArgs<T> a)
-{
-  edge_point<T, ORDER, PHYS, MAT, MODE, KAHAN>(a, blockIdx.x * kFrameBX + threadIdx.x);
-}
-
-// part 1: pair2d_kernel pair 1 (+ frame stage 1); part 2: frame stage 2;
-// part 3: pair2d_kernel pair 2 (+ frame stage 3); part 4: frame stage 4.
-// pair_args is ignored for parts 2 and 4.
-using PairLaunchFn = hipError_t (*)(int part, const void *edge_args, const void *pair_args, int grid,
-                                    hipStream_t s);
+// The two launches of a two-stage step: part 1 = pair2d_kernel pair 1
+// (frame patches and interior tiles, stages 1-2), part 3 = pair 2 (stages
+// 3-4).  stage_a / stage_b are the StageArgs of the launch's two stages.
+using PairLaunchFn = hipError_t (*)(int part, const void *stage_a, const void *stage_b, const void *pair_args,
+                                    int grid, hipStream_t s);
 
 // PF1 / PF2: rows of loads in flight ahead of the row being computed in the
-// pair-1 (one stencil operand per row) and pair-2 (operand + U + Y2) launches.
+// pair-1 (one stencil operand per row) and pair-2 (operand + U) launches.
 template <typename T, int ORDER, int PHYS, int MAT, bool KAHAN, int NT, int PF1, int PF2> struct PairLauncher {
-  static hipError_t launch(int part, const void *ea, const void *pa, int grid, hipStream_t s)
+  static hipError_t launch(int part, const void *sa, const void *sb, const void *pa, int grid, hipStream_t s)
   {
-    const StageArgs<T> &e = *static_cast<const StageArgs<T> *>(ea);
-    switch (part) {
-      case 1:
-      case 3: {
-        const PairArgs<T> &a = *static_cast<const PairArgs<T> *>(pa);
-        if (part == 1 && a.nts)
-          hipLaunchKernelGGL((pair2d_kernel<T, ORDER, PHYS, MAT, 1, KAHAN, NT, PF1, true>), dim3(grid), dim3(NT), 0, s, e, a);
-        else if (part == 1)
-          hipLaunchKernelGGL((pair2d_kernel<T, ORDER, PHYS, MAT, 1, KAHAN, NT, PF1, false>), dim3(grid), dim3(NT), 0, s, e, a);
-        else if (a.nts)
-          hipLaunchKernelGGL((pair2d_kernel<T, ORDER, PHYS, MAT, 2, KAHAN, NT, PF2, true>), dim3(grid), dim3(NT), 0, s, e, a);
-        else
-          hipLaunchKernelGGL((pair2d_kernel<T, ORDER, PHYS, MAT, 2, KAHAN, NT, PF2, false>), dim3(grid), dim3(NT), 0, s, e, a);
-        break;
-      }
-      case 2:
-        hipLaunchKernelGGL((frame2d_kernel<T, ORDER, PHYS, MAT, MODE_S2, KAHAN>), dim3(grid), dim3(kFrameBX), 0, s, e);
-        break;
-      default:
-        hipLaunchKernelGGL((frame2d_kernel<T, ORDER, PHYS, MAT, MODE_S4, KAHAN>), dim3(grid), dim3(kFrameBX), 0, s, e);
-    }
+    const StageArgs<T> &ea = *static_cast<const StageArgs<T> *>(sa);
+    const StageArgs<T> &eb = *static_cast<const StageArgs<T> *>(sb);
+    const PairArgs<T> &a = *static_cast<const PairArgs<T> *>(pa);
+    if (part == 1 && a.nts)
+      hipLaunchKernelGGL((pair2d_kernel<T, ORDER, PHYS, MAT, 1, KAHAN, NT, PF1, true>), dim3(grid), dim3(NT), 0, s, ea,
+                         eb, a);
+    else if (part == 1)
+      hipLaunchKernelGGL((pair2d_kernel<T, ORDER, PHYS, MAT, 1, KAHAN, NT, PF1, false>), dim3(grid), dim3(NT), 0, s, ea,
+                         eb, a);
+    else if (a.nts)
+      hipLaunchKernelGGL((pair2d_kernel<T, ORDER, PHYS, MAT, 2, KAHAN, NT, PF2, true>), dim3(grid), dim3(NT), 0, s, ea,
+                         eb, a);
+    else
+      hipLaunchKernelGGL((pair2d_kernel<T, ORDER, PHYS, MAT, 2, KAHAN, NT, PF2, false>), dim3(grid), dim3(NT), 0, s, ea,
+                         eb, a);
     return hipGetLastError();
   }
 };

@@ -210,8 +210,10 @@ int  sbp_synchronize(sbp_ctx *ctx);
 /* Run nsteps resident steps bracketed by HIP events on the context's stream;
  * *ms receives the elapsed event time.  Used by the benchmark. */
 int  sbp_time_steps(sbp_ctx *ctx, double t0, double dt, long nsteps, double *ms);
-/* Time each of the four stage kernels separately (HIP events around every
- * launch, nsteps steps); ms_stage[4] receives the mean duration per launch. */
+/* Time each launch of a step separately (HIP events around every launch,
+ * nsteps steps); ms_stage[4] receives the mean duration per launch: one per
+ * stage, or with two-stage stepping the launches of stages 1-2 and 3-4 in
+ * ms_stage[0] and ms_stage[2] (ms_stage[1] = ms_stage[3] = 0). */
 int  sbp_time_stages(sbp_ctx *ctx, double dt, long nsteps, double ms_stage[4]);
 
 /* ---- norms ----------------------------------------------------------------

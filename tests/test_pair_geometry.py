@@ -1,28 +1,28 @@
-"""CPU model of the two-stage interior launches (sbp_capi.hip
-compute_pair_geometry / default_pair_rows, sbp_kernels.hpp pair_tile_rw /
-pair2d_kernel, edge_point with a D = CLS + R deep frame).
+"""CPU model of the two-stage launches (sbp_capi.hip compute_pair_geometry /
+default_pair_rows, sbp_kernels.hpp pair_tile_rw / pair2d_kernel and the frame
+patches frame_patch / frame_patch_pair of the D = CLS + R deep frame).
 
 Checks, for many grid shapes and all orders, that
 * every global load of a pair tile (operand rows with prefetch, halo
-  columns, the mid-row and output-row pointwise loads) and of the D-deep
-  frame's table rows stays inside [0,nx) x [0,ny);
-* the pair tiles' output points and the frame partition the grid;
+  columns, the mid-row and output-row pointwise loads) and of the frame
+  patches' table rows stays inside [0,nx) x [0,ny);
+* the pair tiles' output points and the frame patches partition the grid;
 * every mid point an output point differentiates is computed by the same
   tile, at an unclamped column, from input columns the tile staged, and is an
   interior point of the operator (so the interior stencil is the right row);
-* every interior point the frame's stages 2 and 4 read (zero-padded taps
-  included) is one of the points pair 1 stores Y2 and pair 2 stores Y4 at
-  (the band next to the frame: pair 1 stores Y2 nowhere else, pair 2
-  recomputes it);
+* every point the second stage of a frame patch reads (zero-padded taps
+  included: a stale non-finite value times zero would still poison the sum)
+  lies in the patch grown by RW = TAPS - R - 1, where the patch computed the
+  first stage, and the grown patch fits the LDS array;
 * the register windows and FIFOs hold the rows each phase reads
   (pair_tile_rw);
 * the ensemble kernels' persistent loops take every work item of every XCD
-  chunk, and every one-wave part of a frame item, exactly once.
+  chunk exactly once.
 """
 import numpy as np
 import pytest
 
-from test_kernel_geometry import CLS, R, TAPS, NXCD, dline_range, edge_decode, xcd_tile
+from test_kernel_geometry import CLS, R, TAPS, NXCD, dline_range, xcd_tile
 
 def pair_nt(esz, order):
     """Threads per two-stage tile."""
@@ -53,10 +53,38 @@ def pair_geometry(nx, ny, order, rows=None, esz=8):
     bxo = NT - 2 * r
     tiles_x = (wi + bxo - 1) // bxo if wi > 0 else 0
     n_tiles = tiles_x * ((hi + rows - 1) // rows) if (wi > 0 and hi > 0) else 0
-    n_edge = 2 * d * nx + 2 * d * (ny - 2 * d)
-    eb = (n_edge + NT - 1) // NT
-    neb = (eb + NXCD - 1) // NXCD * NXCD
-    return dict(d=d, rows=rows, tiles_x=tiles_x, n_tiles=n_tiles, n_edge=n_edge, n_edge_blocks=neb, nt=NT)
+    n_patches = frame_patch_count(nx, ny, d)
+    neb = (n_patches + NXCD - 1) // NXCD * NXCD
+    return dict(d=d, rows=rows, tiles_x=tiles_x, n_tiles=n_tiles, n_patches=n_patches, n_edge_blocks=neb, nt=NT)
+
+
+PATCH_W = 64  # kPatchW
+
+
+def frame_patch_count(nx, ny, d):
+    return 2 * (-(-nx // PATCH_W)) + 2 * (-(-(ny - 2 * d) // PATCH_W))
+
+
+def frame_patch(p, nx, ny, d, rw, taps):
+    """(i0, i1, j0, j1) of the patch and (ei0, ei1, ej0, ej1): grown by rw,
+    sideways at least to the taps-point lines of the closure rows."""
+    nb = -(-nx // PATCH_W)
+    if p < 2 * nb:
+        top = p >= nb
+        k = p - nb if top else p
+        i0, i1 = k * PATCH_W, min(k * PATCH_W + PATCH_W, nx)
+        j0, j1 = (ny - d, ny) if top else (0, d)
+        ei0, ei1 = max(min(i0 - rw, nx - taps), 0), min(max(i1 + rw, taps), nx)
+        ej0, ej1 = (max(ny - d - rw, 0), ny) if top else (0, min(d + rw, ny))
+    else:
+        nl = -(-(ny - 2 * d) // PATCH_W)
+        right = p - 2 * nb >= nl
+        k = p - 2 * nb - (nl if right else 0)
+        j0, j1 = d + k * PATCH_W, min(d + k * PATCH_W + PATCH_W, ny - d)
+        i0, i1 = (nx - d, nx) if right else (0, d)
+        ej0, ej1 = max(j0 - rw, 0), min(j1 + rw, ny)
+        ei0, ei1 = (max(nx - d - rw, 0), nx) if right else (0, min(d + rw, nx))
+    return (i0, i1, j0, j1), (ei0, ei1, ej0, ej1)
 
 
 def simulate_windows(order, nrows):
@@ -107,26 +135,27 @@ def check(nx, ny, order, rows=None, pf=None, esz=8):
     r, d, cls = R[order], g["d"], CLS[order]
     LW = NT + 2 * r
     count = np.zeros((ny, nx), dtype=np.int32)
-    # frame: stage kernels with depth d
-    frame_reads_interior = set()
-    for e in range(g["n_edge"]):
-        i, j = edge_decode(e, nx, ny, d)
-        assert 0 <= i < nx and 0 <= j < ny
-        count[j, i] += 1
-        # every tap read, the zero-padded ones included: a stale non-finite
-        # value times zero would still poison the sum
-        lo, hi = dline_range(nx, i, order)
-        assert 0 <= lo and hi < nx
-        for x in range(lo, hi + 1):
-            if d <= x < nx - d and d <= j < ny - d:
-                frame_reads_interior.add((j, x))
-        lo, hi = dline_range(ny, j, order)
-        assert 0 <= lo and hi < ny
-        for y in range(lo, hi + 1):
-            if d <= i < nx - d and d <= y < ny - d:
-                frame_reads_interior.add((y, i))
-    assert g["n_edge_blocks"] * NT >= g["n_edge"] and g["n_edge_blocks"] % NXCD == 0
-    ring = set()
+    # frame patches: first stage on the grown patch, second on the patch
+    rw = TAPS[order] - r - 1
+    fp_n = (d + rw) * (PATCH_W + 2 * rw)
+    for p in range(g["n_patches"]):
+        (i0, i1, j0, j1), (ei0, ei1, ej0, ej1) = frame_patch(p, nx, ny, d, rw, TAPS[order])
+        assert 0 <= ei0 <= i0 < i1 <= ei1 <= nx and 0 <= ej0 <= j0 < j1 <= ej1 <= ny
+        assert (ei1 - ei0) * (ej1 - ej0) <= fp_n
+        for j in range(ej0, ej1):       # first stage: every read inside the grid
+            for i in range(ei0, ei1):
+                lo, hi = dline_range(nx, i, order)
+                assert 0 <= lo and hi < nx
+                lo, hi = dline_range(ny, j, order)
+                assert 0 <= lo and hi < ny
+        for j in range(j0, j1):         # second stage: every read inside the grown patch
+            for i in range(i0, i1):
+                count[j, i] += 1
+                lo, hi = dline_range(nx, i, order)
+                assert ei0 <= lo and hi < ei1, (p, i, j, lo, hi, ei0, ei1)
+                lo, hi = dline_range(ny, j, order)
+                assert ej0 <= lo and hi < ej1, (p, i, j, lo, hi, ej0, ej1)
+    assert g["n_edge_blocks"] >= g["n_patches"] and g["n_edge_blocks"] % NXCD == 0
     tiles = [xcd_tile(bi, g["n_tiles"]) for bi in range(g["n_tiles"])]
     assert sorted(tiles) == list(range(g["n_tiles"]))
     t = np.arange(NT)
@@ -169,12 +198,7 @@ def check(nx, ny, order, rows=None, pf=None, esz=8):
             for s in range(2 * r + 1):
                 assert cls <= o - r + s < ny - cls
             count[o, cols] += 1
-            rw = TAPS[order] - r - 1
-            for c in cols:
-                if c - d < rw or nx - d - 1 - c < rw or o - d < rw or ny - d - 1 - o < rw:
-                    ring.add((o, c))
     assert (count == 1).all(), (nx, ny, order, np.argwhere(count != 1)[:5])
-    assert frame_reads_interior <= ring, sorted(frame_reads_interior - ring)[:5]
     return g
 
 
@@ -220,10 +244,10 @@ def test_pair_geometry_large_grid_parameters():
 
 def batch_traversal(counts, grid, sub):
     """Replays the persistent loops of the ensemble kernels (sbp_batch.hpp
-    batch_stage_kernel / batch_pair_kernel with sub = 1, batch_frame_kernel
-    with sub = NT / 64 and grid * sub blocks): block b works on XCD b % 8's
-    chunk, from item (b / 8) / sub on, striding by the XCD's block count.
-    Returns the (xcd, item, part) triples visited."""
+    batch_stage_kernel / batch_pair_kernel: sub = 1; sub > 1 splits every
+    item over sub blocks of grid * sub): block b works on XCD b % 8's chunk,
+    from item (b / 8) / sub on, striding by the XCD's block count.  Returns
+    the (xcd, item, part) triples visited."""
     assert grid % NXCD == 0
     nb = grid * sub
     seen = []
@@ -240,9 +264,9 @@ def batch_traversal(counts, grid, sub):
 @pytest.mark.parametrize("sub", [1, 4])
 @pytest.mark.parametrize("grid", [8, 256, 1024, 4096])
 def test_batch_work_traversal_visits_every_item_once(grid, sub):
-    """Every work item of every XCD chunk (and every wave-sized part of a
-    frame item) is taken by exactly one block, for chunks shorter and longer
-    than the XCD's share of the grid, and empty ones."""
+    """Every work item of every XCD chunk (and every part of an item split
+    over several blocks) is taken by exactly one block, for chunks shorter
+    and longer than the XCD's share of the grid, and empty ones."""
     rng = np.random.default_rng(grid + sub)
     for counts in ([0] * 8, [1, 0, 3, 0, 0, 7, 0, 2], list(rng.integers(0, 2000, 8))):
         seen = batch_traversal(counts, grid, sub)
