@@ -40,8 +40,8 @@ PEAK_HBM_GBS = 8000.0
 # (the factors that turn the counters of a 1 GiB plain copy in the same runs
 # into the bytes it reads and writes), with the commit it was measured at:
 # profiles/pair2d_acoustic_4097sq_order4_fp64_hbm_counters.md.
-MEASURED_TRAFFIC = {(4097, 4, 0, "two-stage", 3): (535295 * 2.0 + 406522 * 1.0) * 1024}
-MEASURED_AT = "885b11b"
+MEASURED_TRAFFIC = {(4097, 4, 0, "two-stage", 3): (555685 * 2.0 + 405237 * 1.0) * 1024}
+MEASURED_AT = "49e0a32"
 
 
 def pkg(sub):
