@@ -16,7 +16,9 @@ Prints one JSON line:
               library's stream), against the 8 TB/s peak; `traffic` is the
               same launch's HBM bytes measured by rocprofv3 counters.  The
               whole step's rate on its own bytes, and on the 16-word basis of
-              a 3-register RK4 with one launch per stage, are reported beside it
+              a 3-register RK4 with one launch per stage, are reported beside
+              it, and so is `stream_ceiling`: the rate of a flat elementwise
+              kernel that reads and writes the launch's planes per point
   cpu_baseline the oracle compiled with the reference's optimisation flags,
               pthreads over a near-square block decomposition, timed on a
               bounded number of steps of the same workload on this host
@@ -95,6 +97,22 @@ def launch_bytes(spec, fused):
     b1 = ni * (2 * C + m) + nf * (3 * C + m)
     b3 = ni * ((3 + k2) * C + m) + nf * ((4 + k2) * C + m)
     return [float(b1 * w), 0.0, float(b3 * w), 0.0]
+
+
+def launch_planes(spec, fused, slot):
+    """(planes read, planes written) per interior point of the launch in stage
+    slot `slot` (1-4), one plane being one component or material array: the
+    byte mix launch_bytes counts, for the flat streaming ceiling."""
+    kahan = spec.precision == 2
+    m = 2 if spec.material == 2 else 0
+    C = spec.components
+    if fused:
+        if slot == 1:
+            return C + m, C
+        return (3 if kahan else 2) * C + m, (2 if kahan else 1) * C
+    reads = {1: 1, 2: 2, 3: 2, 4: 4 + (1 if kahan else 0)}[slot]
+    writes = 1 + (1 if kahan and slot == 4 else 0)
+    return reads * C + m, writes * C
 
 
 TILE_ROWS = 0  # --tile-rows: fixed two-stage tile height instead of sbp_tune_fused (profiling runs)
@@ -356,6 +374,12 @@ def main():
 
     dofs = spec.nx * spec.ny * spec.components
     dom = main_m["dominant_launch"]
+    # the dominant launch's byte mix in a flat elementwise kernel (no stencil,
+    # halo or tiles): the rate those bytes reach as a plain stream
+    nr, nw = launch_planes(spec, main_m["schedule"] == "two-stage", dom)
+    plane_bytes = spec.nx * spec.ny * (8 if spec.precision == 0 else 4)
+    stream_gbs = capi.stream_bandwidth(plane_bytes, nr, nw, 5)
+    log("flat stream, %d planes read + %d written: %.0f GB/s" % (nr, nw, stream_gbs))
     traffic = MEASURED_TRAFFIC.get((spec.nx, spec.order, spec.precision, main_m["schedule"], dom))
     if main_m["schedule"] == "two-stage":
         names = {1: "pair2d_kernel, RK4 stages 1-2 (interior tiles and frame patches)", 2: "(no launch)",
@@ -397,7 +421,12 @@ def main():
                                            "note": "3-register RK4 with one launch per stage (16 words); "
                                                    "the two-stage schedule needs fewer bytes, so this rate "
                                                    "is not bounded by the peak"},
-                     "copy_ceiling_gbs": copy_gbs},
+                     "copy_ceiling_gbs": copy_gbs,
+                     "stream_ceiling": {"planes_read": nr, "planes_written": nw, "plane_bytes": plane_bytes,
+                                        "gbs": stream_gbs, "frac": main_m["dominant_gbs"] / stream_gbs,
+                                        "note": "the dominant launch's byte mix per point in a flat elementwise "
+                                                "kernel (sbp_stream_bandwidth); frac is the launch's "
+                                                "algorithmic rate over it"}},
         "cpu_baseline": cpu,
         "time_to_l2_1e-4": tta,
         "secondary": secondary,

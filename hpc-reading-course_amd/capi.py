@@ -118,6 +118,7 @@ EXPORTS = {
     "sbp_device_arch": (C.c_char_p, []),
     "sbp_version": (C.c_int, []),
     "sbp_copy_bandwidth": (C.c_int, [C.c_long, C.c_int, _DP]),
+    "sbp_stream_bandwidth": (C.c_int, [C.c_long, C.c_int, C.c_int, C.c_int, _DP]),
 }
 
 _lib = None
@@ -604,6 +605,14 @@ def copy_bandwidth(nbytes: int = 1 << 30, reps: int = 5) -> float:
     """GB/s of a plain device copy of nbytes (read + write counted)."""
     g = C.c_double()
     _check(load_library().sbp_copy_bandwidth(int(nbytes), int(reps), C.byref(g)))
+    return g.value
+
+
+def stream_bandwidth(plane_bytes: int, n_read: int, n_write: int, reps: int = 5) -> float:
+    """GB/s (read + write counted) of a flat elementwise kernel reading n_read
+    and writing n_write planes of plane_bytes each."""
+    g = C.c_double()
+    _check(load_library().sbp_stream_bandwidth(int(plane_bytes), int(n_read), int(n_write), int(reps), C.byref(g)))
     return g.value
 
 

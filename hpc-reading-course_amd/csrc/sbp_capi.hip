@@ -1297,6 +1297,112 @@ int sbp_copy_bandwidth(long bytes, int reps, double *gbs)
   return SBP_OK;
 }
 
+}  // extern "C"
+
+// Flat streaming kernel: every point reads NR planes and writes NW planes
+// (16 bytes per plane and thread per pass, grid-stride, nontemporal stores),
+// the byte mix of a launch without its stencils, halos or tiles.
+template <int NR, int NW>
+static __global__ __launch_bounds__(256) void stream_probe_kernel(const double2 *__restrict__ src,
+                                                                  double2 *__restrict__ dst, long n2)
+{
+  for (long i = blockIdx.x * 256L + threadIdx.x; i < n2; i += (long)gridDim.x * 256) {
+    double sx = 0, sy = 0;
+#pragma unroll
+    for (int c = 0; c < NR; c++) {
+      const double2 v = src[c * n2 + i];
+      sx += v.x;
+      sy += v.y;
+    }
+#pragma unroll
+    for (int c = 0; c < NW; c++) {
+      double *d = reinterpret_cast<double *>(dst + c * n2 + i);
+      __builtin_nontemporal_store(sx + c, d);
+      __builtin_nontemporal_store(sy + c, d + 1);
+    }
+  }
+}
+
+typedef void (*StreamProbeFn)(const double2 *, double2 *, long);
+template <int NR, int NW> static void *stream_probe_ptr() { return (void *)stream_probe_kernel<NR, NW>; }
+template <int NR> static void *stream_probe_nw(int nw)
+{
+  switch (nw) {
+  case 1: return stream_probe_ptr<NR, 1>();
+  case 2: return stream_probe_ptr<NR, 2>();
+  case 3: return stream_probe_ptr<NR, 3>();
+  case 4: return stream_probe_ptr<NR, 4>();
+  case 5: return stream_probe_ptr<NR, 5>();
+  case 6: return stream_probe_ptr<NR, 6>();
+  }
+  return nullptr;
+}
+static void *stream_probe_fn(int nr, int nw)
+{
+  switch (nr) {
+  case 1: return stream_probe_nw<1>(nw);
+  case 2: return stream_probe_nw<2>(nw);
+  case 3: return stream_probe_nw<3>(nw);
+  case 4: return stream_probe_nw<4>(nw);
+  case 5: return stream_probe_nw<5>(nw);
+  case 6: return stream_probe_nw<6>(nw);
+  case 7: return stream_probe_nw<7>(nw);
+  case 8: return stream_probe_nw<8>(nw);
+  case 9: return stream_probe_nw<9>(nw);
+  case 10: return stream_probe_nw<10>(nw);
+  case 11: return stream_probe_nw<11>(nw);
+  case 12: return stream_probe_nw<12>(nw);
+  }
+  return nullptr;
+}
+
+extern "C" {
+
+int sbp_stream_bandwidth(long plane_bytes, int n_read, int n_write, int reps, double *gbs)
+{
+  if (plane_bytes < 1024 || n_read < 1 || n_read > 12 || n_write < 1 || n_write > 6 || reps < 1 || !gbs)
+    return fail(SBP_EINVAL, "bad argument");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(SBP_ENODEV, "no HIP device");
+  int dev = 0, ncu = 0;
+  if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev))
+    return fail(SBP_EHIP, "device query failed");
+  const long n2 = plane_bytes / (long)sizeof(double2);
+  void *fn = stream_probe_fn(n_read, n_write);
+  double2 *a = nullptr, *b = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = SBP_OK;
+  float best = 1e30f;
+  if (hipMalloc(&a, n_read * n2 * sizeof(double2)) != hipSuccess ||
+      hipMalloc(&b, n_write * n2 * sizeof(double2)) != hipSuccess) {
+    rc = fail(SBP_ENOMEM, "stream probe allocation failed");
+  } else if (hipMemset(a, 0, n_read * n2 * sizeof(double2)) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+             hipEventCreate(&e1) != hipSuccess) {
+    rc = fail(SBP_EHIP, "stream probe setup failed");
+  } else {
+    // 4, 8 and 16 blocks per CU; the best rate counts
+    for (int bpc = 4; bpc <= 16 && rc == SBP_OK; bpc *= 2) {
+      const int grid = (int)std::min<long>((long)ncu * bpc, (n2 + 255) / 256);
+      for (int r = 0; r <= reps && rc == SBP_OK; r++) {
+        (void)hipEventRecord(e0, nullptr);
+        hipLaunchKernelGGL((StreamProbeFn)fn, dim3(grid), dim3(256), 0, nullptr, (const double2 *)a, b, n2);
+        (void)hipEventRecord(e1, nullptr);
+        if (hipEventSynchronize(e1) != hipSuccess) rc = fail(SBP_EHIP, "stream probe failed");
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (r > 0 && ms < best) best = ms;  // first launch is a warm-up
+      }
+    }
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (a) (void)hipFree(a);
+  if (b) (void)hipFree(b);
+  if (rc) return rc;
+  *gbs = (double)(n_read + n_write) * n2 * sizeof(double2) / (best * 1e-3) / 1e9;
+  return SBP_OK;
+}
+
 int sbp_version(void) { return 100; }
 
 }  // extern "C"

@@ -336,6 +336,12 @@ int  sbp_version(void);
  * of `bytes` bytes: the achievable streaming ceiling, and the known-size
  * kernel that calibrates HBM traffic counters.  No reference counterpart. */
 int  sbp_copy_bandwidth(long bytes, int reps, double *gbs);
+/* Best-of-reps bandwidth of a flat elementwise kernel that reads n_read
+ * (1..12) and writes n_write (1..6) planes of plane_bytes each: the rate a
+ * launch's byte mix reaches without its stencils, halos or tiles (the
+ * dominant launch of the two-stage step reads 6 FP64 planes and writes 3).
+ * No reference counterpart. */
+int  sbp_stream_bandwidth(long plane_bytes, int n_read, int n_write, int reps, double *gbs);
 
 #ifdef __cplusplus
 }

@@ -256,6 +256,17 @@ def test_errors_are_reported(capi):
     assert capi.device_arch().startswith("gfx950")
 
 
+def test_bandwidth_probes(capi):
+    """The copy and flat streaming probes run, report positive rates below the
+    8 TB/s peak, and reject byte mixes outside their instantiated range."""
+    cp = capi.copy_bandwidth(64 << 20, 2)
+    st = capi.stream_bandwidth(16 << 20, 6, 3, 2)
+    assert 0 < cp < 8000 and 0 < st < 8000
+    for nr, nw in ((0, 1), (13, 1), (1, 0), (1, 7)):
+        with pytest.raises(capi.SbpError):
+            capi.stream_bandwidth(1 << 20, nr, nw, 1)
+
+
 @pytest.mark.parametrize("prec", [P.FP64, P.FP32])
 def test_error_norms_on_device(capi, orc, prec):
     """sbp_error_norms (vec_util.h:47-61) reduces on the device in FP64, for
