@@ -136,20 +136,20 @@ bool pair_supported(const sbp_ctx *c)
          c->ny > 2 * D;
 }
 
-// Two-stage stepping is the default where it was measured faster.  2D
-// acoustics: grids of at least 1000^2 interior points (order 4 FP64, ms per
-// step against one launch per stage: 1025^2 0.072 / 0.079, 1537^2 0.120 /
-// 0.157, 2049^2 0.176 / 0.238, 4097^2 0.667 / 0.980).  2D advection: at
-// least 1500^2 (order 4, one-stage / two-stage ms, constant velocity and
-// per-point fields: 1025^2 FP64 0.033 / 0.031 and 0.044 / 0.032, FP32 0.018
-// / 0.022 and 0.026 / 0.023; 2049^2 FP64 0.096 / 0.060 and 0.131 / 0.067,
-// FP32 0.064 / 0.038 and 0.078 / 0.042; 4097^2 FP64 0.316 / 0.202 and 0.450
-// / 0.263, FP32 0.220 / 0.090 and 0.264 / 0.125).
+// Two-stage stepping is the default where it was measured faster: grids of
+// at least 1000^2 interior points.  2D acoustics, order 4 FP64, ms per step
+// against one launch per stage: 1025^2 0.072 / 0.079, 1537^2 0.120 / 0.157,
+// 2049^2 0.176 / 0.238, 4097^2 0.667 / 0.980 (with the boundary frame in
+// launches of its own; 0.466 at 4097^2 with the frame patches).  2D
+// advection, order 4, one-stage / two-stage ms, constant velocity and
+// per-point fields: 1025^2 FP64 0.033 / 0.025 and 0.045 / 0.027, FP32 0.019 /
+// 0.018 and 0.026 / 0.020; 2049^2 FP64 0.096 / 0.052 and 0.131 / 0.063, FP32
+// 0.064 / 0.030 and 0.078 / 0.036; 4097^2 FP64 0.316 / 0.173 and 0.449 /
+// 0.261, FP32 0.220 / 0.079 and 0.264 / 0.116.
 bool pair_default(const sbp_ctx *c)
 {
   const int D = c->cls + c->order / 2;
-  const long side = c->phys == SBP_ADVECTION_2D ? 1500L : 1000L;
-  return pair_supported(c) && (long)(c->nx - 2 * D) * (c->ny - 2 * D) >= side * side;
+  return pair_supported(c) && (long)(c->nx - 2 * D) * (c->ny - 2 * D) >= 1000L * 1000L;
 }
 
 int default_ty_rows(int nx, int ny, int cls)

@@ -422,11 +422,37 @@ struct FusedRows {
 
 template <int V> using IC = std::integral_constant<int, V>;
 
+// Energy of the corrected candidate, reduced in the inference kernel's
+// epilogue (the batched corrected step): image k is the state of
+// inst[k], and block b writes the H-weighted energy of U + out_scale * net
+// over its output points to partial[b] (blocks of image k are contiguous).
+struct NetEnergyArgs {
+  const void *inst;  // BatchInst<T> of every image
+  const float *out_scale;
+  double *partial;
+  int order, band;
+};
+
+// Blocks of fused_net_kernel per image and the rows of each block's chunk.
+inline int net_chunk_rows(int H)
+{
+  const int nchunks = (H + FUSED_MAX_ROWS - 1) / FUSED_MAX_ROWS;  // at most FUSED_MAX_ROWS rows, chunks as even as possible
+  return (H + nchunks - 1) / nchunks;
+}
+inline long net_blocks_per_image(int H, int W)
+{
+  const int rows = net_chunk_rows(H);
+  return (long)((W + FOUT - 1) / FOUT) * ((H + rows - 1) / rows);
+}
+
+// T: type of the state whose corrected energy the epilogue reduces; void for
+// plain inference.
+template <typename T>
 __global__ __launch_bounds__(256) void fused_net_kernel(const bf16 *__restrict__ X, int nimg, int H, int W, int rows,
                                                         const bf16 *__restrict__ w1, const float *__restrict__ b1,
                                                         const bf16 *__restrict__ w2, const float *__restrict__ b2,
                                                         const bf16 *__restrict__ w3, const float *__restrict__ b3,
-                                                        float *__restrict__ out)
+                                                        float *__restrict__ out, NetEnergyArgs ea)
 {
   __shared__ __attribute__((aligned(16))) bf16 s_in[2][FPX * C_INP];
   __shared__ __attribute__((aligned(16))) bf16 s_a1[2][FPX * F1S];
@@ -577,12 +603,61 @@ __global__ __launch_bounds__(256) void fused_net_kernel(const bf16 *__restrict__
     step(t + 5, IC<1>{}, rc, ra, rb);
   }
 
+  // corrected-candidate energy (T != void): this block's U and materials,
+  // loaded before the output is written out
+  using TE = std::conditional_t<std::is_void_v<T>, float, T>;
+  __shared__ double sh[4];
+  constexpr int NP = (FUSED_MAX_ROWS * FOUT + 255) / 256;  // points per thread, at most
+  const int npt = (y1 - y0) * FOUT;
+  TE uv[NP][3], ri[NP], kb[NP];
+  const BatchInst<TE> *einst = static_cast<const BatchInst<TE> *>(ea.inst);
+  if constexpr (!std::is_void_v<T>) {
+    const BatchInst<TE> &d = einst[img];
+    // (every load of a thread issued before the first use: the march's
+    // registers are free by now, and the loads then wait once, not per point)
+#pragma unroll
+    for (int n = 0; n < NP; n++) {
+      const int p = threadIdx.x + 256 * n;
+      const int r = p / FOUT, i = c0 + 3 + (p - r * FOUT), j = y0 + r;
+      const long o = p < npt && i < W ? (long)j * d.pitch + d.xoff + i : (long)d.xoff;  // a valid point otherwise
+#pragma unroll
+      for (int c = 0; c < 3; c++) uv[n][c] = d.U[c * d.plane + o];
+      ri[n] = d.rinv ? d.rinv[o] : T(1);
+      kb[n] = d.bulk ? d.bulk[o] : T(1);
+    }
+  }
+
   // the staged output rows y0 .. y1-1, window columns 3 .. 60
   const int nval = (y1 - y0) * FOUT * C_OUT;
   for (int q = threadIdx.x; q < nval; q += 256) {
     const int r = q / (FOUT * C_OUT), e = q - r * (FOUT * C_OUT);
     const int col = c0 + 3 + e / C_OUT;
     if (col < W) out[(ibase + (long)(y0 + r) * W + c0 + 3) * C_OUT + e] = s_out[q];
+  }
+  if constexpr (!std::is_void_v<T>) {
+    // H-weighted energy of the candidate U + (T)(out_scale * net) at this
+    // block's output points (the points outside the corrected region keep U),
+    // from the staged values the loop above stored
+    const BatchInst<TE> &d = einst[img];
+    const double hx = 1.0 / (double)d.hix, hy = 1.0 / (double)d.hiy;
+    double acc = 0;
+#pragma unroll
+    for (int n = 0; n < NP; n++) {
+      const int p = threadIdx.x + 256 * n;
+      const int r = p / FOUT, i = c0 + 3 + (p - r * FOUT), j = y0 + r;
+      if (p >= npt || i >= W) continue;
+      const double w = hweight(ea.order, W, i, hx) * hweight(ea.order, H, j, hy);
+      const bool on = corrected_at(i, j, W, H, ea.band);
+      double v[3];
+#pragma unroll
+      for (int c = 0; c < 3; c++)
+        v[c] = on ? (double)(T)(uv[n][c] + (T)(s_out[p * C_OUT + c] * ea.out_scale[c])) : (double)uv[n][c];
+      const double rho = d.rinv ? 1.0 / (double)ri[n] : 1.0;
+      const double kinv = d.bulk ? 1.0 / (double)kb[n] : 1.0;
+      acc += w * (rho * (v[0] * v[0] + v[1] * v[1]) + kinv * v[2] * v[2]);
+    }
+    const double sm = block_sum_256(acc, sh);
+    if (threadIdx.x == 0) ea.partial[blockIdx.x] = sm;
   }
 }
 
@@ -845,41 +920,6 @@ __global__ void batch_energy_final_kernel(const double *__restrict__ partial, in
   e[k] = s;
 }
 
-// Energy of the corrected candidate U + (T)(out_scale * net) of every
-// instance, without storing it: blocks [k*bpi, (k+1)*bpi) reduce instance k.
-template <typename T>
-__global__ __launch_bounds__(256) void batch_candidate_energy_kernel(const BatchInst<T> *__restrict__ inst, int bpi,
-                                                                     int order, int H, int W, int band,
-                                                                     const float *__restrict__ net,
-                                                                     const float *__restrict__ out_scale,
-                                                                     double *__restrict__ partial)
-{
-  __shared__ double sh[4];
-  const int k = blockIdx.x / bpi, chunk = blockIdx.x % bpi;
-  const BatchInst<T> &d = inst[k];
-  const long npts = (long)H * W;
-  const float *nk = net + (long)k * npts * C_OUT;
-  const double hx = 1.0 / (double)d.hix, hy = 1.0 / (double)d.hiy;
-  double acc = 0;
-  for (long q = chunk * 256L + threadIdx.x; q < npts; q += (long)bpi * 256) {
-    const int j = (int)(q / W), i = (int)(q - (long)j * W);
-    const long o = (long)j * d.pitch + d.xoff + i;
-    const double w = hweight(order, W, i, hx) * hweight(order, H, j, hy);
-    const bool on = corrected_at(i, j, W, H, band);
-    double v[3];
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-      const T u = d.U[c * d.plane + o];
-      v[c] = on ? (double)(T)(u + (T)(nk[q * C_OUT + c] * out_scale[c])) : (double)u;
-    }
-    const double rho = d.rinv ? 1.0 / (double)d.rinv[o] : 1.0;
-    const double kinv = d.bulk ? 1.0 / (double)d.bulk[o] : 1.0;
-    acc += w * (rho * (v[0] * v[0] + v[1] * v[1]) + kinv * v[2] * v[2]);
-  }
-  const double sm = block_sum_256(acc, sh);
-  if (threadIdx.x == 0) partial[blockIdx.x] = sm;
-}
-
 // U += (T)(out_scale * net) where the candidate's energy passed the bound
 // (the same expression the energy was taken of); count the refusals.  The
 // candidate energies are summed here from their per-block partials (bpi per
@@ -1011,13 +1051,13 @@ int forward(sbp_corrector *c, int nb)
 }
 
 // Network output Yo of the nb packed images X0 (inference: the fused kernel).
-void launch_infer(sbp_corrector *c, int nb, hipStream_t stream)
+// T != void: also the corrected candidate's energy partials (NetEnergyArgs).
+template <typename T = void>
+void launch_infer(sbp_corrector *c, int nb, hipStream_t stream, NetEnergyArgs ea = NetEnergyArgs{})
 {
-  // at most FUSED_MAX_ROWS rows per chunk, the chunks of an image as even as possible
-  const int nchunks = (c->H + FUSED_MAX_ROWS - 1) / FUSED_MAX_ROWS, rows = (c->H + nchunks - 1) / nchunks;
-  const long blocks = (long)nb * ((c->W + FOUT - 1) / FOUT) * ((c->H + rows - 1) / rows);
-  hipLaunchKernelGGL(fused_net_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, c->X0, nb, c->H, c->W, rows,
-                     c->w1, c->b1, c->w2, c->b2, c->w3, c->b3, c->Yo);
+  const long blocks = (long)nb * net_blocks_per_image(c->H, c->W);
+  hipLaunchKernelGGL(fused_net_kernel<T>, dim3((unsigned)blocks), dim3(256), 0, stream, c->X0, nb, c->H, c->W,
+                     net_chunk_rows(c->H), c->w1, c->b1, c->w2, c->b2, c->w3, c->b3, c->Yo, ea);
 }
 
 int infer(sbp_corrector *c, int nb)
@@ -1125,7 +1165,7 @@ int sbp_corr_create(int nx, int ny, int batch, sbp_corrector **out)
   if (!rc) rc = dmalloc(&c->rejected, sizeof(int));
   if (!rc) rc = dmalloc(&c->idx, sizeof(int) * batch);
   if (!rc) rc = dmalloc(&c->bpart, sizeof(double) * batch * 64);
-  if (!rc) rc = dmalloc(&c->cpart, sizeof(double) * batch * (((long)nx * ny + 255) / 256));
+  if (!rc) rc = dmalloc(&c->cpart, sizeof(double) * batch * net_blocks_per_image(ny, nx));
   if (!rc) rc = dmalloc(&c->be_old, sizeof(double) * batch);
   if (!rc) rc = dmalloc(&c->brej, sizeof(int) * batch);
   if (!rc) {
@@ -1425,9 +1465,8 @@ int sbp_corr_advance_batch(sbp_corrector *c, sbp_batch *b, const double *dt, lon
   int rc = sbp::host::batch_upload_dt(b, dt);
   if (rc) return rc;
   const int count = b->count, bpi = 64;
-  // candidate-energy blocks per instance: few enough that batch_apply_kernel's
-  // blocks sum an instance's partials in one load per lane
-  const int cbpi = (int)std::min<long>(64, ((long)c->H * c->W + 255) / 256);
+  // candidate-energy partials per instance: one per inference block
+  const int cbpi = (int)net_blocks_per_image(c->H, c->W);
   const int pth_ = b->esz == 8 ? pth<double> : pth<float>;
   const int ptiles = ((c->W + PTW - 1) / PTW) * ((c->H + pth_ - 1) / pth_);
   const long npix = (long)count * c->H * c->W;
@@ -1453,18 +1492,14 @@ int sbp_corr_advance_batch(sbp_corrector *c, sbp_batch *b, const double *dt, lon
       const BatchInst<double> *inst = static_cast<const BatchInst<double> *>(sbp::host::batch_inst(b));
       hipLaunchKernelGGL(pack_batch_tiled_kernel<double>, dim3(count * ptiles), dim3(256), 0, s, inst, c->H, c->W,
                          c->norm, c->X0);
-      launch_infer(c, count, s);
-      hipLaunchKernelGGL(batch_candidate_energy_kernel<double>, dim3(count * cbpi), dim3(256), 0, s, inst, cbpi,
-                         b->order, c->H, c->W, c->band, c->Yo, c->norm + NORM_OUT, c->cpart);
+      launch_infer<double>(c, count, s, NetEnergyArgs{inst, c->norm + NORM_OUT, c->cpart, b->order, c->band});
       hipLaunchKernelGGL(batch_apply_kernel<double>, dim3(grid_for(npix)), dim3(256), 0, s, inst, count, c->H, c->W,
                          c->band, c->Yo, c->norm + NORM_OUT, c->be_old, c->cpart, cbpi, bound, c->brej);
     } else {
       const BatchInst<float> *inst = static_cast<const BatchInst<float> *>(sbp::host::batch_inst(b));
       hipLaunchKernelGGL(pack_batch_tiled_kernel<float>, dim3(count * ptiles), dim3(256), 0, s, inst, c->H, c->W,
                          c->norm, c->X0);
-      launch_infer(c, count, s);
-      hipLaunchKernelGGL(batch_candidate_energy_kernel<float>, dim3(count * cbpi), dim3(256), 0, s, inst, cbpi,
-                         b->order, c->H, c->W, c->band, c->Yo, c->norm + NORM_OUT, c->cpart);
+      launch_infer<float>(c, count, s, NetEnergyArgs{inst, c->norm + NORM_OUT, c->cpart, b->order, c->band});
       hipLaunchKernelGGL(batch_apply_kernel<float>, dim3(grid_for(npix)), dim3(256), 0, s, inst, count, c->H, c->W,
                          c->band, c->Yo, c->norm + NORM_OUT, c->be_old, c->cpart, cbpi, bound, c->brej);
     }

@@ -108,12 +108,12 @@ long sbp_vector_length(const sbp_ctx *ctx);
 int  sbp_set_tile_rows(sbp_ctx *ctx, int rows);
 
 /* Two-stage stepping (on = 1): each RK4 step runs stages 1-2 and stages 3-4
- * of the interior in one launch each, the inner stage kept on chip, and the
- * (CLS+R)-deep boundary frame in the one-stage kernels; 5 array passes per
- * step instead of 13, same arithmetic per point.  2D acoustic contexts (every
- * material model) and 2D advection contexts (constant or per-point velocity);
- * SBP_EINVAL otherwise.  On by default for 2D acoustics with at least 1000^2
- * interior points and 2D advection with at least 1500^2 (where each was
+ * in one launch each, the inner stage kept on chip: interior tiles, and
+ * patches of the (CLS+R)-deep boundary frame that evaluate both stages
+ * themselves; 5 array passes per step instead of 13, same arithmetic per
+ * point.  2D acoustic contexts (every material model) and 2D advection
+ * contexts (constant or per-point velocity); SBP_EINVAL otherwise.  On by
+ * default for grids of at least 1000^2 interior points (where it was
  * measured faster).  rows: tile height of the two-stage tiles
  * (0 = chosen from the grid).  A performance knob with no reference
  * counterpart: it changes how ts_rk4's steps are scheduled on the device

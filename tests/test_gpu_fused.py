@@ -195,8 +195,8 @@ def test_fused_then_rk45_and_time_stages(capi, orc):
 
 
 def test_fused_default_by_grid_size(capi):
-    """On by default for 2D acoustics with >= 1000^2 interior points and 2D
-    advection with >= 1500^2, off below and for the 1D problems."""
+    """On by default for 2D acoustics and 2D advection with >= 1000^2
+    interior points, off below and for the 1D problems."""
     big, _, _ = P.baseline_acoustic(1025, 4)
     with capi.Context(big) as ctx:
         assert ctx.fused
@@ -205,9 +205,9 @@ def test_fused_default_by_grid_size(capi):
     small, _, _ = P.baseline_acoustic(1001, 4)   # 989^2 interior points
     with capi.Context(small) as ctx:
         assert not ctx.fused
-    with capi.Context(adv_spec("adv", 4, 1513, 1513)) as ctx:   # 1501^2 interior points
+    with capi.Context(adv_spec("adv", 4, 1013, 1013)) as ctx:   # 1001^2 interior points
         assert ctx.fused
-    with capi.Context(adv_spec("adv", 4, 1511, 1511)) as ctx:
+    with capi.Context(adv_spec("adv", 4, 1011, 1011)) as ctx:
         assert not ctx.fused
     with capi.Context(P.adv1d(5000, 4)[0]) as ctx:
         assert not ctx.fused
