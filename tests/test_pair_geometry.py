@@ -37,7 +37,7 @@ def default_pair_rows(nx, ny, order, esz=8):
         return 16
     tx = (wi + NT - 2 * r - 1) // (NT - 2 * r)
     if esz == 8:
-        t = 68 if order == 6 else min(32, max(8, (hi + 50) // 100 // 2 * 2))
+        t = 136 if order == 6 else min(32, max(8, (hi + 50) // 100 // 2 * 2))
     else:
         t = 48 if order == 6 else 16
     while t > 8 and tx * ((hi + t - 1) // t) < 400:
@@ -233,7 +233,7 @@ def test_pair_register_windows(order, nrows):
 def test_pair_geometry_large_grid_parameters():
     g = pair_geometry(4097, 4097, 4)
     assert g["d"] == 6 and g["tiles_x"] == 17 and g["rows"] == 32 and g["n_tiles"] == 17 * 128
-    assert pair_geometry(4097, 4097, 6)["rows"] == 68 and pair_geometry(4097, 4097, 6)["nt"] == 256
+    assert pair_geometry(4097, 4097, 6)["rows"] == 136 and pair_geometry(4097, 4097, 6)["nt"] == 256
     assert pair_geometry(4097, 4097, 6, esz=4)["rows"] == 48
     assert pair_geometry(4097, 4097, 4, esz=4)["rows"] == 16
     assert pair_geometry(1025, 1025, 4)["rows"] == 10
