@@ -219,11 +219,21 @@ def reflection_2d(steps=2000, warmup=0):
             fused = ctx.fused
         dofs = spec.nx * spec.ny * 3
         bpd = bytes_per_dof(prec, True)
-        kbpd = sum(launch_bytes(spec, fused)) / dofs
+        lb = launch_bytes(spec, fused)
+        kbpd = sum(lb) / dofs
         step_s = float(np.sum(stage_ms)) * 1e-3
+        # the dominant launch against the flat stream of its own byte mix
+        dom = int(np.argmax(stage_ms))
+        dom_gbs = lb[dom] / (float(stage_ms[dom]) * 1e-3) / 1e9
+        nr, nw = launch_planes(spec, fused, dom + 1)
+        stream_gbs = capi.stream_bandwidth(spec.nx * spec.ny * (8 if prec == 0 else 4), nr, nw, 3)
         out[name] = {"dof_updates_per_s": dofs * steps / wall, "ms_per_step": wall / steps * 1e3,
                      "bytes_per_dof": bpd, "frac": dofs * bpd / step_s / 1e9 / PEAK_HBM_GBS,
                      "kernel_bytes_per_dof": kbpd, "kernel_frac": dofs * kbpd / step_s / 1e9 / PEAK_HBM_GBS,
+                     "stage_ms": [float(x) for x in stage_ms], "dominant_launch": dom + 1,
+                     "dominant_frac": dom_gbs / PEAK_HBM_GBS,
+                     "stream_ceiling": {"planes_read": nr, "planes_written": nw, "gbs": stream_gbs,
+                                        "frac": dom_gbs / stream_gbs},
                      "schedule": "two-stage" if fused else "one stage per launch"}
         del v0
     ref = finals["fp64"]
@@ -293,6 +303,8 @@ def main():
                     help="only run the time-to-L2-1e-4 comparison (plain FP64, batched FP32, learned)")
     ap.add_argument("--batched-only", action="store_true",
                     help="only run the 256-instance FP32 ensemble (--steps timed steps)")
+    ap.add_argument("--reflection-only", action="store_true",
+                    help="only run the order-6 variable-material 4097^2 comparison (--steps steps per precision)")
     ap.add_argument("--tile-rows", type=int, default=0,
                     help="two-stage tile height of the main workload (default: tuned on the grid)")
     ap.add_argument("--one-stage", action="store_true",
@@ -320,6 +332,9 @@ def main():
         return
     if args.batched_only:
         print(json.dumps(batched(steps=args.steps, warmup=args.warmup)), flush=True)
+        return
+    if args.reflection_only:
+        print(json.dumps(reflection_2d(steps=args.steps)), flush=True)
         return
 
     spec, v0, dt = P.baseline_acoustic(args.n, args.order)
