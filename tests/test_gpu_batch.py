@@ -208,7 +208,8 @@ def test_batch_store_policies(capi, orc, prec, policy):
     """Both store builds of the ensemble's two-stage kernels (parts 1 and 3;
     the 256-instance workload runs the nontemporal ones): every instance
     within 2e-6 (FP32; FP64: bit-identical) of the two-stage single-problem
-    path, FP64 within 1e-12 of the oracle."""
+    path; FP64 within 1e-12 of the oracle, FP32 and Kahan within 2e-5 of it
+    (the FP32 bar of the single-problem path)."""
     print("batch store policy", prec, policy, flush=True)
     ens = ensemble(prec, sizes=(20, 77, 130, 41, 257))
     dts = [e[2] for e in ens]
@@ -225,9 +226,10 @@ def test_batch_store_policies(capi, orc, prec, policy):
             ctx.set_fused(True)
             ctx.set_store_policy(policy)
             ref = ctx.rk4_advance(v0, 0.0, dt, 13)
+        o = orc.rk4(spec, v0, 0.0, dt, 13)
         if prec == P.FP64:
             assert np.array_equal(g, ref)
-            o = orc.rk4(spec, v0, 0.0, dt, 13)
             assert np.max(np.abs(g - o)) <= 1e-12 * np.max(np.abs(o))
         else:
             assert np.max(np.abs(g - ref)) <= 2e-6 * np.max(np.abs(ref))
+            assert np.max(np.abs(g - o)) <= 2e-5 * np.max(np.abs(o)), (spec.nx, prec)
