@@ -72,8 +72,8 @@ def raw_features(spec, state_aos):
     The first three are, up to a constant, the leading truncation error
     h^4/30 f^(5) of the fourth-order central stencil in each equation, so the
     per-step residual is close to a linear function of the inputs.  The device
-    packing kernels (sbp_corr.hip state_features) evaluate the same
-    expressions."""
+    packing kernels (sbp_corr.hip pack_state_tiled_kernel,
+    pack_batch_tiled_kernel) evaluate the same expressions."""
     q = np.asarray(state_aos, dtype=np.float64).reshape(spec.ny, spec.nx, 3)
     u, v, p = q[..., 0], q[..., 1], q[..., 2]
     ri = spec.rho_inv if spec.rho_inv is not None else np.ones((spec.ny, spec.nx))

@@ -1,19 +1,20 @@
 // sbp_corr.hip -- learned coarse-grid corrector: a 3-layer 3x3 convolutional
-// network (5 -> 32 -> 32 -> 3 channels, ReLU) applied to the state after each
-// coarse RK4 step.  Its inputs at each point are the clamped fourth
-// differences d4 = dxxxx + dyyyy of u, v and p, rho and K.  Forward, backward
-// and inference run on bf16 MFMA
-// (v_mfma_f32_16x16x16_bf16) with FP32 accumulation; master weights and Adam
-// moments are FP32.  No counterpart in the reference: the plain fine-grid
-// solve of the same path is its fallback and judge, and a correction that
-// lifts the discrete energy above the energy the solve started from by more
-// than a set bound is rejected.
+// network (8 -> 32 -> 32 -> 3 channels, ReLU) applied to the state after each
+// coarse RK4 step.  Its inputs at each point are eight normalised features of
+// the state and the materials (the fourth-difference terms of the stencil's
+// leading truncation error, the fourth differences of u, v and p, rho and
+// K: pack_state_tiled_kernel / pack_batch_tiled_kernel below).  Training
+// (forward and backward) runs on v_mfma_f32_16x16x16_bf16, inference on
+// v_mfma_f32_16x16x32_bf16 (fused_net_kernel), with FP32 accumulation;
+// master weights and Adam moments are FP32.  No counterpart in the
+// reference: the plain fine-grid solve of the same path is its fallback and
+// reference, and a correction that lifts the discrete energy above the
+// energy the solve started from by more than a set bound is rejected.
 //
 // Activations are channels-last [image][y][x][channel] in bf16, channels
-// padded to a multiple of 4 (inputs: 5 -> 8).  A convolution is a GEMM with
-// M = 16 pixels of one image row, N = output channels, K = 9 taps x input
-// channels (padded to a multiple of 16); one wavefront owns a 16-pixel
-// segment.  Backward-data is the same kernel with the weights transposed and
+// padded to a multiple of 4.  A training convolution is a GEMM with M = 16
+// pixels of one image row, N = output channels, K = 9 taps x input channels
+// (padded to a multiple of 16); one wavefront owns a 16-pixel segment.  Backward-data is the same kernel with the weights transposed and
 // the taps mirrored; weight gradients are a GEMM reduced over pixels, written
 // as per-wavefront partials and summed in a fixed order (deterministic).
 #include <hip/hip_runtime.h>
