@@ -3,10 +3,12 @@
 // Every instance owns a slice of six pooled allocations (U, A, Y0, Y1, the
 // optional Kahan compensation and the rho^-1 / K planes), laid out exactly as
 // a single-problem context lays out its registers.  A step is four launches
-// of the persistent stage kernel over the flattened work list; nsteps are
-// replayed from a captured hipGraph.  The tile shape and the number of
-// persistent blocks come from a small table of candidates, chosen by
-// sbp_batch_autotune on the ensemble at hand.
+// of the persistent stage kernel over the flattened work list (one-stage
+// configurations) or two of the persistent pair kernel (two-stage
+// configurations: stages 1-2 and 3-4, each over every instance's frame
+// patches and pair tiles); nsteps are replayed from a captured hipGraph.  The
+// schedule, tile shape and number of persistent blocks come from a small
+// table of candidates, chosen by sbp_batch_autotune on the ensemble at hand.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>

@@ -169,7 +169,7 @@ __device__ __forceinline__ T stage4_increment(const StageArgs<T> &a, T U, T Y2, 
 // else and stores after everything else: the compiler cannot move a load past
 // a store through these pointers (they may alias as far as it knows), and
 // loads issued late, one load-store pair per component, put several
-// dependent memory round trips on the critical path of the frame launches.
+// dependent memory round trips on the critical path of the edge blocks.
 // Distinct components live in distinct planes, and each component's loads
 // precede its own stores, so the values are unchanged.
 template <typename T, int NC> struct RkIn {

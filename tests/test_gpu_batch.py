@@ -109,8 +109,8 @@ def test_batch_two_stage_parity_and_switching(capi, orc, prec, order):
     """Two-stage configurations: graph replays (10 steps), odd remainders and
     a switch back to a one-stage configuration mid-run keep every instance
     on the single-problem path; FP64 within 1e-12 of the oracle.  Ragged
-    instance sizes, so the frame work items of the frame-only launches end
-    mid-wave."""
+    instance sizes, so the instances' frame patches and tiles of each launch's
+    work list end mid-wave."""
     print("batch two-stage", prec, order, flush=True)
     ens = ensemble(prec, sizes=(20, 77, 130, 41), order=order)
     dts = [e[2] for e in ens]
