@@ -1,10 +1,11 @@
-// sbp_batch.hpp -- persistent stage kernel of the batched acoustic ensemble.
+// sbp_batch.hpp -- persistent kernels of the batched acoustic ensemble.
 //
 // Many independent 2D acoustic problems of different sizes share one launch
-// per RK4 stage.  The host flattens the work of every instance into one list
-// of (instance, item) pairs -- the boundary-frame blocks and the interior
-// tiles of sbp_kernels.hpp -- and a fixed grid of persistent blocks walks
-// that list.  The list is cut into one chunk per XCD: whole instances,
+// per RK4 stage (batch_stage_kernel) or, in the two-stage configurations, one
+// per pair of stages (batch_pair_kernel).  The host flattens the work of every
+// instance into one list of (instance, item) pairs -- the boundary-frame
+// blocks (frame patches) and the interior tiles (pair tiles) of
+// sbp_kernels.hpp -- and a fixed grid of persistent blocks walks that list.  The list is cut into one chunk per XCD: whole instances,
 // bin-packed by work, so every XCD gets the same share of the heavy tiles and
 // an instance's halo rows stay in one L2.  The per-instance descriptors live
 // in device memory, so a captured graph stays valid when the time steps

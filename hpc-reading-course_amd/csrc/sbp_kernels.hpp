@@ -18,7 +18,9 @@
 // step instead of the 16 of a 3-register (U, Y, accumulator) scheme.  The
 // Kahan form compensates the final U += increment with one more register.
 // The 1D kernels keep the stage vectors and follow RK4_custom's operation
-// order exactly.
+// order exactly.  Large 2D grids run the two-stage schedule further down
+// (pair2d_kernel: stages 1-2 and 3-4 in one launch each, the inner stage on
+// chip, 5 words per value and step) with the same arithmetic per point.
 //
 // Layout in HBM: one plane per component, rows of `pitch` elements, column i
 // stored at xoff + i with xoff chosen so that the first interior column
