@@ -246,9 +246,10 @@ int  sbp_error_norms_noalloc(const double *v, const double *w, long n, double hx
 
 /* ---- batched ensemble ---------------------------------------------------
  * Many independent 2D acoustic problems (uneven sizes, own materials, own dt)
- * advanced together: one persistent kernel per RK4 stage walks a flattened
- * list of every instance's tiles, and nsteps are replayed from a captured
- * hipGraph.  No counterpart in the reference, which runs one problem per MPI
+ * advanced together: one persistent kernel per RK4 stage (or, in the
+ * two-stage configurations, per pair of stages) walks a flattened list of
+ * every instance's tiles and boundary blocks, and nsteps are replayed from a
+ * captured hipGraph.  No counterpart in the reference, which runs one problem per MPI
  * job (code/batch.sh:1-7); each instance's arithmetic is the single-problem
  * path's (sbp_rk4_advance).  All instances share order, precision and
  * material model (SBP_MAT_UNIT or SBP_MAT_ARRAYS). */
@@ -265,8 +266,8 @@ int  sbp_batch_synchronize(sbp_batch *b);
 int  sbp_batch_time_steps(sbp_batch *b, const double *dt, long nsteps, double *ms);
 int  sbp_batch_energy(sbp_batch *b, int k, double *E);
 /* Launch-shape candidates (tile width, tile rows, persistent blocks per CU,
- * and whether the step runs two RK4 stages per interior launch: the schedule
- * of sbp_set_fused, which needs every instance larger than 2(CLS+R) points;
+ * and whether the step runs two RK4 stages per launch: the schedule of
+ * sbp_set_fused, which needs every instance larger than 2(CLS+R) points;
  * sbp_batch_set_config fails with SBP_EINVAL otherwise). */
 int  sbp_batch_num_configs(void);
 int  sbp_batch_set_config(sbp_batch *b, int config);
