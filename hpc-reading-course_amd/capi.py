@@ -257,9 +257,9 @@ class Context:
         _check(load_library().sbp_set_tile_rows(self._h, int(rows)))
 
     def set_fused(self, on: bool = True, rows: int = 0):
-        """Two-stage stepping: interior stages 1-2 and 3-4 in one launch each
-        (2D acoustics with unit or array materials); rows = tile height, 0 =
-        chosen from the grid."""
+        """Two-stage stepping: stages 1-2 and 3-4 in one launch each, interior
+        tiles and boundary-frame patches alike (2D acoustics with any material
+        model, 2D advection); rows = tile height, 0 = chosen from the grid."""
         _check(load_library().sbp_set_fused(self._h, int(bool(on)), int(rows)))
 
     @property
