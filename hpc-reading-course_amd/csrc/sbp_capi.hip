@@ -107,8 +107,9 @@ void compute_pair_geometry(sbp_ctx *c)
 }
 
 // Default pair-tile height, from sweeps on gfx950 (ms per two-stage step):
-// FP64 orders 2 and 4 take about ny/100 rows, at most 32 (4097^2: 32 rows
-// 0.48 against 16 rows 0.50; 1025^2: 10 rows 0.063 against 8 rows 0.071);
+// FP64 orders 2 and 4 take about ny/100 rows, at most 40 (4097^2: 40 rows
+// 0.460 against 32 rows 0.465 and 48 rows 0.468; 1025^2: 10 rows 0.063
+// against 8 rows 0.071);
 // FP64 order 6 136 rows (4097^2 with rho/K arrays: 0.70 against 68 rows
 // 0.75 and 48 rows 0.80; rho = K = 1: 0.59 against 0.61), its 4R = 12 halo
 // rows per tile favouring tall tiles; FP32 16 rows (order 6: 48).  Halved
@@ -123,7 +124,7 @@ int default_pair_rows(const sbp_ctx *c)
   const int tx = (wi + nt - 2 * R - 1) / (nt - 2 * R);
   int t;
   if (c->esz == 8)
-    t = c->order == 6 ? 136 : std::min(32, std::max(8, (hi + 50) / 100 / 2 * 2));
+    t = c->order == 6 ? 136 : std::min(40, std::max(8, (hi + 50) / 100 / 2 * 2));
   else
     t = c->order == 6 ? 48 : 16;
   while (t > 8 && (long)tx * ((hi + t - 1) / t) < 400) t /= 2;
@@ -919,7 +920,7 @@ int sbp_tune_fused(sbp_ctx *ctx, double dt, int *rows)
   }
   const int R = ctx->order / 2, D = ctx->cls + R;
   const int hi = ctx->ny - 2 * D;
-  static const int cand[] = {8, 10, 12, 16, 20, 24, 32, 48, 68, 96, 136};
+  static const int cand[] = {8, 10, 12, 16, 20, 24, 32, 40, 48, 68, 96, 136};
   // the trial steps advance the state: save it (and its compensation) first
   const size_t reg = ctx->esz * (size_t)ctx->C * ctx->plane;
   void *save_u = nullptr, *save_c = nullptr;

@@ -33,7 +33,7 @@ StageLaunchFn pick_launcher(int phys, int mat)
 template <typename T, int ORDER, bool KAHAN>
 PairLaunchFn pick_pair(int phys, int mat)
 {
-  constexpr int PF1 = pair_pf(sizeof(T), SBP_PAIR_PF1), PF2 = pair_pf(sizeof(T), SBP_PAIR_PF2);
+  constexpr int PF1 = pair_pf(sizeof(T), 1), PF2 = pair_pf(sizeof(T), 2);
   constexpr int NT = pair_nt(sizeof(T), ORDER);
   if (phys == PH_ACOUSTIC) {
     if (mat == M_UNIT) return &PairLauncher<T, ORDER, PH_ACOUSTIC, M_UNIT, KAHAN, NT, PF1, PF2>::launch;

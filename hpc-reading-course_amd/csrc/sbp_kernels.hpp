@@ -1049,17 +1049,15 @@ __device__ __forceinline__ void frame_patch_pair(const StageArgs<T> &eA, const S
 }
 
 // Rows of loads in flight in the pair-1 and pair-2 tiles (sweep on gfx950):
-// FP64 tiles run SBP_PAIR_PF1/PF2 rows ahead; FP32 rows take half the
-// registers, and their tiles run one row further ahead (4% faster on
-// ensembles of 513^2 grids, where one round of tiles fills the GPU; one
-// more row slowed FP64).
-#ifndef SBP_PAIR_PF1
-#define SBP_PAIR_PF1 2
-#endif
-#ifndef SBP_PAIR_PF2
-#define SBP_PAIR_PF2 2
-#endif
-constexpr int pair_pf(int esz, int pf) { return esz == 4 ? pf + 1 : pf; }
+// Rows of loads in flight per wave of a two-stage tile, from sweeps on
+// gfx950.  FP32 rows take half the registers, and their tiles run three rows
+// ahead (4% faster than two on ensembles of 513^2 grids, where one round of
+// tiles fills the GPU).  FP64 pair 1 runs three too (4097^2 order 4: 0.189
+// against 0.197 ms at 40-row tiles, still three waves per SIMD; order-6
+// rho/K: 0.267 against 0.290), FP64 pair 2 two (a third row cost it time).
+// The ensemble's FP64 tiles keep two in both launches.
+constexpr int pair_pf(int esz, int pair) { return esz == 4 || pair == 1 ? 3 : 2; }
+constexpr int batch_pair_pf(int esz) { return esz == 4 ? 3 : 2; }
 
 // Blocks [0, n_edge_blocks): frame patches, stages 1-2 (pair 1) or 3-4
 // (pair 2) with eA / eB the two stages' arguments; the rest: two-stage

@@ -390,7 +390,7 @@ def test_tune_fused_keeps_state_and_results(capi, prec):
         ctx.set_state(v0)
         before = ctx.get_state()
         rows = ctx.tune_fused(dt)
-        assert rows in (8, 10, 12, 16, 20, 24, 32, 48, 68, 96, 136)
+        assert rows in (8, 10, 12, 16, 20, 24, 32, 40, 48, 68, 96, 136)
         assert np.array_equal(ctx.get_state(), before)
         ctx.step(0.0, dt, 21)
         tuned = ctx.get_state()

@@ -37,7 +37,7 @@ def default_pair_rows(nx, ny, order, esz=8):
         return 16
     tx = (wi + NT - 2 * r - 1) // (NT - 2 * r)
     if esz == 8:
-        t = 136 if order == 6 else min(32, max(8, (hi + 50) // 100 // 2 * 2))
+        t = 136 if order == 6 else min(40, max(8, (hi + 50) // 100 // 2 * 2))
     else:
         t = 48 if order == 6 else 16
     while t > 8 and tx * ((hi + t - 1) // t) < 400:
@@ -122,10 +122,11 @@ def simulate_windows(order, nrows):
             assert fU == list(range(o - r, o + r + 1)) and fU[r] == o and hdx[0] == o
 
 
-def pair_pf(esz, pf=2):
-    """Rows of loads in flight of a two-stage tile (sbp_kernels.hpp pair_pf):
-    one more for FP32."""
-    return pf + 1 if esz == 4 else pf
+def pair_pf(esz):
+    """Rows of loads in flight of a two-stage tile, the deepest of the two
+    launches (sbp_kernels.hpp pair_pf: FP64 pair 1 and every FP32 tile run
+    three rows ahead, FP64 pair 2 two; the ensemble's FP64 tiles two)."""
+    return 3
 
 
 def check(nx, ny, order, rows=None, pf=None, esz=8):
@@ -232,7 +233,7 @@ def test_pair_register_windows(order, nrows):
 
 def test_pair_geometry_large_grid_parameters():
     g = pair_geometry(4097, 4097, 4)
-    assert g["d"] == 6 and g["tiles_x"] == 17 and g["rows"] == 32 and g["n_tiles"] == 17 * 128
+    assert g["d"] == 6 and g["tiles_x"] == 17 and g["rows"] == 40 and g["n_tiles"] == 17 * 103
     assert pair_geometry(4097, 4097, 6)["rows"] == 136 and pair_geometry(4097, 4097, 6)["nt"] == 256
     assert pair_geometry(4097, 4097, 6, esz=4)["rows"] == 48
     assert pair_geometry(4097, 4097, 4, esz=4)["rows"] == 16
