@@ -40,10 +40,11 @@ PEAK_HBM_GBS = 8000.0
 # order 4, rho = K = 1, FP64, two-stage schedule: the pair2d_kernel launch of
 # RK4 stages 3-4), from rocprofv3 as FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 KiB
 # (the factors that turn the counters of a 1 GiB plain copy in the same runs
-# into the bytes it reads and writes), with the commit it was measured at:
-# profiles/pair2d_acoustic_4097sq_order4_fp64_hbm_counters.md.
-MEASURED_TRAFFIC = {(4097, 4, 0, "two-stage", 3): (555685 * 2.0 + 405237 * 1.0) * 1024}
-MEASURED_AT = "49e0a32"
+# into the bytes it reads and writes), with the commit and the tile height it
+# was measured at: profiles/pair2d_acoustic_4097sq_order4_fp64_hbm_counters.md.
+MEASURED_TRAFFIC = {(4097, 4, 0, "two-stage", 3): (491316 * 2.0 + 404713 * 1.0) * 1024}
+MEASURED_AT = "84c602f"
+MEASURED_TILE_ROWS = 40
 
 
 def pkg(sub):
@@ -427,6 +428,7 @@ def main():
                      "traffic_unit": "HBM bytes per launch of that kernel (rocprofv3 FETCH_SIZE x 2.0 + "
                                      "WRITE_SIZE x 1.0 KiB, copy-calibrated)",
                      "traffic_measured_at": MEASURED_AT,
+                     "traffic_tile_rows": MEASURED_TILE_ROWS, "tile_rows": main_m.get("tile_rows"),
                      "launch_bytes": main_m["launch_bytes"], "stage_ms": main_m["stage_ms"],
                      "step": {"bytes": sum(main_m["launch_bytes"]), "bytes_per_dof": main_m["kernel_bytes_per_dof"],
                               "kernel_ms": float(np.sum(main_m["stage_ms"])),
