@@ -34,10 +34,11 @@ template <typename T, int ORDER, bool KAHAN>
 PairLaunchFn pick_pair(int phys, int mat)
 {
   constexpr int PF1 = pair_pf(sizeof(T), 1), PF2 = pair_pf(sizeof(T), 2);
+  constexpr int PF2A = pair_pf(sizeof(T), 2, true);
   constexpr int NT = pair_nt(sizeof(T), ORDER);
   if (phys == PH_ACOUSTIC) {
     if (mat == M_UNIT) return &PairLauncher<T, ORDER, PH_ACOUSTIC, M_UNIT, KAHAN, NT, PF1, PF2>::launch;
-    if (mat == M_ARR) return &PairLauncher<T, ORDER, PH_ACOUSTIC, M_ARR, KAHAN, NT, PF1, PF2>::launch;
+    if (mat == M_ARR) return &PairLauncher<T, ORDER, PH_ACOUSTIC, M_ARR, KAHAN, NT, PF1, PF2A>::launch;
     if (mat == M_REF) return &PairLauncher<T, ORDER, PH_ACOUSTIC, M_REF, KAHAN, NT, PF1, PF2>::launch;
   }
   if (phys == PH_ADV2D) {
