@@ -38,7 +38,7 @@ struct BatchConfig {
 // Candidate launch shapes of the autotuner.
 // Short tiles (4, 8 rows) are for ensembles of small grids, where tall tiles
 // leave most of the persistent blocks without work.
-// The last four are two-stage configurations (sbp_kernels.hpp pair2d_kernel);
+// The last nine are two-stage configurations (sbp_kernels.hpp pair2d_kernel);
 // they need every instance to have an interior beyond its (CLS+R)-deep frame.
 // Their tiles are pair_nt wide; bx only sizes the unused one-stage fields.
 constexpr BatchConfig kConfigs[] = {
