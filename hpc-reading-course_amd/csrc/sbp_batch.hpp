@@ -203,7 +203,7 @@ template <typename T, int ORDER, int MAT, bool KAHAN> struct BatchPairLauncher {
   // part | kBatchCachedStores: the pair tiles store cached (pair_cached_stores)
   static hipError_t launch(int part, const void *inst, const int2 *work, int nwork, int grid, hipStream_t s)
   {
-    constexpr int PF1 = batch_pair_pf(sizeof(T)), PF2 = batch_pair_pf(sizeof(T));
+    constexpr int PF1 = pair_pf(sizeof(T), 1), PF2 = pair_pf(sizeof(T), 2, MAT == M_ARR);
     switch (part) {
       case 1: return go<1, PF1, true>(inst, work, nwork, grid, s);
       case 1 | kBatchCachedStores: return go<1, PF1, false>(inst, work, nwork, grid, s);

@@ -1057,12 +1057,12 @@ __device__ __forceinline__ void frame_patch_pair(const StageArgs<T> &eA, const S
 // rho/K: 0.267 against 0.290).  FP64 pair 2 runs three with rho/K arrays
 // (order 6: 0.419 against 0.446 ms, order 4: 0.335 against 0.342) and two
 // otherwise (rho = K = 1: the same time; rho = 2 + xy: 0.610 against 0.563).
-// The ensemble's FP64 tiles keep two in both launches.
+// The ensemble's tiles follow the same rule (FP64, 16 x 513^2 with rho/K
+// arrays: 0.188 against 0.205 ms per step with two rows in both launches).
 constexpr int pair_pf(int esz, int pair, bool acoustic_arrays = false)
 {
   return esz == 4 || pair == 1 || acoustic_arrays ? 3 : 2;
 }
-constexpr int batch_pair_pf(int esz) { return esz == 4 ? 3 : 2; }
 
 // Blocks [0, n_edge_blocks): frame patches, stages 1-2 (pair 1) or 3-4
 // (pair 2) with eA / eB the two stages' arguments; the rest: two-stage
