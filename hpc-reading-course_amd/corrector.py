@@ -284,9 +284,11 @@ def time_to_accuracy(params, norm, band=BAND, n_eval=32, seed0=900000, t_end=0.5
       plain         FP64 single-problem path, one instance after another, plain_n^2
       batched_fp32  FP32 ensemble, plain_n^2, one persistent kernel per stage
       learned       FP32 ensemble on 257^2 + the corrector after every step
-    the plain path's time-versus-error curve over 257^2 .. plain_n^2 (grids in
-    between measured against their own 4x-refined FP64 references) and the
-    time it needs at the learned form's mean and worst error; and, for comparison, the FP64 ensemble at plain_n^2 (batched_fp64), the
+    the plain path's and the FP32 ensemble's time-versus-error curves over
+    257^2 .. plain_n^2 (grids in between measured against their own
+    4x-refined FP64 references), the time each needs at L2 = 1e-4 and the
+    time the plain path needs at the learned form's mean and worst error;
+    and, for comparison, the FP64 ensemble at plain_n^2 (batched_fp64), the
     learned form with FP64 stages (learned_fp64), the plain 257^2 solve
     (the method without the network), and the plain FP64 path and the FP32
     ensemble with the order-6 operators on 257^2."""
@@ -331,9 +333,8 @@ def time_to_accuracy(params, norm, band=BAND, n_eval=32, seed0=900000, t_end=0.5
     res["plain_fp64_coarse"] = {"grid": N_COARSE, "wall_s": tc, "max_l2": max(ec), "mean_l2": float(np.mean(ec))}
     log("plain FP64 %d^2 (no network): %.3f s, max L2 %.2e" % (N_COARSE, tc, max(ec)))
 
-    # batched ensembles at plain_n (FP32: the chosen form; FP64 for comparison)
-    for prec, key in ((P.FP32, "batched_fp32"), (P.FP64, "batched_fp64")):
-        insts = _instances(seeds, plain_n, prec)
+    def run_batched(n, prec, refs):
+        insts = _instances(seeds, n, prec)
         dt = P.dt_from_cfl(insts[0][0], CFL)
         ns = int(round(t_end / dt))
         with capi.Batch([s for s, _ in insts]) as b:
@@ -345,10 +346,16 @@ def time_to_accuracy(params, norm, band=BAND, n_eval=32, seed0=900000, t_end=0.5
             b.step(dt, ns)
             b.synchronize()
             tb = time.time() - t
-            eb = [_l2(insts[k][0], b.get_state(k), ref_p[k]) for k in range(n_eval)]
+            eb = [_l2(insts[k][0], b.get_state(k), refs[k]) for k in range(n_eval)]
+        return tb, eb
+
+    # batched ensembles at plain_n (FP32: the chosen form; FP64 for comparison)
+    for prec, key in ((P.FP32, "batched_fp32"), (P.FP64, "batched_fp64")):
+        tb, eb = run_batched(plain_n, prec, ref_p)
         res[key] = {"grid": plain_n, "wall_s": tb, "max_l2": max(eb), "mean_l2": float(np.mean(eb))}
         log("%s %d^2: %.3f s, max L2 %.2e" % (key, plain_n, tb, max(eb)))
     tb = res["batched_fp32"]["wall_s"]
+    tbc, ebc = run_batched(N_COARSE, P.FP32, ref_c)
 
     # learned corrector on the coarse ensemble (FP32 stages: rounding is far
     # below the discretisation error at this grid; FP64 for comparison)
@@ -385,6 +392,8 @@ def time_to_accuracy(params, norm, band=BAND, n_eval=32, seed0=900000, t_end=0.5
     # learned form's error (mean and worst instance) is interpolated on it
     # (piecewise linear in log time against log error).
     curve = [(N_COARSE, tc, float(np.mean(ec)), max(ec)), (plain_n, tp, float(np.mean(ep)), max(ep))]
+    eb_p = res["batched_fp32"]
+    bcurve = [(N_COARSE, tbc, float(np.mean(ebc)), max(ebc)), (plain_n, tb, eb_p["mean_l2"], eb_p["max_l2"])]
     for n in equal_error_grids:
         nfine = RATIO * (n - 1) + 1
         fine_n = _instances(seeds, nfine, P.FP64)
@@ -400,8 +409,25 @@ def time_to_accuracy(params, norm, band=BAND, n_eval=32, seed0=900000, t_end=0.5
         curve.append((n, tn, float(np.mean(en)), max(en)))
         log("plain FP64 %d^2 (vs its %d^2 reference): %.3f s, mean L2 %.2e, max %.2e"
             % (n, nfine, tn, float(np.mean(en)), max(en)))
+        tbn, ebn = run_batched(n, P.FP32, refs_n)
+        bcurve.append((n, tbn, float(np.mean(ebn)), max(ebn)))
+        log("batched FP32 %d^2: %.3f s, mean L2 %.2e, max %.2e" % (n, tbn, float(np.mean(ebn)), max(ebn)))
     curve.sort()
+    bcurve.sort()
     res["plain_fp64_curve"] = [{"grid": n, "wall_s": t_, "mean_l2": em, "max_l2": ex} for n, t_, em, ex in curve]
+    res["batched_fp32_curve"] = [{"grid": n, "wall_s": t_, "mean_l2": em, "max_l2": ex} for n, t_, em, ex in bcurve]
+    # time to the target itself: each grid-refined form interpolated at L2 =
+    # target on its own curve (mean over the instances, and worst instance);
+    # the learned form solves one grid, below the target on the mean
+    target_l2 = res["target_l2"]
+    for which, col in (("mean", 2), ("max", 3)):
+        tp_t = interp_time_at_error([(c[1], c[col]) for c in curve], target_l2)
+        tb_t = interp_time_at_error([(c[1], c[col]) for c in bcurve], target_l2)
+        res["plain_time_at_target_%s_l2" % which] = tp_t
+        res["batched_fp32_time_at_target_%s_l2" % which] = tb_t
+        res["speedup_batched_fp32_at_target_%s_l2" % which] = tp_t / tb_t if tp_t and tb_t else None
+        reached = res["learned"][which + "_l2"] <= target_l2
+        res["speedup_learned_at_target_%s_l2" % which] = tp_t / tl if tp_t and reached else None
     for which, col in (("mean", 2), ("max", 3)):
         target = res["learned"][which + "_l2"]
         t_eq = interp_time_at_error([(c[1], c[col]) for c in curve], target)

@@ -139,11 +139,12 @@ bool pair_supported(const sbp_ctx *c)
 }
 
 // Two-stage stepping is the default where it was measured faster: grids of
-// at least 1000^2 interior points.  2D acoustics, order 4 FP64, ms per step
-// against one launch per stage: 1025^2 0.072 / 0.079, 1537^2 0.120 / 0.157,
-// 2049^2 0.176 / 0.238, 4097^2 0.667 / 0.980 (with the boundary frame in
-// launches of its own; 0.466 at 4097^2 with the frame patches).  2D
-// advection, order 4, one-stage / two-stage ms, constant velocity and
+// at least 450^2 interior points.  2D acoustics, order 4 FP64, one-stage /
+// two-stage ms per step with the frame patches (rho = K = 1, then rho/K
+// arrays): 385^2 0.0201 / 0.0236 and 0.0213 / 0.0237; 513^2 0.0278 /
+// 0.0244 and 0.0342 / 0.0246; 769^2 0.053 / 0.035 and 0.059 / 0.037; 1025^2
+// 0.080 / 0.040 and 0.091 / 0.051; 4097^2 0.98 / 0.45.  2D advection, order
+// 4, one-stage / two-stage ms, constant velocity and
 // per-point fields: 1025^2 FP64 0.033 / 0.025 and 0.045 / 0.027, FP32 0.019 /
 // 0.018 and 0.026 / 0.020; 2049^2 FP64 0.096 / 0.052 and 0.131 / 0.063, FP32
 // 0.064 / 0.030 and 0.078 / 0.036; 4097^2 FP64 0.316 / 0.173 and 0.449 /
@@ -151,7 +152,7 @@ bool pair_supported(const sbp_ctx *c)
 bool pair_default(const sbp_ctx *c)
 {
   const int D = c->cls + c->order / 2;
-  return pair_supported(c) && (long)(c->nx - 2 * D) * (c->ny - 2 * D) >= 1000L * 1000L;
+  return pair_supported(c) && (long)(c->nx - 2 * D) * (c->ny - 2 * D) >= 450L * 450L;
 }
 
 int default_ty_rows(int nx, int ny, int cls)

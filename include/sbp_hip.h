@@ -113,7 +113,7 @@ int  sbp_set_tile_rows(sbp_ctx *ctx, int rows);
  * themselves; 5 array passes per step instead of 13, same arithmetic per
  * point.  2D acoustic contexts (every material model) and 2D advection
  * contexts (constant or per-point velocity); SBP_EINVAL otherwise.  On by
- * default for grids of at least 1000^2 interior points (where it was
+ * default for grids of at least 450^2 interior points (where it was
  * measured faster).  rows: tile height of the two-stage tiles
  * (0 = chosen from the grid).  A performance knob with no reference
  * counterpart: it changes how ts_rk4's steps are scheduled on the device

@@ -195,19 +195,19 @@ def test_fused_then_rk45_and_time_stages(capi, orc):
 
 
 def test_fused_default_by_grid_size(capi):
-    """On by default for 2D acoustics and 2D advection with >= 1000^2
+    """On by default for 2D acoustics and 2D advection with >= 450^2
     interior points, off below and for the 1D problems."""
-    big, _, _ = P.baseline_acoustic(1025, 4)
+    big, _, _ = P.baseline_acoustic(463, 4)      # 451^2 interior points
     with capi.Context(big) as ctx:
         assert ctx.fused
     with capi.Context(big.with_(precision=P.FP32_KAHAN)) as ctx:
         assert ctx.fused
-    small, _, _ = P.baseline_acoustic(1001, 4)   # 989^2 interior points
+    small, _, _ = P.baseline_acoustic(461, 4)    # 449^2 interior points
     with capi.Context(small) as ctx:
         assert not ctx.fused
-    with capi.Context(adv_spec("adv", 4, 1013, 1013)) as ctx:   # 1001^2 interior points
+    with capi.Context(adv_spec("adv", 4, 463, 463)) as ctx:
         assert ctx.fused
-    with capi.Context(adv_spec("adv", 4, 1011, 1011)) as ctx:
+    with capi.Context(adv_spec("adv", 4, 461, 461)) as ctx:
         assert not ctx.fused
     with capi.Context(P.adv1d(5000, 4)[0]) as ctx:
         assert not ctx.fused
